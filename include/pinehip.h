@@ -1,0 +1,159 @@
+/* pinehip.h - C ABI of the MI355X (gfx950) path-tracing hot path that sits
+ * behind pine's Accel and Integrator plug-in points.
+ *
+ * Pine itself has no C ABI: its boundary is the C++ virtual interface
+ *   class Accel { Initialize(const Scene*); bool Hit(Ray) const;
+ *                 bool Intersect(Ray&, Interaction&) const; }   (src/core/accel.h:8-14)
+ * called one ray at a time.  A per-ray virtual call is the wrong granularity for
+ * a GPU, so this ABI takes whole ray batches.  Each entry point names the pine
+ * function whose semantics it reproduces.
+ *
+ * Conventions
+ *   - Every function returns 0 on success or a negative PG_E* code; the message
+ *     for the last failure on the calling thread is pg_last_error().
+ *   - Pointers named d_* are device (HBM) pointers; h_* are host pointers.
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream).  Work is
+ *     stream-ordered: results are ready when the stream is synchronised.
+ *   - An accel handle may be traced from several threads at once; building and
+ *     releasing it must not overlap any other call on the same handle.
+ */
+#ifndef PINEHIP_H
+#define PINEHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PG_OK 0
+#define PG_E_ARG (-1)     /* invalid argument (null pointer, bad size, bad handle) */
+#define PG_E_STATE (-2)   /* call out of order (trace before build, add after build) */
+#define PG_E_HIP (-3)     /* a HIP runtime call failed */
+#define PG_E_NOMEM (-4)   /* host or device allocation failed */
+#define PG_E_IO (-5)      /* file could not be read or written */
+#define PG_E_PARSE (-6)   /* scene text could not be parsed */
+
+/* One ray, 32 bytes: pine's Ray{o, d, tmin, tmax} (src/core/ray.h:10-32) with
+ * tmin moved next to o so the record is two 16-byte loads. */
+typedef struct pg_ray {
+    float o[3];
+    float tmin;
+    float d[3];
+    float tmax;
+} pg_ray;
+
+/* Closest-hit result, 8 bytes.  prim is the global primitive id (see
+ * pg_accel_add_*), -1 on a miss; t is the hit distance pine writes into
+ * ray.tmax (bvh.cpp:534-610 + geometry.h:271-294). */
+typedef struct pg_hit {
+    int32_t prim;
+    float t;
+} pg_hit;
+
+/* Barycentric / surface parameters of a closest hit (pine's Interaction::uv):
+ * triangles (u,v) as in Triangle::Intersect, spheres (phi,theta), rects (u,v). */
+typedef struct pg_hit_uv {
+    float u;
+    float v;
+} pg_hit_uv;
+
+typedef struct pg_accel pg_accel;
+
+const char* pg_last_error(void);
+const char* pg_version(void);
+
+/* --- Accel (replaces pine::BVH, src/impl/accel/bvh.h:72-85) ---------------- */
+
+/* Creates an empty accel (host only; the device is the one current at build). */
+int pg_accel_create(pg_accel** out);
+
+/* Adds a triangle mesh: pine's TriangleMesh{vertices, indices} (geometry.h:364-413).
+ * Triangle k of the mesh gets global prim id *first_prim_out + k.  Copies the data. */
+int pg_accel_add_mesh(pg_accel* a, const float* h_vertices, int64_t n_vertices,
+                      const uint32_t* h_indices, int64_t n_triangles, int32_t* first_prim_out);
+
+/* Analytic shapes: pine's Sphere{c, r} (geometry.h:143-168), Rect{position, ex, ey}
+ * (geometry.h:334-362, ex/ey unnormalised as in the scene file), Plane{position,
+ * normal} (geometry.h:118-141).  Each is one prim id. */
+int pg_accel_add_sphere(pg_accel* a, const float c[3], float r, int32_t* prim_out);
+int pg_accel_add_rect(pg_accel* a, const float position[3], const float ex[3],
+                      const float ey[3], int32_t* prim_out);
+int pg_accel_add_plane(pg_accel* a, const float position[3], const float normal[3],
+                       int32_t* prim_out);
+
+/* Build options.  width: 2 (binary) or 4 (4-wide nodes); leaf_max: 1..8 prims;
+ * bins: SAH bins per axis; threads: host build threads (0 = all cores). */
+typedef struct pg_build_options {
+    int32_t width;
+    int32_t leaf_max;
+    int32_t bins;
+    int32_t threads;
+} pg_build_options;
+
+/* Builds the hierarchy on the host and uploads it to HBM.  Replaces
+ * BVH::Initialize (bvh.cpp:612-650).  opts may be NULL for defaults. */
+int pg_accel_build(pg_accel* a, const pg_build_options* opts);
+
+/* Closest hit for n rays: replaces BVH::Intersect (bvh.cpp:669-709).  Hit rule:
+ * the smallest t in (tmin, tmax) over all prims, using pine's per-shape tests;
+ * on an exact tie in t the smaller prim id wins (pine keeps the first found in
+ * its own traversal order).  d_uv may be NULL. */
+int pg_trace_closest(pg_accel* a, const pg_ray* d_rays, int64_t n, pg_hit* d_hits,
+                     pg_hit_uv* d_uv, void* stream);
+
+/* Any hit for n rays: replaces BVH::Hit (bvh.cpp:652-667) with pine's Hit-variant
+ * tests (closed t interval for triangles, strict u+v<1).  d_occluded[i] = 0/1. */
+int pg_trace_any(pg_accel* a, const pg_ray* d_rays, int64_t n, int32_t* d_occluded,
+                 void* stream);
+
+/* pg_trace_closest plus, per ray, the work done: d_counts[2i] = nodes fetched,
+ * d_counts[2i+1] = primitives tested.  For measurement (algorithmic bytes);
+ * no pine counterpart beyond the per-node counter of bvh.cpp:553. */
+int pg_trace_closest_counts(pg_accel* a, const pg_ray* d_rays, int64_t n, pg_hit* d_hits,
+                            int32_t* d_counts, void* stream);
+
+/* Statistics of the uploaded hierarchy (for reporting and roofline accounting). */
+typedef struct pg_accel_stats {
+    int64_t n_prims;
+    int64_t n_nodes;
+    int64_t node_bytes;
+    int64_t prim_bytes;
+    int32_t width;
+    int32_t max_depth;
+    double build_ms;
+} pg_accel_stats;
+int pg_accel_get_stats(const pg_accel* a, pg_accel_stats* out);
+
+/* Maps a global prim id back to (shape index in add order, prim within shape). */
+int pg_accel_prim_shape(const pg_accel* a, int32_t prim, int32_t* shape, int32_t* local);
+
+/* Kernel launch shape used by the trace calls: block = threads per workgroup
+ * (64..1024, multiple of 64); 0 keeps the current value. */
+int pg_accel_set_launch(pg_accel* a, int32_t block);
+
+int pg_accel_release(pg_accel* a);
+
+/* --- Deterministic benchmark workloads (no pine counterpart) ---------------- */
+
+/* Geodesic icosphere of the given frequency (20*freq^2 triangles), displaced
+ * radially by seeded 5-octave value-noise fBm of amplitude `amp`, truncated to
+ * max_triangles (0 = all).  Call with h_vertices/h_indices NULL to get sizes. */
+int pg_gen_icosphere(int32_t freq, float amp, uint64_t seed, int64_t max_triangles,
+                     float* h_vertices, int64_t* n_vertices, uint32_t* h_indices,
+                     int64_t* n_triangles);
+
+/* Ray stream for the traversal benchmark: n_primary pinhole rays of a
+ * res x res camera at `eye` looking at the origin with vertical fov `fov_deg`,
+ * then n_bounce cosine-distributed rays leaving seeded points of the mesh
+ * (origins offset with pine's OffsetRayOrigin, ray.h:34-45). */
+int pg_gen_rays(const float* h_vertices, int64_t n_vertices, const uint32_t* h_indices,
+                int64_t n_triangles, int32_t res, const float eye[3], float fov_deg,
+                int64_t n_bounce, uint64_t seed, pg_ray* h_rays);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PINEHIP_H */

@@ -1,0 +1,217 @@
+"""pine_amd: MI355X (gfx950) implementation of pine's path-tracing hot path.
+
+The native library ``libpinehip.so`` (built in-tree by ``pine_amd.build``)
+exports the C ABI declared in ``include/pinehip.h``; this module binds it with
+ctypes for tests and the benchmark.  Device buffers are torch tensors on the
+current HIP device; torch is only the allocator and stream provider here.
+
+There is no fallback: if the library is missing or fails to load, every entry
+point raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpinehip.so")
+
+PG_OK = 0
+EXPORTED = [
+    "pg_last_error", "pg_version", "pg_accel_create", "pg_accel_add_mesh",
+    "pg_accel_add_sphere", "pg_accel_add_rect", "pg_accel_add_plane", "pg_accel_build",
+    "pg_trace_closest", "pg_trace_any", "pg_accel_get_stats", "pg_accel_prim_shape",
+    "pg_accel_set_launch", "pg_accel_release", "pg_gen_icosphere", "pg_gen_rays",
+    "pg_trace_closest_counts",
+]
+
+
+class PineError(RuntimeError):
+    pass
+
+
+class _BuildOptions(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int32), ("leaf_max", ctypes.c_int32),
+                ("bins", ctypes.c_int32), ("threads", ctypes.c_int32)]
+
+
+class _Stats(ctypes.Structure):
+    _fields_ = [("n_prims", ctypes.c_int64), ("n_nodes", ctypes.c_int64),
+                ("node_bytes", ctypes.c_int64), ("prim_bytes", ctypes.c_int64),
+                ("width", ctypes.c_int32), ("max_depth", ctypes.c_int32),
+                ("build_ms", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libpinehip.so once; raises PineError if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PineError(f"{LIB_PATH} is missing: build it with pine_amd.build.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i64, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.pg_last_error.restype = ctypes.c_char_p
+    L.pg_version.restype = ctypes.c_char_p
+    L.pg_accel_create.argtypes = [ctypes.POINTER(vp)]
+    L.pg_accel_add_mesh.argtypes = [vp, vp, i64, vp, i64, ctypes.POINTER(i32)]
+    L.pg_accel_add_sphere.argtypes = [vp, fp, ctypes.c_float, ctypes.POINTER(i32)]
+    L.pg_accel_add_rect.argtypes = [vp, fp, fp, fp, ctypes.POINTER(i32)]
+    L.pg_accel_add_plane.argtypes = [vp, fp, fp, ctypes.POINTER(i32)]
+    L.pg_accel_build.argtypes = [vp, ctypes.POINTER(_BuildOptions)]
+    L.pg_trace_closest.argtypes = [vp, vp, i64, vp, vp, vp]
+    L.pg_trace_any.argtypes = [vp, vp, i64, vp, vp]
+    L.pg_trace_closest_counts.argtypes = [vp, vp, i64, vp, vp, vp]
+    L.pg_accel_get_stats.argtypes = [vp, ctypes.POINTER(_Stats)]
+    L.pg_accel_prim_shape.argtypes = [vp, i32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
+    L.pg_accel_set_launch.argtypes = [vp, i32]
+    L.pg_accel_release.argtypes = [vp]
+    L.pg_gen_icosphere.argtypes = [i32, ctypes.c_float, ctypes.c_uint64, i64, vp,
+                                   ctypes.POINTER(i64), vp, ctypes.POINTER(i64)]
+    L.pg_gen_rays.argtypes = [vp, i64, vp, i64, i32, fp, ctypes.c_float, i64,
+                              ctypes.c_uint64, vp]
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != PG_OK:
+        raise PineError(f"pinehip error {rc}: {lib().pg_last_error().decode()}")
+    return rc
+
+
+def _f3(v):
+    return (ctypes.c_float * 3)(*[float(x) for x in v])
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _stream_handle():
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+# --- workloads -------------------------------------------------------------------
+
+def gen_icosphere(freq, amp=0.1, seed=1, max_triangles=0):
+    """Displaced geodesic icosphere; returns (vertices float32 [nv,3], indices uint32 [nt,3])."""
+    L = lib()
+    nv, nt = ctypes.c_int64(), ctypes.c_int64()
+    check(L.pg_gen_icosphere(freq, amp, seed, max_triangles, None, ctypes.byref(nv), None,
+                             ctypes.byref(nt)))
+    v = np.zeros((nv.value, 3), np.float32)
+    i = np.zeros((nt.value, 3), np.uint32)
+    check(L.pg_gen_icosphere(freq, amp, seed, max_triangles, _ptr(v), ctypes.byref(nv), _ptr(i),
+                             ctypes.byref(nt)))
+    return v[: nv.value], i[: nt.value]
+
+
+def gen_rays(vertices, indices, res, eye, fov_deg, n_bounce, seed=7):
+    """Traversal-benchmark rays as float32 [n, 8] = (o.xyz, tmin, d.xyz, tmax)."""
+    v = np.ascontiguousarray(vertices, np.float32)
+    i = np.ascontiguousarray(indices, np.uint32)
+    n = res * res + n_bounce
+    rays = np.zeros((n, 8), np.float32)
+    check(lib().pg_gen_rays(_ptr(v), len(v), _ptr(i), len(i), res, _f3(eye), fov_deg, n_bounce,
+                            seed, _ptr(rays)))
+    return rays
+
+
+# --- accel -----------------------------------------------------------------------
+
+class Accel:
+    """Owns a pg_accel handle (pine's Accel, src/core/accel.h:8-14)."""
+
+    def __init__(self):
+        self.h = ctypes.c_void_p()
+        check(lib().pg_accel_create(ctypes.byref(self.h)))
+
+    def add_mesh(self, vertices, indices):
+        v = np.ascontiguousarray(vertices, np.float32)
+        i = np.ascontiguousarray(indices, np.uint32)
+        first = ctypes.c_int32()
+        check(lib().pg_accel_add_mesh(self.h, _ptr(v), len(v), _ptr(i), len(i),
+                                      ctypes.byref(first)))
+        return first.value
+
+    def add_sphere(self, c, r):
+        p = ctypes.c_int32()
+        check(lib().pg_accel_add_sphere(self.h, _f3(c), r, ctypes.byref(p)))
+        return p.value
+
+    def add_rect(self, position, ex, ey):
+        p = ctypes.c_int32()
+        check(lib().pg_accel_add_rect(self.h, _f3(position), _f3(ex), _f3(ey), ctypes.byref(p)))
+        return p.value
+
+    def add_plane(self, position, normal):
+        p = ctypes.c_int32()
+        check(lib().pg_accel_add_plane(self.h, _f3(position), _f3(normal), ctypes.byref(p)))
+        return p.value
+
+    def build(self, width=2, leaf_max=4, bins=32, threads=0):
+        o = _BuildOptions(width, leaf_max, bins, threads)
+        check(lib().pg_accel_build(self.h, ctypes.byref(o)))
+        return self
+
+    def set_launch(self, block):
+        check(lib().pg_accel_set_launch(self.h, block))
+
+    def stats(self):
+        s = _Stats()
+        check(lib().pg_accel_get_stats(self.h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in _Stats._fields_}
+
+    def prim_shape(self, prim):
+        s, l = ctypes.c_int32(), ctypes.c_int32()
+        check(lib().pg_accel_prim_shape(self.h, prim, ctypes.byref(s), ctypes.byref(l)))
+        return s.value, l.value
+
+    def trace_closest(self, d_rays, d_hits=None, d_uv=None):
+        """d_rays: cuda float32 [n,8]. Returns int32 [n,2] (prim, t bits)."""
+        import torch
+        n = d_rays.shape[0]
+        if d_hits is None:
+            d_hits = torch.empty((n, 2), dtype=torch.int32, device=d_rays.device)
+        check(lib().pg_trace_closest(self.h, ctypes.c_void_p(d_rays.data_ptr()), n,
+                                     ctypes.c_void_p(d_hits.data_ptr()),
+                                     ctypes.c_void_p(d_uv.data_ptr()) if d_uv is not None else None,
+                                     _stream_handle()))
+        return d_hits
+
+    def trace_counts(self, d_rays):
+        """Closest hit plus per-ray (nodes fetched, prims tested), int32 [n, 2]."""
+        import torch
+        n = d_rays.shape[0]
+        d_hits = torch.empty((n, 2), dtype=torch.int32, device=d_rays.device)
+        d_cnt = torch.empty((n, 2), dtype=torch.int32, device=d_rays.device)
+        check(lib().pg_trace_closest_counts(self.h, ctypes.c_void_p(d_rays.data_ptr()), n,
+                                            ctypes.c_void_p(d_hits.data_ptr()),
+                                            ctypes.c_void_p(d_cnt.data_ptr()), _stream_handle()))
+        return d_hits, d_cnt
+
+    def trace_any(self, d_rays, d_occ=None):
+        import torch
+        n = d_rays.shape[0]
+        if d_occ is None:
+            d_occ = torch.empty((n,), dtype=torch.int32, device=d_rays.device)
+        check(lib().pg_trace_any(self.h, ctypes.c_void_p(d_rays.data_ptr()), n,
+                                 ctypes.c_void_p(d_occ.data_ptr()), _stream_handle()))
+        return d_occ
+
+    def release(self):
+        if self.h:
+            lib().pg_accel_release(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass

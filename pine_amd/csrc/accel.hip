@@ -1,0 +1,677 @@
+// Accel: host build + gfx950 traversal kernels + the C ABI that replaces pine's
+// BVH (src/impl/accel/bvh.h:72-85, bvh.cpp:612-709).
+//
+// Traversal: one ray per lane, ordered depth-first descent (nearest child
+// first, as pine orders by slab distance, bvh.cpp:584-592), with the
+// per-lane stack kept in LDS, laid out [depth][lane] so that the 64 lanes of a
+// wave touch 64 consecutive dwords whatever their depths.  Node records are
+// read as 16-byte vector loads.  Nodes store the boxes of their children, so
+// one node fetch tests all children at once.
+#include "bvh_build.h"
+#include "pg_api.h"
+#include "pg_isect.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace pg {
+
+thread_local std::string g_last_error;
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+}  // namespace pg
+
+using namespace pg;
+
+#define PG_HIP(call)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return pg_fail(PG_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+struct HostShape {
+    int32_t kind;        // 0 mesh, 1 analytic
+    int32_t first_prim;  // global id of its first prim
+    int32_t count;
+};
+
+struct pg_accel {
+    int device = 0;
+    // Host copies in global prim id order.
+    std::vector<int32_t> prim_kind;   // 0 triangle, 1 analytic
+    std::vector<int32_t> prim_index;  // index into tri_v / ana_rec
+    std::vector<float> tri_v;         // 9 floats per triangle: v0, v1, v2
+    std::vector<float> ana_rec;       // 16 floats per analytic record (prim slot = 0)
+    std::vector<float> ana_box;       // 6 floats per analytic record
+    std::vector<HostShape> shapes;
+    std::vector<int32_t> prim_shape;  // global prim -> shape index
+    // Device copies.
+    float4* d_nodes = nullptr;
+    float4* d_tris = nullptr;
+    float4* d_ana = nullptr;
+    int width = 2;
+    int max_depth = 0;
+    int64_t n_nodes = 0;
+    int64_t n_tri_slots = 0, n_ana_slots = 0;
+    double build_ms = 0.0;
+    bool built = false;
+    int block = 128;
+
+    void free_device() {
+        if (d_nodes)
+            (void)hipFree(d_nodes);
+        if (d_tris)
+            (void)hipFree(d_tris);
+        if (d_ana)
+            (void)hipFree(d_ana);
+        d_nodes = nullptr;
+        d_tris = nullptr;
+        d_ana = nullptr;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Device side
+
+namespace {
+
+// LDS stack entries per lane: a binary tree needs one per level; a 4-wide node
+// can leave up to three siblings per level.
+template <int W>
+struct StackDepth;
+template <>
+struct StackDepth<2> {
+    static constexpr int value = 32;
+};
+template <>
+struct StackDepth<4> {
+    static constexpr int value = 48;
+};
+
+struct RayIn {
+    f3 o, d;
+    float tmin, tmax;
+    f3 inv, oinv;
+};
+
+__device__ __forceinline__ RayIn load_ray(const pg_ray* rays, int64_t i) {
+    const float4* r4 = reinterpret_cast<const float4*>(rays + i);
+    float4 a = r4[0], b = r4[1];
+    RayIn r;
+    r.o = mk(a.x, a.y, a.z);
+    r.tmin = a.w;
+    r.d = mk(b.x, b.y, b.z);
+    r.tmax = b.w;
+    // pine's SafeRcp (vecmath.h:1095-1100)
+    r.inv.x = r.d.x == 0.0f ? 1e20f : 1.0f / r.d.x;
+    r.inv.y = r.d.y == 0.0f ? 1e20f : 1.0f / r.d.y;
+    r.inv.z = r.d.z == 0.0f ? 1e20f : 1.0f / r.d.z;
+    r.oinv = mk(-r.o.x * r.inv.x, -r.o.y * r.inv.y, -r.o.z * r.inv.z);
+    return r;
+}
+
+// Slab test of one box; returns entry distance or +inf-like miss flag.
+__device__ __forceinline__ bool slab(const RayIn& r, float lx, float ly, float lz, float hx,
+                                     float hy, float hz, float tmax, float& tnear) {
+    float x0 = __builtin_fmaf(lx, r.inv.x, r.oinv.x), x1 = __builtin_fmaf(hx, r.inv.x, r.oinv.x);
+    float y0 = __builtin_fmaf(ly, r.inv.y, r.oinv.y), y1 = __builtin_fmaf(hy, r.inv.y, r.oinv.y);
+    float z0 = __builtin_fmaf(lz, r.inv.z, r.oinv.z), z1 = __builtin_fmaf(hz, r.inv.z, r.oinv.z);
+    float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r.tmin));
+    float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
+    tnear = tn;
+    return tn <= tf;
+}
+
+template <bool kAny>
+__device__ __forceinline__ bool leaf_test(int32_t code, const float4* __restrict__ tris,
+                                          const float4* __restrict__ ana, const RayIn& r,
+                                          float& best_t, int32_t& best_prim, float& bu, float& bv,
+                                          int& tested) {
+    const int count = ((code >> kLeafCountShift) & 15) + 1;
+    tested += count;
+    const int32_t start = code & kLeafStartMask;
+    if (!(code & kLeafAnalytic)) {
+        for (int k = 0; k < count; k++) {
+            const float4* t = tris + 3 * (int64_t)(start + k);
+            float4 a = t[0], b = t[1], c = t[2];
+            f3 v0 = mk(a.x, a.y, a.z), e1 = mk(b.x, b.y, b.z), e2 = mk(c.x, c.y, c.z);
+            if (kAny) {
+                if (tri_any(r.o, r.d, r.tmin, r.tmax, v0, e1, e2))
+                    return true;
+            } else {
+                tri_closest(r.o, r.d, r.tmin, v0, e1, e2, __float_as_int(a.w), best_t, best_prim,
+                            bu, bv);
+            }
+        }
+    } else {
+        for (int k = 0; k < count; k++) {
+            const float4* q = ana + 4 * (int64_t)(start + k);
+            float rec[16];
+            float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+            rec[0] = q0.x, rec[1] = q0.y, rec[2] = q0.z, rec[3] = q0.w;
+            rec[4] = q1.x, rec[5] = q1.y, rec[6] = q1.z, rec[7] = q1.w;
+            rec[8] = q2.x, rec[9] = q2.y, rec[10] = q2.z, rec[11] = q2.w;
+            rec[12] = q3.x, rec[13] = q3.y, rec[14] = q3.z, rec[15] = q3.w;
+            if (kAny) {
+                if (analytic_any(rec, r.o, r.d, r.tmin, r.tmax))
+                    return true;
+            } else {
+                analytic_closest(rec, r.o, r.d, r.tmin, best_t, best_prim, bu, bv);
+            }
+        }
+    }
+    return false;
+}
+
+// Binary nodes: 64 B = {lo0.xyz, hi0.x}, {hi0.yz, lo1.xy}, {lo1.z, hi1.xyz}, {c0, c1, -, -}.
+template <int kBlock, bool kAny, bool kStats>
+__global__ __launch_bounds__(kBlock) void k_trace2(const float4* __restrict__ nodes,
+                                                   const float4* __restrict__ tris,
+                                                   const float4* __restrict__ ana,
+                                                   const pg_ray* __restrict__ rays, int64_t n,
+                                                   pg_hit* __restrict__ hits,
+                                                   pg_hit_uv* __restrict__ uvs,
+                                                   int32_t* __restrict__ occluded,
+                                                   int2* __restrict__ counts) {
+    __shared__ int32_t stack[StackDepth<2>::value * kBlock];
+    const int lane = threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + lane;
+    if (i >= n)
+        return;
+    const RayIn r = load_ray(rays, i);
+    float best_t = r.tmax, bu = 0.0f, bv = 0.0f;
+    int32_t best_prim = -1;
+    int sp = 0;
+    int32_t node = 0;
+    bool found_any = false;
+    int visited = 0, tested = 0;
+    while (true) {
+        if (node >= 0) {
+            visited++;
+            const float4* nd = nodes + 4 * (int64_t)node;
+            float4 a = nd[0], b = nd[1], c = nd[2], d = nd[3];
+            const int32_t c0 = __float_as_int(d.x), c1 = __float_as_int(d.y);
+            const float lim = kAny ? r.tmax : best_t;
+            float t0, t1;
+            bool h0 = slab(r, a.x, a.y, a.z, a.w, b.x, b.y, lim, t0) && c0 != kEmptyChild;
+            bool h1 = slab(r, b.z, b.w, c.x, c.y, c.z, c.w, lim, t1) && c1 != kEmptyChild;
+            if (h0 && h1) {
+                int32_t nearc = t0 <= t1 ? c0 : c1;
+                int32_t farc = t0 <= t1 ? c1 : c0;
+                stack[sp * kBlock + lane] = farc;
+                sp++;
+                node = nearc;
+                continue;
+            }
+            if (h0 || h1) {
+                node = h0 ? c0 : c1;
+                continue;
+            }
+        } else {
+            if (leaf_test<kAny>(node, tris, ana, r, best_t, best_prim, bu, bv, tested)) {
+                found_any = true;
+                break;
+            }
+        }
+        if (sp == 0)
+            break;
+        sp--;
+        node = stack[sp * kBlock + lane];
+    }
+    if (kStats)
+        counts[i] = make_int2(visited, tested);
+    if (kAny) {
+        occluded[i] = found_any ? 1 : 0;
+    } else {
+        pg_hit h;
+        h.prim = best_prim;
+        h.t = best_prim >= 0 ? best_t : r.tmax;
+        hits[i] = h;
+        if (uvs) {
+            pg_hit_uv uv;
+            uv.u = bu;
+            uv.v = bv;
+            uvs[i] = uv;
+        }
+    }
+}
+
+// 4-wide nodes: 128 B = lox[4], loy[4], loz[4], hix[4], hiy[4], hiz[4], child[4], pad.
+template <int kBlock, bool kAny, bool kStats>
+__global__ __launch_bounds__(kBlock) void k_trace4(const float4* __restrict__ nodes,
+                                                   const float4* __restrict__ tris,
+                                                   const float4* __restrict__ ana,
+                                                   const pg_ray* __restrict__ rays, int64_t n,
+                                                   pg_hit* __restrict__ hits,
+                                                   pg_hit_uv* __restrict__ uvs,
+                                                   int32_t* __restrict__ occluded,
+                                                   int2* __restrict__ counts) {
+    __shared__ int32_t stack[StackDepth<4>::value * kBlock];
+    const int lane = threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + lane;
+    if (i >= n)
+        return;
+    const RayIn r = load_ray(rays, i);
+    float best_t = r.tmax, bu = 0.0f, bv = 0.0f;
+    int32_t best_prim = -1;
+    int sp = 0;
+    int32_t node = 0;
+    bool found_any = false;
+    int visited = 0, tested = 0;
+    while (true) {
+        if (node >= 0) {
+            visited++;
+            const float4* nd = nodes + 8 * (int64_t)node;
+            float4 lx = nd[0], ly = nd[1], lz = nd[2], hx = nd[3], hy = nd[4], hz = nd[5];
+            int4 ch = *reinterpret_cast<const int4*>(nd + 6);
+            const float lim = kAny ? r.tmax : best_t;
+            float t[4];
+            int32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
+            bool h[4];
+            h[0] = slab(r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, lim, t[0]) && c[0] != kEmptyChild;
+            h[1] = slab(r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, lim, t[1]) && c[1] != kEmptyChild;
+            h[2] = slab(r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, lim, t[2]) && c[2] != kEmptyChild;
+            h[3] = slab(r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, lim, t[3]) && c[3] != kEmptyChild;
+            // Sort hit children by entry distance (misses pushed to +inf), then push
+            // all but the nearest, farthest first.
+            float key[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                key[k] = h[k] ? t[k] : __int_as_float(0x7f800000);
+#define PG_CSWAP(a, b)                                \
+    if (key[b] < key[a]) {                            \
+        float tk = key[a];                            \
+        key[a] = key[b];                              \
+        key[b] = tk;                                  \
+        int32_t tc = c[a];                            \
+        c[a] = c[b];                                  \
+        c[b] = tc;                                    \
+    }
+            PG_CSWAP(0, 1)
+            PG_CSWAP(2, 3)
+            PG_CSWAP(0, 2)
+            PG_CSWAP(1, 3)
+            PG_CSWAP(1, 2)
+#undef PG_CSWAP
+            const int nhit = (int)h[0] + (int)h[1] + (int)h[2] + (int)h[3];
+            if (nhit > 0) {
+                for (int k = nhit - 1; k >= 1; k--) {
+                    stack[sp * kBlock + lane] = c[k];
+                    sp++;
+                }
+                node = c[0];
+                continue;
+            }
+        } else {
+            if (leaf_test<kAny>(node, tris, ana, r, best_t, best_prim, bu, bv, tested)) {
+                found_any = true;
+                break;
+            }
+        }
+        if (sp == 0)
+            break;
+        sp--;
+        node = stack[sp * kBlock + lane];
+    }
+    if (kStats)
+        counts[i] = make_int2(visited, tested);
+    if (kAny) {
+        occluded[i] = found_any ? 1 : 0;
+    } else {
+        pg_hit hh;
+        hh.prim = best_prim;
+        hh.t = best_prim >= 0 ? best_t : r.tmax;
+        hits[i] = hh;
+        if (uvs) {
+            pg_hit_uv uv;
+            uv.u = bu;
+            uv.v = bv;
+            uvs[i] = uv;
+        }
+    }
+}
+
+template <bool kAny, bool kStats>
+hipError_t launch_trace(const pg_accel* a, const pg_ray* rays, int64_t n, pg_hit* hits,
+                        pg_hit_uv* uvs, int32_t* occ, int2* counts, hipStream_t s) {
+    const int block = a->block;
+    const int64_t grid = (n + block - 1) / block;
+#define PG_LAUNCH(K, B)                                                                        \
+    hipLaunchKernelGGL((K<B, kAny, kStats>), dim3((unsigned)grid), dim3(B), 0, s, a->d_nodes,   \
+                       a->d_tris, a->d_ana, rays, n, hits, uvs, occ, counts)
+    if (a->width == 2) {
+        if (block == 64)
+            PG_LAUNCH(k_trace2, 64);
+        else if (block == 256)
+            PG_LAUNCH(k_trace2, 256);
+        else
+            PG_LAUNCH(k_trace2, 128);
+    } else {
+        if (block == 64)
+            PG_LAUNCH(k_trace4, 64);
+        else if (block == 256)
+            PG_LAUNCH(k_trace4, 256);
+        else
+            PG_LAUNCH(k_trace4, 128);
+    }
+#undef PG_LAUNCH
+    return hipGetLastError();
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+extern "C" const char* pg_last_error(void) { return pg::g_last_error.c_str(); }
+extern "C" const char* pg_version(void) { return "pinehip 0.1 gfx950"; }
+
+extern "C" int pg_accel_create(pg_accel** out) {
+    if (!out)
+        return pg_fail(PG_E_ARG, "pg_accel_create: out is null");
+    pg_accel* a = new (std::nothrow) pg_accel();
+    if (!a)
+        return pg_fail(PG_E_NOMEM, "pg_accel_create: out of host memory");
+    *out = a;
+    return PG_OK;
+}
+
+static int check_open(pg_accel* a, const char* fn) {
+    if (!a)
+        return pg_fail(PG_E_ARG, std::string(fn) + ": null accel");
+    if (a->built)
+        return pg_fail(PG_E_STATE, std::string(fn) + ": accel already built");
+    return PG_OK;
+}
+
+extern "C" int pg_accel_add_mesh(pg_accel* a, const float* v, int64_t nv, const uint32_t* idx,
+                                 int64_t nt, int32_t* first_prim_out) {
+    if (int rc = check_open(a, "pg_accel_add_mesh"))
+        return rc;
+    if (!v || !idx || nv <= 0 || nt <= 0)
+        return pg_fail(PG_E_ARG, "pg_accel_add_mesh: empty mesh");
+    // bvh.cpp:148 CHECK_NE(numPrimitives, 0) and geometry.h:390-394 index checks.
+    const int64_t first = (int64_t)a->prim_kind.size();
+    if (first + nt >= (int64_t)kLeafStartMask)
+        return pg_fail(PG_E_ARG, "pg_accel_add_mesh: too many prims for one accel");
+    for (int64_t k = 0; k < 3 * nt; k++)
+        if ((int64_t)idx[k] >= nv)
+            return pg_fail(PG_E_ARG, "pg_accel_add_mesh: index out of range");
+    const int32_t shape = (int32_t)a->shapes.size();
+    a->shapes.push_back({0, (int32_t)first, (int32_t)nt});
+    const int64_t base = (int64_t)a->tri_v.size() / 9;
+    a->tri_v.resize((base + nt) * 9);
+    for (int64_t k = 0; k < nt; k++) {
+        for (int c = 0; c < 3; c++) {
+            const float* p = v + 3 * (int64_t)idx[3 * k + c];
+            a->tri_v[(base + k) * 9 + 3 * c + 0] = p[0];
+            a->tri_v[(base + k) * 9 + 3 * c + 1] = p[1];
+            a->tri_v[(base + k) * 9 + 3 * c + 2] = p[2];
+        }
+        a->prim_kind.push_back(0);
+        a->prim_index.push_back((int32_t)(base + k));
+        a->prim_shape.push_back(shape);
+    }
+    if (first_prim_out)
+        *first_prim_out = (int32_t)first;
+    return PG_OK;
+}
+
+static int add_analytic(pg_accel* a, const float rec[16], const float box[6], int32_t* prim_out) {
+    const int32_t prim = (int32_t)a->prim_kind.size();
+    const int32_t shape = (int32_t)a->shapes.size();
+    a->shapes.push_back({1, prim, 1});
+    a->prim_kind.push_back(1);
+    a->prim_index.push_back((int32_t)(a->ana_rec.size() / 16));
+    a->prim_shape.push_back(shape);
+    a->ana_rec.insert(a->ana_rec.end(), rec, rec + 16);
+    a->ana_box.insert(a->ana_box.end(), box, box + 6);
+    if (prim_out)
+        *prim_out = prim;
+    return PG_OK;
+}
+
+extern "C" int pg_accel_add_sphere(pg_accel* a, const float c[3], float r, int32_t* prim_out) {
+    if (int rc = check_open(a, "pg_accel_add_sphere"))
+        return rc;
+    if (!c || !(r > 0.0f))
+        return pg_fail(PG_E_ARG, "pg_accel_add_sphere: bad centre or radius");
+    float rec[16] = {c[0], c[1], c[2], 0, r, 0, 0, u2f(kSphere), 0, 0, 0, 0, 0, 0, 0, 0};
+    float box[6] = {c[0] - r, c[1] - r, c[2] - r, c[0] + r, c[1] + r, c[2] + r};
+    return add_analytic(a, rec, box, prim_out);
+}
+
+extern "C" int pg_accel_add_rect(pg_accel* a, const float pos[3], const float ex[3],
+                                 const float ey[3], int32_t* prim_out) {
+    if (int rc = check_open(a, "pg_accel_add_rect"))
+        return rc;
+    if (!pos || !ex || !ey)
+        return pg_fail(PG_E_ARG, "pg_accel_add_rect: null vector");
+    // Rect ctor (geometry.h:337-343), with pine's Normalize/Length.
+    f3 X = mk(ex[0], ex[1], ex[2]), Y = mk(ey[0], ey[1], ey[2]), P = mk(pos[0], pos[1], pos[2]);
+    f3 exn = normalize(X), eyn = normalize(Y), n = normalize(cross(X, Y));
+    float lx = length(X), ly = length(Y);
+    float rec[16] = {P.x,   P.y,   P.z,   0,  exn.x, exn.y, exn.z, u2f(kRect),
+                     eyn.x, eyn.y, eyn.z, lx, n.x,   n.y,   n.z,   ly};
+    // Conservative box over the four corners.
+    float box[6] = {3.4e38f, 3.4e38f, 3.4e38f, -3.4e38f, -3.4e38f, -3.4e38f};
+    for (int sx = -1; sx <= 1; sx += 2)
+        for (int sy = -1; sy <= 1; sy += 2) {
+            f3 q = P + X * (0.5f * sx) + Y * (0.5f * sy);
+            float qq[3] = {q.x, q.y, q.z};
+            for (int k = 0; k < 3; k++) {
+                box[k] = std::min(box[k], qq[k]);
+                box[3 + k] = std::max(box[3 + k], qq[k]);
+            }
+        }
+    return add_analytic(a, rec, box, prim_out);
+}
+
+extern "C" int pg_accel_add_plane(pg_accel* a, const float pos[3], const float normal[3],
+                                  int32_t* prim_out) {
+    if (int rc = check_open(a, "pg_accel_add_plane"))
+        return rc;
+    if (!pos || !normal)
+        return pg_fail(PG_E_ARG, "pg_accel_add_plane: null vector");
+    // Plane::Create normalises (geometry.cpp:340-342); ctor builds u, v (geometry.h:121-125).
+    f3 n = normalize(mk(normal[0], normal[1], normal[2]));
+    f3 u, v;
+    coordinate_system(n, u, v);
+    float rec[16] = {pos[0], pos[1], pos[2], 0, n.x, n.y, n.z, u2f(kPlane),
+                     u.x,    u.y,    u.z,    0, v.x, v.y, v.z, 0};
+    // Plane::GetAABB (geometry.cpp:110-112)
+    float box[6] = {-1e6f, -1e6f, -1e6f, 1e6f, 1e6f, 1e6f};
+    return add_analytic(a, rec, box, prim_out);
+}
+
+extern "C" int pg_accel_build(pg_accel* a, const pg_build_options* opts) {
+    if (int rc = check_open(a, "pg_accel_build"))
+        return rc;
+    BuildParams bp;
+    if (opts) {
+        if (opts->width != 0 && opts->width != 2 && opts->width != 4)
+            return pg_fail(PG_E_ARG, "pg_accel_build: width must be 2 or 4");
+        if (opts->leaf_max < 0 || opts->leaf_max > 8 || opts->bins < 0 || opts->bins > 256)
+            return pg_fail(PG_E_ARG, "pg_accel_build: leaf_max 1..8, bins 4..256");
+        if (opts->width)
+            bp.width = opts->width;
+        if (opts->leaf_max)
+            bp.leaf_max = opts->leaf_max;
+        if (opts->bins)
+            bp.bins = opts->bins;
+        bp.threads = opts->threads;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    const int64_t n = (int64_t)a->prim_kind.size();
+    std::vector<BuildPrim> prims((size_t)n);
+    float scene_mag = 0.0f;
+    for (int64_t i = 0; i < n; i++) {
+        BuildPrim& p = prims[i];
+        p.id = (int32_t)i;
+        p.kind = a->prim_kind[i];
+        if (p.kind == 0) {
+            const float* t = &a->tri_v[(size_t)a->prim_index[i] * 9];
+            for (int k = 0; k < 3; k++) {
+                p.lo[k] = std::min(t[k], std::min(t[3 + k], t[6 + k]));
+                p.hi[k] = std::max(t[k], std::max(t[3 + k], t[6 + k]));
+            }
+        } else {
+            const float* b = &a->ana_box[(size_t)a->prim_index[i] * 6];
+            for (int k = 0; k < 3; k++) {
+                p.lo[k] = b[k];
+                p.hi[k] = b[3 + k];
+            }
+        }
+        for (int k = 0; k < 3; k++)
+            scene_mag = std::max(scene_mag, std::max(std::fabs(p.lo[k]), std::fabs(p.hi[k])));
+    }
+    // Grow child boxes by 2^-18 of the scene's magnitude: float slab tests then
+    // never cull a box that holds the true closest hit.
+    const float inflate = scene_mag * (1.0f / 262144.0f) + 1e-30f;
+    FlatBVH flat = build_bvh(prims, bp, inflate);
+    const int capacity = bp.width == 2 ? StackDepth<2>::value : StackDepth<4>::value;
+    if ((bp.width - 1) * flat.max_depth > capacity)
+        return pg_fail(PG_E_STATE, "pg_accel_build: hierarchy deeper than the traversal stack");
+
+    // Leaf records in slot order.
+    std::vector<float4> tris(flat.tri_order.size() * 3);
+    for (size_t s = 0; s < flat.tri_order.size(); s++) {
+        int32_t prim = flat.tri_order[s];
+        const float* t = &a->tri_v[(size_t)a->prim_index[prim] * 9];
+        f3 v0 = mk(t[0], t[1], t[2]), v1 = mk(t[3], t[4], t[5]), v2 = mk(t[6], t[7], t[8]);
+        f3 e1 = v1 - v0, e2 = v2 - v0;  // geometry.h:272-273
+        tris[3 * s + 0] = make_float4(v0.x, v0.y, v0.z, u2f((uint32_t)prim));
+        tris[3 * s + 1] = make_float4(e1.x, e1.y, e1.z, 0.0f);
+        tris[3 * s + 2] = make_float4(e2.x, e2.y, e2.z, 0.0f);
+    }
+    std::vector<float4> ana(flat.ana_order.size() * 4);
+    for (size_t s = 0; s < flat.ana_order.size(); s++) {
+        int32_t prim = flat.ana_order[s];
+        const float* q = &a->ana_rec[(size_t)a->prim_index[prim] * 16];
+        ana[4 * s + 0] = make_float4(q[0], q[1], q[2], u2f((uint32_t)prim));
+        ana[4 * s + 1] = make_float4(q[4], q[5], q[6], q[7]);
+        ana[4 * s + 2] = make_float4(q[8], q[9], q[10], q[11]);
+        ana[4 * s + 3] = make_float4(q[12], q[13], q[14], q[15]);
+    }
+    a->free_device();
+    PG_HIP(hipGetDevice(&a->device));
+    const size_t node_bytes = flat.nodes.size() * sizeof(float);
+    PG_HIP(hipMalloc(&a->d_nodes, node_bytes));
+    PG_HIP(hipMemcpy(a->d_nodes, flat.nodes.data(), node_bytes, hipMemcpyHostToDevice));
+    if (!tris.empty()) {
+        PG_HIP(hipMalloc(&a->d_tris, tris.size() * sizeof(float4)));
+        PG_HIP(hipMemcpy(a->d_tris, tris.data(), tris.size() * sizeof(float4),
+                         hipMemcpyHostToDevice));
+    }
+    if (!ana.empty()) {
+        PG_HIP(hipMalloc(&a->d_ana, ana.size() * sizeof(float4)));
+        PG_HIP(hipMemcpy(a->d_ana, ana.data(), ana.size() * sizeof(float4), hipMemcpyHostToDevice));
+    }
+    a->width = flat.width;
+    a->max_depth = flat.max_depth;
+    a->n_nodes = flat.n_nodes;
+    a->n_tri_slots = (int64_t)flat.tri_order.size();
+    a->n_ana_slots = (int64_t)flat.ana_order.size();
+    a->built = true;
+    a->build_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return PG_OK;
+}
+
+static int check_trace(pg_accel* a, const pg_ray* rays, int64_t n, const void* out,
+                       const char* fn) {
+    if (!a)
+        return pg_fail(PG_E_ARG, std::string(fn) + ": null accel");
+    if (!a->built)
+        return pg_fail(PG_E_STATE, std::string(fn) + ": accel not built");
+    if (n < 0 || (n > 0 && (!rays || !out)))
+        return pg_fail(PG_E_ARG, std::string(fn) + ": bad ray/hit buffers");
+    return PG_OK;
+}
+
+extern "C" int pg_trace_closest(pg_accel* a, const pg_ray* d_rays, int64_t n, pg_hit* d_hits,
+                                pg_hit_uv* d_uv, void* stream) {
+    if (int rc = check_trace(a, d_rays, n, d_hits, "pg_trace_closest"))
+        return rc;
+    if (n == 0)
+        return PG_OK;
+    PG_HIP((launch_trace<false, false>(a, d_rays, n, d_hits, d_uv, nullptr, nullptr,
+                                       (hipStream_t)stream)));
+    return PG_OK;
+}
+
+extern "C" int pg_trace_any(pg_accel* a, const pg_ray* d_rays, int64_t n, int32_t* d_occluded,
+                            void* stream) {
+    if (int rc = check_trace(a, d_rays, n, d_occluded, "pg_trace_any"))
+        return rc;
+    if (n == 0)
+        return PG_OK;
+    PG_HIP((launch_trace<true, false>(a, d_rays, n, nullptr, nullptr, d_occluded, nullptr,
+                                      (hipStream_t)stream)));
+    return PG_OK;
+}
+
+extern "C" int pg_trace_closest_counts(pg_accel* a, const pg_ray* d_rays, int64_t n,
+                                       pg_hit* d_hits, int32_t* d_counts, void* stream) {
+    if (int rc = check_trace(a, d_rays, n, d_hits, "pg_trace_closest_counts"))
+        return rc;
+    if (n > 0 && !d_counts)
+        return pg_fail(PG_E_ARG, "pg_trace_closest_counts: null counts buffer");
+    if (n == 0)
+        return PG_OK;
+    PG_HIP((launch_trace<false, true>(a, d_rays, n, d_hits, nullptr, nullptr,
+                                      reinterpret_cast<int2*>(d_counts), (hipStream_t)stream)));
+    return PG_OK;
+}
+
+extern "C" int pg_accel_get_stats(const pg_accel* a, pg_accel_stats* out) {
+    if (!a || !out)
+        return pg_fail(PG_E_ARG, "pg_accel_get_stats: null argument");
+    out->n_prims = (int64_t)a->prim_kind.size();
+    out->n_nodes = a->n_nodes;
+    out->node_bytes = a->n_nodes * (a->width == 2 ? 64 : 128);
+    out->prim_bytes = a->n_tri_slots * 48 + a->n_ana_slots * 64;
+    out->width = a->width;
+    out->max_depth = a->max_depth;
+    out->build_ms = a->build_ms;
+    return PG_OK;
+}
+
+extern "C" int pg_accel_prim_shape(const pg_accel* a, int32_t prim, int32_t* shape,
+                                   int32_t* local) {
+    if (!a || !shape || !local)
+        return pg_fail(PG_E_ARG, "pg_accel_prim_shape: null argument");
+    if (prim < 0 || prim >= (int32_t)a->prim_shape.size())
+        return pg_fail(PG_E_ARG, "pg_accel_prim_shape: prim out of range");
+    *shape = a->prim_shape[prim];
+    *local = prim - a->shapes[*shape].first_prim;
+    return PG_OK;
+}
+
+extern "C" int pg_accel_set_launch(pg_accel* a, int32_t block) {
+    if (!a)
+        return pg_fail(PG_E_ARG, "pg_accel_set_launch: null accel");
+    if (block != 0 && block != 64 && block != 128 && block != 256)
+        return pg_fail(PG_E_ARG, "pg_accel_set_launch: block must be 64, 128 or 256");
+    if (block)
+        a->block = block;
+    return PG_OK;
+}
+
+extern "C" int pg_accel_release(pg_accel* a) {
+    if (!a)
+        return PG_OK;
+    a->free_device();
+    delete a;
+    return PG_OK;
+}

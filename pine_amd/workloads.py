@@ -1,0 +1,88 @@
+"""Benchmark workloads, named by what they are.
+
+``ICOSPHERE_1M``: geodesic icosphere of frequency 229 (20*229^2 = 1,048,820
+triangles) with its last 244 triangles dropped, giving exactly 2^20 =
+1,048,576 triangles, displaced radially by seeded 5-octave value-noise fBm of
+amplitude 0.1.  (20*n^2 never equals 2^20, so a truncated frequency-229 sphere
+is the closest geodesic construction.)
+
+``RAYS_16M``: 4,194,304 primary rays of a 2048x2048 pinhole camera at
+(0, 0.6, 3) looking at the origin with a 45 degree vertical field of view,
+followed by 12,582,912 cosine-distributed rays leaving seeded points of the
+mesh, origins offset by pine's OffsetRayOrigin.
+"""
+from dataclasses import dataclass
+
+
+@dataclass(frozen=True)
+class MeshSpec:
+    freq: int
+    amp: float
+    seed: int
+    max_triangles: int
+
+
+@dataclass(frozen=True)
+class RaySpec:
+    res: int
+    eye: tuple
+    fov_deg: float
+    n_bounce: int
+    seed: int
+
+    @property
+    def n(self):
+        return self.res * self.res + self.n_bounce
+
+
+ICOSPHERE_1M = MeshSpec(freq=229, amp=0.1, seed=1, max_triangles=1 << 20)
+RAYS_16M = RaySpec(res=2048, eye=(0.0, 0.6, 3.0), fov_deg=45.0, n_bounce=12 << 20, seed=7)
+
+# Smaller cases of the same construction for parity tests.
+ICOSPHERE_5K = MeshSpec(freq=16, amp=0.1, seed=1, max_triangles=0)
+ICOSPHERE_82K = MeshSpec(freq=64, amp=0.1, seed=1, max_triangles=0)
+RAYS_SMALL = RaySpec(res=96, eye=(0.0, 0.6, 3.0), fov_deg=45.0, n_bounce=8192, seed=7)
+
+
+def make_mesh(spec):
+    from . import gen_icosphere
+    return gen_icosphere(spec.freq, spec.amp, spec.seed, spec.max_triangles)
+
+
+def make_rays(spec, vertices, indices):
+    from . import gen_rays
+    return gen_rays(vertices, indices, spec.res, spec.eye, spec.fov_deg, spec.n_bounce, spec.seed)
+
+
+def spheres_scene(seed=0, n_spheres=64):
+    """Seeded spheres scene: n_spheres spheres with radius U(0.2, 1.0) inside a
+    20^3 box standing on a 100x100 ground rect, one 6x6 rect area light of
+    emission 10 above them.  Materials: 50% diffuse (albedo U(0.1, 0.9)), 30%
+    dielectric (IOR 1.5, roughness 0), 20% conductor (roughness U(0.05, 0.5),
+    albedo U(0.5, 0.95)).  Returns a dict of plain lists (scene-file terms)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    spheres = []
+    for k in range(n_spheres):
+        r = float(rng.uniform(0.2, 1.0))
+        c = (float(rng.uniform(-10, 10)), float(r + rng.uniform(0, 20 - 2 * r)),
+             float(rng.uniform(-10, 10)))
+        u = float(rng.uniform())
+        if u < 0.5:
+            mat = {"type": "Diffuse", "albedo": float(rng.uniform(0.1, 0.9))}
+        elif u < 0.8:
+            mat = {"type": "Dielectric", "eta": 1.5, "roughness": 0.0}
+        else:
+            mat = {"type": "Conductor", "roughness": float(rng.uniform(0.05, 0.5)),
+                   "albedo": float(rng.uniform(0.5, 0.95))}
+        spheres.append({"c": c, "r": r, "material": mat})
+    return {
+        "spheres": spheres,
+        # Rect{position, ex, ey}: normal = normalize(cross(ex, ey)) = +y
+        "ground": {"position": (0.0, 0.0, 0.0), "ex": (0.0, 0.0, 100.0), "ey": (100.0, 0.0, 0.0),
+                   "material": {"type": "Diffuse", "albedo": 0.5}},
+        # Light faces down: cross(ex, ey) = -y
+        "light": {"position": (0.0, 24.0, 0.0), "ex": (6.0, 0.0, 0.0), "ey": (0.0, 0.0, 6.0),
+                  "emission": 10.0},
+        "camera": {"from": (0.0, 12.0, 38.0), "to": (0.0, 6.0, 0.0), "fov": 0.9},
+    }

@@ -1,0 +1,199 @@
+"""Writes the golden fixtures under tests/golden/ by running the reference
+pine library (built from its own sources by oracle/Makefile into
+oracle/_ref/) through oracle/_ref/ref_harness.  Run in a container that holds
+the reference sources; the fixtures are committed so tests never need it.
+
+    python tests/golden/make_golden.py [case ...]
+
+Cases
+  trace_5k    freq-16 icosphere (5,120 triangles), 17,408 rays, all of them
+  trace_82k   freq-64 icosphere (81,920 triangles), 17,408 rays
+  trace_1m    the 2^20-triangle benchmark mesh, 16,384 rays sampled every
+              1,024th from the 16,777,216-ray benchmark stream (pine's own
+              BVH build takes minutes here)
+  shapes      64 spheres + ground rect + light rect + plane + small mesh,
+              12,288 rays
+  pstd        pine's libm on fixed grids
+  halton      pine's Halton sampler values for a spread of pixels
+
+Trace fixtures store ray *indices* into the deterministic ray generator plus a
+checksum of the selected rays, and pine's answers: hit shape, prim within the
+shape, t, (u, v) and the any-hit flag.
+"""
+import hashlib
+import os
+import struct
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+import pine_amd as pa  # noqa: E402
+from pine_amd import workloads as W  # noqa: E402
+
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def ray_digest(rays):
+    return hashlib.sha256(np.ascontiguousarray(rays, np.float32).tobytes()).hexdigest()
+
+
+def write_scene(path, shapes):
+    with open(path, "wb") as f:
+        f.write(struct.pack("<II", 0x31534750, len(shapes)))
+        for s in shapes:
+            kind = s[0]
+            if kind == "mesh":
+                v, i = s[1], s[2]
+                f.write(struct.pack("<III", 0, len(v), len(i)))
+                f.write(np.ascontiguousarray(v, np.float32).tobytes())
+                f.write(np.ascontiguousarray(i, np.uint32).tobytes())
+            elif kind == "sphere":
+                f.write(struct.pack("<I4f", 1, *s[1], s[2]))
+            elif kind == "rect":
+                f.write(struct.pack("<I9f", 2, *s[1], *s[2], *s[3]))
+            elif kind == "plane":
+                f.write(struct.pack("<I6f", 3, *s[1], *s[2]))
+            else:
+                raise ValueError(kind)
+
+
+def run_harness(shapes, rays, threads=8):
+    with tempfile.TemporaryDirectory() as d:
+        sp, rp, hp = (os.path.join(d, x) for x in ("s.bin", "r.bin", "h.bin"))
+        write_scene(sp, shapes)
+        with open(rp, "wb") as f:
+            f.write(struct.pack("<Q", len(rays)))
+            f.write(np.ascontiguousarray(rays, np.float32).tobytes())
+        p = subprocess.run([HARNESS, "trace", sp, rp, hp, str(threads)], capture_output=True,
+                           text=True)
+        print(p.stdout.strip().splitlines()[-1] if p.stdout.strip() else "", flush=True)
+        if p.returncode != 0:
+            raise RuntimeError(f"harness failed ({p.returncode}): {p.stderr[-2000:]}")
+        h = np.fromfile(hp, dtype=np.int32).reshape(-1, 6)
+    return h
+
+
+def save_trace(name, mesh_spec, ray_spec, sel, rays_sel, h, extra=None):
+    hf = h.view(np.float32)
+    out = dict(
+        mesh=np.array([mesh_spec.freq, mesh_spec.seed, mesh_spec.max_triangles], np.int64),
+        mesh_amp=np.float32(mesh_spec.amp),
+        rayspec=np.array([ray_spec.res, ray_spec.n_bounce, ray_spec.seed], np.int64),
+        eye=np.array(ray_spec.eye, np.float32), fov=np.float32(ray_spec.fov_deg),
+        ray_index=sel.astype(np.int64), ray_sha256=np.bytes_(ray_digest(rays_sel)),
+        shape=h[:, 0].astype(np.int32), prim=h[:, 1].astype(np.int32),
+        t=hf[:, 2].copy(), u=hf[:, 3].copy(), v=hf[:, 4].copy(),
+        occluded=h[:, 5].astype(np.int8))
+    if extra:
+        out.update(extra)
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **out)
+    print(f"{name}: {len(sel)} rays, {int((h[:, 1] >= 0).sum())} hits", flush=True)
+
+
+def case_trace(name, mesh_spec, ray_spec, stride=1, offset=0, limit=None):
+    v, i = W.make_mesh(mesh_spec)
+    rays = W.make_rays(ray_spec, v, i)
+    sel = np.arange(offset, len(rays), stride)
+    if limit:
+        sel = sel[:limit]
+    rs = rays[sel]
+    h = run_harness([("mesh", v, i)], rs)
+    save_trace(name, mesh_spec, ray_spec, sel, rs, h)
+
+
+def shapes_scene():
+    sc = W.spheres_scene(seed=3)
+    shapes = []
+    for s in sc["spheres"]:
+        shapes.append(("sphere", s["c"], s["r"]))
+    g = sc["ground"]
+    shapes.append(("rect", g["position"], g["ex"], g["ey"]))
+    li = sc["light"]
+    shapes.append(("rect", li["position"], li["ex"], li["ey"]))
+    shapes.append(("plane", (0.0, -2.0, 0.0), (0.0, 1.0, 0.2)))
+    v, i = W.make_mesh(W.MeshSpec(freq=6, amp=0.1, seed=5, max_triangles=0))
+    v = v * 3.0 + np.array([4.0, 5.0, -3.0], np.float32)
+    shapes.append(("mesh", v.astype(np.float32), i))
+    return shapes
+
+
+def shapes_rays(n=12288, seed=11):
+    rng = np.random.default_rng(seed)
+    o = np.empty((n, 3), np.float32)
+    d = np.empty((n, 3), np.float32)
+    # Half from a camera-like point, half from random points inside the box.
+    o[: n // 2] = np.array([0.0, 12.0, 38.0], np.float32)
+    o[n // 2:] = rng.uniform([-12, 0.5, -12], [12, 22, 12], (n - n // 2, 3)).astype(np.float32)
+    dd = rng.normal(size=(n, 3))
+    dd[: n // 2] = np.array([0, -6, -38]) + rng.uniform(-14, 14, (n // 2, 3))
+    dd /= np.linalg.norm(dd, axis=1, keepdims=True)
+    d[:] = dd.astype(np.float32)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3], rays[:, 4:7] = o, d
+    rays[:, 3] = 0.0
+    rays[:, 7] = np.float32(3.4028235e38)
+    rays[n - 512:, 7] = np.float32(15.0)  # a few bounded rays
+    return rays
+
+
+def case_shapes():
+    shapes = shapes_scene()
+    rays = shapes_rays()
+    h = run_harness(shapes, rays)
+    np.savez_compressed(
+        os.path.join(OUT, "shapes.npz"), rays=rays, shape=h[:, 0], prim=h[:, 1],
+        t=h.view(np.float32)[:, 2].copy(), u=h.view(np.float32)[:, 3].copy(),
+        v=h.view(np.float32)[:, 4].copy(), occluded=h[:, 5].astype(np.int8))
+    print(f"shapes: {len(rays)} rays, {int((h[:, 0] >= 0).sum())} hits", flush=True)
+
+
+def case_pstd():
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "m.bin")
+        subprocess.run([HARNESS, "math", p], check=True, capture_output=True)
+        a = np.fromfile(p, dtype=np.float32)
+    N = 4096
+    names = ["sqrt", "exp", "log", "cos", "sin", "acos", "pow5", "pow_gamma"]
+    out = {}
+    off = 0
+    for nm in names:
+        xy = a[off: off + 2 * N].reshape(N, 2)
+        out[nm] = xy
+        off += 2 * N
+    out["atan2"] = a[off: off + 3 * N].reshape(N, 3)
+    np.savez_compressed(os.path.join(OUT, "pstd.npz"), **out)
+    print("pstd:", ", ".join(out), flush=True)
+
+
+def case_halton():
+    out = {}
+    for (w, h, spp, nd, npix) in [(512, 512, 16, 24, 64), (64, 48, 8, 12, 32), (1024, 96, 4, 8, 32)]:
+        with tempfile.TemporaryDirectory() as d:
+            p = os.path.join(d, "h.bin")
+            subprocess.run([HARNESS, "halton", p, str(w), str(h), str(spp), str(nd), "3", "1",
+                            str(npix)], check=True, capture_output=True)
+            out[f"{w}x{h}_spp{spp}_d{nd}_n{npix}"] = np.fromfile(p, dtype=np.float32).reshape(
+                npix, spp, nd)
+    np.savez_compressed(os.path.join(OUT, "halton.npz"), **out)
+    print("halton:", ", ".join(out), flush=True)
+
+
+CASES = {
+    "trace_5k": lambda: case_trace("trace_5k", W.ICOSPHERE_5K, W.RAYS_SMALL),
+    "trace_82k": lambda: case_trace("trace_82k", W.ICOSPHERE_82K, W.RAYS_SMALL),
+    "trace_1m": lambda: case_trace("trace_1m", W.ICOSPHERE_1M, W.RAYS_16M, stride=1024,
+                                   offset=17),
+    "shapes": case_shapes,
+    "pstd": case_pstd,
+    "halton": case_halton,
+}
+
+if __name__ == "__main__":
+    todo = sys.argv[1:] or [c for c in CASES if c != "trace_1m"]
+    for c in todo:
+        CASES[c]()

@@ -1,0 +1,96 @@
+"""The C ABI (include/pinehip.h) on the CPU: the library loads, exports every
+declared symbol, validates arguments, and its host-only workload generators are
+deterministic.  No GPU work is launched here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", fn)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names |= set(re.findall(r"\b(pg_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol(pg):
+    L = pg.lib()
+    names = declared_functions()
+    assert len(names) >= 16
+    for n in names:
+        assert hasattr(L, n), f"{n} declared in include/ but not exported"
+    assert set(pg.EXPORTED) <= set(names)
+
+
+def test_version_string(pg):
+    assert b"gfx950" in pg.lib().pg_version()
+
+
+def test_argument_errors_without_gpu(pg):
+    L = pg.lib()
+    assert L.pg_accel_create(None) == -1
+    h = ctypes.c_void_p()
+    assert L.pg_accel_create(ctypes.byref(h)) == 0
+    v = np.zeros((3, 3), np.float32)
+    bad = np.array([[0, 1, 7]], np.uint32)
+    first = ctypes.c_int32()
+    rc = L.pg_accel_add_mesh(h, ctypes.c_void_p(v.ctypes.data), 3, ctypes.c_void_p(bad.ctypes.data),
+                             1, ctypes.byref(first))
+    assert rc == -1 and b"out of range" in L.pg_last_error()
+    assert L.pg_accel_add_mesh(h, None, 0, None, 0, None) == -1
+    # Trace before build is a state error, caught before any device work.
+    assert L.pg_trace_closest(h, ctypes.c_void_p(16), 1, ctypes.c_void_p(16), None, None) == -2
+    assert L.pg_trace_any(h, ctypes.c_void_p(16), 1, ctypes.c_void_p(16), None) == -2
+    assert L.pg_accel_set_launch(h, 96) == -1
+    c = (ctypes.c_float * 3)(0, 0, 0)
+    assert L.pg_accel_add_sphere(h, c, ctypes.c_float(-1.0), None) == -1
+    assert L.pg_accel_release(h) == 0
+    assert L.pg_accel_release(None) == 0
+
+
+def test_icosphere_sizes_and_watertightness(pg):
+    v, i = pg.gen_icosphere(8, 0.1, 3)
+    assert i.shape == (20 * 64, 3) and v.shape == (10 * 64 + 2, 3)
+    # Closed 2-manifold: every undirected edge is used by exactly two triangles,
+    # once in each direction.
+    e = np.concatenate([i[:, [0, 1]], i[:, [1, 2]], i[:, [2, 0]]])
+    directed = {tuple(x) for x in e.tolist()}
+    assert len(directed) == len(e)
+    assert all((b, a) in directed for a, b in directed)
+    r = np.linalg.norm(v, axis=1)
+    assert r.min() > 0.85 and r.max() < 1.15
+
+
+def test_icosphere_truncation(pg):
+    v, i = pg.gen_icosphere(229, 0.1, 1, 1 << 20)
+    assert len(i) == 1 << 20
+    assert i.max() < len(v)
+
+
+def test_generators_are_deterministic(pg):
+    v1, i1 = pg.gen_icosphere(12, 0.1, 5)
+    v2, i2 = pg.gen_icosphere(12, 0.1, 5)
+    assert v1.tobytes() == v2.tobytes() and i1.tobytes() == i2.tobytes()
+    r1 = pg.gen_rays(v1, i1, 16, (0, 0.6, 3), 45.0, 500, 9)
+    r2 = pg.gen_rays(v1, i1, 16, (0, 0.6, 3), 45.0, 500, 9)
+    assert r1.tobytes() == r2.tobytes()
+    d = r1[:, 4:7]
+    assert np.allclose(np.linalg.norm(d, axis=1), 1.0, atol=1e-5)
+    # Bounce rays leave the surface outward (cosine lobe around the normal).
+    o = r1[256:, 0:3]
+    assert (np.einsum("ij,ij->i", o, d[256:]) > -0.2).mean() > 0.95
+
+
+def test_generator_rejects_bad_arguments(pg):
+    L = pg.lib()
+    nv, nt = ctypes.c_int64(), ctypes.c_int64()
+    assert L.pg_gen_icosphere(0, ctypes.c_float(0.1), 1, 0, None, ctypes.byref(nv), None,
+                              ctypes.byref(nt)) == -1

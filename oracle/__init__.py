@@ -52,6 +52,7 @@ def lib():
         L.po_bvh_free.argtypes = [vp]
         L.po_bvh_trace.argtypes = [vp, vp, i64, vp, vp, ctypes.c_int]
         L.po_bvh_trace.restype = ctypes.c_double
+        L.po_halton_values.argtypes = [ctypes.c_int] * 7 + [vp]
         _lib = L
     return _lib
 
@@ -84,6 +85,13 @@ def trace_brute(tri, rays, analytic=None, threads=0):
     lib().po_trace_brute(_p(tri), len(tri), _p(ana), len(ana), _p(rays), n, _p(prim), _p(t),
                          _p(u), _p(v), _p(occ), threads)
     return prim, t, u, v, occ
+
+
+def halton_values(w, h, spp, ndims, px0, py0, npix):
+    """pine's HaltonSampler values, layout [npix, spp, ndims] (see pinecpu.cpp)."""
+    out = np.zeros(npix * spp * ndims, np.float32)
+    lib().po_halton_values(w, h, spp, ndims, px0, py0, npix, _p(out))
+    return out.reshape(npix, spp, ndims)
 
 
 class CpuBVH:

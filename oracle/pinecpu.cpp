@@ -358,6 +358,174 @@ bool AnaHit(const Analytic& s, const Ray& r) {  // Sphere/Rect/Plane::Hit
     return !(v < -0.5f || v > 0.5f);
 }
 
+
+// ---- RNG + Halton sampler ------------------------------------------------------
+// rng.h:39-44 SplitMix64, rng.h:62-105 RNG (xoroshiro128+ step).
+uint64_t SplitMix64(uint64_t& s) {
+    uint64_t r = s += 0x9E3779B97f4A7C15ULL;
+    r = (r ^ (r >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    r = (r ^ (r >> 27)) * 0x94D049BB133111EBULL;
+    return r ^ (r >> 31);
+}
+struct Rng {
+    uint64_t s[2];
+    explicit Rng(uint64_t seed = 0) {
+        s[0] = SplitMix64(seed);
+        s[1] = SplitMix64(seed);
+    }
+    uint64_t U64() {
+        const uint64_t s0 = s[0];
+        uint64_t s1 = s[1];
+        const uint64_t result = s0 + s1;
+        s1 ^= s0;
+        s[0] = ((s0 << 24) | (s0 >> 40)) ^ s1 ^ (s1 << 16);
+        s[1] = (s1 << 37) | (s1 >> 27);
+        return result;
+    }
+    uint32_t U32() {
+        uint64_t h = U64();
+        return (uint32_t)(h ^ (h >> 32));
+    }
+};
+
+struct HaltonTables {
+    std::vector<int> primes, sums;
+    std::vector<uint16_t> perms;
+    HaltonTables() {
+        // First 1000 primes (util/primes.cpp) and their prefix sums.
+        for (int n = 2; (int)primes.size() < 1000; n++) {
+            bool prime = true;
+            for (int p : primes) {
+                if (p * p > n)
+                    break;
+                if (n % p == 0) {
+                    prime = false;
+                    break;
+                }
+            }
+            if (prime)
+                primes.push_back(n);
+        }
+        int acc = 0;
+        for (int p : primes) {
+            sums.push_back(acc);
+            acc += p;
+        }
+        // ComputeRadicalInversePermutations (lowdiscrepancy.cpp:5-21) with
+        // Shuffle (lowdiscrepancy.h:55-62) driven by RNG(0).
+        perms.resize(acc);
+        Rng rng(0);
+        uint16_t* p = perms.data();
+        for (int i = 0; i < 1000; i++) {
+            for (int j = 0; j < primes[i]; j++)
+                p[j] = (uint16_t)j;
+            for (int k = 0; k < primes[i]; k++) {
+                int other = k + (int)(rng.U32() % (uint32_t)(primes[i] - k));
+                std::swap(p[k], p[other]);
+            }
+            p += primes[i];
+        }
+    }
+};
+const HaltonTables& Tables() {
+    static HaltonTables t;
+    return t;
+}
+
+// HaltonSampler (sampler.cpp:32-96).
+struct Halton {
+    int baseScales[2], baseExponents[2], multInverse[2];
+    int stride;
+    uint64_t index = 0;
+    int dim = 0;
+
+    static void Egcd(uint64_t a, uint64_t b, int64_t& g, int64_t& x, int64_t& y) {
+        int d = (int)(a / b);
+        int r = (int)(a - (uint64_t)d * b);
+        if (r == 0) {
+            g = (int64_t)b;
+            x = 0;
+            y = 1;
+            return;
+        }
+        int64_t nx, ny;
+        Egcd(b, (uint64_t)r, g, nx, ny);
+        x = ny;
+        y = nx - d * ny;
+    }
+    static uint64_t MulInv(int64_t a, int64_t n) {
+        int64_t g, x, y;
+        Egcd((uint64_t)a, (uint64_t)n, g, x, y);
+        int64_t r = x - (x / n) * n;
+        return (uint64_t)(r < 0 ? r + n : r);
+    }
+    Halton(int w, int h) {
+        int film[2] = {w, h};
+        for (int i = 0; i < 2; i++) {
+            int base = i == 0 ? 2 : 3, scale = 1, exp = 0;
+            while (scale < std::min(film[i], 128)) {
+                scale *= base;
+                ++exp;
+            }
+            baseScales[i] = scale;
+            baseExponents[i] = exp;
+        }
+        multInverse[0] = (int)MulInv(baseScales[0], baseScales[1]);
+        multInverse[1] = (int)MulInv(baseScales[1], baseScales[0]);
+        stride = baseScales[0] * baseScales[1];
+    }
+    static uint64_t InverseRadicalInverse(uint64_t inverse, int base, int nDigits) {
+        uint64_t index = 0;
+        for (int i = 0; i < nDigits; i++) {
+            uint64_t digit = inverse % base;
+            inverse /= base;
+            index = index * base + digit;
+        }
+        return index;
+    }
+    void StartPixel(int px, int py, int sampleIndex) {
+        int64_t hi = 0;
+        if (stride > 1) {
+            int pm[2] = {px % 128, py % 128};
+            for (int i = 0; i < 2; i++) {
+                uint64_t off = InverseRadicalInverse((uint64_t)pm[i], i == 0 ? 2 : 3, baseExponents[i]);
+                hi += (int64_t)(off * (uint64_t)baseScales[1 - i] * (uint64_t)multInverse[1 - i]);
+            }
+            hi %= stride;
+        }
+        hi += (int64_t)sampleIndex * stride;
+        index = (uint64_t)hi;
+        dim = 2;
+    }
+    void NextSample() {
+        index += (uint64_t)stride;
+        dim = 2;
+    }
+    // ScrambledRadicalInverse (lowdiscrepancy.h:28-42)
+    static float Scrambled(int baseIndex, uint64_t a) {
+        const HaltonTables& T = Tables();
+        int base = T.primes[baseIndex];
+        const uint16_t* perm = &T.perms[T.sums[baseIndex]];
+        float invBase = 1.0f / base, invBaseN = 1;
+        uint64_t reversed = 0;
+        while (a) {
+            uint64_t next = a / base;
+            uint64_t digit = a - next * base;
+            reversed = reversed * base + perm[digit];
+            invBaseN *= invBase;
+            a = next;
+        }
+        float series = perm[0] / (base + 1.0f);
+        float v = ((float)reversed + series) * invBaseN;
+        return v < 0x1.fffffep-1f ? v : 0x1.fffffep-1f;
+    }
+    float Get1D() {
+        if (dim >= 1000)
+            dim = 2;
+        return Scrambled(dim++, index);
+    }
+};
+
 Ray LoadRay(const float* f) {
     Ray r;
     r.o = v3(f[0], f[1], f[2]);
@@ -534,6 +702,22 @@ bool BoxHit(const float* lo, const float* hi, const V3& inv, const V3& nod, cons
 }  // namespace
 
 extern "C" {
+
+// Halton values in the harness's layout: npix pixels spread as k*977 from
+// (px0, py0), spp samples each, ndims successive Get1D values per sample.
+void po_halton_values(int w, int h, int spp, int ndims, int px0, int py0, int npix, float* out) {
+    Halton s(w, h);
+    for (int k = 0; k < npix; k++) {
+        int idx = py0 * w + px0 + k * 977;
+        int px = idx % w, py = (idx / w) % h;
+        s.StartPixel(px, py, 0);
+        for (int si = 0; si < spp; si++) {
+            for (int d = 0; d < ndims; d++)
+                *out++ = s.Get1D();
+            s.NextSample();
+        }
+    }
+}
 
 // pstd libm entry points for the golden-vector tests.
 float po_pstd_sqrt(float x) { return Sqrt(x); }

@@ -18,6 +18,14 @@
 //       pstd libm outputs (pstd/math.h:141-351) on a fixed input grid.
 //   halton <out.bin> <filmW> <filmH> <spp> <ndims> <px0> <py0> <npix>
 //       HaltonSampler values (sampler.cpp:32-96) for a run of pixels.
+//   render <scene.txt> <out.bin> [threads]
+//       Loads a scene with pine's own LoadScene (fileio.cpp:251-281) and renders
+//       it with the scene's integrator (PathIntegrator::Li, path.cpp:6-87) in
+//       pine's per-pixel sample order (integrator.cpp:121-158), accumulating
+//       each sample with the film's filter-table weight (film.h:26-45) into a
+//       linear float image; pine itself only writes tone-mapped 8-bit BMPs.
+//       out.bin: i32 w, i32 h, then h*w*4 floats (sum w*r, sum w*g, sum w*b,
+//       sum w) in pine's pixel order (row 0 = top, film.h:60-62).
 //
 // File formats (little endian):
 //   scene.bin: u32 'PGS1', u32 nshapes, then per shape u32 type and payload:
@@ -28,6 +36,9 @@
 //   rays.bin : u64 n, then n x {f32 o[3], tmin, d[3], tmax}
 //   hits.bin : n x {i32 shape, i32 prim, f32 t, f32 u, f32 v, i32 occluded}
 #include <core/scene.h>
+#include <core/integrator.h>
+#include <core/filter.h>
+#include <util/fileio.h>
 #include <core/sampler.h>
 #include <core/lowdiscrepancy.h>
 #include <impl/accel/bvh.h>
@@ -39,6 +50,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -429,6 +441,87 @@ int ModeHalton(int argc, char** argv) {
     return 0;
 }
 
+int ModeRender(int argc, char** argv) {
+    if (argc < 4)
+        return 2;
+    int threads = Threads(argc, argv, 4);
+    auto scene = pstd::make_shared<Scene>();
+    Parameters params = LoadScene(argv[2], scene.get());
+    auto* ri = dynamic_cast<RadianceIntegrator*>(scene->integrator.get());
+    if (!ri) {
+        fprintf(stderr, "integrator is not a RadianceIntegrator\n");
+        return 3;
+    }
+    Film& film = scene->camera.GetFilm();
+    vec2i size = film.Size();
+    int spp = ri->samplesPerPixel;
+    // Filter table exactly as Film's constructor builds it (film.cpp:22-28).
+    Filter filter = CreateFilter(params["Camera"]["film"]["filter"]);
+    const int TW = 16;
+    float table[TW * TW];
+    for (int y = 0, o = 0; y < TW; y++)
+        for (int x = 0; x < TW; x++)
+            table[o++] = filter.Evaluate(
+                {(x + 0.5f) / TW * filter.Radius().x, (y + 0.5f) / TW * filter.Radius().y});
+    std::vector<float> acc((size_t)size.x * size.y * 4, 0.0f);
+    std::vector<std::mutex> locks(1024);
+    double t0 = Now();
+    std::atomic<int64_t> next{0};
+    std::vector<std::thread> pool;
+    for (int th = 0; th < threads; th++)
+        pool.emplace_back([&]() {
+            Sampler sampler = ri->samplers[0].Clone();
+            while (true) {
+                int64_t index = next.fetch_add(1);
+                if (index >= (int64_t)size.x * size.y)
+                    break;
+                vec2i p = {int(index % size.x), int(index / size.x)};
+                sampler.StartPixel(p, 0);
+                for (int s = 0; s < spp; s++) {
+                    // RadianceIntegrator::Compute (integrator.cpp:152-158)
+                    vec2 pFilm = (p + sampler.Get2D()) / film.Size();
+                    Ray ray = scene->camera.GenRay(pFilm, sampler.Get2D());
+                    Spectrum L = ri->Li(ray, sampler);
+                    // Film::AddSample (film.h:26-45)
+                    vec2 pf = pFilm * size;
+                    pf -= vec2(0.5f);
+                    vec2i p0 = Ceil(pf - filter.Radius());
+                    vec2i p1 = Floor(pf + filter.Radius());
+                    p0 = Max(p0, vec2i(0));
+                    p1 = Min(p1, size - vec2i(1));
+                    vec3 rgb = L.ToRGB();
+                    for (int y = p0.y; y <= p1.y; y++)
+                        for (int x = p0.x; x <= p1.x; x++) {
+                            vec2 d = vec2(x, y) - pf;
+                            vec2i pi = TW * Min(Abs(d) / filter.Radius(), vec2(OneMinusEpsilon));
+                            float w = table[pi.y * TW + pi.x];
+                            size_t pix = (size_t)(size.y - 1 - y) * size.x + x;
+                            std::lock_guard<std::mutex> lk(locks[pix % locks.size()]);
+                            acc[pix * 4 + 0] += rgb[0] * w;
+                            acc[pix * 4 + 1] += rgb[1] * w;
+                            acc[pix * 4 + 2] += rgb[2] * w;
+                            acc[pix * 4 + 3] += w;
+                        }
+                    sampler.StartNextSample();
+                }
+            }
+        });
+    for (auto& t : pool)
+        t.join();
+    double secs = Now() - t0;
+    FILE* f = fopen(argv[3], "wb");
+    int32_t hdr[2] = {size.x, size.y};
+    fwrite(hdr, 4, 2, f);
+    fwrite(acc.data(), sizeof(float), acc.size(), f);
+    fclose(f);
+    float lightPdf = ri->lightSampler.SampleLight(0.5f).pdf;
+    printf("{\"w\": %d, \"h\": %d, \"spp\": %d, \"render_s\": %.3f, \"msamples_per_s\": %.4f, "
+           "\"threads\": %d, \"light_select_pdf\": %.9g, \"n_lights\": %zu}\n",
+           size.x, size.y, spp, secs, (double)size.x * size.y * spp / secs * 1e-6, threads,
+           lightPdf, scene->lights.size());
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -445,6 +538,8 @@ int main(int argc, char** argv) {
         return ModeMath(argc, argv);
     if (mode == "halton")
         return ModeHalton(argc, argv);
+    if (mode == "render")
+        return ModeRender(argc, argv);
     fprintf(stderr, "unknown mode\n");
     return 2;
 }

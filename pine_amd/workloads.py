@@ -86,3 +86,69 @@ def spheres_scene(seed=0, n_spheres=64):
                   "emission": 10.0},
         "camera": {"from": (0.0, 12.0, 38.0), "to": (0.0, 6.0, 0.0), "fov": 0.9},
     }
+
+
+def _fmt(x):
+    # Plain decimals only: pine's stof (pstd/string.cpp:50-68) has no exponents.
+    s = f"{float(x):.6f}".rstrip("0").rstrip(".")
+    return s if s not in ("", "-0") else "0"
+
+
+def _v3(v):
+    return " ".join(_fmt(x) for x in v)
+
+
+def spheres_scene_text(seed=0, width=512, height=512, spp=64, max_depth=8, n_spheres=64,
+                       output="spheres.bmp"):
+    """The seeded spheres scene as pine scene-file text (parser.cpp grammar)."""
+    sc = spheres_scene(seed, n_spheres)
+    cam = sc["camera"]
+    out = [
+        "Integrator: Path{",
+        f"    maxDepth: {max_depth}",
+        "    sampler: Halton{",
+        f"        samplesPerPixel: {spp}",
+        "    }",
+        "}",
+        "Camera: ThinLen{",
+        "    film{",
+        f"        outputFileName: {output}",
+        f"        size: {width} {height}",
+        "        filter: LanczosSinc{",
+        "            radius: 0.5",
+        "        }",
+        "    }",
+        f"    from: {_v3(cam['from'])}",
+        f"    to: {_v3(cam['to'])}",
+        f"    fov: {_fmt(cam['fov'])}",
+        "}",
+        "Material light: Emissive{",
+        f"    color: {_fmt(sc['light']['emission'])}",
+        "}",
+        "Material ground: Layered{",
+        "    layer0: Diffuse{",
+        f"        albedo: {_fmt(sc['ground']['material']['albedo'])}",
+        "    }",
+        "}",
+    ]
+    for k, s in enumerate(sc["spheres"]):
+        m = s["material"]
+        out.append(f"Material m{k}: Layered{{")
+        if m["type"] == "Diffuse":
+            out += ["    layer0: Diffuse{", f"        albedo: {_fmt(m['albedo'])}", "    }"]
+        elif m["type"] == "Dielectric":
+            out += ["    layer0: Dielectric{", f"        eta: {_fmt(m['eta'])}",
+                    f"        roughness: {_fmt(m['roughness'])}", "    }"]
+        else:
+            out += ["    layer0: Conductor{", f"        albedo: {_fmt(m['albedo'])}",
+                    f"        roughness: {_fmt(m['roughness'])}", "    }"]
+        out.append("}")
+    g, li = sc["ground"], sc["light"]
+    out += ["Shape: Rect{", f"    position: {_v3(g['position'])}", f"    ex: {_v3(g['ex'])}",
+            f"    ey: {_v3(g['ey'])}", "    material: ground", "}"]
+    out += ["Shape: Rect{", f"    position: {_v3(li['position'])}", f"    ex: {_v3(li['ex'])}",
+            f"    ey: {_v3(li['ey'])}", "    material: light", "}"]
+    for k, s in enumerate(sc["spheres"]):
+        out += ["Shape: Sphere{", f"    position: {_v3(s['c'])}", f"    radius: {_fmt(s['r'])}",
+                f"    material: m{k}", "}"]
+    return "\n".join(out) + "\n"

@@ -166,3 +166,29 @@ def test_cpu_bvh_port_agrees_with_brute_force(pg, oracle):
     assert np.array_equal(t1.view(np.int32), t2.view(np.int32))
     tie_free = p1 == p2
     assert tie_free.mean() > 0.999
+
+
+def test_halton_matches_pine(oracle):
+    g = golden("halton")
+    for key in g.files:
+        w, h = (int(x) for x in key.split("_")[0].split("x"))
+        spp = int(key.split("spp")[1].split("_")[0])
+        nd = int(key.split("_d")[1].split("_")[0])
+        npix = int(key.split("_n")[1])
+        got = oracle.halton_values(w, h, spp, nd, 3, 1, npix)
+        assert np.array_equal(got.view(np.int32), g[key].view(np.int32)), key
+
+
+def test_halton_is_a_permuted_van_der_corput(oracle):
+    # Dimension 2 uses base 5: over 5^k consecutive sample indices of one pixel
+    # every stratum [j/5^k, (j+1)/5^k) holds exactly one value.
+    # Pixel (1, 0): its first Halton index is not 0 (index 0 has no digits and
+    # maps to the tail term alone).
+    vals = oracle.halton_values(5, 5, 125, 3, 1, 0, 1)[0, :, 0]
+    # Samples of one pixel step the Halton index by the stride (2*3 here), which
+    # is coprime with 5, so the base-5 digits of 125 consecutive samples cover
+    # all residues.
+    # Coarse strata (2 digits) so that float rounding at a boundary cannot move
+    # a value: each of the 25 strata holds exactly 5 of the 125 values.
+    strata = np.floor(vals * 25).astype(int)
+    assert np.bincount(strata, minlength=25).tolist() == [5] * 25

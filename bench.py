@@ -50,14 +50,13 @@ def main():
               "shards by rays with the hierarchy replicated, see DESIGN.md.", file=sys.stderr)
         sys.exit(2)
 
-    import torch
     import pine_amd as pg
+    from pine_amd import hip
     from pine_amd import workloads as W
 
-    if not torch.cuda.is_available():
+    if hip.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
-    dev = torch.device("cuda:0")
 
     t0 = time.time()
     v, i = W.make_mesh(W.ICOSPHERE_1M)
@@ -71,41 +70,42 @@ def main():
     st = acc.stats()
     log(f"[bench] accel built in {st['build_ms']:.0f} ms: {st['n_nodes']} nodes, depth {st['max_depth']}")
 
-    d_rays = torch.from_numpy(rays).to(dev)
+    d_rays = hip.DeviceArray.from_numpy(rays)
     n = d_rays.shape[0]
-    d_hits = torch.empty((n, 2), dtype=torch.int32, device=dev)
+    d_hits = hip.DeviceArray((n, 2), np.int32)
 
     # Instrumented pass: work per ray over the same rays and tree.
-    _, cnt = acc.trace_counts(d_rays)
-    torch.cuda.synchronize()
-    nodes_fetched = int(cnt[:, 0].sum(dtype=torch.int64).item())
-    prims_tested = int(cnt[:, 1].sum(dtype=torch.int64).item())
-    del cnt
+    ch, cnt = acc.trace_counts(d_rays)
+    hip.synchronize()
+    c = cnt.numpy()
+    nodes_fetched = int(c[:, 0].sum(dtype=np.int64))
+    prims_tested = int(c[:, 1].sum(dtype=np.int64))
+    del c, cnt, ch
     node_bytes = 64 if st["width"] == 2 else 128
     prim_bytes = 48
     compulsory = 40 * n
     traversal = nodes_fetched * node_bytes + prims_tested * prim_bytes
     algo_bytes = compulsory + traversal
 
-    stream = torch.cuda.current_stream()
+    # Launches go to the default stream; events are recorded on that stream.
     for _ in range(args.warmup):
         acc.trace_closest(d_rays, d_hits)
-    torch.cuda.synchronize()
+    hip.synchronize()
 
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    torch.cuda.synchronize()
+    starts = [hip.Event() for _ in range(args.steps)]
+    ends = [hip.Event() for _ in range(args.steps)]
+    hip.synchronize()
     w0 = time.perf_counter()
     for k in range(args.steps):
-        starts[k].record(stream)
+        starts[k].record()
         acc.trace_closest(d_rays, d_hits)
-        ends[k].record(stream)
-    torch.cuda.synchronize()
+        ends[k].record()
+    hip.synchronize()
     wall = time.perf_counter() - w0
-    kern_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
+    kern_ms = float(np.mean([s.elapsed_ms(e) for s, e in zip(starts, ends)]))
     ms_per_step = wall * 1e3 / args.steps
     mrays = n / (ms_per_step * 1e-3) * 1e-6
-    hits = d_hits.cpu().numpy()
+    hits = d_hits.numpy()
     hit_frac = float((hits[:, 0] >= 0).mean())
     log(f"[bench] {ms_per_step:.3f} ms/step, kernel {kern_ms:.3f} ms, {mrays:.1f} Mrays/s, "
         f"hit fraction {hit_frac:.3f}")

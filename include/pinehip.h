@@ -108,6 +108,12 @@ int pg_trace_closest(pg_accel* a, const pg_ray* d_rays, int64_t n, pg_hit* d_hit
 int pg_trace_any(pg_accel* a, const pg_ray* d_rays, int64_t n, int32_t* d_occluded,
                  void* stream);
 
+/* Shadow-ray visibility as pine's IntersectTr decides it for surfaces that all
+ * carry a material (integrator.cpp:68-87): occluded when any prim's Intersect
+ * test accepts a t in (tmin, tmax).  d_occluded[i] = 0/1. */
+int pg_trace_occluded(pg_accel* a, const pg_ray* d_rays, int64_t n, int32_t* d_occluded,
+                      void* stream);
+
 /* pg_trace_closest plus, per ray, the work done: d_counts[2i] = nodes fetched,
  * d_counts[2i+1] = primitives tested.  For measurement (algorithmic bytes);
  * no pine counterpart beyond the per-node counter of bvh.cpp:553. */
@@ -134,6 +140,52 @@ int pg_accel_prim_shape(const pg_accel* a, int32_t prim, int32_t* shape, int32_t
 int pg_accel_set_launch(pg_accel* a, int32_t block);
 
 int pg_accel_release(pg_accel* a);
+
+/* --- Scene + Path integrator (replaces LoadScene and PathIntegrator) --------- */
+
+typedef struct pg_scene pg_scene;
+
+/* Loads a pine scene file: the scene-file grammar of parser.cpp:45-110 with
+ * pine's number parsing, the object order of LoadScene (fileio.cpp:251-281) and
+ * the Create* factories, then builds the accel (opts may be NULL).  Supported
+ * on the GPU: Integrator Path + Halton sampler, ThinLen camera, LanczosSinc/Box
+ * filters, Layered (Diffuse/Conductor/Dielectric) and Emissive materials with
+ * constant inputs, Sphere/Rect/Plane/Triangle/TriangleMesh shapes, Point,
+ * Directional and area lights.  Anything else fails with PG_E_PARSE naming it. */
+int pg_scene_load(const char* path, const pg_build_options* opts, pg_scene** out);
+/* Same from scene text; `dir` resolves relative file names (may be NULL). */
+int pg_scene_load_text(const char* text, const char* dir, const pg_build_options* opts,
+                       pg_scene** out);
+
+typedef struct pg_scene_info {
+    int32_t width, height;   /* film size */
+    int32_t spp, max_depth;  /* sampler samplesPerPixel, integrator maxDepth */
+    int32_t n_shapes, n_lights;
+    int64_t n_prims;
+    double last_render_ms;   /* wall time of the last pg_render on its stream */
+    int64_t last_rays;       /* rays traced by the last pg_render (extension + shadow) */
+} pg_scene_info;
+int pg_scene_get_info(const pg_scene* s, pg_scene_info* info);
+
+typedef struct pg_render_options {
+    int32_t spp;                /* 0 = the scene's samplesPerPixel */
+    int32_t samples_per_batch;  /* 0 = automatic (about 4M paths per batch) */
+} pg_render_options;
+
+/* Renders the whole film: PixelIntegrator::Render + RadianceIntegrator::Compute
+ * + PathIntegrator::Li (integrator.cpp:121-158, path.cpp:6-87) for every pixel
+ * and sample, pine's Halton sample sequence per pixel.  Synchronous. */
+int pg_render(pg_scene* s, const pg_render_options* opts, void* stream);
+
+/* Film accumulators of the last render, h*w*4 floats (sum w*r, sum w*g, sum w*b,
+ * sum w) in pine's pixel order (film.h:60-62: row 0 is the top of the image).
+ * Linear radiance = rgb / w, as Film::CopyToRGBArray divides (film.cpp:72-89). */
+int pg_scene_film(const pg_scene* s, float* h_rgbw);
+
+/* The film's outputFileName (film.cpp:9), NUL-terminated, truncated to len. */
+int pg_scene_output_name(const pg_scene* s, char* buf, int32_t len);
+
+int pg_scene_release(pg_scene* s);
 
 /* --- Deterministic benchmark workloads (no pine counterpart) ---------------- */
 

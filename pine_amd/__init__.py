@@ -2,8 +2,8 @@
 
 The native library ``libpinehip.so`` (built in-tree by ``pine_amd.build``)
 exports the C ABI declared in ``include/pinehip.h``; this module binds it with
-ctypes for tests and the benchmark.  Device buffers are torch tensors on the
-current HIP device; torch is only the allocator and stream provider here.
+ctypes for tests and the benchmark.  Device buffers are ``pine_amd.hip.DeviceArray``
+allocations made through the same HIP runtime the library links.
 
 There is no fallback: if the library is missing or fails to load, every entry
 point raises.
@@ -22,7 +22,8 @@ EXPORTED = [
     "pg_accel_add_sphere", "pg_accel_add_rect", "pg_accel_add_plane", "pg_accel_build",
     "pg_trace_closest", "pg_trace_any", "pg_accel_get_stats", "pg_accel_prim_shape",
     "pg_accel_set_launch", "pg_accel_release", "pg_gen_icosphere", "pg_gen_rays",
-    "pg_trace_closest_counts",
+    "pg_trace_closest_counts", "pg_trace_occluded", "pg_scene_load", "pg_scene_load_text",
+    "pg_scene_get_info", "pg_render", "pg_scene_film", "pg_scene_output_name", "pg_scene_release",
 ]
 
 
@@ -40,6 +41,17 @@ class _Stats(ctypes.Structure):
                 ("node_bytes", ctypes.c_int64), ("prim_bytes", ctypes.c_int64),
                 ("width", ctypes.c_int32), ("max_depth", ctypes.c_int32),
                 ("build_ms", ctypes.c_double)]
+
+
+class _SceneInfo(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("spp", ctypes.c_int32),
+                ("max_depth", ctypes.c_int32), ("n_shapes", ctypes.c_int32),
+                ("n_lights", ctypes.c_int32), ("n_prims", ctypes.c_int64),
+                ("last_render_ms", ctypes.c_double), ("last_rays", ctypes.c_int64)]
+
+
+class _RenderOptions(ctypes.Structure):
+    _fields_ = [("spp", ctypes.c_int32), ("samples_per_batch", ctypes.c_int32)]
 
 
 _lib = None
@@ -66,6 +78,15 @@ def lib():
     L.pg_trace_closest.argtypes = [vp, vp, i64, vp, vp, vp]
     L.pg_trace_any.argtypes = [vp, vp, i64, vp, vp]
     L.pg_trace_closest_counts.argtypes = [vp, vp, i64, vp, vp, vp]
+    L.pg_trace_occluded.argtypes = [vp, vp, i64, vp, vp]
+    L.pg_scene_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(_BuildOptions), ctypes.POINTER(vp)]
+    L.pg_scene_load_text.argtypes = [ctypes.c_char_p, ctypes.c_char_p,
+                                     ctypes.POINTER(_BuildOptions), ctypes.POINTER(vp)]
+    L.pg_scene_get_info.argtypes = [vp, ctypes.POINTER(_SceneInfo)]
+    L.pg_render.argtypes = [vp, ctypes.POINTER(_RenderOptions), vp]
+    L.pg_scene_film.argtypes = [vp, vp]
+    L.pg_scene_output_name.argtypes = [vp, ctypes.c_char_p, i32]
+    L.pg_scene_release.argtypes = [vp]
     L.pg_accel_get_stats.argtypes = [vp, ctypes.POINTER(_Stats)]
     L.pg_accel_prim_shape.argtypes = [vp, i32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
     L.pg_accel_set_launch.argtypes = [vp, i32]
@@ -93,8 +114,11 @@ def _ptr(a):
 
 
 def _stream_handle():
-    import torch
-    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    return None  # the default stream
+
+
+def _dptr(a):
+    return a.ptr if a is not None else None
 
 
 # --- workloads -------------------------------------------------------------------
@@ -174,40 +198,94 @@ class Accel:
         return s.value, l.value
 
     def trace_closest(self, d_rays, d_hits=None, d_uv=None):
-        """d_rays: cuda float32 [n,8]. Returns int32 [n,2] (prim, t bits)."""
-        import torch
+        """d_rays: DeviceArray float32 [n, 8].  Returns DeviceArray int32 [n, 2]
+        (prim, bits of t)."""
+        from .hip import DeviceArray
         n = d_rays.shape[0]
         if d_hits is None:
-            d_hits = torch.empty((n, 2), dtype=torch.int32, device=d_rays.device)
-        check(lib().pg_trace_closest(self.h, ctypes.c_void_p(d_rays.data_ptr()), n,
-                                     ctypes.c_void_p(d_hits.data_ptr()),
-                                     ctypes.c_void_p(d_uv.data_ptr()) if d_uv is not None else None,
+            d_hits = DeviceArray((n, 2), np.int32)
+        check(lib().pg_trace_closest(self.h, d_rays.ptr, n, _dptr(d_hits), _dptr(d_uv),
                                      _stream_handle()))
         return d_hits
 
     def trace_counts(self, d_rays):
         """Closest hit plus per-ray (nodes fetched, prims tested), int32 [n, 2]."""
-        import torch
+        from .hip import DeviceArray
         n = d_rays.shape[0]
-        d_hits = torch.empty((n, 2), dtype=torch.int32, device=d_rays.device)
-        d_cnt = torch.empty((n, 2), dtype=torch.int32, device=d_rays.device)
-        check(lib().pg_trace_closest_counts(self.h, ctypes.c_void_p(d_rays.data_ptr()), n,
-                                            ctypes.c_void_p(d_hits.data_ptr()),
-                                            ctypes.c_void_p(d_cnt.data_ptr()), _stream_handle()))
+        d_hits = DeviceArray((n, 2), np.int32)
+        d_cnt = DeviceArray((n, 2), np.int32)
+        check(lib().pg_trace_closest_counts(self.h, d_rays.ptr, n, d_hits.ptr, d_cnt.ptr,
+                                            _stream_handle()))
         return d_hits, d_cnt
 
     def trace_any(self, d_rays, d_occ=None):
-        import torch
+        from .hip import DeviceArray
         n = d_rays.shape[0]
         if d_occ is None:
-            d_occ = torch.empty((n,), dtype=torch.int32, device=d_rays.device)
-        check(lib().pg_trace_any(self.h, ctypes.c_void_p(d_rays.data_ptr()), n,
-                                 ctypes.c_void_p(d_occ.data_ptr()), _stream_handle()))
+            d_occ = DeviceArray((n,), np.int32)
+        check(lib().pg_trace_any(self.h, d_rays.ptr, n, d_occ.ptr, _stream_handle()))
+        return d_occ
+
+    def trace_occluded(self, d_rays, d_occ=None):
+        from .hip import DeviceArray
+        n = d_rays.shape[0]
+        if d_occ is None:
+            d_occ = DeviceArray((n,), np.int32)
+        check(lib().pg_trace_occluded(self.h, d_rays.ptr, n, d_occ.ptr, _stream_handle()))
         return d_occ
 
     def release(self):
         if self.h:
             lib().pg_accel_release(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+# --- scene + Path integrator ---------------------------------------------------------
+
+class Scene:
+    """A loaded pine scene on the GPU (pine's LoadScene + Path integrator)."""
+
+    def __init__(self, path=None, text=None, directory=None, width=2):
+        self.h = ctypes.c_void_p()
+        o = _BuildOptions(width, 4, 32, 0)
+        if path is not None:
+            check(lib().pg_scene_load(path.encode(), ctypes.byref(o), ctypes.byref(self.h)))
+        else:
+            check(lib().pg_scene_load_text(text.encode(), (directory or ".").encode(),
+                                           ctypes.byref(o), ctypes.byref(self.h)))
+
+    def info(self):
+        s = _SceneInfo()
+        check(lib().pg_scene_get_info(self.h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in _SceneInfo._fields_}
+
+    def render(self, spp=0, samples_per_batch=0):
+        o = _RenderOptions(spp, samples_per_batch)
+        check(lib().pg_render(self.h, ctypes.byref(o), _stream_handle()))
+        return self.info()
+
+    def film(self):
+        """Film sums [h, w, 4] = (sum w*r, sum w*g, sum w*b, sum w), row 0 on top."""
+        inf = self.info()
+        out = np.zeros((inf["height"], inf["width"], 4), np.float32)
+        check(lib().pg_scene_film(self.h, _ptr(out)))
+        return out
+
+    def image(self):
+        """Linear radiance [h, w, 3] as Film::CopyToRGBArray computes it."""
+        f = self.film()
+        w = f[..., 3:4]
+        return np.where(w != 0, f[..., :3] / np.where(w != 0, w, 1), 0).astype(np.float32)
+
+    def release(self):
+        if self.h:
+            lib().pg_scene_release(self.h)
             self.h = ctypes.c_void_p()
 
     def __del__(self):

@@ -18,8 +18,8 @@ CLI = os.path.join(HERE, "pine")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
-HOST_SRC = ["bvh_build.cpp", "workload.cpp"]
-HIP_SRC = ["accel.hip"]
+HOST_SRC = ["bvh_build.cpp", "workload.cpp", "scene_parse.cpp"]
+HIP_SRC = ["accel.hip", "render.hip"]
 
 CXXFLAGS = ["-std=c++17", "-O3", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
 HIPFLAGS = [
@@ -54,6 +54,10 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp, *objs, "-lpthread"])
     os.replace(tmp, LIB)
+    # The `pine` command line, linked against the library next to it.
+    _run(["g++", *CXXFLAGS, "-o", CLI + ".tmp", os.path.join(CSRC, "pine_cli.cpp"), LIB,
+          "-Wl,-rpath,$ORIGIN"])
+    os.replace(CLI + ".tmp", CLI)
     return LIB
 
 

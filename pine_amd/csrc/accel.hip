@@ -132,7 +132,11 @@ __device__ __forceinline__ bool slab(const RayIn& r, float lx, float ly, float l
     return tn <= tf;
 }
 
-template <bool kAny>
+// kMode: 0 closest hit, 1 any hit with pine's Hit tests, 2 occlusion with
+// pine's Intersect tests (first accepted prim ends the ray).
+enum TraceMode : int { kClosest = 0, kAnyHit = 1, kOccluded = 2 };
+
+template <int kMode>
 __device__ __forceinline__ bool leaf_test(int32_t code, const float4* __restrict__ tris,
                                           const float4* __restrict__ ana, const RayIn& r,
                                           float& best_t, int32_t& best_prim, float& bu, float& bv,
@@ -145,12 +149,14 @@ __device__ __forceinline__ bool leaf_test(int32_t code, const float4* __restrict
             const float4* t = tris + 3 * (int64_t)(start + k);
             float4 a = t[0], b = t[1], c = t[2];
             f3 v0 = mk(a.x, a.y, a.z), e1 = mk(b.x, b.y, b.z), e2 = mk(c.x, c.y, c.z);
-            if (kAny) {
+            if (kMode == kAnyHit) {
                 if (tri_any(r.o, r.d, r.tmin, r.tmax, v0, e1, e2))
                     return true;
             } else {
-                tri_closest(r.o, r.d, r.tmin, v0, e1, e2, __float_as_int(a.w), best_t, best_prim,
-                            bu, bv);
+                bool acc = tri_closest(r.o, r.d, r.tmin, v0, e1, e2, __float_as_int(a.w), best_t,
+                                       best_prim, bu, bv);
+                if (kMode == kOccluded && acc)
+                    return true;
             }
         }
     } else {
@@ -162,11 +168,13 @@ __device__ __forceinline__ bool leaf_test(int32_t code, const float4* __restrict
             rec[4] = q1.x, rec[5] = q1.y, rec[6] = q1.z, rec[7] = q1.w;
             rec[8] = q2.x, rec[9] = q2.y, rec[10] = q2.z, rec[11] = q2.w;
             rec[12] = q3.x, rec[13] = q3.y, rec[14] = q3.z, rec[15] = q3.w;
-            if (kAny) {
+            if (kMode == kAnyHit) {
                 if (analytic_any(rec, r.o, r.d, r.tmin, r.tmax))
                     return true;
             } else {
-                analytic_closest(rec, r.o, r.d, r.tmin, best_t, best_prim, bu, bv);
+                bool acc = analytic_closest(rec, r.o, r.d, r.tmin, best_t, best_prim, bu, bv);
+                if (kMode == kOccluded && acc)
+                    return true;
             }
         }
     }
@@ -174,7 +182,7 @@ __device__ __forceinline__ bool leaf_test(int32_t code, const float4* __restrict
 }
 
 // Binary nodes: 64 B = {lo0.xyz, hi0.x}, {hi0.yz, lo1.xy}, {lo1.z, hi1.xyz}, {c0, c1, -, -}.
-template <int kBlock, bool kAny, bool kStats>
+template <int kBlock, int kMode, bool kStats>
 __global__ __launch_bounds__(kBlock) void k_trace2(const float4* __restrict__ nodes,
                                                    const float4* __restrict__ tris,
                                                    const float4* __restrict__ ana,
@@ -201,7 +209,7 @@ __global__ __launch_bounds__(kBlock) void k_trace2(const float4* __restrict__ no
             const float4* nd = nodes + 4 * (int64_t)node;
             float4 a = nd[0], b = nd[1], c = nd[2], d = nd[3];
             const int32_t c0 = __float_as_int(d.x), c1 = __float_as_int(d.y);
-            const float lim = kAny ? r.tmax : best_t;
+            const float lim = kMode == kAnyHit ? r.tmax : best_t;
             float t0, t1;
             bool h0 = slab(r, a.x, a.y, a.z, a.w, b.x, b.y, lim, t0) && c0 != kEmptyChild;
             bool h1 = slab(r, b.z, b.w, c.x, c.y, c.z, c.w, lim, t1) && c1 != kEmptyChild;
@@ -218,7 +226,7 @@ __global__ __launch_bounds__(kBlock) void k_trace2(const float4* __restrict__ no
                 continue;
             }
         } else {
-            if (leaf_test<kAny>(node, tris, ana, r, best_t, best_prim, bu, bv, tested)) {
+            if (leaf_test<kMode>(node, tris, ana, r, best_t, best_prim, bu, bv, tested)) {
                 found_any = true;
                 break;
             }
@@ -230,7 +238,7 @@ __global__ __launch_bounds__(kBlock) void k_trace2(const float4* __restrict__ no
     }
     if (kStats)
         counts[i] = make_int2(visited, tested);
-    if (kAny) {
+    if (kMode != kClosest) {
         occluded[i] = found_any ? 1 : 0;
     } else {
         pg_hit h;
@@ -247,7 +255,7 @@ __global__ __launch_bounds__(kBlock) void k_trace2(const float4* __restrict__ no
 }
 
 // 4-wide nodes: 128 B = lox[4], loy[4], loz[4], hix[4], hiy[4], hiz[4], child[4], pad.
-template <int kBlock, bool kAny, bool kStats>
+template <int kBlock, int kMode, bool kStats>
 __global__ __launch_bounds__(kBlock) void k_trace4(const float4* __restrict__ nodes,
                                                    const float4* __restrict__ tris,
                                                    const float4* __restrict__ ana,
@@ -274,7 +282,7 @@ __global__ __launch_bounds__(kBlock) void k_trace4(const float4* __restrict__ no
             const float4* nd = nodes + 8 * (int64_t)node;
             float4 lx = nd[0], ly = nd[1], lz = nd[2], hx = nd[3], hy = nd[4], hz = nd[5];
             int4 ch = *reinterpret_cast<const int4*>(nd + 6);
-            const float lim = kAny ? r.tmax : best_t;
+            const float lim = kMode == kAnyHit ? r.tmax : best_t;
             float t[4];
             int32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
             bool h[4];
@@ -313,7 +321,7 @@ __global__ __launch_bounds__(kBlock) void k_trace4(const float4* __restrict__ no
                 continue;
             }
         } else {
-            if (leaf_test<kAny>(node, tris, ana, r, best_t, best_prim, bu, bv, tested)) {
+            if (leaf_test<kMode>(node, tris, ana, r, best_t, best_prim, bu, bv, tested)) {
                 found_any = true;
                 break;
             }
@@ -325,7 +333,7 @@ __global__ __launch_bounds__(kBlock) void k_trace4(const float4* __restrict__ no
     }
     if (kStats)
         counts[i] = make_int2(visited, tested);
-    if (kAny) {
+    if (kMode != kClosest) {
         occluded[i] = found_any ? 1 : 0;
     } else {
         pg_hit hh;
@@ -341,13 +349,13 @@ __global__ __launch_bounds__(kBlock) void k_trace4(const float4* __restrict__ no
     }
 }
 
-template <bool kAny, bool kStats>
+template <int kMode, bool kStats>
 hipError_t launch_trace(const pg_accel* a, const pg_ray* rays, int64_t n, pg_hit* hits,
                         pg_hit_uv* uvs, int32_t* occ, int2* counts, hipStream_t s) {
     const int block = a->block;
     const int64_t grid = (n + block - 1) / block;
 #define PG_LAUNCH(K, B)                                                                        \
-    hipLaunchKernelGGL((K<B, kAny, kStats>), dim3((unsigned)grid), dim3(B), 0, s, a->d_nodes,   \
+    hipLaunchKernelGGL((K<B, kMode, kStats>), dim3((unsigned)grid), dim3(B), 0, s, a->d_nodes,   \
                        a->d_tris, a->d_ana, rays, n, hits, uvs, occ, counts)
     if (a->width == 2) {
         if (block == 64)
@@ -605,7 +613,7 @@ extern "C" int pg_trace_closest(pg_accel* a, const pg_ray* d_rays, int64_t n, pg
         return rc;
     if (n == 0)
         return PG_OK;
-    PG_HIP((launch_trace<false, false>(a, d_rays, n, d_hits, d_uv, nullptr, nullptr,
+    PG_HIP((launch_trace<kClosest, false>(a, d_rays, n, d_hits, d_uv, nullptr, nullptr,
                                        (hipStream_t)stream)));
     return PG_OK;
 }
@@ -616,8 +624,19 @@ extern "C" int pg_trace_any(pg_accel* a, const pg_ray* d_rays, int64_t n, int32_
         return rc;
     if (n == 0)
         return PG_OK;
-    PG_HIP((launch_trace<true, false>(a, d_rays, n, nullptr, nullptr, d_occluded, nullptr,
+    PG_HIP((launch_trace<kAnyHit, false>(a, d_rays, n, nullptr, nullptr, d_occluded, nullptr,
                                       (hipStream_t)stream)));
+    return PG_OK;
+}
+
+extern "C" int pg_trace_occluded(pg_accel* a, const pg_ray* d_rays, int64_t n,
+                                  int32_t* d_occluded, void* stream) {
+    if (int rc = check_trace(a, d_rays, n, d_occluded, "pg_trace_occluded"))
+        return rc;
+    if (n == 0)
+        return PG_OK;
+    PG_HIP((launch_trace<kOccluded, false>(a, d_rays, n, nullptr, nullptr, d_occluded, nullptr,
+                                          (hipStream_t)stream)));
     return PG_OK;
 }
 
@@ -629,7 +648,7 @@ extern "C" int pg_trace_closest_counts(pg_accel* a, const pg_ray* d_rays, int64_
         return pg_fail(PG_E_ARG, "pg_trace_closest_counts: null counts buffer");
     if (n == 0)
         return PG_OK;
-    PG_HIP((launch_trace<false, true>(a, d_rays, n, d_hits, nullptr, nullptr,
+    PG_HIP((launch_trace<kClosest, true>(a, d_rays, n, d_hits, nullptr, nullptr,
                                       reinterpret_cast<int2*>(d_counts), (hipStream_t)stream)));
     return PG_OK;
 }

@@ -1,0 +1,1206 @@
+// Wavefront path tracer behind pine's `Integrator: Path` (PathIntegrator::Li,
+// src/impl/integrator/path.cpp:6-87; PixelIntegrator::Render and
+// RadianceIntegrator::Compute, src/core/integrator.cpp:121-158) plus the scene
+// loader behind pine's LoadScene (src/util/fileio.cpp:251-281).
+//
+// One batch = every pixel times `spb` samples, one path per lane.  Each bounce
+// is a sequence of kernels over compacted queues in HBM:
+//   extend   closest hit for the active rays            (accel kernels)
+//   shade    emission + MIS, light sample -> shadow ray, BSDF sample -> next
+//            ray, Russian roulette; survivors appended to the next queue
+//   shadow   occlusion of the shadow rays               (accel kernels)
+//   resolve  adds the unoccluded light contributions to the paths' radiance
+// and after the last bounce `film` adds each path to its pixel with pine's
+// filter-table weight, sample by sample, so the film sum is deterministic.
+// Dimension bookkeeping of the Halton sampler follows pine's call order, so a
+// path sees exactly the sample values pine's would.
+#include "bvh_build.h"
+#include "pg_api.h"
+#include "pg_isect.h"
+#include "pg_shade.h"
+#include "scene_parse.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+using namespace pg;
+
+#define PG_HIP(call)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return pg_fail(PG_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+namespace pg {
+
+enum ShapeType : int32_t { kShapeMesh = 0, kShapeSphere = 1, kShapeRect = 2, kShapePlane = 3 };
+
+struct ShapeDev {
+    int32_t type;
+    int32_t material;   // index into materials, -1 = none
+    int32_t first_prim; // global prim id of the shape's first prim
+    int32_t tri_offset; // meshes: first triangle in tri_verts
+    f3 p0, e0, e1, n;   // sphere: p0 = c; rect: p0 = pos, e0/e1 = ex/ey unit, n; plane: p0, n, e0 = u, e1 = v
+    float lx, ly, r, area;
+};
+
+enum LightType : int32_t { kLightArea = 0, kLightPoint = 1, kLightDirectional = 2 };
+struct LightDev {
+    int32_t type;
+    int32_t shape;
+    f3 v;      // point: position; directional: unit direction
+    f3 color;
+};
+
+struct CameraDev {
+    f3 cx, cy, cz, cw;  // LookAt columns (vecmath.h:1212-1221)
+    float fov2d[2];
+    float lensRadius, focalDistance;
+};
+
+struct SceneDev {
+    const int32_t* prim_shape;
+    const float* tri_verts;  // 9 floats per mesh triangle (v0, v1, v2)
+    const ShapeDev* shapes;
+    const MaterialDev* materials;
+    const LightDev* lights;
+    const float* light_func;  // Distribution1D func (lightsampler.cpp:20-25)
+    const float* light_cdf;   // n+1 entries
+    int32_t n_lights;
+    int32_t light_uniform;    // UniformLightSampler
+    float light_func_int;
+    CameraDev cam;
+    HaltonDev halton;
+    float filter_table[256];
+    float filter_radius[2];
+    int32_t width, height;
+    int32_t max_depth;
+};
+
+// Path state, structure of arrays (one entry per path of the batch).
+struct PathsDev {
+    float* L;       // 3 per path
+    float* beta;    // 3 per path
+    float* bsdf_pdf;
+    uint64_t* sidx; // Halton index
+    int32_t* sdim;  // next Halton dimension
+    int32_t* depth;
+    float* pfilm;   // 2 per path
+    // queues
+    int32_t* queue_in;
+    int32_t* queue_out;
+    pg_ray* rays_in;
+    pg_ray* rays_out;
+    pg_hit* hits;
+    pg_hit_uv* uvs;
+    int32_t* counters;  // [0] next-queue length, [1] shadow-queue length
+    pg_ray* shadow_rays;
+    int32_t* shadow_path;
+    float* shadow_contrib;  // 3 per shadow ray
+    int32_t* occluded;
+};
+
+}  // namespace pg
+
+// ----------------------------------------------------------------------------
+// Device code
+
+namespace {
+
+// ThinLenCamera::GenRay (camera.cpp:31-41) for pFilm in [0,1]^2.
+__device__ void gen_ray(const CameraDev& c, float fx, float fy, float u0, float u1, f3& o, f3& d) {
+    float sx = (fx - 0.5f) * 2.0f * c.fov2d[0];
+    float sy = (fy - 0.5f) * 2.0f * c.fov2d[1];
+    f3 dir = normalize(mk(sx, sy, 1.0f));
+    f3 pFocus = c.focalDistance * dir / dir.z;
+    float lx, ly;
+    sample_disk_polar(u0, u1, lx, ly);
+    f3 pLen = c.lensRadius * mk(lx, ly, 0.0f);
+    // c2w * vec4(pLen, 1) and (mat3)c2w * Normalize(pFocus - pLen)
+    o = c.cx * pLen.x + c.cy * pLen.y + c.cz * pLen.z + c.cw * 1.0f;
+    f3 dl = normalize(pFocus - pLen);
+    d = c.cx * dl.x + c.cy * dl.y + c.cz * dl.z;
+}
+
+__device__ __forceinline__ void store_ray(pg_ray* r, f3 o, f3 d, float tmin, float tmax) {
+    float4* p = reinterpret_cast<float4*>(r);
+    p[0] = make_float4(o.x, o.y, o.z, tmin);
+    p[1] = make_float4(d.x, d.y, d.z, tmax);
+}
+
+__global__ void k_raygen(SceneDev s, PathsDev P, int32_t n_paths, int32_t spb, int32_t sample0) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_paths)
+        return;
+    const int32_t pixel = i / spb, sl = i - pixel * spb;
+    const int px = pixel % s.width, py = pixel / s.width;
+    SamplerState ss;
+    ss.index = halton_index(s.halton, px, py, sample0 + sl);
+    ss.dim = 2;
+    float u0, u1, l0, l1;
+    get_2d(s.halton, ss, u0, u1);
+    // (p + Get2D()) / film->Size()  (integrator.cpp:153)
+    float fx = ((float)px + u0) / (float)s.width;
+    float fy = ((float)py + u1) / (float)s.height;
+    get_2d(s.halton, ss, l0, l1);
+    f3 o, d;
+    gen_ray(s.cam, fx, fy, l0, l1, o, d);
+    P.L[3 * i + 0] = P.L[3 * i + 1] = P.L[3 * i + 2] = 0.0f;
+    P.beta[3 * i + 0] = P.beta[3 * i + 1] = P.beta[3 * i + 2] = 1.0f;
+    P.bsdf_pdf[i] = 0.0f;
+    P.sidx[i] = ss.index;
+    P.sdim[i] = ss.dim;
+    P.depth[i] = 0;
+    P.pfilm[2 * i + 0] = fx;
+    P.pfilm[2 * i + 1] = fy;
+    P.queue_in[i] = i;
+    store_ray(&P.rays_in[i], o, d, 0.0f, kFloatMax);
+}
+
+struct Hit {
+    f3 p, n;
+    int32_t shape;
+};
+
+// Surface data after a closest hit, as BVH::Intersect's post-hit closure and the
+// shapes' Intersect members fill the Interaction (bvh.cpp:690-708,
+// geometry.cpp:95-195).
+__device__ Hit surface(const SceneDev& s, int32_t prim, float t, float u, float v, f3 o, f3 d) {
+    Hit h;
+    h.shape = s.prim_shape[prim];
+    const ShapeDev& sh = s.shapes[h.shape];
+    if (sh.type == kShapeMesh) {
+        const float* tv = s.tri_verts + 9 * (int64_t)(sh.tri_offset + (prim - sh.first_prim));
+        f3 v0 = mk(tv[0], tv[1], tv[2]), v1 = mk(tv[3], tv[4], tv[5]), v2 = mk(tv[6], tv[7], tv[8]);
+        h.p = (1.0f - u - v) * v0 + u * v1 + v * v2;  // InterpolatePosition (geometry.h:308-310)
+        h.n = normalize(cross(v0 - v1, v0 - v2));
+    } else if (sh.type == kShapeSphere) {
+        // Normalize(ray(t) - c) with ray(t) - c evaluated as t*d + (o - c).
+        f3 q = mk(d.x * t + (o.x - sh.p0.x), d.y * t + (o.y - sh.p0.y), d.z * t + (o.z - sh.p0.z));
+        h.n = normalize(q);
+        h.p = sh.p0 + h.n * sh.r;
+    } else if (sh.type == kShapeRect) {
+        float ur = u - 0.5f, vr = v - 0.5f;
+        h.p = sh.p0 + sh.lx * sh.e0 * ur + sh.ly * sh.e1 * vr;
+        h.n = sh.n;
+    } else {
+        f3 p = o + t * d;
+        float pu = dot(p - sh.p0, sh.e0), pv = dot(p - sh.p0, sh.e1);
+        h.p = sh.p0 + pu * sh.e0 + pv * sh.e1;
+        h.n = sh.n;
+    }
+    return h;
+}
+
+// EmissiveMaterial::Le through MaterialEvalCtx (material.h:14-33, 60-65): the
+// emission when wi, expressed in the frame of n, points above the surface.
+__device__ f3 emitted(const MaterialDev& m, f3 n, f3 wi) {
+    M3 w2n = inverse(frame_from(n));
+    f3 wl = mul(w2n, wi);
+    return cos_theta(wl) > 0.0f ? m.emission : mk(0, 0, 0);
+}
+
+// PowerLightSampler::SampleLight / UniformLightSampler::SampleLight
+// (lightsampler.cpp:6-34) with Distribution1D::SampleDiscrete
+// (distribution.h:27-52): pdf is func[i]/n, not normalised, as in pine.
+__device__ int sample_light(const SceneDev& s, float u, float& pdf) {
+    const int n = s.n_lights;
+    if (s.light_uniform) {
+        int idx = (int)(u * (float)n);
+        idx = idx < n - 1 ? idx : n - 1;
+        pdf = 1.0f / (float)n;
+        return idx;
+    }
+    float cont, fpdf;
+    if (s.light_func_int == 0.0f) {
+        fpdf = 1.0f;
+        cont = u;
+    } else {
+        // FindInterval(size, cdf[i] <= u) (math.h FindInterval), clamped.
+        int first = 0, len = n + 1;
+        while (len > 0) {
+            int half = len >> 1, middle = first + half;
+            if (s.light_cdf[middle] <= u) {
+                first = middle + 1;
+                len -= half + 1;
+            } else {
+                len = half;
+            }
+        }
+        int offset = first - 1;
+        offset = offset < 0 ? 0 : (offset > n - 1 ? n - 1 : offset);
+        float du = u - s.light_cdf[offset];
+        if (s.light_cdf[offset + 1] - s.light_cdf[offset] > 0)
+            du /= s.light_cdf[offset + 1] - s.light_cdf[offset];
+        fpdf = s.light_func[offset];
+        cont = ((float)offset + du) / (float)n;
+        cont = cont < kOneMinusEps ? cont : kOneMinusEps;
+    }
+    int idx = (int)(cont * (float)n);
+    pdf = fpdf * (1.0f / (float)n);
+    return idx;
+}
+
+struct LightSampleD {
+    f3 Le, wo;
+    float distance, pdf;
+    bool delta;
+};
+
+// Light::Sample (light.cpp:9-63) and Shape::Sample (geometry.h:435-442).
+__device__ LightSampleD sample_light_point(const SceneDev& s, const LightDev& L, f3 p, float u0,
+                                           float u1) {
+    LightSampleD ls;
+    ls.delta = false;
+    if (L.type == kLightPoint) {
+        ls.wo = normalize(L.v - p, ls.distance);
+        ls.pdf = ls.distance * ls.distance;
+        ls.Le = L.color;
+        ls.delta = true;
+        return ls;
+    }
+    if (L.type == kLightDirectional) {
+        ls.Le = L.color;
+        ls.wo = L.v;
+        ls.distance = 1e10f;
+        ls.pdf = 1.0f;
+        ls.delta = true;
+        return ls;
+    }
+    const ShapeDev& sh = s.shapes[L.shape];
+    f3 sp, sn;
+    if (sh.type == kShapeRect) {
+        sp = sh.p0 + sh.e0 * sh.lx * (u0 - 0.5f) + sh.e1 * sh.ly * (u1 - 0.5f);
+        sn = sh.n;
+        sp = offset_ray_origin(sp, sn);
+    } else if (sh.type == kShapeSphere) {
+        // UniformSphereSampling (sampling.h:41-43) via SphericalToCartesian.
+        float phi = u0 * kPi * 2.0f, theta = pstd::acos(1.0f - 2.0f * u1);
+        float st = pstd::sin(theta);
+        sn = mk(st * pstd::cos(phi), st * pstd::sin(phi), pstd::cos(theta));
+        sp = sh.p0 + sh.r * sn;
+        sp = offset_ray_origin(sp, sn);
+    } else {
+        // Single-triangle emitters (meshes cannot be sampled in pine).
+        const float* tv = s.tri_verts + 9 * (int64_t)sh.tri_offset;
+        f3 v0 = mk(tv[0], tv[1], tv[2]), v1 = mk(tv[3], tv[4], tv[5]), v2 = mk(tv[6], tv[7], tv[8]);
+        if (u0 + u1 > 1.0f) {
+            u0 = 1.0f - u0;
+            u1 = 1.0f - u1;
+        }
+        sp = (1.0f - u0 - u1) * v0 + u0 * v1 + u1 * v2;
+        sn = normalize(cross(v0 - v1, v0 - v2));
+        sp = offset_ray_origin(sp, sn);
+    }
+    ls.wo = normalize(sp - p, ls.distance);
+    const MaterialDev& m = s.materials[sh.material];
+    ls.Le = emitted(m, sn, -ls.wo);
+    ls.pdf = ls.distance * ls.distance / (abs_dot(-ls.wo, sn) * sh.area);
+    return ls;
+}
+
+__global__ void k_shade(SceneDev s, PathsDev P, int32_t n_active) {
+    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_active)
+        return;
+    const int32_t i = P.queue_in[k];
+    const float4* r4 = reinterpret_cast<const float4*>(&P.rays_in[k]);
+    const float4 ra = r4[0], rb = r4[1];
+    const f3 ro = mk(ra.x, ra.y, ra.z), rd = mk(rb.x, rb.y, rb.z);
+    const pg_hit hit = P.hits[k];
+    int depth = P.depth[i];
+    f3 L = mk(P.L[3 * i], P.L[3 * i + 1], P.L[3 * i + 2]);
+    f3 beta = mk(P.beta[3 * i], P.beta[3 * i + 1], P.beta[3 * i + 2]);
+    const float bsdfPdf = P.bsdf_pdf[i];
+    SamplerState ss;
+    ss.index = P.sidx[i];
+    ss.dim = P.sdim[i];
+    bool alive = false;
+    f3 nextO, nextD;
+
+    if (hit.prim >= 0) {
+        const pg_hit_uv uv = P.uvs[k];
+        Hit h = surface(s, hit.prim, hit.t, uv.u, uv.v, ro, rd);
+        const ShapeDev& sh = s.shapes[h.shape];
+        if (sh.material < 0) {
+            // Surface without a material: continue straight through (path.cpp:42-46).
+            nextD = rd;
+            nextO = offset_ray_origin(h.p, face_same_hemisphere(h.n, rd));
+            alive = true;
+        } else {
+            const MaterialDev& m = s.materials[sh.material];
+            const f3 wi = -rd;
+            if (m.kind == kMatEmissive) {
+                f3 le = emitted(m, h.n, wi);
+                if (depth == 0) {
+                    L = L + beta * le;
+                } else {
+                    // Shape::Pdf (geometry.h:432-434); BSDF-hit MIS (path.cpp:51-60)
+                    float lightPdf = hit.t * hit.t / (abs_dot(-rd, h.n) * sh.area);
+                    L = L + beta * le * power_heuristic(bsdfPdf, lightPdf);
+                }
+            } else if (depth + 1 < s.max_depth) {
+                const M3 n2w = frame_from(h.n);
+                const M3 w2n = inverse(n2w);
+                const f3 wiL = mul(w2n, wi);
+                // EstimateDirect (integrator.cpp:88-119)
+                float lpdf;
+                const float ul = get_1d(s.halton, ss);
+                float la, lb;
+                if (s.n_lights > 0) {
+                    int li = sample_light(s, ul, lpdf);
+                    get_2d(s.halton, ss, la, lb);
+                    LightSampleD ls = sample_light_point(s, s.lights[li], h.p, la, lb);
+                    ls.pdf *= lpdf;
+                    const f3 woL = mul(w2n, ls.wo);
+                    f3 f = material_f(m, wiL, woL) * abs_dot(ls.wo, h.n);
+                    float scatPdf = 0.0f;
+                    if (!ls.delta)
+                        scatPdf = material_pdf(m, wiL, woL);
+                    float w = power_heuristic(ls.pdf, scatPdf);
+                    f3 ld = ls.delta ? f * ls.Le / ls.pdf : f * w * ls.Le / ls.pdf;
+                    f3 contrib = beta * ld;
+                    if (!is_black(ld)) {
+                        int slot = atomicAdd(&P.counters[1], 1);
+                        f3 so = offset_ray_origin(h.p, face_same_hemisphere(h.n, ls.wo));
+                        store_ray(&P.shadow_rays[slot], so, ls.wo, 0.0f,
+                                  ls.distance * (1.0f - 1e-3f));
+                        P.shadow_path[slot] = i;
+                        P.shadow_contrib[3 * slot + 0] = contrib.x;
+                        P.shadow_contrib[3 * slot + 1] = contrib.y;
+                        P.shadow_contrib[3 * slot + 2] = contrib.z;
+                    }
+                }
+                // BSDF sample (path.cpp:66-86)
+                const float u1 = get_1d(s.halton, ss);
+                float u2a, u2b;
+                get_2d(s.halton, ss, u2a, u2b);
+                BSDFSample bs;
+                if (material_sample(m, wiL, u1, u2a, u2b, bs)) {
+                    bs.wo = mul(n2w, bs.wo);
+                    if (!is_black(bs.f) && bs.pdf != 0.0f) {
+                        beta = beta * (abs_dot(bs.wo, h.n) * bs.f / bs.pdf);
+                        P.bsdf_pdf[i] = bs.pdf;
+                        nextD = bs.wo;
+                        nextO = offset_ray_origin(h.p, face_same_hemisphere(h.n, nextD));
+                        alive = true;
+                        if (depth > 2) {
+                            float q = pstd::clamp(1.0f - luminance(beta), 0.05f, 1.0f);
+                            if (get_1d(s.halton, ss) < q)
+                                alive = false;
+                            else
+                                beta = beta / (1.0f - q);
+                        }
+                    }
+                }
+                depth++;
+            }
+        }
+    }
+    P.L[3 * i + 0] = L.x;
+    P.L[3 * i + 1] = L.y;
+    P.L[3 * i + 2] = L.z;
+    P.beta[3 * i + 0] = beta.x;
+    P.beta[3 * i + 1] = beta.y;
+    P.beta[3 * i + 2] = beta.z;
+    P.sdim[i] = ss.dim;
+    P.depth[i] = depth;
+    if (alive) {
+        int slot = atomicAdd(&P.counters[0], 1);
+        P.queue_out[slot] = i;
+        store_ray(&P.rays_out[slot], nextO, nextD, 0.0f, kFloatMax);
+    }
+}
+
+__global__ void k_resolve_shadow(PathsDev P, int32_t n_shadow) {
+    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_shadow || P.occluded[k])
+        return;
+    const int32_t i = P.shadow_path[k];
+    // One shadow ray per path per bounce: no two lanes share a path here.
+    P.L[3 * i + 0] += P.shadow_contrib[3 * k + 0];
+    P.L[3 * i + 1] += P.shadow_contrib[3 * k + 1];
+    P.L[3 * i + 2] += P.shadow_contrib[3 * k + 2];
+}
+
+// Film::AddSample (film.h:26-45) for every path of a pixel, in sample order.
+__global__ void k_film(SceneDev s, PathsDev P, int32_t n_pixels, int32_t spb, float* film) {
+    const int32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pixel >= n_pixels)
+        return;
+    const int px = pixel % s.width, py = pixel / s.width;
+    const size_t own = (size_t)(s.height - 1 - py) * s.width + px;
+    float acc[4] = {film[4 * own + 0], film[4 * own + 1], film[4 * own + 2], film[4 * own + 3]};
+    for (int sl = 0; sl < spb; sl++) {
+        const int32_t i = pixel * spb + sl;
+        const float fx = P.pfilm[2 * i] * (float)s.width - 0.5f;
+        const float fy = P.pfilm[2 * i + 1] * (float)s.height - 0.5f;
+        const float rx = s.filter_radius[0], ry = s.filter_radius[1];
+        int x0 = (int)pstd::ceil(fx - rx), y0 = (int)pstd::ceil(fy - ry);
+        int x1 = (int)pstd::floor(fx + rx), y1 = (int)pstd::floor(fy + ry);
+        x0 = x0 > 0 ? x0 : 0;
+        y0 = y0 > 0 ? y0 : 0;
+        x1 = x1 < s.width - 1 ? x1 : s.width - 1;
+        y1 = y1 < s.height - 1 ? y1 : s.height - 1;
+        const float L0 = P.L[3 * i], L1 = P.L[3 * i + 1], L2 = P.L[3 * i + 2];
+        for (int y = y0; y <= y1; y++)
+            for (int x = x0; x <= x1; x++) {
+                float dx = fabsp((float)x - fx) / rx, dy = fabsp((float)y - fy) / ry;
+                int tx = (int)(16.0f * (dx < kOneMinusEps ? dx : kOneMinusEps));
+                int ty = (int)(16.0f * (dy < kOneMinusEps ? dy : kOneMinusEps));
+                const float w = s.filter_table[ty * 16 + tx];
+                const size_t pix = (size_t)(s.height - 1 - y) * s.width + x;
+                if (pix == own) {
+                    acc[0] += L0 * w;
+                    acc[1] += L1 * w;
+                    acc[2] += L2 * w;
+                    acc[3] += w;
+                } else {
+                    // A sample exactly on a pixel border also feeds the neighbour.
+                    atomicAdd(&film[4 * pix + 0], L0 * w);
+                    atomicAdd(&film[4 * pix + 1], L1 * w);
+                    atomicAdd(&film[4 * pix + 2], L2 * w);
+                    atomicAdd(&film[4 * pix + 3], w);
+                }
+            }
+    }
+    film[4 * own + 0] = acc[0];
+    film[4 * own + 1] = acc[1];
+    film[4 * own + 2] = acc[2];
+    film[4 * own + 3] = acc[3];
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------------
+// Host: scene loading (fileio.cpp:251-281 and the Create* factories)
+
+struct pg_scene {
+    std::string dir;
+    Params root;
+    // host scene
+    std::vector<ShapeDev> shapes;
+    std::vector<MaterialDev> materials;
+    std::vector<LightDev> lights;
+    std::vector<float> tri_verts;
+    std::vector<int32_t> prim_shape;
+    std::vector<float> light_func, light_cdf;
+    float light_func_int = 0.0f;
+    bool light_uniform = false;
+    SceneDev dev{};
+    int spp = 1, max_depth = 4;
+    std::string output = "result.png";
+    bool tone_map = true;
+    pg_accel* accel = nullptr;
+    // device buffers owned by the scene
+    std::vector<void*> allocations;
+    float* d_film = nullptr;
+    PathsDev paths{};
+    int64_t path_capacity = 0;
+    double last_render_ms = 0.0;
+    int64_t last_rays = 0;
+
+    ~pg_scene() {
+        for (void* p : allocations)
+            (void)hipFree(p);
+        if (accel)
+            pg_accel_release(accel);
+    }
+};
+
+namespace {
+
+std::runtime_error unsupported(const std::string& what) {
+    return std::runtime_error("unsupported by the GPU path: " + what);
+}
+
+// A node input evaluated once: pine's CreateNode (node.cpp:33-98) for the
+// constant forms a Path scene can use on the GPU.
+struct ConstNode {
+    float f;
+    f3 v;
+};
+ConstNode const_node(const Params& p, const std::string& what) {
+    std::string type = p.get_string("type");
+    if (type == "Constant") {
+        float vec[3];
+        if (!p.has_value("float") || !p.has_value("vec3"))
+            throw std::runtime_error(what + ": Constant needs float and vec3");
+        p.get_vec3("vec3", vec);
+        return {p.get_float("float"), mk(vec[0], vec[1], vec[2])};
+    }
+    std::string at = p.get_string("@");
+    if (!at.empty() && type == at) {
+        // A plain value: Constant(stof(v), stofs(v)).
+        float vec[3];
+        pine_stofs(at, vec, 3);
+        float fv[3] = {vec[0], vec[1], vec[2]};
+        if (p.has_value("vec3"))
+            p.get_vec3("vec3", fv);
+        return {p.has_value("float") ? p.get_float("float") : pine_stof(at), mk(fv[0], fv[1], fv[2])};
+    }
+    if (type.empty())
+        throw std::runtime_error(what + ": missing value");
+    throw unsupported("node type \"" + type + "\" for " + what);
+}
+
+Layer make_layer(const Params& p) {
+    std::string type = p.get_string("type");
+    Layer l{};
+    if (type == "Dielectric") {
+        l.type = kDielectric;
+        l.albedo = p.has_subset("albedo") ? const_node(p.sub("albedo"), "albedo").v : mk(1, 1, 1);
+        l.roughness = const_node(p.sub("roughness"), "roughness").f;
+        l.eta = const_node(p.sub("eta"), "eta").f;
+    } else if (type == "Conductor") {
+        l.type = kConductor;
+        l.albedo = const_node(p.sub("albedo"), "albedo").v;
+        l.roughness = const_node(p.sub("roughness"), "roughness").f;
+    } else {
+        if (type != "Diffuse")
+            fprintf(stderr, "[BSDF][Create]Unknown type \"%s\"\n", type.c_str());
+        l.type = kDiffuse;
+        l.albedo = const_node(p.sub("albedo"), "albedo").v;
+    }
+    return l;
+}
+
+MaterialDev make_material(const Params& p) {
+    MaterialDev m{};
+    if (p.has_subset("bumpMap"))
+        throw unsupported("bump maps");
+    std::string type = p.get_string("type");
+    if (type == "Emissive") {
+        m.kind = kMatEmissive;
+        m.emission = const_node(p.sub("color"), "color").v;
+        return m;
+    }
+    if (type != "Layered")
+        fprintf(stderr, "[Material][Create]Unknown type \"%s\"\n", type.c_str());
+    m.kind = kMatLayered;
+    // LayeredMaterial ctor (material.cpp:37-47): layers by descending index.
+    std::vector<std::pair<int, const Params*>> layers;
+    for (auto& kv : p.subsets)
+        if (kv.first.compare(0, 5, "layer") == 0 && !kv.second.empty())
+            layers.push_back({pine_stoi(kv.first.substr(5)), &kv.second.back()});
+    std::stable_sort(layers.begin(), layers.end(),
+                     [](const auto& a, const auto& b) { return a.first > b.first; });
+    if ((int)layers.size() > kMaxLayers)
+        throw unsupported("more than 4 material layers");
+    m.nlayers = (int)layers.size();
+    for (int i = 0; i < m.nlayers; i++)
+        m.layers[i] = make_layer(*layers[i].second);
+    return m;
+}
+
+std::string read_file(const std::string& path) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f)
+        throw std::runtime_error("cannot open \"" + path + "\"");
+    std::stringstream ss;
+    ss << f.rdbuf();
+    return ss.str();
+}
+
+// LoadObj (objloader.cpp:12-101) for the v / f forms it accepts.
+void load_obj(const std::string& path, std::vector<float>& v, std::vector<uint32_t>& idx) {
+    std::string text = read_file(path);
+    std::istringstream in(text);
+    std::string line;
+    while (std::getline(in, line)) {
+        if (line.size() < 2)
+            continue;
+        if (line[0] == 'v' && line[1] != 't' && line[1] != 'n') {
+            float p[3];
+            pine_stofs(line.substr(2), p, 3);
+            v.insert(v.end(), p, p + 3);
+        } else if (line[0] == 'f') {
+            std::istringstream fs(line.substr(2));
+            std::vector<long> f;
+            std::string tok;
+            while (fs >> tok) {
+                long k = pine_stoi(tok.substr(0, tok.find('/')));
+                long nv = (long)(v.size() / 3);
+                f.push_back(k < 0 ? nv + k : k - 1);
+            }
+            if (f.size() == 3) {
+                idx.insert(idx.end(), {(uint32_t)f[0], (uint32_t)f[1], (uint32_t)f[2]});
+            } else if (f.size() == 4) {
+                idx.insert(idx.end(), {(uint32_t)f[0], (uint32_t)f[1], (uint32_t)f[2],
+                                       (uint32_t)f[0], (uint32_t)f[2], (uint32_t)f[3]});
+            }
+        }
+    }
+    if (!idx.empty()) {
+        uint32_t mn = *std::min_element(idx.begin(), idx.end());
+        for (auto& k : idx)
+            k -= mn;
+    }
+}
+
+void build_scene(pg_scene& S) {
+    const Params& root = S.root;
+    // Integrator block (integrator.cpp:20-56)
+    const Params& ip = root.sub("Integrator");
+    std::string itype = ip.get_string("type");
+    if (itype != "Path")
+        throw unsupported("integrator \"" + itype + "\" (the GPU path is pine's Path integrator)");
+    S.max_depth = ip.get_int("maxDepth", 4);
+    const Params& sp = ip.sub("sampler");
+    std::string stype = sp.get_string("type");
+    if (stype != "Halton")
+        throw unsupported("sampler \"" + stype + "\" (Halton only)");
+    S.spp = sp.get_int("samplesPerPixel");
+    std::string lstype = ip.sub("lightSampler").get_string("type", "Power");
+    S.light_uniform = lstype == "Uniform";
+    if (lstype != "Power" && lstype != "Uniform") {
+        fprintf(stderr, "[LightSampler][Create]Unknown type \"%s\"\n", lstype.c_str());
+        S.light_uniform = true;
+    }
+
+    // Materials by name.
+    std::map<std::string, int> mat_index;
+    for (const Params& p : root.all("Material")) {
+        mat_index[p.get_string("name")] = (int)S.materials.size();
+        S.materials.push_back(make_material(p));
+    }
+    if (!root.all("Medium").empty())
+        throw unsupported("participating media in a Path scene");
+    // Lights from Light blocks (fileio.cpp:264-265).
+    for (const Params& p : root.all("Light")) {
+        std::string t = p.get_string("type");
+        LightDev l{};
+        float a[3], c[3];
+        if (t == "Directional") {
+            l.type = kLightDirectional;
+            p.get_vec3("direction", a);
+            l.v = normalize(mk(a[0], a[1], a[2]));
+        } else if (t == "Point") {
+            l.type = kLightPoint;
+            p.get_vec3("position", a);
+            l.v = mk(a[0], a[1], a[2]);
+        } else {
+            throw unsupported("light type \"" + t + "\"");
+        }
+        p.get_vec3("color", c);
+        l.color = mk(c[0], c[1], c[2]);
+        S.lights.push_back(l);
+    }
+
+    // Shapes (geometry.cpp:366-411).
+    pg_accel* a = nullptr;
+    if (int rc = pg_accel_create(&a))
+        throw std::runtime_error(pg_last_error());
+    S.accel = a;
+    std::vector<int> area_light_shapes;
+    for (const Params& p : root.all("Shape")) {
+        std::string t = p.get_string("type");
+        ShapeDev sh{};
+        sh.material = -1;
+        std::string mname;
+        if (p.try_string("material", mname)) {
+            auto it = mat_index.find(mname);
+            if (it == mat_index.end())
+                throw std::runtime_error("material \"" + mname + "\" is not found");
+            sh.material = it->second;
+        }
+        if (p.has_value("medium") || p.has_value("mediumInside") || p.has_value("mediumOutside"))
+            throw unsupported("media on shapes");
+        float pos[3], e0[3], e1[3];
+        int32_t first = 0;
+        const int shape_index = (int)S.shapes.size();
+        if (t == "Sphere") {
+            p.get_vec3("position", pos);
+            float r = p.get_float("radius");
+            sh.type = kShapeSphere;
+            sh.p0 = mk(pos[0], pos[1], pos[2]);
+            sh.r = r;
+            sh.area = 4.0f * kPi * r * r;
+            if (pg_accel_add_sphere(a, pos, r, &first))
+                throw std::runtime_error(pg_last_error());
+        } else if (t == "Rect") {
+            p.get_vec3("position", pos);
+            p.get_vec3("ex", e0);
+            p.get_vec3("ey", e1);
+            f3 X = mk(e0[0], e0[1], e0[2]), Y = mk(e1[0], e1[1], e1[2]);
+            sh.type = kShapeRect;
+            sh.p0 = mk(pos[0], pos[1], pos[2]);
+            sh.e0 = normalize(X);
+            sh.e1 = normalize(Y);
+            sh.n = normalize(cross(X, Y));
+            sh.lx = length(X);
+            sh.ly = length(Y);
+            sh.area = sh.lx * sh.ly;
+            if (pg_accel_add_rect(a, pos, e0, e1, &first))
+                throw std::runtime_error(pg_last_error());
+        } else if (t == "Plane") {
+            p.get_vec3("position", pos);
+            p.get_vec3("normal", e0);
+            sh.type = kShapePlane;
+            sh.p0 = mk(pos[0], pos[1], pos[2]);
+            sh.n = normalize(mk(e0[0], e0[1], e0[2]));
+            coordinate_system(sh.n, sh.e0, sh.e1);
+            sh.area = kFloatMax;
+            if (pg_accel_add_plane(a, pos, e0, &first))
+                throw std::runtime_error(pg_last_error());
+        } else if (t == "Triangle" || t == "TriangleMesh") {
+            std::vector<float> v;
+            std::vector<uint32_t> idx;
+            if (t == "Triangle") {
+                float q[3];
+                for (const char* k : {"v0", "v1", "v2"}) {
+                    p.get_vec3(k, q);
+                    v.insert(v.end(), q, q + 3);
+                }
+                idx = {0, 1, 2};
+            } else {
+                std::string file = p.get_string("file");
+                if (!file.empty() && file[0] != '/')
+                    file = S.dir + "/" + file;
+                load_obj(file, v, idx);
+                float sc[3], of[3];
+                if (p.try_vec3("scale", sc))
+                    for (size_t k = 0; k < v.size(); k++)
+                        v[k] *= sc[k % 3];
+                if (p.try_vec3("position", of))
+                    for (size_t k = 0; k < v.size(); k++)
+                        v[k] += of[k % 3];
+            }
+            sh.type = kShapeMesh;
+            sh.tri_offset = (int32_t)(S.tri_verts.size() / 9);
+            const int64_t nt = (int64_t)idx.size() / 3;
+            for (int64_t k = 0; k < 3 * nt; k++)
+                for (int c = 0; c < 3; c++)
+                    S.tri_verts.push_back(v[3 * (size_t)idx[k] + c]);
+            if (pg_accel_add_mesh(a, v.data(), (int64_t)v.size() / 3, idx.data(), nt, &first))
+                throw std::runtime_error(pg_last_error());
+            if (t == "Triangle")
+                sh.area = length(cross(mk(v[3] - v[0], v[4] - v[1], v[5] - v[2]),
+                                       mk(v[6] - v[0], v[7] - v[1], v[8] - v[2]))) * 0.5f;
+        } else {
+            throw unsupported("shape type \"" + t + "\"");
+        }
+        sh.first_prim = first;
+        int64_t nprims = (t == "Triangle" || t == "TriangleMesh") ? (int64_t)S.tri_verts.size() / 9 - sh.tri_offset : 1;
+        for (int64_t k = 0; k < nprims; k++)
+            S.prim_shape.push_back(shape_index);
+        if (sh.material >= 0 && S.materials[sh.material].kind == kMatEmissive) {
+            if (t == "TriangleMesh" || t == "Plane")
+                throw unsupported("emissive " + t + " (pine cannot sample it)");
+            area_light_shapes.push_back(shape_index);
+        }
+        S.shapes.push_back(sh);
+    }
+    // Area lights follow the Light blocks (fileio.cpp:269-271).
+    for (int si : area_light_shapes) {
+        LightDev l{};
+        l.type = kLightArea;
+        l.shape = si;
+        S.lights.push_back(l);
+    }
+    for (const Params& p : root.all("Light"))
+        if (p.get_string("type") == "Environment")
+            throw unsupported("environment lights");
+
+    // PowerLightSampler (lightsampler.cpp:20-25): Power().y() per light.  An
+    // area light's Power evaluates its emission with a zero normal, which
+    // yields no emission in pine's build (observed: selection pdf 1 for one
+    // area light), so area lights count as zero power.
+    const int nl = (int)S.lights.size();
+    S.light_func.resize(nl);
+    for (int k = 0; k < nl; k++) {
+        const LightDev& l = S.lights[k];
+        S.light_func[k] = l.type == kLightPoint ? luminance(l.color * (kPi * 4.0f)) : 0.0f;
+    }
+    S.light_cdf.assign(nl + 1, 0.0f);
+    if (nl > 0) {
+        float dx = 1.0f / (float)nl;
+        for (int k = 1; k <= nl; k++)
+            S.light_cdf[k] = S.light_cdf[k - 1] + S.light_func[k - 1] * dx;
+        S.light_func_int = S.light_cdf[nl];
+        if (S.light_func_int != 0)
+            for (int k = 0; k <= nl; k++)
+                S.light_cdf[k] /= S.light_func_int;
+    }
+
+    // Camera (camera.cpp:8-12, camera.h:22-33) and film (film.cpp:8-28).
+    const Params& cp = root.sub("Camera");
+    std::string ctype = cp.get_string("type");
+    if (ctype != "ThinLen")
+        fprintf(stderr, "[Camera][Create]Unknown type \"%s\"\n", ctype.c_str());
+    if (cp.has_value("medium"))
+        throw unsupported("camera media");
+    const Params& fp = cp.sub("film");
+    int size[2];
+    const int defsize[2] = {720, 480};
+    fp.get_vec2i("size", size, defsize);
+    S.output = fp.get_string("outputFileName", "result.png");
+    S.tone_map = fp.get_bool("applyToneMapping", true);
+    const Params& flp = fp.sub("filter");
+    std::string ftype = flp.get_string("type", "LanczosSinc");
+    float radius[2];
+    const float defr[2] = {0.5f, 0.5f};
+    flp.get_vec2("radius", radius, defr);
+    float table[256];
+    for (int y = 0, o = 0; y < 16; y++)
+        for (int x = 0; x < 16; x++) {
+            float px = ((float)x + 0.5f) / 16.0f * radius[0];
+            float py = ((float)y + 0.5f) / 16.0f * radius[1];
+            float w;
+            if (ftype == "Box") {
+                w = 1.0f;
+            } else if (ftype == "LanczosSinc") {
+                // LanczosSincFilter (filter.h:72-96)
+                float tau = flp.get_float("tau", 3.0f);
+                auto sinc = [](float x) {
+                    x = fabsp(x);
+                    if (x < 1e-5f)
+                        return 1.0f;
+                    return pstd::sin(kPi * x) / (kPi * x);
+                };
+                auto wsinc = [&](float x, float r) {
+                    x = fabsp(x);
+                    if (x > r)
+                        return 0.0f;
+                    return sinc(x) * sinc(x / tau);
+                };
+                w = wsinc(px, radius[0]) * wsinc(py, radius[1]);
+            } else {
+                throw unsupported("film filter \"" + ftype + "\"");
+            }
+            table[o++] = w;
+        }
+    float from[3], to[3];
+    cp.get_vec3("from", from);
+    cp.get_vec3("to", to);
+    float fov = cp.get_float("fov");
+    float lensRadius = cp.get_float("lensRadius", 0.0f);
+    float focus = cp.get_float("focusDistance", 1.0f);
+    // LookAt (vecmath.h:1212-1221)
+    f3 F = mk(from[0], from[1], from[2]), T = mk(to[0], to[1], to[2]), up = mk(0, 1, 0);
+    f3 z = normalize(T - F);
+    if (fabsp(dot(z, up)) > 0.999f)
+        z = normalize(z + mk(0.0f, 0.0f, 1e-5f));
+    f3 x = normalize(cross(up, z));
+    f3 y = cross(z, x);
+    CameraDev cam;
+    cam.cx = x;
+    cam.cy = y;
+    cam.cz = z;
+    cam.cw = F;
+    float tf = pstd::tan(fov / 2.0f);
+    float aspect = (float)size[0] / (float)size[1];
+    cam.fov2d[0] = tf * aspect;
+    cam.fov2d[1] = tf;
+    cam.lensRadius = lensRadius;
+    cam.focalDistance = focus;
+
+    SceneDev& d = S.dev;
+    d.cam = cam;
+    memcpy(d.filter_table, table, sizeof(table));
+    d.filter_radius[0] = radius[0];
+    d.filter_radius[1] = radius[1];
+    d.width = size[0];
+    d.height = size[1];
+    d.max_depth = S.max_depth;
+    d.n_lights = nl;
+    d.light_uniform = S.light_uniform ? 1 : 0;
+    d.light_func_int = S.light_func_int;
+}
+
+// Halton tables: primes, prefix sums and RNG(0)-shuffled digit permutations
+// (lowdiscrepancy.cpp:5-21, rng.h:39-105).
+void halton_tables(std::vector<int>& primes, std::vector<int>& sums, std::vector<uint16_t>& perms) {
+    for (int n = 2; (int)primes.size() < 1000; n++) {
+        bool prime = true;
+        for (int p : primes) {
+            if (p * p > n)
+                break;
+            if (n % p == 0) {
+                prime = false;
+                break;
+            }
+        }
+        if (prime)
+            primes.push_back(n);
+    }
+    int acc = 0;
+    for (int p : primes) {
+        sums.push_back(acc);
+        acc += p;
+    }
+    perms.resize(acc);
+    uint64_t seed = 0, st[2];
+    auto splitmix = [&](uint64_t& s) {
+        uint64_t r = s += 0x9E3779B97f4A7C15ULL;
+        r = (r ^ (r >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        r = (r ^ (r >> 27)) * 0x94D049BB133111EBULL;
+        return r ^ (r >> 31);
+    };
+    st[0] = splitmix(seed);
+    st[1] = splitmix(seed);
+    auto next32 = [&]() {
+        const uint64_t s0 = st[0];
+        uint64_t s1 = st[1];
+        const uint64_t result = s0 + s1;
+        s1 ^= s0;
+        st[0] = ((s0 << 24) | (s0 >> 40)) ^ s1 ^ (s1 << 16);
+        st[1] = (s1 << 37) | (s1 >> 27);
+        return (uint32_t)(result ^ (result >> 32));
+    };
+    uint16_t* p = perms.data();
+    for (int i = 0; i < 1000; i++) {
+        for (int j = 0; j < primes[i]; j++)
+            p[j] = (uint16_t)j;
+        for (int k = 0; k < primes[i]; k++) {
+            int other = k + (int)(next32() % (uint32_t)(primes[i] - k));
+            std::swap(p[k], p[other]);
+        }
+        p += primes[i];
+    }
+}
+
+void halton_setup(HaltonDev& h, int w, int hgt) {
+    const int film[2] = {w, hgt};
+    for (int i = 0; i < 2; i++) {
+        int base = i == 0 ? 2 : 3, scale = 1, exp = 0;
+        while (scale < std::min(film[i], 128)) {
+            scale *= base;
+            ++exp;
+        }
+        h.baseScales[i] = scale;
+        h.baseExponents[i] = exp;
+    }
+    // multiplicativeInverse (sampler.cpp:9-30) by the extended Euclid algorithm.
+    auto mulinv = [](int64_t a, int64_t n) {
+        int64_t r0 = n, r1 = a % n, t0 = 0, t1 = 1;
+        while (r1) {
+            int64_t q = r0 / r1;
+            int64_t r2 = r0 - q * r1, t2 = t0 - q * t1;
+            r0 = r1, r1 = r2, t0 = t1, t1 = t2;
+        }
+        return (int)(((t0 % n) + n) % n);
+    };
+    h.multInverse[0] = mulinv(h.baseScales[0], h.baseScales[1]);
+    h.multInverse[1] = mulinv(h.baseScales[1], h.baseScales[0]);
+    h.stride = h.baseScales[0] * h.baseScales[1];
+}
+
+template <typename T>
+int upload(pg_scene& S, const std::vector<T>& v, const T** out) {
+    if (v.empty()) {
+        *out = nullptr;
+        return PG_OK;
+    }
+    void* p = nullptr;
+    PG_HIP(hipMalloc(&p, v.size() * sizeof(T)));
+    S.allocations.push_back(p);
+    PG_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    *out = (const T*)p;
+    return PG_OK;
+}
+
+template <typename T>
+int dalloc(pg_scene& S, int64_t n, T** out) {
+    void* p = nullptr;
+    PG_HIP(hipMalloc(&p, std::max<int64_t>(n, 1) * sizeof(T)));
+    S.allocations.push_back(p);
+    *out = (T*)p;
+    return PG_OK;
+}
+
+int finish_scene(pg_scene& S, const pg_build_options* opts) {
+    if (int rc = pg_accel_build(S.accel, opts))
+        return rc;
+    std::vector<int> primes, sums;
+    std::vector<uint16_t> perms;
+    halton_tables(primes, sums, perms);
+    halton_setup(S.dev.halton, S.dev.width, S.dev.height);
+    int rc;
+    if ((rc = upload(S, perms, &S.dev.halton.perms)) || (rc = upload(S, primes, &S.dev.halton.primes)) ||
+        (rc = upload(S, sums, &S.dev.halton.primeSums)) || (rc = upload(S, S.prim_shape, &S.dev.prim_shape)) ||
+        (rc = upload(S, S.tri_verts, &S.dev.tri_verts)) || (rc = upload(S, S.shapes, &S.dev.shapes)) ||
+        (rc = upload(S, S.materials, &S.dev.materials)) || (rc = upload(S, S.lights, &S.dev.lights)) ||
+        (rc = upload(S, S.light_func, &S.dev.light_func)) || (rc = upload(S, S.light_cdf, &S.dev.light_cdf)))
+        return rc;
+    if ((rc = dalloc(S, (int64_t)S.dev.width * S.dev.height * 4, &S.d_film)))
+        return rc;
+    PG_HIP(hipMemset(S.d_film, 0, (size_t)S.dev.width * S.dev.height * 4 * sizeof(float)));
+    return PG_OK;
+}
+
+int ensure_paths(pg_scene& S, int64_t n) {
+    if (n <= S.path_capacity)
+        return PG_OK;
+    PathsDev& P = S.paths;
+    int rc;
+    if ((rc = dalloc(S, 3 * n, &P.L)) || (rc = dalloc(S, 3 * n, &P.beta)) ||
+        (rc = dalloc(S, n, &P.bsdf_pdf)) || (rc = dalloc(S, n, &P.sidx)) || (rc = dalloc(S, n, &P.sdim)) ||
+        (rc = dalloc(S, n, &P.depth)) || (rc = dalloc(S, 2 * n, &P.pfilm)) ||
+        (rc = dalloc(S, n, &P.queue_in)) || (rc = dalloc(S, n, &P.queue_out)) ||
+        (rc = dalloc(S, n, &P.rays_in)) || (rc = dalloc(S, n, &P.rays_out)) ||
+        (rc = dalloc(S, n, &P.hits)) || (rc = dalloc(S, n, &P.uvs)) || (rc = dalloc(S, 2, &P.counters)) ||
+        (rc = dalloc(S, n, &P.shadow_rays)) || (rc = dalloc(S, n, &P.shadow_path)) ||
+        (rc = dalloc(S, 3 * n, &P.shadow_contrib)) || (rc = dalloc(S, n, &P.occluded)))
+        return rc;
+    S.path_capacity = n;
+    return PG_OK;
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------------
+// C ABI
+
+static int load_common(pg_scene* S, const std::string& text, pg_scene** out,
+                       const pg_build_options* opts) {
+    try {
+        S->root = parse_scene_text(text);
+        build_scene(*S);
+    } catch (const std::exception& e) {
+        delete S;
+        return pg_fail(PG_E_PARSE, e.what());
+    }
+    if (int rc = finish_scene(*S, opts)) {
+        delete S;
+        return rc;
+    }
+    *out = S;
+    return PG_OK;
+}
+
+extern "C" int pg_scene_load_text(const char* text, const char* dir, const pg_build_options* opts,
+                                  pg_scene** out) {
+    if (!text || !out)
+        return pg_fail(PG_E_ARG, "pg_scene_load_text: null argument");
+    pg_scene* S = new pg_scene();
+    S->dir = dir ? dir : ".";
+    return load_common(S, text, out, opts);
+}
+
+extern "C" int pg_scene_load(const char* path, const pg_build_options* opts, pg_scene** out) {
+    if (!path || !out)
+        return pg_fail(PG_E_ARG, "pg_scene_load: null argument");
+    std::string text;
+    try {
+        text = read_file(path);
+    } catch (const std::exception& e) {
+        return pg_fail(PG_E_IO, e.what());
+    }
+    pg_scene* S = new pg_scene();
+    // GetFileDirectory (fileio.cpp:256): everything up to the last '/'.
+    std::string p(path);
+    size_t slash = p.find_last_of('/');
+    S->dir = slash == std::string::npos ? "." : p.substr(0, slash);
+    return load_common(S, text, out, opts);
+}
+
+extern "C" int pg_scene_get_info(const pg_scene* S, pg_scene_info* info) {
+    if (!S || !info)
+        return pg_fail(PG_E_ARG, "pg_scene_get_info: null argument");
+    info->width = S->dev.width;
+    info->height = S->dev.height;
+    info->spp = S->spp;
+    info->max_depth = S->max_depth;
+    info->n_shapes = (int32_t)S->shapes.size();
+    info->n_lights = (int32_t)S->lights.size();
+    info->n_prims = (int64_t)S->prim_shape.size();
+    info->last_render_ms = S->last_render_ms;
+    info->last_rays = S->last_rays;
+    return PG_OK;
+}
+
+extern "C" int pg_render(pg_scene* S, const pg_render_options* opts, void* stream) {
+    if (!S)
+        return pg_fail(PG_E_ARG, "pg_render: null scene");
+    const int spp = opts && opts->spp > 0 ? opts->spp : S->spp;
+    const int64_t npix = (int64_t)S->dev.width * S->dev.height;
+    int spb = opts && opts->samples_per_batch > 0 ? opts->samples_per_batch : 0;
+    if (spb <= 0) {
+        // About 4M paths per batch: enough lanes for 256 CUs, bounded state.
+        spb = (int)std::max<int64_t>(1, std::min<int64_t>(spp, (4 << 20) / std::max<int64_t>(npix, 1)));
+    }
+    spb = std::min(spb, spp);
+    const int64_t n_paths = npix * spb;
+    if (n_paths > (int64_t)INT32_MAX)
+        return pg_fail(PG_E_ARG, "pg_render: batch too large");
+    if (int rc = ensure_paths(*S, n_paths))
+        return rc;
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    PG_HIP(hipEventCreate(&e0));
+    PG_HIP(hipEventCreate(&e1));
+    PG_HIP(hipEventRecord(e0, st));
+    PG_HIP(hipMemsetAsync(S->d_film, 0, (size_t)npix * 4 * sizeof(float), st));
+    PathsDev P = S->paths;
+    int64_t rays = 0;
+    const int B = 256;
+    for (int s0 = 0; s0 < spp; s0 += spb) {
+        const int nb = std::min(spb, spp - s0);
+        const int32_t np = (int32_t)(npix * nb);
+        hipLaunchKernelGGL(k_raygen, dim3((np + B - 1) / B), dim3(B), 0, st, S->dev, P, np, nb, s0);
+        int32_t active = np;
+        PathsDev Q = P;
+        for (int bounce = 0; active > 0; bounce++) {
+            PG_HIP(hipMemsetAsync(Q.counters, 0, 2 * sizeof(int32_t), st));
+            if (int rc = pg_trace_closest(S->accel, Q.rays_in, active, Q.hits, Q.uvs, st))
+                return rc;
+            hipLaunchKernelGGL(k_shade, dim3((active + B - 1) / B), dim3(B), 0, st, S->dev, Q, active);
+            int32_t counts[2];
+            PG_HIP(hipMemcpyAsync(counts, Q.counters, sizeof(counts), hipMemcpyDeviceToHost, st));
+            PG_HIP(hipStreamSynchronize(st));
+            rays += active + counts[1];
+            if (counts[1] > 0) {
+                if (int rc = pg_trace_occluded(S->accel, Q.shadow_rays, counts[1], Q.occluded, st))
+                    return rc;
+                hipLaunchKernelGGL(k_resolve_shadow, dim3((counts[1] + B - 1) / B), dim3(B), 0, st,
+                                   Q, counts[1]);
+            }
+            std::swap(Q.queue_in, Q.queue_out);
+            std::swap(Q.rays_in, Q.rays_out);
+            active = counts[0];
+        }
+        hipLaunchKernelGGL(k_film, dim3((npix + B - 1) / B), dim3(B), 0, st, S->dev, P, (int32_t)npix,
+                           nb, S->d_film);
+    }
+    PG_HIP(hipGetLastError());
+    PG_HIP(hipEventRecord(e1, st));
+    PG_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    PG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    S->last_render_ms = ms;
+    S->last_rays = rays;
+    return PG_OK;
+}
+
+extern "C" int pg_scene_film(const pg_scene* S, float* h_rgbw) {
+    if (!S || !h_rgbw)
+        return pg_fail(PG_E_ARG, "pg_scene_film: null argument");
+    PG_HIP(hipMemcpy(h_rgbw, S->d_film, (size_t)S->dev.width * S->dev.height * 4 * sizeof(float),
+                     hipMemcpyDeviceToHost));
+    return PG_OK;
+}
+
+extern "C" int pg_scene_output_name(const pg_scene* S, char* buf, int32_t len) {
+    if (!S || !buf || len <= 0)
+        return pg_fail(PG_E_ARG, "pg_scene_output_name: bad argument");
+    snprintf(buf, (size_t)len, "%s", S->output.c_str());
+    return PG_OK;
+}
+
+extern "C" int pg_scene_release(pg_scene* S) {
+    delete S;
+    return PG_OK;
+}

@@ -152,3 +152,104 @@ def spheres_scene_text(seed=0, width=512, height=512, spp=64, max_depth=8, n_sph
         out += ["Shape: Sphere{", f"    position: {_v3(s['c'])}", f"    radius: {_fmt(s['r'])}",
                 f"    material: m{k}", "}"]
     return "\n".join(out) + "\n"
+
+
+def write_obj(path, vertices, indices):
+    """Writes a mesh as OBJ text with plain decimals (pine's stof has no exponent)."""
+    with open(path, "w") as f:
+        for v in vertices:
+            f.write("v " + " ".join(_fmt(x) for x in v) + "\n")
+        for t in indices:
+            f.write("f %d %d %d\n" % (t[0] + 1, t[1] + 1, t[2] + 1))
+
+
+def mixed_scene_text(obj_file, width=96, height=72, spp=64, max_depth=6):
+    """A small scene exercising what the spheres scene does not: point and
+    directional lights next to an area light (power-based light selection),
+    two-layer materials, a rough dielectric, an OBJ triangle mesh, a plane and
+    a scaled/positioned mesh, a Box filter and a thin-lens camera."""
+    return f"""Integrator: Path{{
+    maxDepth: {max_depth}
+    sampler: Halton{{
+        samplesPerPixel: {spp}
+    }}
+}}
+Camera: ThinLen{{
+    film{{
+        outputFileName: mixed.bmp
+        size: {width} {height}
+        filter: Box{{
+            radius: 0.5
+        }}
+    }}
+    from: 0.3 2.2 6.5
+    to: 0 0.6 0
+    fov: 0.8
+    lensRadius: 0.02
+    focusDistance: 6
+}}
+Material floor: Layered{{
+    layer0: Diffuse{{
+        albedo: 0.6 0.55 0.5
+    }}
+}}
+Material coated: Layered{{
+    layer0: Diffuse{{
+        albedo: 0.2 0.4 0.8
+    }}
+    layer1: Conductor{{
+        albedo: 0.9
+        roughness: 0.3
+    }}
+}}
+Material frosted: Layered{{
+    layer0: Dielectric{{
+        eta: 1.33
+        roughness: 0.25
+    }}
+}}
+Material gold: Layered{{
+    layer0: Conductor{{
+        albedo: 0.95 0.7 0.3
+        roughness: 0.15
+    }}
+}}
+Material lamp: Emissive{{
+    color: 4 3.5 3
+}}
+Light: Point{{
+    position: -2 3 2
+    color: 6 6 8
+}}
+Light: Directional{{
+    direction: 0.3 1 0.5
+    color: 0.8 0.8 0.7
+}}
+Shape: Plane{{
+    position: 0 0 0
+    normal: 0 1 0
+    material: floor
+}}
+Shape: Sphere{{
+    position: -1.2 0.7 0
+    radius: 0.7
+    material: coated
+}}
+Shape: Sphere{{
+    position: 1.2 0.6 0.4
+    radius: 0.6
+    material: frosted
+}}
+Shape: TriangleMesh{{
+    file: {obj_file}
+    scale: 0.5 0.5 0.5
+    position: 0 0.5 -1.4
+    material: gold
+}}
+Shape: Rect{{
+    position: 0 3 0
+    ex: 1.2 0 0
+    ey: 0 0 1.2
+    material: lamp
+}}
+"""

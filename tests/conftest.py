@@ -39,8 +39,9 @@ def oracle():
 
 
 @pytest.fixture(scope="session")
-def cuda():
-    import torch
-    if not torch.cuda.is_available():
+def cuda(pg):
+    """Skips unless a HIP device is visible to the library's runtime."""
+    from pine_amd import hip
+    if hip.device_count() < 1:
         pytest.skip("no GPU")
-    return torch.device("cuda:0")
+    return hip

@@ -15,6 +15,10 @@ Cases
               12,288 rays
   pstd        pine's libm on fixed grids
   halton      pine's Halton sampler values for a spread of pixels
+  render_spheres  the seeded spheres scene (64 spheres, area light) at
+              96x96, 256 spp, maxDepth 8, rendered by pine's PathIntegrator
+  render_mixed    point + directional + area lights, layered materials, a
+              rough dielectric, an OBJ mesh, thin lens, Box filter (96x72, 256 spp)
 
 Trace fixtures store ray *indices* into the deterministic ray generator plus a
 checksum of the selected rays, and pine's answers: hit shape, prim within the
@@ -183,6 +187,45 @@ def case_halton():
     print("halton:", ", ".join(out), flush=True)
 
 
+def run_render(scene_text, directory):
+    path = os.path.join(directory, "scene.txt")
+    with open(path, "w") as f:
+        f.write(scene_text)
+    out = os.path.join(directory, "film.bin")
+    p = subprocess.run([HARNESS, "render", "./scene.txt", out, "8"], capture_output=True,
+                       text=True, cwd=directory)
+    if p.returncode != 0:
+        raise RuntimeError(f"harness render failed: {p.stderr[-2000:]}")
+    line = [l for l in p.stdout.splitlines() if l.startswith("{")][-1]
+    a = np.fromfile(out, dtype=np.float32)
+    w, h = a[:2].view(np.int32)
+    return a[2:].reshape(h, w, 4), line
+
+
+def case_render_spheres():
+    text = W.spheres_scene_text(seed=0, width=96, height=96, spp=256, max_depth=8)
+    with tempfile.TemporaryDirectory() as d:
+        film, line = run_render(text, d)
+    np.savez_compressed(os.path.join(OUT, "render_spheres.npz"), scene=np.bytes_(text), film=film,
+                        harness=np.bytes_(line))
+    print("render_spheres:", line, flush=True)
+
+
+def mixed_mesh():
+    return W.make_mesh(W.MeshSpec(freq=5, amp=0.15, seed=9, max_triangles=0))
+
+
+def case_render_mixed():
+    v, i = mixed_mesh()
+    with tempfile.TemporaryDirectory() as d:
+        W.write_obj(os.path.join(d, "blob.obj"), v, i)
+        text = W.mixed_scene_text("blob.obj", spp=256)
+        film, line = run_render(text, d)
+    np.savez_compressed(os.path.join(OUT, "render_mixed.npz"), scene=np.bytes_(text), film=film,
+                        harness=np.bytes_(line))
+    print("render_mixed:", line, flush=True)
+
+
 CASES = {
     "trace_5k": lambda: case_trace("trace_5k", W.ICOSPHERE_5K, W.RAYS_SMALL),
     "trace_82k": lambda: case_trace("trace_82k", W.ICOSPHERE_82K, W.RAYS_SMALL),
@@ -191,6 +234,8 @@ CASES = {
     "shapes": case_shapes,
     "pstd": case_pstd,
     "halton": case_halton,
+    "render_spheres": case_render_spheres,
+    "render_mixed": case_render_mixed,
 }
 
 if __name__ == "__main__":

@@ -24,13 +24,12 @@ def _accel(pg, v, i, width, block=128):
 
 
 def _trace(a, rays, cuda):
-    import torch
-    d = torch.from_numpy(np.ascontiguousarray(rays, np.float32)).to(cuda)
+    d = cuda.DeviceArray.from_numpy(np.ascontiguousarray(rays, np.float32))
     h = a.trace_closest(d)
     occ = a.trace_any(d)
-    torch.cuda.synchronize()
-    h = h.cpu().numpy()
-    return h[:, 0].copy(), h[:, 1].view(np.float32).copy(), occ.cpu().numpy()
+    cuda.synchronize()
+    h = h.numpy()
+    return h[:, 0].copy(), h[:, 1].view(np.float32).copy(), occ.numpy()
 
 
 def _mesh_rays(fx):
@@ -113,14 +112,13 @@ def test_analytic_shapes_match_pine(pg, cuda, width):
 
 
 def test_uv_output_matches_pine(pg, cuda):
-    import torch
     fx = golden("trace_5k")
     v, i, rays = _mesh_rays(fx)
     a = _accel(pg, v, i, 2)
-    d = torch.from_numpy(rays).to(cuda)
-    uv = torch.empty((len(rays), 2), dtype=torch.float32, device=cuda)
-    h = a.trace_closest(d, d_uv=uv).cpu().numpy()
-    uv = uv.cpu().numpy()
+    d = cuda.DeviceArray.from_numpy(rays)
+    uv = cuda.DeviceArray((len(rays), 2), np.float32)
+    h = a.trace_closest(d, d_uv=uv).numpy()
+    uv = uv.numpy()
     hit = (fx["prim"] >= 0) & (h[:, 0] == fx["prim"])
     assert np.array_equal(uv[hit, 0].view(np.int32), fx["u"][hit].view(np.int32))
     assert np.array_equal(uv[hit, 1].view(np.int32), fx["v"][hit].view(np.int32))
@@ -172,7 +170,6 @@ def test_edge_cases_vs_oracle(pg, oracle, cuda):
 
 
 def test_single_triangle_and_empty_batch(pg, oracle, cuda):
-    import torch
     v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
     i = np.array([[0, 1, 2]], np.uint32)
     rays = np.array([[0.2, 0.2, -1, 0, 0, 0, 1, 3.4e38], [2, 2, -1, 0, 0, 0, 1, 3.4e38]],
@@ -182,7 +179,7 @@ def test_single_triangle_and_empty_batch(pg, oracle, cuda):
         prim, t, occ = _trace(a, rays, cuda)
         assert prim.tolist() == [0, -1] and t[0] == np.float32(1.0)
         assert occ.tolist() == [1, 0]
-        empty = torch.empty((0, 8), dtype=torch.float32, device=cuda)
+        empty = cuda.DeviceArray((0, 8), np.float32)
         assert a.trace_closest(empty).shape == (0, 2)
 
 
@@ -190,15 +187,14 @@ def test_full_benchmark_stream_is_tree_independent(pg, oracle, cuda):
     """All 16,777,216 benchmark rays: the closest hit does not depend on the
     hierarchy, so the binary and 4-wide trees must agree bit for bit, and a
     sample must agree with the CPU port of pine's traversal."""
-    import torch
     from pine_amd import workloads as W
     v, i = W.make_mesh(W.ICOSPHERE_1M)
     rays = W.make_rays(W.RAYS_16M, v, i)
-    d = torch.from_numpy(rays).to(cuda)
+    d = cuda.DeviceArray.from_numpy(rays)
     outs = []
     for width in WIDTHS:
         a = _accel(pg, v, i, width)
-        outs.append(a.trace_closest(d).cpu().numpy())
+        outs.append(a.trace_closest(d).numpy())
         a.release()
     assert np.array_equal(outs[0], outs[1])
     hitfrac = (outs[0][:, 0] >= 0).mean()

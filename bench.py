@@ -42,7 +42,12 @@ def main():
     ap.add_argument("--width", type=int, default=2)
     ap.add_argument("--block", type=int, default=128)
     ap.add_argument("--leaf", type=int, default=4)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
+    ap.add_argument("--no-pt", action="store_true", help="skip the path-tracing measurement")
+    ap.add_argument("--pt-spp", type=int, default=64)
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run one device-to-device copy of the ray buffer (512 MiB), the "
+                         "known-size kernel that scales rocprof's FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -107,6 +112,11 @@ def main():
     mrays = n / (ms_per_step * 1e-3) * 1e-6
     hits = d_hits.numpy()
     hit_frac = float((hits[:, 0] >= 0).mean())
+    if args.copy_calib:
+        dst = hip.DeviceArray(d_rays.shape, np.float32)
+        hip.check(hip.rt().hipMemcpy(dst.ptr, d_rays.ptr, d_rays.nbytes, hip.hipMemcpyDeviceToDevice))
+        hip.synchronize()
+        dst.free()
     log(f"[bench] {ms_per_step:.3f} ms/step, kernel {kern_ms:.3f} ms, {mrays:.1f} Mrays/s, "
         f"hit fraction {hit_frac:.3f}")
 
@@ -128,16 +138,18 @@ def main():
         tb = time.time()
         cb = oracle.CpuBVH(tri)
         build_s = time.time() - tb
-        sel = np.arange(0, n, 4)  # every 4th ray of the stream
+        sel = np.arange(n)  # the whole stream
         prim, t, secs = cb.trace(rays[sel], threads=threads)
         cpu = {"value": round(len(sel) / secs * 1e-6, 3), "unit": "Mrays/s", "cores": threads,
                "kind": "port",
-               "sample": f"{len(sel)} rays (every 4th of the stream), same mesh; "
-                         f"pine-order BVH2 traversal, tree built in {build_s:.1f}s (not timed)"}
+               "sample": f"all {len(sel)} rays of the stream, same mesh; pine-order BVH2 "
+                         f"traversal, tree built in {build_s:.1f}s (not timed)"}
         # Sanity: same closest distances as the GPU on the sample.
         same_t = float((t.view(np.int32) == hits[sel, 1]).mean())
         cpu["agrees_with_gpu_t"] = round(same_t, 6)
         log(f"[bench] cpu baseline {cpu['value']} Mrays/s on {threads} threads ({secs:.2f}s)")
+
+    pt = None if args.no_pt else path_tracing(args, pg, W)
 
     out = {
         "metric": "Mrays/s at 1 GPU (closest hit)", "value": round(mrays, 2), "unit": "Mrays/s",
@@ -150,8 +162,48 @@ def main():
                    "build_ms": round(st["build_ms"], 1)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "path_tracing": pt,
     }
     print(json.dumps(out), flush=True)
+
+
+def path_tracing(args, pg, W):
+    """Secondary measurement: samples/s of pine's Path integrator on the seeded
+    spheres scene (64 spheres, one area light, 512x512, maxDepth 8), beside pine
+    itself (oracle/_ref/ref_harness render mode) on the host cores."""
+    import subprocess
+    import tempfile
+    text = W.spheres_scene_text(seed=0, width=512, height=512, spp=args.pt_spp, max_depth=8)
+    scene = pg.Scene(text=text)
+    scene.render(spp=4)  # warm-up: allocations and first launches
+    inf = scene.render()
+    ms = inf["last_render_ms"]
+    samples = 512 * 512 * args.pt_spp
+    res = {"workload": f"spheres scene 512x512, {args.pt_spp} spp, maxDepth 8",
+           "samples_per_s": round(samples / (ms * 1e-3), 1), "render_ms": round(ms, 2),
+           "mrays_per_s": round(inf["last_rays"] / (ms * 1e-3) * 1e-6, 2)}
+    scene.release()
+    log(f"[bench] path tracing {res['samples_per_s'] / 1e6:.2f} M samples/s "
+        f"({res['mrays_per_s']} Mrays/s incl. shadow rays)")
+    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if not args.no_cpu and os.path.exists(harness):
+        threads = min(os.cpu_count() or 1, 16)
+        with tempfile.TemporaryDirectory() as d:
+            with open(os.path.join(d, "scene.txt"), "w") as f:
+                f.write(text)
+            p = subprocess.run([harness, "render", "./scene.txt", os.path.join(d, "film.bin"),
+                                str(threads)], capture_output=True, text=True, cwd=d, timeout=300)
+        line = [l for l in p.stdout.splitlines() if l.startswith("{")]
+        if p.returncode == 0 and line:
+            ref = json.loads(line[-1])
+            cps = ref["msamples_per_s"] * 1e6
+            res["cpu_baseline"] = {"value": round(cps, 1), "unit": "samples/s", "cores": threads,
+                                   "kind": "reference",
+                                   "sample": "same scene file rendered by pine's PathIntegrator "
+                                             f"({ref['render_s']:.1f}s)"}
+            res["speedup_vs_cpu"] = round(res["samples_per_s"] / cps, 1)
+            log(f"[bench] pine on {threads} cores: {cps / 1e6:.3f} M samples/s")
+    return res
 
 
 if __name__ == "__main__":

@@ -482,7 +482,10 @@ int ModeRender(int argc, char** argv) {
                     vec2 pFilm = (p + sampler.Get2D()) / film.Size();
                     Ray ray = scene->camera.GenRay(pFilm, sampler.Get2D());
                     Spectrum L = ri->Li(ray, sampler);
-                    // Film::AddSample (film.h:26-45)
+                    // Film::AddSample (film.h:26-45).  The barrier keeps g++ from
+                    // folding (x / size) * size back to x here; pine's own Compute
+                    // keeps the divide and the multiply (integrator.o).
+                    asm volatile("" : "+m"(pFilm));
                     vec2 pf = pFilm * size;
                     pf -= vec2(0.5f);
                     vec2i p0 = Ceil(pf - filter.Radius());

@@ -108,6 +108,11 @@ struct PathsDev {
     int32_t* shadow_path;
     float* shadow_contrib;  // 3 per shadow ray
     int32_t* occluded;
+    // Film contributions a sample makes to a neighbouring pixel (a sample on a
+    // pixel border): (pixel, w*r, w*g, w*b, w), applied after the main pass.
+    int32_t* spill_count;
+    float* spill;  // 5 floats per entry (pixel index stored as bits)
+    int32_t spill_capacity;
 };
 
 }  // namespace pg
@@ -466,11 +471,16 @@ __global__ void k_film(SceneDev s, PathsDev P, int32_t n_pixels, int32_t spb, fl
                     acc[2] += L2 * w;
                     acc[3] += w;
                 } else {
-                    // A sample exactly on a pixel border also feeds the neighbour.
-                    atomicAdd(&film[4 * pix + 0], L0 * w);
-                    atomicAdd(&film[4 * pix + 1], L1 * w);
-                    atomicAdd(&film[4 * pix + 2], L2 * w);
-                    atomicAdd(&film[4 * pix + 3], w);
+                    // A sample exactly on a pixel border also feeds the neighbour;
+                    // the neighbour's own pass may be running, so defer it.
+                    int slot = atomicAdd(P.spill_count, 1);
+                    if (slot < P.spill_capacity) {
+                        P.spill[5 * slot + 0] = __int_as_float((int)pix);
+                        P.spill[5 * slot + 1] = L0 * w;
+                        P.spill[5 * slot + 2] = L1 * w;
+                        P.spill[5 * slot + 3] = L2 * w;
+                        P.spill[5 * slot + 4] = w;
+                    }
                 }
             }
     }
@@ -478,6 +488,17 @@ __global__ void k_film(SceneDev s, PathsDev P, int32_t n_pixels, int32_t spb, fl
     film[4 * own + 1] = acc[1];
     film[4 * own + 2] = acc[2];
     film[4 * own + 3] = acc[3];
+}
+
+__global__ void k_film_spill(PathsDev P, float* film) {
+    const int32_t n = min(*P.spill_count, P.spill_capacity);
+    for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const size_t pix = (size_t)__float_as_int(P.spill[5 * k]);
+        atomicAdd(&film[4 * pix + 0], P.spill[5 * k + 1]);
+        atomicAdd(&film[4 * pix + 1], P.spill[5 * k + 2]);
+        atomicAdd(&film[4 * pix + 2], P.spill[5 * k + 3]);
+        atomicAdd(&film[4 * pix + 3], P.spill[5 * k + 4]);
+    }
 }
 
 }  // namespace
@@ -1051,7 +1072,11 @@ int ensure_paths(pg_scene& S, int64_t n) {
         (rc = dalloc(S, n, &P.rays_in)) || (rc = dalloc(S, n, &P.rays_out)) ||
         (rc = dalloc(S, n, &P.hits)) || (rc = dalloc(S, n, &P.uvs)) || (rc = dalloc(S, 2, &P.counters)) ||
         (rc = dalloc(S, n, &P.shadow_rays)) || (rc = dalloc(S, n, &P.shadow_path)) ||
-        (rc = dalloc(S, 3 * n, &P.shadow_contrib)) || (rc = dalloc(S, n, &P.occluded)))
+        (rc = dalloc(S, 3 * n, &P.shadow_contrib)) || (rc = dalloc(S, n, &P.occluded)) ||
+        (rc = dalloc(S, 1, &P.spill_count)))
+        return rc;
+    P.spill_capacity = (int32_t)std::min<int64_t>(n, 1 << 20);
+    if ((rc = dalloc(S, 5 * (int64_t)P.spill_capacity, &P.spill)))
         return rc;
     S.path_capacity = n;
     return PG_OK;
@@ -1170,8 +1195,10 @@ extern "C" int pg_render(pg_scene* S, const pg_render_options* opts, void* strea
             std::swap(Q.rays_in, Q.rays_out);
             active = counts[0];
         }
+        PG_HIP(hipMemsetAsync(P.spill_count, 0, sizeof(int32_t), st));
         hipLaunchKernelGGL(k_film, dim3((npix + B - 1) / B), dim3(B), 0, st, S->dev, P, (int32_t)npix,
                            nb, S->d_film);
+        hipLaunchKernelGGL(k_film_spill, dim3(64), dim3(B), 0, st, P, S->d_film);
     }
     PG_HIP(hipGetLastError());
     PG_HIP(hipEventRecord(e1, st));

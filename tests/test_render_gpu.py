@@ -35,9 +35,12 @@ def render_text(pg, text, directory, spp=0, spb=0):
 
 def check_film(film, ref, tol):
     assert film.shape == ref.shape
-    np.testing.assert_allclose(film[..., 3], ref[..., 3], rtol=1e-5, atol=1e-6)
-    e = rel_mse(linear(film), linear(ref))
     assert np.isfinite(film).all()
+    e = rel_mse(linear(film), linear(ref))
+    wdiff = int((np.abs(film[..., 3] - ref[..., 3]) > 1e-5 * np.abs(ref[..., 3]) + 1e-6).sum())
+    exact = float((film[..., :3] == ref[..., :3]).all(-1).mean())
+    print(f"relMSE {e:.3e}, weight mismatches {wdiff}, bit-exact pixels {exact:.4f}")
+    np.testing.assert_allclose(film[..., 3], ref[..., 3], rtol=1e-5, atol=1e-6)
     assert e < tol, f"relMSE {e:.3e} >= {tol}"
     return e
 

@@ -41,6 +41,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--width", type=int, default=2)
     ap.add_argument("--block", type=int, default=128)
+    ap.add_argument("--flags", type=int, default=-1,
+                    help="pg_accel_set_launch flags (1 postpone leaves, 2 XCD remap); -1 = default")
     ap.add_argument("--leaf", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-pt", action="store_true", help="skip the path-tracing measurement")
@@ -71,8 +73,9 @@ def main():
     acc = pg.Accel()
     acc.add_mesh(v, i)
     acc.build(width=args.width, leaf_max=args.leaf)
-    acc.set_launch(args.block)
+    acc.set_launch(args.block, args.flags)
     st = acc.stats()
+    launch_flags = args.flags if args.flags >= 0 else pg.LAUNCH_DEFAULT
     log(f"[bench] accel built in {st['build_ms']:.0f} ms: {st['n_nodes']} nodes, depth {st['max_depth']}")
 
     d_rays = hip.DeviceArray.from_numpy(rays)
@@ -123,7 +126,8 @@ def main():
     achieved = algo_bytes / (kern_ms * 1e-3) / 1e9
     roofline = {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": measured_traffic(st["width"], args.leaf, args.block, launch_flags),
         "kernel": f"k_trace{st['width']}",
         "bytes_per_launch": {"compulsory": compulsory, "traversal": traversal},
         "compulsory_frac": round(compulsory / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
@@ -158,13 +162,30 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "bvh-traversal: icosphere 1,048,576 tris, 16,777,216 rays",
                    "triangles": int(len(i)), "rays": int(n), "node_width": st["width"],
-                   "leaf_max": args.leaf, "block": args.block, "hit_fraction": round(hit_frac, 4),
+                   "leaf_max": args.leaf, "block": args.block, "launch_flags": launch_flags, "hit_fraction": round(hit_frac, 4),
                    "build_ms": round(st["build_ms"], 1)},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "path_tracing": pt,
     }
     print(json.dumps(out), flush=True)
+
+
+def measured_traffic(width, leaf, block, flags):
+    """HBM bytes per launch of the trace kernel for this launch configuration,
+    as measured by rocprofv3 FETCH_SIZE + WRITE_SIZE (scaled by a known-size
+    copy) and recorded in the kernel's profile summary; None when that
+    configuration has no recorded measurement."""
+    import re
+    path = os.path.join(ROOT, "profiles", "k_trace2_icosphere1m_rays16m.md")
+    if not os.path.exists(path):
+        return None
+    key = f"width={width} leaf={leaf} block={block} flags={flags}"
+    for line in open(path):
+        m = re.match(r"\s*traffic_bytes_per_launch k_trace\S* (.*): (\d+)\s*$", line)
+        if m and m.group(1).strip() == key:
+            return int(m.group(2))
+    return None
 
 
 def path_tracing(args, pg, W):

@@ -135,9 +135,15 @@ int pg_accel_get_stats(const pg_accel* a, pg_accel_stats* out);
 /* Maps a global prim id back to (shape index in add order, prim within shape). */
 int pg_accel_prim_shape(const pg_accel* a, int32_t prim, int32_t* shape, int32_t* local);
 
-/* Kernel launch shape used by the trace calls: block = threads per workgroup
- * (64..1024, multiple of 64); 0 keeps the current value. */
-int pg_accel_set_launch(pg_accel* a, int32_t block);
+/* Kernel launch shape used by the trace calls (no counterpart in pine, whose
+ * BVH::Intersect is one ray on one thread):
+ *   block: threads per workgroup, 64, 128 or 256; 0 keeps the current value;
+ *   flags: PG_LAUNCH_* bits, -1 keeps the current value.
+ * Every setting returns identical hits; only the speed differs. */
+#define PG_LAUNCH_POSTPONE_LEAVES 1 /* wave tests its leaves together (while-while loop) */
+#define PG_LAUNCH_XCD_REMAP 2       /* each XCD traces one contiguous range of rays */
+#define PG_LAUNCH_DEFAULT 0
+int pg_accel_set_launch(pg_accel* a, int32_t block, int32_t flags);
 
 int pg_accel_release(pg_accel* a);
 

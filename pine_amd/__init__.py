@@ -17,6 +17,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpinehip.so")
 
 PG_OK = 0
+# pg_accel_set_launch flags (include/pinehip.h)
+LAUNCH_POSTPONE_LEAVES = 1
+LAUNCH_XCD_REMAP = 2
+LAUNCH_DEFAULT = 0
 EXPORTED = [
     "pg_last_error", "pg_version", "pg_accel_create", "pg_accel_add_mesh",
     "pg_accel_add_sphere", "pg_accel_add_rect", "pg_accel_add_plane", "pg_accel_build",
@@ -89,7 +93,7 @@ def lib():
     L.pg_scene_release.argtypes = [vp]
     L.pg_accel_get_stats.argtypes = [vp, ctypes.POINTER(_Stats)]
     L.pg_accel_prim_shape.argtypes = [vp, i32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
-    L.pg_accel_set_launch.argtypes = [vp, i32]
+    L.pg_accel_set_launch.argtypes = [vp, i32, i32]
     L.pg_accel_release.argtypes = [vp]
     L.pg_gen_icosphere.argtypes = [i32, ctypes.c_float, ctypes.c_uint64, i64, vp,
                                    ctypes.POINTER(i64), vp, ctypes.POINTER(i64)]
@@ -184,8 +188,9 @@ class Accel:
         check(lib().pg_accel_build(self.h, ctypes.byref(o)))
         return self
 
-    def set_launch(self, block):
-        check(lib().pg_accel_set_launch(self.h, block))
+    def set_launch(self, block=0, flags=-1):
+        """Trace launch shape: block 64/128/256 (0 keeps), flags LAUNCH_* bits (-1 keeps)."""
+        check(lib().pg_accel_set_launch(self.h, block, flags))
 
     def stats(self):
         s = _Stats()

@@ -66,6 +66,8 @@ struct pg_accel {
     double build_ms = 0.0;
     bool built = false;
     int block = 128;
+    int32_t launch_flags = PG_LAUNCH_DEFAULT;
+    int stack_entries = 1;  // LDS stack entries per lane, from the built depth
 
     void free_device() {
         if (d_nodes)
@@ -85,18 +87,13 @@ struct pg_accel {
 
 namespace {
 
-// LDS stack entries per lane: a binary tree needs one per level; a 4-wide node
-// can leave up to three siblings per level.
-template <int W>
-struct StackDepth;
-template <>
-struct StackDepth<2> {
-    static constexpr int value = 32;
-};
-template <>
-struct StackDepth<4> {
-    static constexpr int value = 48;
-};
+// LDS stack entries per lane: a binary node pushes at most one sibling per
+// level, a 4-wide node at most three.  The kernels size the stack to the built
+// tree (dynamic LDS), so shallow trees leave room for more waves per CU.
+constexpr int kMaxStackEntries = 64;
+inline int stack_entries(int width, int max_depth) {
+    return std::max(1, (width - 1) * max_depth);
+}
 
 struct RayIn {
     f3 o, d;
@@ -181,60 +178,179 @@ __device__ __forceinline__ bool leaf_test(int32_t code, const float4* __restrict
     return false;
 }
 
-// Binary nodes: 64 B = {lo0.xyz, hi0.x}, {hi0.yz, lo1.xy}, {lo1.z, hi1.xyz}, {c0, c1, -, -}.
-template <int kBlock, int kMode, bool kStats>
-__global__ __launch_bounds__(kBlock) void k_trace2(const float4* __restrict__ nodes,
-                                                   const float4* __restrict__ tris,
-                                                   const float4* __restrict__ ana,
-                                                   const pg_ray* __restrict__ rays, int64_t n,
-                                                   pg_hit* __restrict__ hits,
-                                                   pg_hit_uv* __restrict__ uvs,
-                                                   int32_t* __restrict__ occluded,
-                                                   int2* __restrict__ counts) {
-    __shared__ int32_t stack[StackDepth<2>::value * kBlock];
+// MI355X dispatches workgroup b to XCD b % 8, and each XCD has its own L2.
+// With the remap, XCD x works through one contiguous range of rays (its
+// share of blocks, in order), so neighbouring rays - which touch the same
+// nodes - meet in the same L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int64_t logical_block(bool remap) {
+    const int64_t b = blockIdx.x;
+    if (!remap)
+        return b;
+    const int64_t g = gridDim.x, q = g / kXcds, rem = g % kXcds;
+    const int64_t x = b % kXcds, k = b / kXcds;
+    return x * q + (x < rem ? x : rem) + k;
+}
+
+struct Stack {
+    int32_t* base;
+    int lane, stride, sp;
+    __device__ __forceinline__ void push(int32_t v) {
+        base[sp * stride + lane] = v;
+        sp++;
+    }
+    // Next node from the stack; false when it is empty (traversal finished).
+    __device__ __forceinline__ bool pop(int32_t& node) {
+        if (sp == 0)
+            return false;
+        sp--;
+        node = base[sp * stride + lane];
+        return true;
+    }
+};
+
+// Binary node, 64 B = {lo0.xyz, hi0.x}, {hi0.yz, lo1.xy}, {lo1.z, hi1.xyz}, {c0, c1, -, -}.
+// On return `node` is the next node to visit; `live` turns false when the
+// stack ran empty.
+__device__ __forceinline__ void visit2(const float4* __restrict__ nodes, const RayIn& r,
+                                       float lim, Stack& st, int32_t& node, bool& live) {
+    const float4* nd = nodes + 4 * (int64_t)node;
+    float4 a = nd[0], b = nd[1], c = nd[2], d = nd[3];
+    const int32_t c0 = __float_as_int(d.x), c1 = __float_as_int(d.y);
+    float t0, t1;
+    bool h0 = slab(r, a.x, a.y, a.z, a.w, b.x, b.y, lim, t0) && c0 != kEmptyChild;
+    bool h1 = slab(r, b.z, b.w, c.x, c.y, c.z, c.w, lim, t1) && c1 != kEmptyChild;
+    if (h0 && h1) {
+        st.push(t0 <= t1 ? c1 : c0);
+        node = t0 <= t1 ? c0 : c1;
+    } else if (h0 || h1) {
+        node = h0 ? c0 : c1;
+    } else {
+        live = st.pop(node);
+    }
+}
+
+// 4-wide node, 128 B = lox[4], loy[4], loz[4], hix[4], hiy[4], hiz[4], child[4], pad.
+// Hit children are ordered by entry distance; all but the nearest are pushed,
+// farthest first.
+__device__ __forceinline__ void visit4(const float4* __restrict__ nodes, const RayIn& r,
+                                       float lim, Stack& st, int32_t& node, bool& live) {
+    const float4* nd = nodes + 8 * (int64_t)node;
+    float4 lx = nd[0], ly = nd[1], lz = nd[2], hx = nd[3], hy = nd[4], hz = nd[5];
+    int4 ch = *reinterpret_cast<const int4*>(nd + 6);
+    float t[4];
+    int32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
+    bool h[4];
+    h[0] = slab(r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, lim, t[0]) && c[0] != kEmptyChild;
+    h[1] = slab(r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, lim, t[1]) && c[1] != kEmptyChild;
+    h[2] = slab(r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, lim, t[2]) && c[2] != kEmptyChild;
+    h[3] = slab(r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, lim, t[3]) && c[3] != kEmptyChild;
+    float key[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        key[k] = h[k] ? t[k] : __int_as_float(0x7f800000);
+#define PG_CSWAP(a, b)          \
+    if (key[b] < key[a]) {      \
+        float tk = key[a];      \
+        key[a] = key[b];        \
+        key[b] = tk;            \
+        int32_t tc = c[a];      \
+        c[a] = c[b];            \
+        c[b] = tc;              \
+    }
+    PG_CSWAP(0, 1)
+    PG_CSWAP(2, 3)
+    PG_CSWAP(0, 2)
+    PG_CSWAP(1, 3)
+    PG_CSWAP(1, 2)
+#undef PG_CSWAP
+    const int nhit = (int)h[0] + (int)h[1] + (int)h[2] + (int)h[3];
+    if (nhit == 0) {
+        live = st.pop(node);
+        return;
+    }
+    for (int k = nhit - 1; k >= 1; k--)
+        st.push(c[k]);
+    node = c[0];
+}
+
+template <int W>
+__device__ __forceinline__ void visit(const float4* __restrict__ nodes, const RayIn& r,
+                                      float lim, Stack& st, int32_t& node, bool& live) {
+    if (W == 2)
+        visit2(nodes, r, lim, st, node, live);
+    else
+        visit4(nodes, r, lim, st, node, live);
+}
+
+// One ray per lane.  kPostpone selects the traversal loop:
+//  - false: each iteration visits one node or tests one leaf, so a wave whose
+//    lanes are at different kinds of node executes both paths;
+//  - true: leaves are postponed (Aila and Laine's while-while loop with
+//    speculative traversal): a lane that reaches a leaf records it and keeps
+//    descending until every lane of the wave holds a leaf, then the wave tests
+//    its leaves together.  Any valid visiting order returns the same closest
+//    hit, because ties are broken by (t, prim id).
+template <int W, int kBlock, int kMode, bool kStats, bool kPostpone>
+__global__ __launch_bounds__(kBlock) void k_trace(const float4* __restrict__ nodes,
+                                                  const float4* __restrict__ tris,
+                                                  const float4* __restrict__ ana,
+                                                  const pg_ray* __restrict__ rays, int64_t n,
+                                                  pg_hit* __restrict__ hits,
+                                                  pg_hit_uv* __restrict__ uvs,
+                                                  int32_t* __restrict__ occluded,
+                                                  int2* __restrict__ counts, int xcd_remap) {
+    extern __shared__ int32_t lds_stack[];
     const int lane = threadIdx.x;
-    const int64_t i = (int64_t)blockIdx.x * kBlock + lane;
+    const int64_t i = logical_block(xcd_remap != 0) * kBlock + lane;
     if (i >= n)
         return;
     const RayIn r = load_ray(rays, i);
     float best_t = r.tmax, bu = 0.0f, bv = 0.0f;
     int32_t best_prim = -1;
-    int sp = 0;
+    Stack st{lds_stack, lane, kBlock, 0};
     int32_t node = 0;
-    bool found_any = false;
+    bool live = true, found_any = false;
     int visited = 0, tested = 0;
-    while (true) {
-        if (node >= 0) {
-            visited++;
-            const float4* nd = nodes + 4 * (int64_t)node;
-            float4 a = nd[0], b = nd[1], c = nd[2], d = nd[3];
-            const int32_t c0 = __float_as_int(d.x), c1 = __float_as_int(d.y);
-            const float lim = kMode == kAnyHit ? r.tmax : best_t;
-            float t0, t1;
-            bool h0 = slab(r, a.x, a.y, a.z, a.w, b.x, b.y, lim, t0) && c0 != kEmptyChild;
-            bool h1 = slab(r, b.z, b.w, c.x, c.y, c.z, c.w, lim, t1) && c1 != kEmptyChild;
-            if (h0 && h1) {
-                int32_t nearc = t0 <= t1 ? c0 : c1;
-                int32_t farc = t0 <= t1 ? c1 : c0;
-                stack[sp * kBlock + lane] = farc;
-                sp++;
-                node = nearc;
-                continue;
-            }
-            if (h0 || h1) {
-                node = h0 ? c0 : c1;
-                continue;
-            }
-        } else {
-            if (leaf_test<kMode>(node, tris, ana, r, best_t, best_prim, bu, bv, tested)) {
-                found_any = true;
-                break;
+    if (!kPostpone) {
+        while (live) {
+            if (node >= 0) {
+                visited++;
+                visit<W>(nodes, r, kMode == kAnyHit ? r.tmax : best_t, st, node, live);
+            } else {
+                if (leaf_test<kMode>(node, tris, ana, r, best_t, best_prim, bu, bv, tested)) {
+                    found_any = true;
+                    break;
+                }
+                live = st.pop(node);
             }
         }
-        if (sp == 0)
-            break;
-        sp--;
-        node = stack[sp * kBlock + lane];
+    } else {
+        int32_t leaf = 0;  // postponed leaf code; leaf codes are negative, 0 = none
+        while (live) {
+            while (live && node >= 0) {
+                visited++;
+                visit<W>(nodes, r, kMode == kAnyHit ? r.tmax : best_t, st, node, live);
+                if (live && node < 0 && leaf == 0) {
+                    leaf = node;
+                    live = st.pop(node);
+                }
+                if (__all(leaf != 0))
+                    break;
+            }
+            while (leaf != 0) {
+                if (leaf_test<kMode>(leaf, tris, ana, r, best_t, best_prim, bu, bv, tested)) {
+                    found_any = true;
+                    live = false;
+                    break;
+                }
+                leaf = 0;
+                if (live && node < 0) {
+                    leaf = node;
+                    live = st.pop(node);
+                }
+            }
+        }
     }
     if (kStats)
         counts[i] = make_int2(visited, tested);
@@ -254,125 +370,40 @@ __global__ __launch_bounds__(kBlock) void k_trace2(const float4* __restrict__ no
     }
 }
 
-// 4-wide nodes: 128 B = lox[4], loy[4], loz[4], hix[4], hiy[4], hiz[4], child[4], pad.
-template <int kBlock, int kMode, bool kStats>
-__global__ __launch_bounds__(kBlock) void k_trace4(const float4* __restrict__ nodes,
-                                                   const float4* __restrict__ tris,
-                                                   const float4* __restrict__ ana,
-                                                   const pg_ray* __restrict__ rays, int64_t n,
-                                                   pg_hit* __restrict__ hits,
-                                                   pg_hit_uv* __restrict__ uvs,
-                                                   int32_t* __restrict__ occluded,
-                                                   int2* __restrict__ counts) {
-    __shared__ int32_t stack[StackDepth<4>::value * kBlock];
-    const int lane = threadIdx.x;
-    const int64_t i = (int64_t)blockIdx.x * kBlock + lane;
-    if (i >= n)
-        return;
-    const RayIn r = load_ray(rays, i);
-    float best_t = r.tmax, bu = 0.0f, bv = 0.0f;
-    int32_t best_prim = -1;
-    int sp = 0;
-    int32_t node = 0;
-    bool found_any = false;
-    int visited = 0, tested = 0;
-    while (true) {
-        if (node >= 0) {
-            visited++;
-            const float4* nd = nodes + 8 * (int64_t)node;
-            float4 lx = nd[0], ly = nd[1], lz = nd[2], hx = nd[3], hy = nd[4], hz = nd[5];
-            int4 ch = *reinterpret_cast<const int4*>(nd + 6);
-            const float lim = kMode == kAnyHit ? r.tmax : best_t;
-            float t[4];
-            int32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
-            bool h[4];
-            h[0] = slab(r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, lim, t[0]) && c[0] != kEmptyChild;
-            h[1] = slab(r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, lim, t[1]) && c[1] != kEmptyChild;
-            h[2] = slab(r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, lim, t[2]) && c[2] != kEmptyChild;
-            h[3] = slab(r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, lim, t[3]) && c[3] != kEmptyChild;
-            // Sort hit children by entry distance (misses pushed to +inf), then push
-            // all but the nearest, farthest first.
-            float key[4];
-#pragma unroll
-            for (int k = 0; k < 4; k++)
-                key[k] = h[k] ? t[k] : __int_as_float(0x7f800000);
-#define PG_CSWAP(a, b)                                \
-    if (key[b] < key[a]) {                            \
-        float tk = key[a];                            \
-        key[a] = key[b];                              \
-        key[b] = tk;                                  \
-        int32_t tc = c[a];                            \
-        c[a] = c[b];                                  \
-        c[b] = tc;                                    \
-    }
-            PG_CSWAP(0, 1)
-            PG_CSWAP(2, 3)
-            PG_CSWAP(0, 2)
-            PG_CSWAP(1, 3)
-            PG_CSWAP(1, 2)
-#undef PG_CSWAP
-            const int nhit = (int)h[0] + (int)h[1] + (int)h[2] + (int)h[3];
-            if (nhit > 0) {
-                for (int k = nhit - 1; k >= 1; k--) {
-                    stack[sp * kBlock + lane] = c[k];
-                    sp++;
-                }
-                node = c[0];
-                continue;
-            }
-        } else {
-            if (leaf_test<kMode>(node, tris, ana, r, best_t, best_prim, bu, bv, tested)) {
-                found_any = true;
-                break;
-            }
-        }
-        if (sp == 0)
-            break;
-        sp--;
-        node = stack[sp * kBlock + lane];
-    }
-    if (kStats)
-        counts[i] = make_int2(visited, tested);
-    if (kMode != kClosest) {
-        occluded[i] = found_any ? 1 : 0;
-    } else {
-        pg_hit hh;
-        hh.prim = best_prim;
-        hh.t = best_prim >= 0 ? best_t : r.tmax;
-        hits[i] = hh;
-        if (uvs) {
-            pg_hit_uv uv;
-            uv.u = bu;
-            uv.v = bv;
-            uvs[i] = uv;
-        }
-    }
+template <int W, int kMode, bool kStats, bool kPostpone>
+void launch_w(const pg_accel* a, int64_t grid, size_t lds, const pg_ray* rays, int64_t n,
+              pg_hit* hits, pg_hit_uv* uvs, int32_t* occ, int2* counts, hipStream_t s) {
+    const int remap = (a->launch_flags & PG_LAUNCH_XCD_REMAP) ? 1 : 0;
+#define PG_LAUNCH(B)                                                                           \
+    hipLaunchKernelGGL((k_trace<W, B, kMode, kStats, kPostpone>), dim3((unsigned)grid), dim3(B), \
+                       lds, s, a->d_nodes, a->d_tris, a->d_ana, rays, n, hits, uvs, occ,         \
+                       counts, remap)
+    if (a->block == 64)
+        PG_LAUNCH(64);
+    else if (a->block == 256)
+        PG_LAUNCH(256);
+    else
+        PG_LAUNCH(128);
+#undef PG_LAUNCH
 }
 
 template <int kMode, bool kStats>
 hipError_t launch_trace(const pg_accel* a, const pg_ray* rays, int64_t n, pg_hit* hits,
                         pg_hit_uv* uvs, int32_t* occ, int2* counts, hipStream_t s) {
-    const int block = a->block;
-    const int64_t grid = (n + block - 1) / block;
-#define PG_LAUNCH(K, B)                                                                        \
-    hipLaunchKernelGGL((K<B, kMode, kStats>), dim3((unsigned)grid), dim3(B), 0, s, a->d_nodes,   \
-                       a->d_tris, a->d_ana, rays, n, hits, uvs, occ, counts)
+    const int64_t grid = (n + a->block - 1) / a->block;
+    const size_t lds = (size_t)a->stack_entries * a->block * sizeof(int32_t);
+    const bool postpone = (a->launch_flags & PG_LAUNCH_POSTPONE_LEAVES) != 0;
     if (a->width == 2) {
-        if (block == 64)
-            PG_LAUNCH(k_trace2, 64);
-        else if (block == 256)
-            PG_LAUNCH(k_trace2, 256);
+        if (postpone)
+            launch_w<2, kMode, kStats, true>(a, grid, lds, rays, n, hits, uvs, occ, counts, s);
         else
-            PG_LAUNCH(k_trace2, 128);
+            launch_w<2, kMode, kStats, false>(a, grid, lds, rays, n, hits, uvs, occ, counts, s);
     } else {
-        if (block == 64)
-            PG_LAUNCH(k_trace4, 64);
-        else if (block == 256)
-            PG_LAUNCH(k_trace4, 256);
+        if (postpone)
+            launch_w<4, kMode, kStats, true>(a, grid, lds, rays, n, hits, uvs, occ, counts, s);
         else
-            PG_LAUNCH(k_trace4, 128);
+            launch_w<4, kMode, kStats, false>(a, grid, lds, rays, n, hits, uvs, occ, counts, s);
     }
-#undef PG_LAUNCH
     return hipGetLastError();
 }
 
@@ -547,8 +578,7 @@ extern "C" int pg_accel_build(pg_accel* a, const pg_build_options* opts) {
     // never cull a box that holds the true closest hit.
     const float inflate = scene_mag * (1.0f / 262144.0f) + 1e-30f;
     FlatBVH flat = build_bvh(prims, bp, inflate);
-    const int capacity = bp.width == 2 ? StackDepth<2>::value : StackDepth<4>::value;
-    if ((bp.width - 1) * flat.max_depth > capacity)
+    if (stack_entries(flat.width, flat.max_depth) > kMaxStackEntries)
         return pg_fail(PG_E_STATE, "pg_accel_build: hierarchy deeper than the traversal stack");
 
     // Leaf records in slot order.
@@ -587,6 +617,7 @@ extern "C" int pg_accel_build(pg_accel* a, const pg_build_options* opts) {
     }
     a->width = flat.width;
     a->max_depth = flat.max_depth;
+    a->stack_entries = stack_entries(flat.width, flat.max_depth);
     a->n_nodes = flat.n_nodes;
     a->n_tri_slots = (int64_t)flat.tri_order.size();
     a->n_ana_slots = (int64_t)flat.ana_order.size();
@@ -677,13 +708,17 @@ extern "C" int pg_accel_prim_shape(const pg_accel* a, int32_t prim, int32_t* sha
     return PG_OK;
 }
 
-extern "C" int pg_accel_set_launch(pg_accel* a, int32_t block) {
+extern "C" int pg_accel_set_launch(pg_accel* a, int32_t block, int32_t flags) {
     if (!a)
         return pg_fail(PG_E_ARG, "pg_accel_set_launch: null accel");
     if (block != 0 && block != 64 && block != 128 && block != 256)
         return pg_fail(PG_E_ARG, "pg_accel_set_launch: block must be 64, 128 or 256");
+    if (flags < -1 || flags > (PG_LAUNCH_POSTPONE_LEAVES | PG_LAUNCH_XCD_REMAP))
+        return pg_fail(PG_E_ARG, "pg_accel_set_launch: unknown flags");
     if (block)
         a->block = block;
+    if (flags >= 0)
+        a->launch_flags = flags;
     return PG_OK;
 }
 

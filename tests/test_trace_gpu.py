@@ -15,11 +15,11 @@ pytestmark = pytest.mark.gpu
 WIDTHS = [2, 4]
 
 
-def _accel(pg, v, i, width, block=128):
+def _accel(pg, v, i, width, block=64, flags=-1):
     a = pg.Accel()
     a.add_mesh(v, i)
     a.build(width=width)
-    a.set_launch(block)
+    a.set_launch(block, flags)
     return a
 
 
@@ -126,12 +126,22 @@ def test_uv_output_matches_pine(pg, cuda):
 
 @pytest.mark.parametrize("width", WIDTHS)
 @pytest.mark.parametrize("block", [64, 128, 256])
-def test_launch_shapes_agree(pg, cuda, width, block):
+@pytest.mark.parametrize("flags", [0, 1, 2, 3])
+def test_launch_shapes_agree(pg, oracle, cuda, width, block, flags):
+    """Every block size, traversal loop (leaves postponed or not) and XCD
+    remap gives pine's hits; the any-hit and occlusion modes too."""
     fx = golden("trace_82k")
     v, i, rays = _mesh_rays(fx)
-    a = _accel(pg, v, i, width, block)
+    a = _accel(pg, v, i, width, block, flags)
     prim, t, occ = _trace(a, rays, cuda)
     check_vs_pine(prim, t, occ, fx["prim"], fx["t"], fx["occluded"])
+    d = cuda.DeviceArray.from_numpy(rays)
+    occl = a.trace_occluded(d).numpy()
+    cuda.synchronize()
+    _, _, _, _, bocc = oracle.trace_brute(oracle.tri9(v, i), rays)
+    # pg_trace_occluded uses pine's Intersect tests (open t interval).
+    assert np.array_equal(occl, (prim >= 0).astype(occl.dtype))
+    assert np.array_equal(occ, bocc)
 
 
 def test_edge_cases_vs_oracle(pg, oracle, cuda):

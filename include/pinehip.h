@@ -170,6 +170,9 @@ typedef struct pg_scene_info {
     int64_t n_prims;
     double last_render_ms;   /* wall time of the last pg_render on its stream */
     int64_t last_rays;       /* rays traced by the last pg_render (extension + shadow) */
+    int32_t n_media;         /* Medium blocks */
+    int32_t general_mode;    /* 1: media or material-less shapes (split bounce) */
+    int64_t medium_steps;    /* reserved, 0 */
 } pg_scene_info;
 int pg_scene_get_info(const pg_scene* s, pg_scene_info* info);
 
@@ -209,6 +212,13 @@ int pg_gen_icosphere(int32_t freq, float amp, uint64_t seed, int64_t max_triangl
 int pg_gen_rays(const float* h_vertices, int64_t n_vertices, const uint32_t* h_indices,
                 int64_t n_triangles, int32_t res, const float eye[3], float fov_deg,
                 int64_t n_bounce, uint64_t seed, pg_ray* h_rays);
+
+/* Heterogeneous-medium workload: an nx*ny*nz density grid, voxel value
+ * max(0, fBm6(voxel centre in [0,1]^3) - threshold) * scale, fBm of value noise
+ * with base frequency base_freq, x fastest (pine's raw volume order,
+ * fileio.cpp:194-203).  Deterministic for a seed. */
+int pg_gen_density_grid(int32_t nx, int32_t ny, int32_t nz, float base_freq, float threshold,
+                        float scale, uint64_t seed, float* h_density);
 
 #ifdef __cplusplus
 }

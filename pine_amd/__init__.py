@@ -28,6 +28,7 @@ EXPORTED = [
     "pg_accel_set_launch", "pg_accel_release", "pg_gen_icosphere", "pg_gen_rays",
     "pg_trace_closest_counts", "pg_trace_occluded", "pg_scene_load", "pg_scene_load_text",
     "pg_scene_get_info", "pg_render", "pg_scene_film", "pg_scene_output_name", "pg_scene_release",
+    "pg_gen_density_grid",
 ]
 
 
@@ -51,7 +52,9 @@ class _SceneInfo(ctypes.Structure):
     _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("spp", ctypes.c_int32),
                 ("max_depth", ctypes.c_int32), ("n_shapes", ctypes.c_int32),
                 ("n_lights", ctypes.c_int32), ("n_prims", ctypes.c_int64),
-                ("last_render_ms", ctypes.c_double), ("last_rays", ctypes.c_int64)]
+                ("last_render_ms", ctypes.c_double), ("last_rays", ctypes.c_int64),
+                ("n_media", ctypes.c_int32), ("general_mode", ctypes.c_int32),
+                ("medium_steps", ctypes.c_int64)]
 
 
 class _RenderOptions(ctypes.Structure):
@@ -99,6 +102,8 @@ def lib():
                                    ctypes.POINTER(i64), vp, ctypes.POINTER(i64)]
     L.pg_gen_rays.argtypes = [vp, i64, vp, i64, i32, fp, ctypes.c_float, i64,
                               ctypes.c_uint64, vp]
+    L.pg_gen_density_grid.argtypes = [i32, i32, i32, ctypes.c_float, ctypes.c_float,
+                                      ctypes.c_float, ctypes.c_uint64, vp]
     _lib = L
     return L
 
@@ -149,6 +154,24 @@ def gen_rays(vertices, indices, res, eye, fov_deg, n_bounce, seed=7):
     check(lib().pg_gen_rays(_ptr(v), len(v), _ptr(i), len(i), res, _f3(eye), fov_deg, n_bounce,
                             seed, _ptr(rays)))
     return rays
+
+
+def gen_density_grid(n, base_freq=4.0, threshold=0.3, scale=40.0, seed=3):
+    """fBm density grid, float32 [n, n, n] indexed [z, y, x] (x fastest in memory):
+    max(0, fbm6 - threshold) * scale."""
+    out = np.zeros((n, n, n), np.float32)
+    check(lib().pg_gen_density_grid(n, n, n, base_freq, threshold, scale, seed, _ptr(out)))
+    return out
+
+
+def write_volume(path, density):
+    """Writes pine's raw volume file (fileio.cpp:194-203): int32 size x, y, z, then
+    the floats, x fastest.  `density` is indexed [z, y, x]."""
+    d = np.ascontiguousarray(density, np.float32)
+    nz, ny, nx = d.shape
+    with open(path, "wb") as f:
+        f.write(np.array([nx, ny, nz], np.int32).tobytes())
+        f.write(d.tobytes())
 
 
 # --- accel -----------------------------------------------------------------------

@@ -17,6 +17,7 @@
 #include "bvh_build.h"
 #include "pg_api.h"
 #include "pg_isect.h"
+#include "pg_medium.h"
 #include "pg_shade.h"
 #include "scene_parse.h"
 
@@ -52,9 +53,10 @@ struct ShapeDev {
     int32_t tri_offset; // meshes: first triangle in tri_verts
     f3 p0, e0, e1, n;   // sphere: p0 = c; rect: p0 = pos, e0/e1 = ex/ey unit, n; plane: p0, n, e0 = u, e1 = v
     float lx, ly, r, area;
+    int32_t mi_inside, mi_outside;  // MediumInterface (geometry.cpp:392-409), -1 = none
 };
 
-enum LightType : int32_t { kLightArea = 0, kLightPoint = 1, kLightDirectional = 2 };
+enum LightType : int32_t { kLightArea = 0, kLightPoint = 1, kLightDirectional = 2, kLightSky = 3 };
 struct LightDev {
     int32_t type;
     int32_t shape;
@@ -85,6 +87,12 @@ struct SceneDev {
     float filter_radius[2];
     int32_t width, height;
     int32_t max_depth;
+    // Media (medium.cpp) and the environment light (scene->envLight).
+    const MediumDev* media;
+    int32_t n_media;
+    int32_t camera_medium;  // ThinLenCamera medium (camera.cpp:98-105), -1 = none
+    int32_t has_env;        // a Sky environment light is present
+    f3 sky_dir, sky_color;  // Sky sunDirection (normalised) and sunColor
 };
 
 // Path state, structure of arrays (one entry per path of the batch).
@@ -113,6 +121,20 @@ struct PathsDev {
     int32_t* spill_count;
     float* spill;  // 5 floats per entry (pixel index stored as bits)
     int32_t spill_capacity;
+    // General mode (media or material-less shapes): the bounce is split at the
+    // shadow ray, because GridMedium::Tr consumes sampler dimensions between
+    // the light sample and the BSDF / phase sample.
+    int32_t* medium;       // medium of the path's current ray, -1 = none
+    float* vtx;            // 9 per path: vertex p, n, wi
+    int32_t* vinfo;        // 4 per path: kind (0 surface, 1 medium), material, medium in, out
+    int32_t* queue_b;      // paths that sample a direction after their shadow ray
+    float* sh_rec;         // 12 per shadow record: f.xyz, w, Le.xyz, pdf, p2.xyz, delta
+    float* sh_tr;          // 3 per shadow record: transmittance so far
+    pg_ray* seg_rays[2];   // shadow segments (IntersectTr loop), ping-pong
+    int32_t* seg_rec[2];
+    int32_t* seg_med[2];
+    pg_hit* seg_hits;
+    pg_hit_uv* seg_uvs;
 };
 
 }  // namespace pg
@@ -143,7 +165,7 @@ __device__ __forceinline__ void store_ray(pg_ray* r, f3 o, f3 d, float tmin, flo
     p[1] = make_float4(d.x, d.y, d.z, tmax);
 }
 
-__global__ void k_raygen(SceneDev s, PathsDev P, int32_t n_paths, int32_t spb, int32_t sample0) {
+__global__ __launch_bounds__(256) void k_raygen(SceneDev s, PathsDev P, int32_t n_paths, int32_t spb, int32_t sample0) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_paths)
         return;
@@ -170,6 +192,20 @@ __global__ void k_raygen(SceneDev s, PathsDev P, int32_t n_paths, int32_t spb, i
     P.pfilm[2 * i + 1] = fy;
     P.queue_in[i] = i;
     store_ray(&P.rays_in[i], o, d, 0.0f, kFloatMax);
+    if (P.medium)
+        P.medium[i] = s.camera_medium;  // camera.h:61 ray.medium = medium
+}
+
+// Sky::Color / SkyColor (light.cpp:74-76, color.cpp:91-99): a gradient in the
+// angle from +y (SkyColor swaps y and z, then takes acos of z), 8 x sunColor
+// exactly along the sun direction.
+__device__ f3 sky_color(const SceneDev& s, f3 d) {
+    if (d.x == s.sky_dir.x && d.y == s.sky_dir.y && d.z == s.sky_dir.z)
+        return 8.0f * s.sky_color;
+    const float theta = pstd::acos(d.y) * kInvPi;
+    const f3 lo = mk(0.01f, 0.03f, 0.3f), hi = mk(0.3f, 0.5f, 0.8f);
+    const f3 l = lo * (1.0f - theta) + hi * theta;  // pstd::lerp (math.h:96-98)
+    return s.sky_color * (l * l);
 }
 
 struct Hit {
@@ -274,6 +310,15 @@ __device__ LightSampleD sample_light_point(const SceneDev& s, const LightDev& L,
         ls.delta = true;
         return ls;
     }
+    if (L.type == kLightSky) {
+        // Sky::Sample (light.cpp:65-73): the sun as a delta light.
+        ls.distance = 1e10f;
+        ls.delta = true;
+        ls.wo = s.sky_dir;
+        ls.Le = 8.0f * s.sky_color;
+        ls.pdf = 1.0f;
+        return ls;
+    }
     if (L.type == kLightDirectional) {
         ls.Le = L.color;
         ls.wo = L.v;
@@ -314,7 +359,7 @@ __device__ LightSampleD sample_light_point(const SceneDev& s, const LightDev& L,
     return ls;
 }
 
-__global__ void k_shade(SceneDev s, PathsDev P, int32_t n_active) {
+__global__ __launch_bounds__(256) void k_shade(SceneDev s, PathsDev P, int32_t n_active) {
     const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n_active)
         return;
@@ -333,7 +378,14 @@ __global__ void k_shade(SceneDev s, PathsDev P, int32_t n_active) {
     bool alive = false;
     f3 nextO, nextD;
 
-    if (hit.prim >= 0) {
+    if (hit.prim < 0) {
+        if (s.has_env) {
+            // path.cpp:29-39; Sky::Pdf is 0, so the MIS weight is the power
+            // heuristic against a zero light pdf.
+            const f3 le = sky_color(s, rd);
+            L = depth == 0 ? L + beta * le : L + beta * le * power_heuristic(bsdfPdf, 0.0f);
+        }
+    } else {
         const pg_hit_uv uv = P.uvs[k];
         Hit h = surface(s, hit.prim, hit.t, uv.u, uv.v, ro, rd);
         const ShapeDev& sh = s.shapes[h.shape];
@@ -427,7 +479,7 @@ __global__ void k_shade(SceneDev s, PathsDev P, int32_t n_active) {
     }
 }
 
-__global__ void k_resolve_shadow(PathsDev P, int32_t n_shadow) {
+__global__ __launch_bounds__(256) void k_resolve_shadow(PathsDev P, int32_t n_shadow) {
     const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n_shadow || P.occluded[k])
         return;
@@ -438,8 +490,328 @@ __global__ void k_resolve_shadow(PathsDev P, int32_t n_shadow) {
     P.L[3 * i + 2] += P.shadow_contrib[3 * k + 2];
 }
 
+
+// ---------------------------------------------------------------------------
+// General mode: media and material-less shapes.  One bounce of
+// PathIntegrator::Li (path.cpp:6-87) becomes
+//   k_shade_gen   medium sample, env / emission, light sample -> shadow record
+//   shadow loop   trace_closest + k_shadow_tr per segment (IntersectTr,
+//                 integrator.cpp:68-87), consuming Tr's sampler dimensions
+//   k_resolve_gen L += beta * tr * f * w * Le / pdf
+//   k_shade_gen_b BSDF or phase sample, Russian roulette -> next ray
+
+// Brick majorants of medium m: the LDS copy when this launch staged it.
+__device__ __forceinline__ const float* majorants(const SceneDev& s, int m, int lds_medium,
+                                                  const float* lds) {
+    return m == lds_medium ? lds : s.media[m].majorant;
+}
+
+// Loads the brick majorants of `lds_medium` into LDS (once per workgroup of a
+// grid-stride kernel).
+__device__ const float* stage_majorants(const SceneDev& s, int lds_medium, float* lds) {
+    if (lds_medium < 0)
+        return nullptr;
+    const MediumDev& m = s.media[lds_medium];
+    const int nb = m.nbricks[0] * m.nbricks[1] * m.nbricks[2];
+    for (int k = threadIdx.x; k < nb; k += blockDim.x)
+        lds[k] = m.majorant[k];
+    __syncthreads();
+    return lds;
+}
+
+__device__ __forceinline__ int medium_for(f3 w, f3 n, int inside, int outside) {
+    return dot(w, n) > 0.0f ? outside : inside;  // Interaction::GetMedium (interaction.h:54-56)
+}
+
+// EstimateDirect's light sample (integrator.cpp:88-119) up to the shadow ray,
+// for a surface (mat >= 0) or medium (mat < 0, phase g) vertex.  The record is
+// written even when the contribution is black: IntersectTr still runs, and in
+// media it consumes sampler dimensions.
+__device__ void light_record(const SceneDev& s, PathsDev& P, SamplerState& ss, int32_t i, f3 p,
+                             f3 n, f3 wi, int mat, float g, int med_in, int med_out) {
+    const float ul = get_1d(s.halton, ss);  // SampleLight's Get1D runs even with no lights
+    if (s.n_lights <= 0)
+        return;
+    float lpdf, la, lb;
+    const int li = sample_light(s, ul, lpdf);
+    get_2d(s.halton, ss, la, lb);
+    LightSampleD ls = sample_light_point(s, s.lights[li], p, la, lb);
+    ls.pdf *= lpdf;
+    f3 f;
+    float scatPdf = 0.0f;
+    if (mat >= 0) {
+        const MaterialDev& m = s.materials[mat];
+        const M3 w2n = inverse(frame_from(n));
+        f = material_f(m, mul(w2n, wi), mul(w2n, ls.wo)) * abs_dot(ls.wo, n);
+        if (!ls.delta)
+            scatPdf = material_pdf(m, mul(w2n, wi), mul(w2n, ls.wo));
+    } else {
+        const float ph = phase_p(g, wi, ls.wo);
+        f = mk(ph, ph, ph);
+        scatPdf = ph;
+    }
+    const float w = power_heuristic(ls.pdf, scatPdf);
+    // it.SpawnRay(ls.wo, ls.distance) (interaction.h:36-44)
+    const f3 so = offset_ray_origin(p, face_same_hemisphere(n, ls.wo));
+    const float stmax = ls.distance * (1.0f - 1e-3f);
+    const int r = atomicAdd(&P.counters[1], 1);
+    float4* rec = reinterpret_cast<float4*>(P.sh_rec + 12 * (int64_t)r);
+    const f3 p2 = so + stmax * ls.wo;  // IntersectTr's target ray(ray.tmax)
+    rec[0] = make_float4(f.x, f.y, f.z, w);
+    rec[1] = make_float4(ls.Le.x, ls.Le.y, ls.Le.z, ls.pdf);
+    rec[2] = make_float4(p2.x, p2.y, p2.z, ls.delta ? 1.0f : 0.0f);
+    P.sh_tr[3 * r + 0] = P.sh_tr[3 * r + 1] = P.sh_tr[3 * r + 2] = 1.0f;
+    P.shadow_path[r] = i;
+    store_ray(&P.seg_rays[0][r], so, ls.wo, 0.0f, stmax);
+    P.seg_rec[0][r] = r;
+    P.seg_med[0][r] = medium_for(ls.wo, n, med_in, med_out);
+}
+
+__device__ void save_vertex(PathsDev& P, int32_t i, int kind, f3 p, f3 n, f3 wi, int mat,
+                            int med_in, int med_out) {
+    float* v = P.vtx + 9 * (int64_t)i;
+    v[0] = p.x, v[1] = p.y, v[2] = p.z;
+    v[3] = n.x, v[4] = n.y, v[5] = n.z;
+    v[6] = wi.x, v[7] = wi.y, v[8] = wi.z;
+    int4 info = make_int4(kind, mat, med_in, med_out);
+    *reinterpret_cast<int4*>(P.vinfo + 4 * (int64_t)i) = info;
+    P.queue_b[atomicAdd(&P.counters[2], 1)] = i;
+}
+
+__global__ __launch_bounds__(256) void k_shade_gen(SceneDev s, PathsDev P, int32_t n_active, int32_t lds_medium) {
+    extern __shared__ float lds_maj[];
+    const float* lds = stage_majorants(s, lds_medium, lds_maj);
+    for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n_active;
+         k += gridDim.x * blockDim.x) {
+        const int32_t i = P.queue_in[k];
+        const float4* r4 = reinterpret_cast<const float4*>(&P.rays_in[k]);
+        const float4 ra = r4[0], rb = r4[1];
+        const f3 ro = mk(ra.x, ra.y, ra.z), rd = mk(rb.x, rb.y, rb.z);
+        const pg_hit hit = P.hits[k];
+        const int depth = P.depth[i];
+        f3 L = mk(P.L[3 * i], P.L[3 * i + 1], P.L[3 * i + 2]);
+        f3 beta = mk(P.beta[3 * i], P.beta[3 * i + 1], P.beta[3 * i + 2]);
+        const float bsdfPdf = P.bsdf_pdf[i];
+        SamplerState ss;
+        ss.index = P.sidx[i];
+        ss.dim = P.sdim[i];
+        const int med = P.medium[i];
+        bool alive = false;
+        f3 nextO, nextD;
+        int next_med = med;
+        // Intersect leaves ray.tmax at the hit distance (bvh.cpp:652-709).
+        const float tmax = hit.prim >= 0 ? hit.t : kFloatMax;
+        MediumEvent ev;
+        ev.scattered = false;
+        if (med >= 0) {
+            ev = medium_sample(s.media[med], majorants(s, med, lds_medium, lds), s.halton, ss, ro,
+                               rd, tmax);
+            beta = beta * ev.weight;
+        }
+        if (ev.scattered) {
+            if (depth + 1 != s.max_depth) {
+                const f3 zero = mk(0.0f, 0.0f, 0.0f);
+                light_record(s, P, ss, i, ev.p, zero, -rd, -1, s.media[med].g, med, med);
+                save_vertex(P, i, 1, ev.p, zero, -rd, -1, med, med);
+            }
+        } else if (hit.prim < 0) {
+            if (s.has_env) {
+                const f3 le = sky_color(s, rd);
+                L = depth == 0 ? L + beta * le : L + beta * le * power_heuristic(bsdfPdf, 0.0f);
+            }
+        } else {
+            const pg_hit_uv uv = P.uvs[k];
+            const Hit h = surface(s, hit.prim, hit.t, uv.u, uv.v, ro, rd);
+            const ShapeDev& sh = s.shapes[h.shape];
+            // bvh.cpp:702-707: a shape's own interface if it is a transition,
+            // else the ray's medium on both sides.
+            const bool transition = sh.mi_inside != sh.mi_outside;
+            const int min_ = transition ? sh.mi_inside : med;
+            const int mout = transition ? sh.mi_outside : med;
+            if (sh.material < 0) {
+                nextD = rd;
+                nextO = offset_ray_origin(h.p, face_same_hemisphere(h.n, rd));
+                next_med = medium_for(rd, h.n, min_, mout);
+                alive = true;
+            } else {
+                const MaterialDev& m = s.materials[sh.material];
+                if (m.kind == kMatEmissive) {
+                    const f3 le = emitted(m, h.n, -rd);
+                    if (depth == 0) {
+                        L = L + beta * le;
+                    } else {
+                        const float lightPdf = hit.t * hit.t / (abs_dot(-rd, h.n) * sh.area);
+                        L = L + beta * le * power_heuristic(bsdfPdf, lightPdf);
+                    }
+                } else if (depth + 1 != s.max_depth) {
+                    light_record(s, P, ss, i, h.p, h.n, -rd, sh.material, 0.0f, min_, mout);
+                    save_vertex(P, i, 0, h.p, h.n, -rd, sh.material, min_, mout);
+                }
+            }
+        }
+        P.L[3 * i + 0] = L.x;
+        P.L[3 * i + 1] = L.y;
+        P.L[3 * i + 2] = L.z;
+        P.beta[3 * i + 0] = beta.x;
+        P.beta[3 * i + 1] = beta.y;
+        P.beta[3 * i + 2] = beta.z;
+        P.sdim[i] = ss.dim;
+        if (alive) {
+            const int slot = atomicAdd(&P.counters[0], 1);
+            P.queue_out[slot] = i;
+            P.medium[i] = next_med;
+            store_ray(&P.rays_out[slot], nextO, nextD, 0.0f, kFloatMax);
+        }
+    }
+}
+
+// One segment of IntersectTr (integrator.cpp:68-87): transmittance of the
+// segment's medium up to the closest hit, then stop at a surface with a
+// material, or continue towards p2 through a material-less one.
+__global__ __launch_bounds__(256) void k_shadow_tr(SceneDev s, PathsDev P, int32_t n_seg, int32_t src, int32_t lds_medium) {
+    extern __shared__ float lds_maj[];
+    const float* lds = stage_majorants(s, lds_medium, lds_maj);
+    const int dst = 1 - src;
+    for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n_seg; k += gridDim.x * blockDim.x) {
+        const int32_t r = P.seg_rec[src][k];
+        const int32_t i = P.shadow_path[r];
+        const float4* r4 = reinterpret_cast<const float4*>(&P.seg_rays[src][k]);
+        const float4 ra = r4[0], rb = r4[1];
+        const f3 ro = mk(ra.x, ra.y, ra.z), rd = mk(rb.x, rb.y, rb.z);
+        const pg_hit hit = P.seg_hits[k];
+        const int med = P.seg_med[src][k];
+        f3 tr = mk(P.sh_tr[3 * r], P.sh_tr[3 * r + 1], P.sh_tr[3 * r + 2]);
+        if (med >= 0) {
+            SamplerState ss;
+            ss.index = P.sidx[i];
+            ss.dim = P.sdim[i];
+            const float tmax = hit.prim >= 0 ? hit.t : rb.w;
+            tr = tr * medium_tr(s.media[med], majorants(s, med, lds_medium, lds), s.halton, ss, ro,
+                                rd, tmax);
+            P.sdim[i] = ss.dim;
+        }
+        if (hit.prim >= 0) {
+            const int shape = s.prim_shape[hit.prim];
+            const ShapeDev& sh = s.shapes[shape];
+            if (sh.material >= 0) {
+                tr = mk(0.0f, 0.0f, 0.0f);
+            } else {
+                const pg_hit_uv uv = P.seg_uvs[k];
+                const Hit h = surface(s, hit.prim, hit.t, uv.u, uv.v, ro, rd);
+                const bool transition = sh.mi_inside != sh.mi_outside;
+                const int min_ = transition ? sh.mi_inside : med;
+                const int mout = transition ? sh.mi_outside : med;
+                // Interaction::SpawnRayTo(p2) (interaction.h:27-35)
+                const float* rec = P.sh_rec + 12 * (int64_t)r;
+                const f3 p2 = mk(rec[8], rec[9], rec[10]);
+                float len;
+                const f3 d2 = normalize(p2 - h.p, len);
+                const f3 o2 = offset_ray_origin(h.p, face_same_hemisphere(h.n, d2));
+                const int slot = atomicAdd(&P.counters[3], 1);
+                store_ray(&P.seg_rays[dst][slot], o2, d2, 0.0f, len * (1.0f - 1e-3f));
+                P.seg_rec[dst][slot] = r;
+                P.seg_med[dst][slot] = medium_for(d2, h.n, min_, mout);
+            }
+        }
+        P.sh_tr[3 * r + 0] = tr.x;
+        P.sh_tr[3 * r + 1] = tr.y;
+        P.sh_tr[3 * r + 2] = tr.z;
+    }
+}
+
+// EstimateDirect's tail (integrator.cpp:101-118) and `L += beta * ...`.
+__global__ __launch_bounds__(256) void k_resolve_gen(PathsDev P, int32_t n_rec) {
+    const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_rec)
+        return;
+    const f3 tr = mk(P.sh_tr[3 * r], P.sh_tr[3 * r + 1], P.sh_tr[3 * r + 2]);
+    if (is_black(tr))
+        return;
+    const int32_t i = P.shadow_path[r];
+    const float4* rec = reinterpret_cast<const float4*>(P.sh_rec + 12 * (int64_t)r);
+    const float4 a = rec[0], b = rec[1], c = rec[2];
+    const f3 f = mk(a.x, a.y, a.z), le = mk(b.x, b.y, b.z);
+    const float w = a.w, pdf = b.w;
+    const f3 ld = c.w != 0.0f ? tr * f * le / pdf : tr * f * w * le / pdf;
+    const f3 beta = mk(P.beta[3 * i], P.beta[3 * i + 1], P.beta[3 * i + 2]);
+    const f3 add = beta * ld;
+    P.L[3 * i + 0] += add.x;
+    P.L[3 * i + 1] += add.y;
+    P.L[3 * i + 2] += add.z;
+}
+
+// Direction sampling after the shadow ray: PhaseFunction::Sample at a medium
+// vertex (path.cpp:19-25), the material's Sample + Russian roulette at a
+// surface (path.cpp:66-86).
+__global__ __launch_bounds__(256) void k_shade_gen_b(SceneDev s, PathsDev P, int32_t n_b) {
+    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_b)
+        return;
+    const int32_t i = P.queue_b[k];
+    const float* v = P.vtx + 9 * (int64_t)i;
+    const f3 p = mk(v[0], v[1], v[2]), n = mk(v[3], v[4], v[5]), wi = mk(v[6], v[7], v[8]);
+    const int4 info = *reinterpret_cast<const int4*>(P.vinfo + 4 * (int64_t)i);
+    f3 beta = mk(P.beta[3 * i], P.beta[3 * i + 1], P.beta[3 * i + 2]);
+    SamplerState ss;
+    ss.index = P.sidx[i];
+    ss.dim = P.sdim[i];
+    int depth = P.depth[i];
+    bool alive = false;
+    f3 nextO, nextD;
+    int next_med = -1;
+    if (info.x == 1) {
+        float u0, u1;
+        get_2d(s.halton, ss, u0, u1);
+        f3 wo;
+        P.bsdf_pdf[i] = phase_sample(s.media[info.z].g, wi, u0, u1, wo);
+        nextD = wo;
+        nextO = offset_ray_origin(p, face_same_hemisphere(n, wo));
+        next_med = info.z;
+        alive = true;
+    } else {
+        const MaterialDev& m = s.materials[info.y];
+        const M3 n2w = frame_from(n);
+        const M3 w2n = inverse(n2w);
+        const f3 wiL = mul(w2n, wi);
+        const float u1 = get_1d(s.halton, ss);
+        float u2a, u2b;
+        get_2d(s.halton, ss, u2a, u2b);
+        BSDFSample bs;
+        if (material_sample(m, wiL, u1, u2a, u2b, bs)) {
+            bs.wo = mul(n2w, bs.wo);
+            if (!is_black(bs.f) && bs.pdf != 0.0f) {
+                beta = beta * (abs_dot(bs.wo, n) * bs.f / bs.pdf);
+                P.bsdf_pdf[i] = bs.pdf;
+                nextD = bs.wo;
+                nextO = offset_ray_origin(p, face_same_hemisphere(n, nextD));
+                next_med = medium_for(nextD, n, info.z, info.w);
+                alive = true;
+                if (depth > 2) {
+                    const float q = pstd::clamp(1.0f - luminance(beta), 0.05f, 1.0f);
+                    if (get_1d(s.halton, ss) < q)
+                        alive = false;
+                    else
+                        beta = beta / (1.0f - q);
+                }
+            }
+        }
+    }
+    depth++;
+    P.beta[3 * i + 0] = beta.x;
+    P.beta[3 * i + 1] = beta.y;
+    P.beta[3 * i + 2] = beta.z;
+    P.sdim[i] = ss.dim;
+    P.depth[i] = depth;
+    if (alive) {
+        const int slot = atomicAdd(&P.counters[0], 1);
+        P.queue_out[slot] = i;
+        P.medium[i] = next_med;
+        store_ray(&P.rays_out[slot], nextO, nextD, 0.0f, kFloatMax);
+    }
+}
+
 // Film::AddSample (film.h:26-45) for every path of a pixel, in sample order.
-__global__ void k_film(SceneDev s, PathsDev P, int32_t n_pixels, int32_t spb, float* film) {
+__global__ __launch_bounds__(256) void k_film(SceneDev s, PathsDev P, int32_t n_pixels, int32_t spb, float* film) {
     const int32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
     if (pixel >= n_pixels)
         return;
@@ -490,7 +862,7 @@ __global__ void k_film(SceneDev s, PathsDev P, int32_t n_pixels, int32_t spb, fl
     film[4 * own + 3] = acc[3];
 }
 
-__global__ void k_film_spill(PathsDev P, float* film) {
+__global__ __launch_bounds__(256) void k_film_spill(PathsDev P, float* film) {
     const int32_t n = min(*P.spill_count, P.spill_capacity);
     for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         const size_t pix = (size_t)__float_as_int(P.spill[5 * k]);
@@ -530,6 +902,11 @@ struct pg_scene {
     int64_t path_capacity = 0;
     double last_render_ms = 0.0;
     int64_t last_rays = 0;
+    // media
+    std::vector<MediumDev> media;
+    std::vector<std::vector<float>> med_density, med_majorant;
+    int lds_medium = -1;  // medium whose brick majorants the tracking kernels stage in LDS
+    bool general = false;
 
     ~pg_scene() {
         for (void* p : allocations)
@@ -540,6 +917,9 @@ struct pg_scene {
 };
 
 namespace {
+
+// Brick majorants staged in LDS by the tracking kernels: at most 32 KB.
+constexpr size_t kMaxLdsMajorants = 8192;
 
 std::runtime_error unsupported(const std::string& what) {
     return std::runtime_error("unsupported by the GPU path: " + what);
@@ -669,6 +1049,148 @@ void load_obj(const std::string& path, std::vector<float>& v, std::vector<uint32
     }
 }
 
+// LoadVolume (fileio.cpp:194-203): three int32 sizes, then the floats, x fastest.
+void load_volume(const std::string& path, std::vector<float>& density, int32_t size[3]) {
+    if (path.size() >= 11 && path.compare(path.size() - 11, 11, ".compressed") == 0)
+        throw unsupported("compressed volume files (\"" + path + "\"); use a raw volume");
+    std::ifstream f(path, std::ios::binary);
+    if (!f)
+        throw std::runtime_error("cannot open volume \"" + path + "\"");
+    f.read(reinterpret_cast<char*>(size), 3 * sizeof(int32_t));
+    if (!f || size[0] <= 0 || size[1] <= 0 || size[2] <= 0 ||
+        (int64_t)size[0] * size[1] * size[2] > ((int64_t)1 << 31))
+        throw std::runtime_error("bad volume header in \"" + path + "\"");
+    density.resize((size_t)size[0] * size[1] * size[2]);
+    f.read(reinterpret_cast<char*>(density.data()), (std::streamsize)(density.size() * sizeof(float)));
+    if (!f)
+        throw std::runtime_error("truncated volume \"" + path + "\"");
+}
+
+// Restates Inverse(mat4) (vecmath.h:1147-1172) for the affine w2m of a grid;
+// m[c][r] column-major as pine's Matrix4.
+void inverse4(const float m[4][4], float r[4][4]) {
+    float det = 0.0f;
+    for (int i = 0; i < 4; i++)
+        det += (m[(1 + i) % 4][0] * (m[(2 + i) % 4][1] * m[(3 + i) % 4][2] -
+                                     m[(3 + i) % 4][1] * m[(2 + i) % 4][2]) +
+                m[(2 + i) % 4][0] * (m[(3 + i) % 4][1] * m[(1 + i) % 4][2] -
+                                     m[(1 + i) % 4][1] * m[(3 + i) % 4][2]) +
+                m[(3 + i) % 4][0] * (m[(1 + i) % 4][1] * m[(2 + i) % 4][2] -
+                                     m[(2 + i) % 4][1] * m[(1 + i) % 4][2])) *
+               m[i % 4][3] * (i % 2 ? -1.0f : 1.0f);
+    if (det == 0.0f)
+        throw std::runtime_error("GridMedium: singular world-to-grid transform");
+    const float inv = 1.0f / det;
+    for (int v = 0; v < 4; v++)
+        for (int i = 0; i < 4; i++)
+            r[v][i] = (m[(1 + i) % 4][(1 + v) % 4] * (m[(2 + i) % 4][(2 + v) % 4] * m[(3 + i) % 4][(3 + v) % 4] -
+                                                      m[(3 + i) % 4][(2 + v) % 4] * m[(2 + i) % 4][(3 + v) % 4]) +
+                       m[(2 + i) % 4][(1 + v) % 4] * (m[(3 + i) % 4][(2 + v) % 4] * m[(1 + i) % 4][(3 + v) % 4] -
+                                                      m[(1 + i) % 4][(2 + v) % 4] * m[(3 + i) % 4][(3 + v) % 4]) +
+                       m[(3 + i) % 4][(1 + v) % 4] * (m[(1 + i) % 4][(2 + v) % 4] * m[(2 + i) % 4][(3 + v) % 4] -
+                                                      m[(2 + i) % 4][(2 + v) % 4] * m[(1 + i) % 4][(3 + v) % 4])) *
+                      ((v + i) % 2 ? 1.0f : -1.0f) * inv;
+}
+
+// CreateMedium (medium.cpp:185-196) with HomogeneousMedium / GridMedium::Create
+// (:162-183).  Extension: `majorantBricks: B` on a Grid tracks against one
+// majorant per B^3-voxel brick instead of the global one.
+void make_medium(pg_scene& S, const Params& p) {
+    std::string type = p.get_string("type");
+    MediumDev m{};
+    float a[3], sc[3];
+    p.get_vec3("sigma_a", a);
+    p.get_vec3("sigma_s", sc);
+    m.sigma_a = mk(a[0], a[1], a[2]);
+    m.sigma_s = mk(sc[0], sc[1], sc[2]);
+    m.sigma_t3 = m.sigma_a + m.sigma_s;
+    m.g = p.get_float("g", 0.0f);
+    std::vector<float> density, majorant;
+    if (type == "Grid") {
+        m.type = kMediumGrid;
+        std::string file = p.get_string("file");
+        if (!file.empty() && file[0] != '/')
+            file = S.dir + "/" + file;
+        load_volume(file, density, m.size);
+        std::string method = p.get_string("samplingMethod", "deltaTracking");
+        m.method = kDeltaTracking;
+        if (method == "rayMarching")
+            m.method = kRayMarching;
+        else if (method != "deltaTracking")
+            fprintf(stderr, "[GridMedium]Unknown sampling method \"%s\"\n", method.c_str());
+        float step = p.get_float("rayMarchingStepSize", 2.0f);
+        if (step < 1.0f)
+            step = 1.0f;
+        const int32_t mx = std::max(m.size[0], std::max(m.size[1], m.size[2]));
+        m.march_step = step / (float)mx;
+        m.interpolate = p.get_bool("interpolate", true) ? 1 : 0;
+        m.sigma_t = (m.sigma_a + m.sigma_s).x;
+        float maxd = 0.0f;
+        for (float d : density)
+            maxd = std::max(d, maxd);
+        m.inv_max_density = 1.0f / maxd;
+        m.ln2_inv_max = pstd::Ln2 * m.inv_max_density;
+        // w2m = Translate(0.5 - position) * Scale(1 / normalizedSize / scale)
+        float pos[3];
+        const float zero3[3] = {0.0f, 0.0f, 0.0f};
+        p.get_vec3("position", pos, zero3);
+        const float scale = p.get_float("scale", 1.0f);
+        const float rm = 1.0f / (float)mx, rs = 1.0f / scale;
+        float s3[3], t3[3];
+        for (int k = 0; k < 3; k++) {
+            s3[k] = (1.0f / ((float)m.size[k] * rm)) * rs;
+            t3[k] = 0.5f - pos[k];
+        }
+        m.w2m_s = mk(s3[0], s3[1], s3[2]);
+        m.w2m_t = mk(t3[0], t3[1], t3[2]);
+        float w2m[4][4] = {{s3[0], 0, 0, 0}, {0, s3[1], 0, 0}, {0, 0, s3[2], 0}, {t3[0], t3[1], t3[2], 1}};
+        float m2w[4][4];
+        inverse4(w2m, m2w);
+        for (int c = 0; c < 4; c++)
+            for (int r = 0; r < 3; r++)
+                m.m2w[3 * c + r] = m2w[c][r];
+        m.brick = p.get_int("majorantBricks", 0);
+        if (m.brick < 0)
+            throw std::runtime_error("GridMedium: majorantBricks must be >= 0");
+        if (m.brick > 0 && m.method == kDeltaTracking) {
+            // Brick b bounds the trilinear density of every point inside it:
+            // voxels b*B - 1 .. (b+1)*B (one voxel of halo on each side).
+            const int B = m.brick;
+            for (int k = 0; k < 3; k++)
+                m.nbricks[k] = (m.size[k] + B - 1) / B;
+            majorant.assign((size_t)m.nbricks[0] * m.nbricks[1] * m.nbricks[2], 0.0f);
+            for (int z = 0; z < m.size[2]; z++)
+                for (int y = 0; y < m.size[1]; y++)
+                    for (int x = 0; x < m.size[0]; x++) {
+                        const float d = density[(size_t)x + (size_t)m.size[0] * (y + (size_t)m.size[1] * z)];
+                        if (!(d > 0.0f))
+                            continue;
+                        int lo[3], hi[3];
+                        const int c[3] = {x, y, z};
+                        for (int k = 0; k < 3; k++) {
+                            lo[k] = std::max(0, (c[k] - 1) / B);
+                            hi[k] = std::min(m.nbricks[k] - 1, (c[k] + 1) / B);
+                        }
+                        for (int bz = lo[2]; bz <= hi[2]; bz++)
+                            for (int by = lo[1]; by <= hi[1]; by++)
+                                for (int bx = lo[0]; bx <= hi[0]; bx++) {
+                                    float& mj = majorant[(size_t)bx + (size_t)m.nbricks[0] * (by + (size_t)m.nbricks[1] * bz)];
+                                    mj = std::max(mj, d);
+                                }
+                    }
+        } else {
+            m.brick = 0;
+        }
+    } else {
+        if (type != "Homogeneous")
+            fprintf(stderr, "[Medium][Create]Unknown type \"%s\"\n", type.c_str());
+        m.type = kMediumHomogeneous;
+    }
+    S.media.push_back(m);
+    S.med_density.push_back(std::move(density));
+    S.med_majorant.push_back(std::move(majorant));
+}
+
 void build_scene(pg_scene& S) {
     const Params& root = S.root;
     // Integrator block (integrator.cpp:20-56)
@@ -695,8 +1217,20 @@ void build_scene(pg_scene& S) {
         mat_index[p.get_string("name")] = (int)S.materials.size();
         S.materials.push_back(make_material(p));
     }
-    if (!root.all("Medium").empty())
-        throw unsupported("participating media in a Path scene");
+    // Media by name (fileio.cpp:262-263).
+    std::map<std::string, int> med_index;
+    for (const Params& p : root.all("Medium")) {
+        med_index[p.get_string("name")] = (int)S.media.size();
+        make_medium(S, p);
+    }
+    auto find_medium = [&](const std::string& name) {
+        auto it = med_index.find(name);
+        if (it == med_index.end())
+            throw std::runtime_error("medium \"" + name + "\" is not found");
+        return it->second;
+    };
+    bool uses_media = false;
+    S.dev.has_env = 0;
     // Lights from Light blocks (fileio.cpp:264-265).
     for (const Params& p : root.all("Light")) {
         std::string t = p.get_string("type");
@@ -710,6 +1244,23 @@ void build_scene(pg_scene& S) {
             l.type = kLightPoint;
             p.get_vec3("position", a);
             l.v = mk(a[0], a[1], a[2]);
+        } else if (t == "Environment") {
+            // EnvironmentLight::Create (light.cpp:157-168): Sky only.
+            const Params& ep = p.sub("environment");
+            std::string et = ep.get_string("type");
+            if (et != "Sky")
+                throw unsupported("environment light \"" + et + "\" (Sky only)");
+            ep.get_vec3("sunDirection", a);
+            ep.get_vec3("sunColor", c);
+            l.type = kLightSky;
+            l.v = normalize(mk(a[0], a[1], a[2]));
+            l.color = mk(c[0], c[1], c[2]);
+            // scene->envLight is the last environment light (fileio.cpp:272-274).
+            S.dev.has_env = 1;
+            S.dev.sky_dir = l.v;
+            S.dev.sky_color = l.color;
+            S.lights.push_back(l);
+            continue;
         } else {
             throw unsupported("light type \"" + t + "\"");
         }
@@ -735,8 +1286,20 @@ void build_scene(pg_scene& S) {
                 throw std::runtime_error("material \"" + mname + "\" is not found");
             sh.material = it->second;
         }
-        if (p.has_value("medium") || p.has_value("mediumInside") || p.has_value("mediumOutside"))
-            throw unsupported("media on shapes");
+        // geometry.cpp:392-409: "medium" and "mediumInside" set the inside,
+        // "mediumOutside" the outside.
+        sh.mi_inside = sh.mi_outside = -1;
+        std::string mn;
+        if (p.try_string("medium", mn))
+            sh.mi_inside = find_medium(mn);
+        if (p.try_string("mediumInside", mn))
+            sh.mi_inside = find_medium(mn);
+        if (p.try_string("mediumOutside", mn))
+            sh.mi_outside = find_medium(mn);
+        if (sh.mi_inside >= 0 || sh.mi_outside >= 0)
+            uses_media = true;
+        if (sh.material < 0)
+            S.general = true;
         float pos[3], e0[3], e1[3];
         int32_t first = 0;
         const int shape_index = (int)S.shapes.size();
@@ -829,9 +1392,6 @@ void build_scene(pg_scene& S) {
         l.shape = si;
         S.lights.push_back(l);
     }
-    for (const Params& p : root.all("Light"))
-        if (p.get_string("type") == "Environment")
-            throw unsupported("environment lights");
 
     // PowerLightSampler (lightsampler.cpp:20-25): Power().y() per light.  An
     // area light's Power evaluates its emission with a zero normal, which
@@ -859,8 +1419,14 @@ void build_scene(pg_scene& S) {
     std::string ctype = cp.get_string("type");
     if (ctype != "ThinLen")
         fprintf(stderr, "[Camera][Create]Unknown type \"%s\"\n", ctype.c_str());
-    if (cp.has_value("medium"))
-        throw unsupported("camera media");
+    S.dev.camera_medium = -1;
+    std::string cmed;
+    if (cp.try_string("medium", cmed)) {
+        S.dev.camera_medium = find_medium(cmed);
+        uses_media = true;
+    }
+    if (uses_media)
+        S.general = true;
     const Params& fp = cp.sub("film");
     int size[2];
     const int defsize[2] = {720, 480};
@@ -1054,6 +1620,24 @@ int finish_scene(pg_scene& S, const pg_build_options* opts) {
         (rc = upload(S, S.materials, &S.dev.materials)) || (rc = upload(S, S.lights, &S.dev.lights)) ||
         (rc = upload(S, S.light_func, &S.dev.light_func)) || (rc = upload(S, S.light_cdf, &S.dev.light_cdf)))
         return rc;
+    // Media: density grids and brick majorants, then the MediumDev table.
+    S.lds_medium = -1;
+    int bricked = 0;
+    for (size_t k = 0; k < S.media.size(); k++) {
+        MediumDev& m = S.media[k];
+        if ((rc = upload(S, S.med_density[k], &m.density)) || (rc = upload(S, S.med_majorant[k], &m.majorant)))
+            return rc;
+        if (m.brick > 0) {
+            bricked++;
+            if (S.med_majorant[k].size() <= kMaxLdsMajorants)
+                S.lds_medium = (int)k;
+        }
+    }
+    if (bricked != 1)
+        S.lds_medium = -1;
+    if ((rc = upload(S, S.media, &S.dev.media)))
+        return rc;
+    S.dev.n_media = (int32_t)S.media.size();
     if ((rc = dalloc(S, (int64_t)S.dev.width * S.dev.height * 4, &S.d_film)))
         return rc;
     PG_HIP(hipMemset(S.d_film, 0, (size_t)S.dev.width * S.dev.height * 4 * sizeof(float)));
@@ -1070,7 +1654,7 @@ int ensure_paths(pg_scene& S, int64_t n) {
         (rc = dalloc(S, n, &P.depth)) || (rc = dalloc(S, 2 * n, &P.pfilm)) ||
         (rc = dalloc(S, n, &P.queue_in)) || (rc = dalloc(S, n, &P.queue_out)) ||
         (rc = dalloc(S, n, &P.rays_in)) || (rc = dalloc(S, n, &P.rays_out)) ||
-        (rc = dalloc(S, n, &P.hits)) || (rc = dalloc(S, n, &P.uvs)) || (rc = dalloc(S, 2, &P.counters)) ||
+        (rc = dalloc(S, n, &P.hits)) || (rc = dalloc(S, n, &P.uvs)) || (rc = dalloc(S, 4, &P.counters)) ||
         (rc = dalloc(S, n, &P.shadow_rays)) || (rc = dalloc(S, n, &P.shadow_path)) ||
         (rc = dalloc(S, 3 * n, &P.shadow_contrib)) || (rc = dalloc(S, n, &P.occluded)) ||
         (rc = dalloc(S, 1, &P.spill_count)))
@@ -1078,6 +1662,16 @@ int ensure_paths(pg_scene& S, int64_t n) {
     P.spill_capacity = (int32_t)std::min<int64_t>(n, 1 << 20);
     if ((rc = dalloc(S, 5 * (int64_t)P.spill_capacity, &P.spill)))
         return rc;
+    if (S.general) {
+        if ((rc = dalloc(S, n, &P.medium)) || (rc = dalloc(S, 9 * n, &P.vtx)) ||
+            (rc = dalloc(S, 4 * n, &P.vinfo)) || (rc = dalloc(S, n, &P.queue_b)) ||
+            (rc = dalloc(S, 12 * n, &P.sh_rec)) || (rc = dalloc(S, 3 * n, &P.sh_tr)) ||
+            (rc = dalloc(S, n, &P.seg_rays[0])) || (rc = dalloc(S, n, &P.seg_rays[1])) ||
+            (rc = dalloc(S, n, &P.seg_rec[0])) || (rc = dalloc(S, n, &P.seg_rec[1])) ||
+            (rc = dalloc(S, n, &P.seg_med[0])) || (rc = dalloc(S, n, &P.seg_med[1])) ||
+            (rc = dalloc(S, n, &P.seg_hits)) || (rc = dalloc(S, n, &P.seg_uvs)))
+            return rc;
+    }
     S.path_capacity = n;
     return PG_OK;
 }
@@ -1142,6 +1736,58 @@ extern "C" int pg_scene_get_info(const pg_scene* S, pg_scene_info* info) {
     info->n_prims = (int64_t)S->prim_shape.size();
     info->last_render_ms = S->last_render_ms;
     info->last_rays = S->last_rays;
+    info->n_media = (int32_t)S->media.size();
+    info->general_mode = S->general ? 1 : 0;
+    info->medium_steps = 0;
+    return PG_OK;
+}
+
+constexpr int kMaxShadowSegments = 1024;
+constexpr int kMaxBounces = 1 << 16;
+
+// Grid of a grid-stride tracking kernel: enough workgroups to fill the chip,
+// each staging the brick majorants once.
+static int tracking_grid(int64_t n, int block) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>((n + block - 1) / block, 4096));
+}
+
+// One bounce in general mode; `active` becomes the next queue's length.
+static int bounce_general(pg_scene* S, PathsDev& Q, int32_t& active, hipStream_t st, int64_t& rays) {
+    const int B = 256;
+    const size_t lds = S->lds_medium >= 0 ? S->med_majorant[S->lds_medium].size() * sizeof(float) : 0;
+    hipLaunchKernelGGL(k_shade_gen, dim3(tracking_grid(active, B)), dim3(B), lds, st, S->dev, Q, active,
+                       S->lds_medium);
+    int32_t counts[4];
+    PG_HIP(hipMemcpyAsync(counts, Q.counters, sizeof(counts), hipMemcpyDeviceToHost, st));
+    PG_HIP(hipStreamSynchronize(st));
+    rays += active;
+    const int32_t n_rec = counts[1], n_b = counts[2];
+    // IntersectTr: trace each segment, track its medium, continue through
+    // material-less surfaces until every record is resolved.
+    int32_t n_seg = n_rec;
+    int src = 0;
+    // A shadow ray crosses each material-less surface once; the bound only
+    // guards against a surface that re-hits itself.
+    for (int pass = 0; n_seg > 0 && pass < kMaxShadowSegments; pass++) {
+        rays += n_seg;
+        PG_HIP(hipMemsetAsync(Q.counters + 3, 0, sizeof(int32_t), st));
+        if (int rc = pg_trace_closest(S->accel, Q.seg_rays[src], n_seg, Q.seg_hits, Q.seg_uvs, st))
+            return rc;
+        hipLaunchKernelGGL(k_shadow_tr, dim3(tracking_grid(n_seg, B)), dim3(B), lds, st, S->dev, Q, n_seg,
+                           src, S->lds_medium);
+        int32_t next = 0;
+        PG_HIP(hipMemcpyAsync(&next, Q.counters + 3, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        PG_HIP(hipStreamSynchronize(st));
+        n_seg = next;
+        src = 1 - src;
+    }
+    if (n_rec > 0)
+        hipLaunchKernelGGL(k_resolve_gen, dim3((n_rec + B - 1) / B), dim3(B), 0, st, Q, n_rec);
+    if (n_b > 0)
+        hipLaunchKernelGGL(k_shade_gen_b, dim3((n_b + B - 1) / B), dim3(B), 0, st, S->dev, Q, n_b);
+    PG_HIP(hipMemcpyAsync(counts, Q.counters, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    PG_HIP(hipStreamSynchronize(st));
+    active = counts[0];
     return PG_OK;
 }
 
@@ -1176,24 +1822,31 @@ extern "C" int pg_render(pg_scene* S, const pg_render_options* opts, void* strea
         hipLaunchKernelGGL(k_raygen, dim3((np + B - 1) / B), dim3(B), 0, st, S->dev, P, np, nb, s0);
         int32_t active = np;
         PathsDev Q = P;
-        for (int bounce = 0; active > 0; bounce++) {
-            PG_HIP(hipMemsetAsync(Q.counters, 0, 2 * sizeof(int32_t), st));
+        // Depth bounds every path except those crossing material-less surfaces,
+        // which pine does not count (path.cpp:42-46); the bound guards those.
+        for (int bounce = 0; active > 0 && bounce < kMaxBounces; bounce++) {
+            PG_HIP(hipMemsetAsync(Q.counters, 0, 4 * sizeof(int32_t), st));
             if (int rc = pg_trace_closest(S->accel, Q.rays_in, active, Q.hits, Q.uvs, st))
                 return rc;
-            hipLaunchKernelGGL(k_shade, dim3((active + B - 1) / B), dim3(B), 0, st, S->dev, Q, active);
-            int32_t counts[2];
-            PG_HIP(hipMemcpyAsync(counts, Q.counters, sizeof(counts), hipMemcpyDeviceToHost, st));
-            PG_HIP(hipStreamSynchronize(st));
-            rays += active + counts[1];
-            if (counts[1] > 0) {
-                if (int rc = pg_trace_occluded(S->accel, Q.shadow_rays, counts[1], Q.occluded, st))
+            if (!S->general) {
+                hipLaunchKernelGGL(k_shade, dim3((active + B - 1) / B), dim3(B), 0, st, S->dev, Q, active);
+                int32_t counts[2];
+                PG_HIP(hipMemcpyAsync(counts, Q.counters, sizeof(counts), hipMemcpyDeviceToHost, st));
+                PG_HIP(hipStreamSynchronize(st));
+                rays += active + counts[1];
+                if (counts[1] > 0) {
+                    if (int rc = pg_trace_occluded(S->accel, Q.shadow_rays, counts[1], Q.occluded, st))
+                        return rc;
+                    hipLaunchKernelGGL(k_resolve_shadow, dim3((counts[1] + B - 1) / B), dim3(B), 0, st,
+                                       Q, counts[1]);
+                }
+                active = counts[0];
+            } else {
+                if (int rc = bounce_general(S, Q, active, st, rays))
                     return rc;
-                hipLaunchKernelGGL(k_resolve_shadow, dim3((counts[1] + B - 1) / B), dim3(B), 0, st,
-                                   Q, counts[1]);
             }
             std::swap(Q.queue_in, Q.queue_out);
             std::swap(Q.rays_in, Q.rays_out);
-            active = counts[0];
         }
         PG_HIP(hipMemsetAsync(P.spill_count, 0, sizeof(int32_t), st));
         hipLaunchKernelGGL(k_film, dim3((npix + B - 1) / B), dim3(B), 0, st, S->dev, P, (int32_t)npix,

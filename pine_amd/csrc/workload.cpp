@@ -60,6 +60,18 @@ float fbm5(f3 p, uint64_t seed) {
     return sum / norm;
 }
 
+// fBm of value noise with `octaves` octaves, amplitudes 1/2, 1/4, ... (so the
+// sum stays inside (-1, 1)) and lacunarity 2.
+float fbm(float x, float y, float z, int octaves, float freq, uint64_t seed) {
+    float sum = 0.0f, amp = 0.5f;
+    for (int o = 0; o < octaves; o++) {
+        sum += amp * value_noise(x * freq, y * freq, z * freq, seed + (uint64_t)o * 7919u);
+        amp *= 0.5f;
+        freq *= 2.0f;
+    }
+    return sum;
+}
+
 f3 normalize(f3 v) {
     float l = std::sqrt(dot(v, v));
     return v / l;
@@ -224,5 +236,32 @@ extern "C" int pg_gen_rays(const float* h_vertices, int64_t n_vertices, const ui
     }
     for (auto& t : pool)
         t.join();
+    return PG_OK;
+}
+
+// Heterogeneous-medium density grid: voxel (x, y, z) holds
+// max(0, fbm6(centre) - threshold) * scale, x fastest, the order pine's raw
+// volume files store (fileio.cpp:194-203).
+extern "C" int pg_gen_density_grid(int32_t nx, int32_t ny, int32_t nz, float base_freq,
+                                   float threshold, float scale, uint64_t seed, float* h_density) {
+    if (nx < 1 || ny < 1 || nz < 1 || (int64_t)nx * ny * nz > (int64_t)1 << 31 || !h_density)
+        return pg_fail(PG_E_ARG, "pg_gen_density_grid: bad size or null output");
+    const int64_t plane = (int64_t)nx * ny;
+    const int nthreads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nthreads; t++)
+        pool.emplace_back([=]() {
+            for (int32_t z = t; z < nz; z += nthreads)
+                for (int32_t y = 0; y < ny; y++)
+                    for (int32_t x = 0; x < nx; x++) {
+                        float px = ((float)x + 0.5f) / (float)nx;
+                        float py = ((float)y + 0.5f) / (float)ny;
+                        float pz = ((float)z + 0.5f) / (float)nz;
+                        float d = fbm(px, py, pz, 6, base_freq, seed) - threshold;
+                        h_density[z * plane + (int64_t)y * nx + x] = d > 0.0f ? d * scale : 0.0f;
+                    }
+        });
+    for (auto& th : pool)
+        th.join();
     return PG_OK;
 }

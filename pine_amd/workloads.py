@@ -253,3 +253,113 @@ Shape: Rect{{
     material: lamp
 }}
 """
+
+
+# --- heterogeneous medium ------------------------------------------------------
+
+# The volume benchmark: a 256^3 fBm density grid, max(0, fbm6 - 0.3) * 40, with
+# 16^3-voxel majorant bricks, HG g = 0.85, single-scattering albedo 0.99.
+CLOUD_GRID = 256
+CLOUD_BRICK = 16
+
+
+def cloud_scene_text(volume_file, width=1024, height=1024, spp=256, max_depth=64, bricks=0,
+                     sigma_a=0.01, sigma_s=0.99, g=0.85, scale=2.0, passthrough=False,
+                     output="cloud.bmp"):
+    """A density grid seen from outside, the camera inside the medium as in
+    pine's scenes/cloud.txt (the grid bounds the density), a diffuse ground and
+    pine's Sky environment light (its sun is sampled as a delta light).
+    `bricks` > 0 adds the brick-majorant extension parameter.  `passthrough`
+    adds a material-less rect in front of the grid (surfaces that rays and
+    shadow rays cross, path.cpp:42-46 and integrator.cpp:80-84)."""
+    out = [
+        "Integrator: Path{",
+        f"    maxDepth: {max_depth}",
+        "    sampler: Halton{",
+        f"        samplesPerPixel: {spp}",
+        "    }",
+        "}",
+        "Camera: ThinLen{",
+        "    film{",
+        f"        outputFileName: {output}",
+        f"        size: {width} {height}",
+        "        filter: LanczosSinc{",
+        "            radius: 0.5",
+        "        }",
+        "    }",
+        "    from: 0.3 0.4 3.6",
+        "    to: 0 0 0",
+        "    fov: 0.7",
+        "    medium: cloud",
+        "}",
+        "Material ground: Layered{",
+        "    layer0: Diffuse{",
+        "        albedo: 0.35 0.3 0.25",
+        "    }",
+        "}",
+        "Light: Environment{",
+        "    environment: Sky{",
+        "        sunDirection: -1 6 1",
+        "        sunColor: 1 1 1",
+        "    }",
+        "}",
+        "Shape: Rect{",
+        "    position: 0 -1.2 0",
+        "    ex: 0 0 100",
+        "    ey: 100 0 0",
+        "    material: ground",
+        "}",
+        "Medium cloud: Grid{",
+        f"    file: {volume_file}",
+        f"    sigma_a: {_fmt(sigma_a)} {_fmt(sigma_a)} {_fmt(sigma_a)}",
+        f"    sigma_s: {_fmt(sigma_s)} {_fmt(sigma_s)} {_fmt(sigma_s)}",
+        f"    g: {_fmt(g)}",
+        "    position: 0 0 0",
+        f"    scale: {_fmt(scale)}",
+    ]
+    if bricks:
+        out.append(f"    majorantBricks: {bricks}")
+    out.append("}")
+    if passthrough:
+        out += ["Shape: Rect{", "    position: 0.2 0.1 1.6", "    ex: 0.8 0 0", "    ey: 0 0.6 0.1",
+                "}"]
+    return "\n".join(out) + "\n"
+
+
+def slab_scene_text(volume_file, width=48, height=48, spp=256, sigma=1.0, albedo=0.0,
+                    view="through", bricks=0):
+    """Closed-form transmittance scenes on a grid of constant density, scale 1
+    (the grid is the cube [-0.5, 0.5]^3, model distance = world distance):
+      view="through": the camera looks along -z through the cube at an emissive
+        wall (Le = 1); with albedo 0 a pixel's expectation is exp(-sigma * d * len)
+        (delta tracking, GridMedium::Sample);
+      view="shadow": a directional light straight down casts the cube's
+        shadow on a white ground seen from the side; direct light only
+        (maxDepth 2), so shadowed / lit radiance = exp(-sigma * d * 1)
+        (ratio tracking, GridMedium::Tr)."""
+    head = ["Integrator: Path{", "    maxDepth: " + ("2" if view == "shadow" else "8"),
+            "    sampler: Halton{", f"        samplesPerPixel: {spp}", "    }", "}"]
+    med = ["Medium fog: Grid{", f"    file: {volume_file}",
+           f"    sigma_a: {_fmt(sigma * (1 - albedo))} {_fmt(sigma * (1 - albedo))} {_fmt(sigma * (1 - albedo))}",
+           f"    sigma_s: {_fmt(sigma * albedo)} {_fmt(sigma * albedo)} {_fmt(sigma * albedo)}",
+           "    g: 0", "    position: 0 0 0", "    scale: 1"]
+    if bricks:
+        med.append(f"    majorantBricks: {bricks}")
+    med.append("}")
+    film = ["    film{", "        outputFileName: slab.bmp", f"        size: {width} {height}",
+            "        filter: Box{", "            radius: 0.5", "        }", "    }"]
+    if view == "through":
+        cam = ["Camera: ThinLen{", *film, "    from: 0 0 3", "    to: 0 0 0", "    fov: 0.25",
+               "    medium: fog", "}"]
+        scene = ["Material wall: Emissive{", "    color: 1 1 1", "}",
+                 "Shape: Rect{", "    position: 0 0 -2", "    ex: 20 0 0", "    ey: 0 20 0",
+                 "    material: wall", "}"]
+    else:
+        # The camera stays below the cube, so only shadow rays cross it.
+        cam = ["Camera: ThinLen{", *film, "    from: 0 -0.7 3", "    to: 0 -1 0",
+               "    fov: 0.9", "    medium: fog", "}"]
+        scene = ["Material white: Layered{", "    layer0: Diffuse{", "        albedo: 1 1 1", "    }",
+                 "}", "Light: Directional{", "    direction: 0 1 0", "    color: 1 1 1", "}",
+                 "Shape: Rect{", "    position: 0 -1 0", "    ex: 0 0 40", "    ey: 40 0 0",
+                 "    material: white", "}"]
+    return "\n".join(head + cam + scene + med) + "\n"

@@ -19,6 +19,12 @@ Cases
               96x96, 256 spp, maxDepth 8, rendered by pine's PathIntegrator
   render_mixed    point + directional + area lights, layered materials, a
               rough dielectric, an OBJ mesh, thin lens, Box filter (96x72, 256 spp)
+  render_cloud    a 32^3 fBm density grid (GridMedium, delta + ratio tracking,
+              HG g = 0.85), camera in the medium, Sky environment light, a
+              material-less rect crossed by paths and shadow rays
+              (64x64, 512 spp, maxDepth 16)
+  slab_through, slab_shadow   constant-density grid scenes whose expectation
+              has a closed form (transmittance exp(-sigma d L)), 48x48, 1024 spp
 
 Trace fixtures store ray *indices* into the deterministic ray generator plus a
 checksum of the selected rays, and pine's answers: hit shape, prim within the
@@ -226,6 +232,39 @@ def case_render_mixed():
     print("render_mixed:", line, flush=True)
 
 
+CLOUD_GRID_SMALL = dict(n=32, base_freq=2.0, threshold=0.3, scale=40.0, seed=3)
+
+
+def cloud_volume(directory):
+    d = pa.gen_density_grid(**CLOUD_GRID_SMALL)
+    pa.write_volume(os.path.join(directory, "cloud32.vol"), d)
+
+
+def slab_volume(directory, density=1.0):
+    pa.write_volume(os.path.join(directory, "const8.vol"), np.full((8, 8, 8), density, np.float32))
+
+
+def case_render_cloud():
+    text = W.cloud_scene_text("cloud32.vol", width=64, height=64, spp=512, max_depth=16,
+                              passthrough=True)
+    with tempfile.TemporaryDirectory() as d:
+        cloud_volume(d)
+        film, line = run_render(text, d)
+    np.savez_compressed(os.path.join(OUT, "render_cloud.npz"), scene=np.bytes_(text), film=film,
+                        harness=np.bytes_(line))
+    print("render_cloud:", line, flush=True)
+
+
+def case_slab(view):
+    text = W.slab_scene_text("const8.vol", view=view, spp=1024, sigma=1.0)
+    with tempfile.TemporaryDirectory() as d:
+        slab_volume(d)
+        film, line = run_render(text, d)
+    np.savez_compressed(os.path.join(OUT, f"slab_{view}.npz"), scene=np.bytes_(text), film=film,
+                        harness=np.bytes_(line))
+    print(f"slab_{view}:", line, flush=True)
+
+
 CASES = {
     "trace_5k": lambda: case_trace("trace_5k", W.ICOSPHERE_5K, W.RAYS_SMALL),
     "trace_82k": lambda: case_trace("trace_82k", W.ICOSPHERE_82K, W.RAYS_SMALL),
@@ -236,6 +275,9 @@ CASES = {
     "halton": case_halton,
     "render_spheres": case_render_spheres,
     "render_mixed": case_render_mixed,
+    "render_cloud": case_render_cloud,
+    "slab_through": lambda: case_slab("through"),
+    "slab_shadow": lambda: case_slab("shadow"),
 }
 
 if __name__ == "__main__":

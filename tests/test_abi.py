@@ -96,3 +96,29 @@ def test_generator_rejects_bad_arguments(pg):
     nv, nt = ctypes.c_int64(), ctypes.c_int64()
     assert L.pg_gen_icosphere(0, ctypes.c_float(0.1), 1, 0, None, ctypes.byref(nv), None,
                               ctypes.byref(nt)) == -1
+
+
+def test_density_grid_generator_is_deterministic_and_sparse(pg):
+    """The volume workload (and the render_cloud fixture's grid) is a pure
+    function of its parameters."""
+    import hashlib
+    a = pg.gen_density_grid(32, base_freq=2.0)
+    b = pg.gen_density_grid(32, base_freq=2.0)
+    assert np.array_equal(a, b)
+    assert a.dtype == np.float32 and a.shape == (32, 32, 32)
+    assert (a >= 0).all() and 0.02 < (a > 0).mean() < 0.3 and a.max() < 40 * 0.7
+    assert not np.array_equal(a, pg.gen_density_grid(32, base_freq=2.0, seed=4))
+    # Voxels at threshold 0 and scale 1 are the raw fBm clipped at zero.
+    raw = pg.gen_density_grid(16, threshold=0.0, scale=1.0)
+    assert raw.max() < 1.0
+    h = hashlib.sha256(a.tobytes()).hexdigest()
+    print("cloud32 sha256", h)
+
+
+def test_volume_file_layout(pg, tmp_path):
+    d = np.arange(2 * 3 * 4, dtype=np.float32).reshape(4, 3, 2)  # [z, y, x]
+    pg.write_volume(str(tmp_path / "v.vol"), d)
+    raw = (tmp_path / "v.vol").read_bytes()
+    assert np.frombuffer(raw[:12], np.int32).tolist() == [2, 3, 4]
+    vals = np.frombuffer(raw[12:], np.float32)
+    assert vals[1] == d[0, 0, 1] and vals[2] == d[0, 1, 0] and vals[6] == d[1, 0, 0]

@@ -172,7 +172,7 @@ typedef struct pg_scene_info {
     int64_t last_rays;       /* rays traced by the last pg_render (extension + shadow) */
     int32_t n_media;         /* Medium blocks */
     int32_t general_mode;    /* 1: media or material-less shapes (split bounce) */
-    int64_t medium_steps;    /* reserved, 0 */
+    int64_t medium_steps;    /* density lookups (8 voxel taps each) of the last render */
 } pg_scene_info;
 int pg_scene_get_info(const pg_scene* s, pg_scene_info* info);
 

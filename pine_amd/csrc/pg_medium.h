@@ -241,7 +241,7 @@ struct MediumEvent {
 // Medium::Sample (medium.h:70-73) for a ray of the path; `maj` holds the brick
 // majorants of grid media in brick mode (LDS copy or global).
 PG_HD MediumEvent medium_sample(const MediumDev& m, const float* maj, const HaltonDev& hs,
-                                SamplerState& ss, f3 o, f3 d, float tmax) {
+                                SamplerState& ss, f3 o, f3 d, float tmax, int& lookups) {
     MediumEvent ev;
     ev.scattered = false;
     ev.weight = mk(1.0f, 1.0f, 1.0f);
@@ -279,6 +279,7 @@ PG_HD MediumEvent medium_sample(const MediumDev& m, const float* maj, const Halt
         const float jitter = stepSize * (get_1d(hs, ss) - 0.5f);
         float t = t0, sum = 0.0f;
         while (t < t1) {
+            lookups++;
             sum += grid_density(m, ray_at(r, t)) * m.sigma_t * stepSize;
             if (sum >= lnXi) {
                 ev.p = to_world(m, ray_at(r, t + jitter));
@@ -297,6 +298,7 @@ PG_HD MediumEvent medium_sample(const MediumDev& m, const float* maj, const Halt
             if (!(t < t1))  // also ends on NaN, where pine would loop forever
                 return ev;
             const f3 pm = ray_at(r, t);
+            lookups++;
             if (grid_density(m, pm) * m.inv_max_density > get_1d(hs, ss)) {
                 ev.p = to_world(m, pm);
                 ev.scattered = true;
@@ -320,6 +322,7 @@ PG_HD MediumEvent medium_sample(const MediumDev& m, const float* maj, const Halt
                 if (!(t < texit))
                     break;
                 const f3 pm = ray_at(r, t);
+                lookups++;
                 if (grid_density(m, pm) * inv_mj > get_1d(hs, ss)) {
                     ev.p = to_world(m, pm);
                     ev.scattered = true;
@@ -338,7 +341,7 @@ PG_HD MediumEvent medium_sample(const MediumDev& m, const float* maj, const Halt
 
 // Medium::Tr (medium.h:66-69): transmittance over [0, tmax] of the ray.
 PG_HD f3 medium_tr(const MediumDev& m, const float* maj, const HaltonDev& hs, SamplerState& ss,
-                   f3 o, f3 d, float tmax) {
+                   f3 o, f3 d, float tmax, int& lookups) {
     if (m.type == kMediumHomogeneous) {
         // HomogeneousMedium::Tr (medium.cpp:33-35): Exp(-ray.tmax * sigma_t)
         return mk(pstd::exp(-tmax * m.sigma_t3.x), pstd::exp(-tmax * m.sigma_t3.y),
@@ -355,6 +358,7 @@ PG_HD f3 medium_tr(const MediumDev& m, const float* maj, const HaltonDev& hs, Sa
         const float jitter = stepSize * (get_1d(hs, ss) - 0.5f);
         float t = t0;
         while (t < t1) {
+            lookups++;
             tr += grid_density(m, ray_at(r, t + jitter)) * m.sigma_t * stepSize;
             t += stepSize;
         }
@@ -369,6 +373,7 @@ PG_HD f3 medium_tr(const MediumDev& m, const float* maj, const HaltonDev& hs, Sa
             t = track_step(m, t, get_1d(hs, ss));
             if (!(t < t1))
                 break;
+            lookups++;
             const float density = grid_density(m, ray_at(r, t));
             tr *= 1.0f - max0(density * m.inv_max_density, 0.0f);
             if (tr < rrThreshold) {
@@ -393,6 +398,7 @@ PG_HD f3 medium_tr(const MediumDev& m, const float* maj, const HaltonDev& hs, Sa
                 t = t - pstd::log(1.0f - get_1d(hs, ss)) / rate;
                 if (!(t < texit))
                     break;
+                lookups++;
                 const float density = grid_density(m, ray_at(r, t));
                 tr *= 1.0f - max0(density * inv_mj, 0.0f);
                 if (tr < rrThreshold) {

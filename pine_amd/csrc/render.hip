@@ -135,6 +135,7 @@ struct PathsDev {
     int32_t* seg_med[2];
     pg_hit* seg_hits;
     pg_hit_uv* seg_uvs;
+    unsigned long long* lookups;  // density lookups of the tracking kernels (8 taps each)
 };
 
 }  // namespace pg
@@ -500,6 +501,15 @@ __global__ __launch_bounds__(256) void k_resolve_shadow(PathsDev P, int32_t n_sh
 //   k_resolve_gen L += beta * tr * f * w * Le / pdf
 //   k_shade_gen_b BSDF or phase sample, Russian roulette -> next ray
 
+// Adds a thread's density-lookup count to the global counter: one atomic per
+// wave after a cross-lane sum.  Every lane of the block reaches this call.
+__device__ void add_lookups(unsigned long long* counter, int n) {
+    for (int off = 32; off > 0; off >>= 1)
+        n += __shfl_down(n, off, 64);
+    if ((threadIdx.x & 63) == 0 && n > 0)
+        atomicAdd(counter, (unsigned long long)n);
+}
+
 // Brick majorants of medium m: the LDS copy when this launch staged it.
 __device__ __forceinline__ const float* majorants(const SceneDev& s, int m, int lds_medium,
                                                   const float* lds) {
@@ -581,6 +591,7 @@ __device__ void save_vertex(PathsDev& P, int32_t i, int kind, f3 p, f3 n, f3 wi,
 __global__ __launch_bounds__(256) void k_shade_gen(SceneDev s, PathsDev P, int32_t n_active, int32_t lds_medium) {
     extern __shared__ float lds_maj[];
     const float* lds = stage_majorants(s, lds_medium, lds_maj);
+    int lookups = 0;
     for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n_active;
          k += gridDim.x * blockDim.x) {
         const int32_t i = P.queue_in[k];
@@ -605,7 +616,7 @@ __global__ __launch_bounds__(256) void k_shade_gen(SceneDev s, PathsDev P, int32
         ev.scattered = false;
         if (med >= 0) {
             ev = medium_sample(s.media[med], majorants(s, med, lds_medium, lds), s.halton, ss, ro,
-                               rd, tmax);
+                               rd, tmax, lookups);
             beta = beta * ev.weight;
         }
         if (ev.scattered) {
@@ -663,6 +674,7 @@ __global__ __launch_bounds__(256) void k_shade_gen(SceneDev s, PathsDev P, int32
             store_ray(&P.rays_out[slot], nextO, nextD, 0.0f, kFloatMax);
         }
     }
+    add_lookups(P.lookups, lookups);
 }
 
 // One segment of IntersectTr (integrator.cpp:68-87): transmittance of the
@@ -672,6 +684,7 @@ __global__ __launch_bounds__(256) void k_shadow_tr(SceneDev s, PathsDev P, int32
     extern __shared__ float lds_maj[];
     const float* lds = stage_majorants(s, lds_medium, lds_maj);
     const int dst = 1 - src;
+    int lookups = 0;
     for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n_seg; k += gridDim.x * blockDim.x) {
         const int32_t r = P.seg_rec[src][k];
         const int32_t i = P.shadow_path[r];
@@ -687,7 +700,7 @@ __global__ __launch_bounds__(256) void k_shadow_tr(SceneDev s, PathsDev P, int32
             ss.dim = P.sdim[i];
             const float tmax = hit.prim >= 0 ? hit.t : rb.w;
             tr = tr * medium_tr(s.media[med], majorants(s, med, lds_medium, lds), s.halton, ss, ro,
-                                rd, tmax);
+                                rd, tmax, lookups);
             P.sdim[i] = ss.dim;
         }
         if (hit.prim >= 0) {
@@ -717,6 +730,7 @@ __global__ __launch_bounds__(256) void k_shadow_tr(SceneDev s, PathsDev P, int32
         P.sh_tr[3 * r + 1] = tr.y;
         P.sh_tr[3 * r + 2] = tr.z;
     }
+    add_lookups(P.lookups, lookups);
 }
 
 // EstimateDirect's tail (integrator.cpp:101-118) and `L += beta * ...`.
@@ -906,6 +920,7 @@ struct pg_scene {
     std::vector<MediumDev> media;
     std::vector<std::vector<float>> med_density, med_majorant;
     int lds_medium = -1;  // medium whose brick majorants the tracking kernels stage in LDS
+    int64_t last_lookups = 0;
     bool general = false;
 
     ~pg_scene() {
@@ -1669,7 +1684,8 @@ int ensure_paths(pg_scene& S, int64_t n) {
             (rc = dalloc(S, n, &P.seg_rays[0])) || (rc = dalloc(S, n, &P.seg_rays[1])) ||
             (rc = dalloc(S, n, &P.seg_rec[0])) || (rc = dalloc(S, n, &P.seg_rec[1])) ||
             (rc = dalloc(S, n, &P.seg_med[0])) || (rc = dalloc(S, n, &P.seg_med[1])) ||
-            (rc = dalloc(S, n, &P.seg_hits)) || (rc = dalloc(S, n, &P.seg_uvs)))
+            (rc = dalloc(S, n, &P.seg_hits)) || (rc = dalloc(S, n, &P.seg_uvs)) ||
+            (rc = dalloc(S, 1, &P.lookups)))
             return rc;
     }
     S.path_capacity = n;
@@ -1738,7 +1754,7 @@ extern "C" int pg_scene_get_info(const pg_scene* S, pg_scene_info* info) {
     info->last_rays = S->last_rays;
     info->n_media = (int32_t)S->media.size();
     info->general_mode = S->general ? 1 : 0;
-    info->medium_steps = 0;
+    info->medium_steps = S->last_lookups;
     return PG_OK;
 }
 
@@ -1814,6 +1830,8 @@ extern "C" int pg_render(pg_scene* S, const pg_render_options* opts, void* strea
     PG_HIP(hipEventRecord(e0, st));
     PG_HIP(hipMemsetAsync(S->d_film, 0, (size_t)npix * 4 * sizeof(float), st));
     PathsDev P = S->paths;
+    if (P.lookups)
+        PG_HIP(hipMemsetAsync(P.lookups, 0, sizeof(unsigned long long), st));
     int64_t rays = 0;
     const int B = 256;
     for (int s0 = 0; s0 < spp; s0 += spb) {
@@ -1862,6 +1880,12 @@ extern "C" int pg_render(pg_scene* S, const pg_render_options* opts, void* strea
     (void)hipEventDestroy(e1);
     S->last_render_ms = ms;
     S->last_rays = rays;
+    S->last_lookups = 0;
+    if (P.lookups) {
+        unsigned long long n = 0;
+        PG_HIP(hipMemcpy(&n, P.lookups, sizeof(n), hipMemcpyDeviceToHost));
+        S->last_lookups = (int64_t)n;
+    }
     return PG_OK;
 }
 

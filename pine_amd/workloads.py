@@ -363,3 +363,61 @@ def slab_scene_text(volume_file, width=48, height=48, spp=256, sigma=1.0, albedo
                  "Shape: Rect{", "    position: 0 -1 0", "    ex: 0 0 40", "    ey: 40 0 0",
                  "    material: white", "}"]
     return "\n".join(head + cam + scene + med) + "\n"
+
+
+def mesh_scene_text(obj_file, width=512, height=512, spp=64, max_depth=8, output="mesh.bmp"):
+    """The path-tracing scene around the 2^20-triangle traversal mesh: the
+    displaced icosphere (TriangleMesh from an OBJ file) with a coated diffuse
+    material on a diffuse ground, lit by one rectangular area light."""
+    return f"""Integrator: Path{{
+    maxDepth: {max_depth}
+    sampler: Halton{{
+        samplesPerPixel: {spp}
+    }}
+}}
+Camera: ThinLen{{
+    film{{
+        outputFileName: {output}
+        size: {width} {height}
+        filter: LanczosSinc{{
+            radius: 0.5
+        }}
+    }}
+    from: 0 0.8 3.4
+    to: 0 0 0
+    fov: 0.8
+}}
+Material ground: Layered{{
+    layer0: Diffuse{{
+        albedo: 0.5 0.5 0.5
+    }}
+}}
+Material body: Layered{{
+    layer0: Diffuse{{
+        albedo: 0.7 0.45 0.3
+    }}
+    layer1: Conductor{{
+        albedo: 0.9
+        roughness: 0.35
+    }}
+}}
+Material lamp: Emissive{{
+    color: 12 11 10
+}}
+Shape: TriangleMesh{{
+    file: {obj_file}
+    material: body
+}}
+Shape: Rect{{
+    position: 0 -1.15 0
+    ex: 0 0 60
+    ey: 60 0 0
+    material: ground
+}}
+Shape: Rect{{
+    position: 1.5 3 1
+    ex: 1.5 0 0
+    ey: 0 0 1.5
+    material: lamp
+}}
+"""

@@ -220,6 +220,43 @@ int pg_gen_rays(const float* h_vertices, int64_t n_vertices, const uint32_t* h_i
 int pg_gen_density_grid(int32_t nx, int32_t ny, int32_t nz, float base_freq, float threshold,
                         float scale, uint64_t seed, float* h_density);
 
+/* --- Learned path guiding (no counterpart in pine) ---------------------------
+ * A 225-64-64-64-41 MLP (BF16 weights, FP32 accumulation on the matrix cores)
+ * mapping a surface point to an 8-lobe von Mises-Fisher mixture over incident
+ * directions plus the share alpha of directions drawn from it; the rest come
+ * from the BSDF and the two are combined by one-sample MIS.  Inputs x[9]:
+ * position normalised to the scene bounds, shading normal, outgoing direction.
+ * Parameters: PG_GUIDE_N_PARAMS floats, the padded layout
+ *   W0[64][256] b0[64] W1[64][64] b1[64] W2[64][64] b2[64] W3[48][64] b3[48]
+ * (W0 columns >= 225 and W3/b3 rows >= 41 are padding, kept at zero). */
+#define PG_GUIDE_N_PARAMS 27888
+typedef struct pg_guide pg_guide;
+typedef struct pg_guide_config {
+    uint64_t seed;        /* initial weights */
+    float learning_rate;  /* Adam step size; 0 = 1e-3 */
+} pg_guide_config;
+int pg_guide_create(const pg_guide_config* cfg, pg_guide** out);
+int pg_guide_set_params(pg_guide* g, const float* h_params);
+int pg_guide_get_params(const pg_guide* g, float* h_params);
+/* Raw network outputs, n x 48 (41 live), for n inputs (n x 9), device buffers. */
+int pg_guide_forward(pg_guide* g, const float* d_inputs, int64_t n, float* d_out, void* stream);
+/* n x 3 out: guide density q(w), alpha, and q_mix = alpha q + (1 - alpha) p_bsdf. */
+int pg_guide_pdf(pg_guide* g, const float* d_inputs, const float* d_dirs, const float* d_pbsdf,
+                 int64_t n, float* d_out, void* stream);
+/* n x 4 out: a direction drawn from the mixture with uniforms d_u (n x 3), and q there. */
+int pg_guide_sample(pg_guide* g, const float* d_inputs, const float* d_u, int64_t n, float* d_out,
+                    void* stream);
+/* Gradient (host, PG_GUIDE_N_PARAMS floats) of the mean over the n samples of
+ * -weight * log q_mix(dir), and that mean loss; weights unchanged.  Synchronous. */
+int pg_guide_gradient(pg_guide* g, const float* d_inputs, const float* d_dirs,
+                      const float* d_weight, const float* d_pbsdf, int64_t n, float* h_grad,
+                      float* h_loss);
+/* `steps` Adam steps over consecutive batches of `batch` samples (wrapping);
+ * h_loss receives the last step's mean loss.  Synchronous. */
+int pg_guide_train(pg_guide* g, const float* d_inputs, const float* d_dirs, const float* d_weight,
+                   const float* d_pbsdf, int64_t n, int32_t steps, int32_t batch, float* h_loss);
+int pg_guide_release(pg_guide* g);
+
 #ifdef __cplusplus
 }
 #endif

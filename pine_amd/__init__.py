@@ -28,8 +28,11 @@ EXPORTED = [
     "pg_accel_set_launch", "pg_accel_release", "pg_gen_icosphere", "pg_gen_rays",
     "pg_trace_closest_counts", "pg_trace_occluded", "pg_scene_load", "pg_scene_load_text",
     "pg_scene_get_info", "pg_render", "pg_scene_film", "pg_scene_output_name", "pg_scene_release",
-    "pg_gen_density_grid",
+    "pg_gen_density_grid", "pg_guide_create", "pg_guide_set_params", "pg_guide_get_params",
+    "pg_guide_forward", "pg_guide_pdf", "pg_guide_sample", "pg_guide_gradient", "pg_guide_train",
+    "pg_guide_release",
 ]
+GUIDE_N_PARAMS = 27888
 
 
 class PineError(RuntimeError):
@@ -46,6 +49,10 @@ class _Stats(ctypes.Structure):
                 ("node_bytes", ctypes.c_int64), ("prim_bytes", ctypes.c_int64),
                 ("width", ctypes.c_int32), ("max_depth", ctypes.c_int32),
                 ("build_ms", ctypes.c_double)]
+
+
+class _GuideConfig(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("learning_rate", ctypes.c_float)]
 
 
 class _SceneInfo(ctypes.Structure):
@@ -104,6 +111,15 @@ def lib():
                               ctypes.c_uint64, vp]
     L.pg_gen_density_grid.argtypes = [i32, i32, i32, ctypes.c_float, ctypes.c_float,
                                       ctypes.c_float, ctypes.c_uint64, vp]
+    L.pg_guide_create.argtypes = [ctypes.POINTER(_GuideConfig), ctypes.POINTER(vp)]
+    L.pg_guide_set_params.argtypes = [vp, vp]
+    L.pg_guide_get_params.argtypes = [vp, vp]
+    L.pg_guide_forward.argtypes = [vp, vp, i64, vp, vp]
+    L.pg_guide_pdf.argtypes = [vp, vp, vp, vp, i64, vp, vp]
+    L.pg_guide_sample.argtypes = [vp, vp, vp, i64, vp, vp]
+    L.pg_guide_gradient.argtypes = [vp, vp, vp, vp, vp, i64, vp, ctypes.POINTER(ctypes.c_float)]
+    L.pg_guide_train.argtypes = [vp, vp, vp, vp, vp, i64, i32, i32, ctypes.POINTER(ctypes.c_float)]
+    L.pg_guide_release.argtypes = [vp]
     _lib = L
     return L
 
@@ -314,6 +330,72 @@ class Scene:
     def release(self):
         if self.h:
             lib().pg_scene_release(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+# --- learned path guiding ----------------------------------------------------------
+
+class Guide:
+    """Owns a pg_guide (the path-guiding MLP).  Batch methods take and return
+    device arrays (pine_amd.hip.DeviceArray)."""
+
+    def __init__(self, seed=1, learning_rate=1e-3):
+        self.h = ctypes.c_void_p()
+        cfg = _GuideConfig(seed, learning_rate)
+        check(lib().pg_guide_create(ctypes.byref(cfg), ctypes.byref(self.h)))
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.shape == (GUIDE_N_PARAMS,)
+        check(lib().pg_guide_set_params(self.h, _ptr(p)))
+
+    def params(self):
+        p = np.zeros(GUIDE_N_PARAMS, np.float32)
+        check(lib().pg_guide_get_params(self.h, _ptr(p)))
+        return p
+
+    def forward(self, d_x):
+        from .hip import DeviceArray
+        out = DeviceArray((d_x.shape[0], 48), np.float32)
+        check(lib().pg_guide_forward(self.h, d_x.ptr, d_x.shape[0], out.ptr, _stream_handle()))
+        return out
+
+    def pdf(self, d_x, d_dirs, d_pbsdf):
+        from .hip import DeviceArray
+        out = DeviceArray((d_x.shape[0], 3), np.float32)
+        check(lib().pg_guide_pdf(self.h, d_x.ptr, d_dirs.ptr, d_pbsdf.ptr, d_x.shape[0], out.ptr,
+                                 _stream_handle()))
+        return out
+
+    def sample(self, d_x, d_u):
+        from .hip import DeviceArray
+        out = DeviceArray((d_x.shape[0], 4), np.float32)
+        check(lib().pg_guide_sample(self.h, d_x.ptr, d_u.ptr, d_x.shape[0], out.ptr,
+                                    _stream_handle()))
+        return out
+
+    def gradient(self, d_x, d_dirs, d_weight, d_pbsdf):
+        g = np.zeros(GUIDE_N_PARAMS, np.float32)
+        loss = ctypes.c_float()
+        check(lib().pg_guide_gradient(self.h, d_x.ptr, d_dirs.ptr, d_weight.ptr, d_pbsdf.ptr,
+                                      d_x.shape[0], _ptr(g), ctypes.byref(loss)))
+        return g, loss.value
+
+    def train(self, d_x, d_dirs, d_weight, d_pbsdf, steps, batch):
+        loss = ctypes.c_float()
+        check(lib().pg_guide_train(self.h, d_x.ptr, d_dirs.ptr, d_weight.ptr, d_pbsdf.ptr,
+                                   d_x.shape[0], steps, batch, ctypes.byref(loss)))
+        return loss.value
+
+    def release(self):
+        if self.h:
+            lib().pg_guide_release(self.h)
             self.h = ctypes.c_void_p()
 
     def __del__(self):

@@ -192,3 +192,48 @@ def test_halton_is_a_permuted_van_der_corput(oracle):
     # a value: each of the 25 strata holds exactly 5 of the 125 values.
     strata = np.floor(vals * 25).astype(int)
     assert np.bincount(strata, minlength=25).tolist() == [5] * 25
+
+
+def test_guide_oracle_backward_pinned_by_finite_differences():
+    """The guide network's float64 oracle: its analytic gradient (the GPU's
+    checker) against central differences of its own loss."""
+    from oracle import guide as G
+    rng = np.random.default_rng(0)
+    n = 12
+    p = G.init_params(3) + G.live_mask() * rng.normal(scale=0.05, size=G.N_PARAMS)
+    x = np.concatenate([rng.uniform(0, 1, (n, 3)), rng.normal(size=(n, 3)), rng.normal(size=(n, 3))], 1)
+    w = rng.normal(size=(n, 3))
+    w /= np.linalg.norm(w, axis=1, keepdims=True)
+    wt, pb = rng.uniform(0.1, 2, n), rng.uniform(0.05, 0.5, n)
+    _, g = G.loss_and_grad(p, x, w, wt, pb)
+    live = np.nonzero(G.live_mask())[0]
+    sel = np.concatenate([rng.choice(live, 150, replace=False),
+                          np.arange(G.N_PARAMS - 48 - 48 * 64, G.N_PARAMS - 48 * 64 + 64, 7),
+                          np.arange(G.N_PARAMS - 48, G.N_PARAMS - 7)])
+    eps = 1e-6
+    for k in sel:
+        d = np.zeros_like(p)
+        d[k] = eps
+        num = (G.loss_and_grad(p + d, x, w, wt, pb)[0] - G.loss_and_grad(p - d, x, w, wt, pb)[0]) / (2 * eps)
+        assert abs(num - g[k]) <= 1e-4 * (abs(num) + abs(g[k])) + 1e-9, (k, num, g[k])
+    assert np.all(g[G.live_mask() == 0] == 0)
+
+
+def test_guide_oracle_vmf_closed_forms():
+    """vMF density integrates to 1 and its samples have the closed-form mean
+    cosine coth(kappa) - 1/kappa; PHI is a permutation."""
+    from oracle import guide as G
+    rng = np.random.default_rng(1)
+    assert sorted(G.PHI.tolist()) == list(range(64))
+    n = 400000
+    u = rng.normal(size=(n, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    for kappa in (0.05, 1.0, 8.0, 50.0):
+        mu = np.array([[[0.6, 0.0, 0.8]]])
+        v = G.vmf(np.broadcast_to(mu, (n, 1, 3)), np.full((n, 1), kappa), u)[:, 0]
+        if kappa <= 8:
+            assert abs(4 * np.pi * v.mean() - 1) < 0.02
+        s = G.sample_vmf(np.broadcast_to(mu[0], (n, 3)), np.full(n, kappa), rng.uniform(size=n),
+                         rng.uniform(size=n))
+        mean_cos = (s @ mu[0, 0]).mean()
+        assert abs(mean_cos - (1 / np.tanh(kappa) - 1 / kappa)) < 4e-3

@@ -173,12 +173,16 @@ typedef struct pg_scene_info {
     int32_t n_media;         /* Medium blocks */
     int32_t general_mode;    /* 1: media or material-less shapes (split bounce) */
     int64_t medium_steps;    /* density lookups (8 voxel taps each) of the last render */
+    double last_guided_ms;   /* last pg_render_guided: training passes + final render */
+    int64_t guide_steps;     /* last pg_render_guided: Adam steps taken */
+    int64_t guide_records;   /* last pg_render_guided: nonzero-weight records trained on */
 } pg_scene_info;
 int pg_scene_get_info(const pg_scene* s, pg_scene_info* info);
 
 typedef struct pg_render_options {
     int32_t spp;                /* 0 = the scene's samplesPerPixel */
     int32_t samples_per_batch;  /* 0 = automatic (about 4M paths per batch) */
+    int32_t sample_offset;      /* first sample index per pixel (progressive rendering) */
 } pg_render_options;
 
 /* Renders the whole film: PixelIntegrator::Render + RadianceIntegrator::Compute
@@ -232,8 +236,9 @@ int pg_gen_density_grid(int32_t nx, int32_t ny, int32_t nz, float base_freq, flo
 #define PG_GUIDE_N_PARAMS 27888
 typedef struct pg_guide pg_guide;
 typedef struct pg_guide_config {
-    uint64_t seed;        /* initial weights */
-    float learning_rate;  /* Adam step size; 0 = 1e-3 */
+    uint64_t seed;         /* initial weights */
+    float learning_rate;   /* Adam step size; 0 = 1e-3 */
+    int32_t freq_offset;   /* encoding frequencies pi 2^(k + freq_offset), k = 0..11 */
 } pg_guide_config;
 int pg_guide_create(const pg_guide_config* cfg, pg_guide** out);
 int pg_guide_set_params(pg_guide* g, const float* h_params);
@@ -256,6 +261,26 @@ int pg_guide_gradient(pg_guide* g, const float* d_inputs, const float* d_dirs,
 int pg_guide_train(pg_guide* g, const float* d_inputs, const float* d_dirs, const float* d_weight,
                    const float* d_pbsdf, int64_t n, int32_t steps, int32_t batch, float* h_loss);
 int pg_guide_release(pg_guide* g);
+
+/* Guided rendering (no counterpart in pine): `train_passes` passes of
+ * `train_spp` samples per pixel sample directions from the current guide
+ * (one-sample MIS with the BSDF at single-layer diffuse vertices), record the
+ * radiance each sampled direction brought back, and train the guide on the
+ * records that brought radiance (`steps_per_pass` Adam steps of `batch`
+ * records; 0 = one epoch, every record once, in at most 32 steps of at least
+ * `batch` records); then the film is cleared and `final_spp` samples per
+ * pixel are rendered with the trained guide.  The estimator stays unbiased
+ * for any guide.  Zero fields take the defaults 16 / 8 / one epoch / 1024 /
+ * 5 / the scene's spp.  Synchronous. */
+typedef struct pg_guided_options {
+    int32_t train_passes;
+    int32_t train_spp;
+    int32_t steps_per_pass;
+    int32_t batch;
+    int32_t record_depth;  /* path vertices recorded per path */
+    int32_t final_spp;
+} pg_guided_options;
+int pg_render_guided(pg_scene* s, pg_guide* g, const pg_guided_options* o, void* stream);
 
 #ifdef __cplusplus
 }

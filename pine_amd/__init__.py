@@ -30,7 +30,7 @@ EXPORTED = [
     "pg_scene_get_info", "pg_render", "pg_scene_film", "pg_scene_output_name", "pg_scene_release",
     "pg_gen_density_grid", "pg_guide_create", "pg_guide_set_params", "pg_guide_get_params",
     "pg_guide_forward", "pg_guide_pdf", "pg_guide_sample", "pg_guide_gradient", "pg_guide_train",
-    "pg_guide_release",
+    "pg_guide_release", "pg_render_guided",
 ]
 GUIDE_N_PARAMS = 27888
 
@@ -52,7 +52,8 @@ class _Stats(ctypes.Structure):
 
 
 class _GuideConfig(ctypes.Structure):
-    _fields_ = [("seed", ctypes.c_uint64), ("learning_rate", ctypes.c_float)]
+    _fields_ = [("seed", ctypes.c_uint64), ("learning_rate", ctypes.c_float),
+                ("freq_offset", ctypes.c_int32)]
 
 
 class _SceneInfo(ctypes.Structure):
@@ -61,11 +62,19 @@ class _SceneInfo(ctypes.Structure):
                 ("n_lights", ctypes.c_int32), ("n_prims", ctypes.c_int64),
                 ("last_render_ms", ctypes.c_double), ("last_rays", ctypes.c_int64),
                 ("n_media", ctypes.c_int32), ("general_mode", ctypes.c_int32),
-                ("medium_steps", ctypes.c_int64)]
+                ("medium_steps", ctypes.c_int64), ("last_guided_ms", ctypes.c_double),
+                ("guide_steps", ctypes.c_int64), ("guide_records", ctypes.c_int64)]
+
+
+class _GuidedOptions(ctypes.Structure):
+    _fields_ = [("train_passes", ctypes.c_int32), ("train_spp", ctypes.c_int32),
+                ("steps_per_pass", ctypes.c_int32), ("batch", ctypes.c_int32),
+                ("record_depth", ctypes.c_int32), ("final_spp", ctypes.c_int32)]
 
 
 class _RenderOptions(ctypes.Structure):
-    _fields_ = [("spp", ctypes.c_int32), ("samples_per_batch", ctypes.c_int32)]
+    _fields_ = [("spp", ctypes.c_int32), ("samples_per_batch", ctypes.c_int32),
+                ("sample_offset", ctypes.c_int32)]
 
 
 _lib = None
@@ -120,6 +129,7 @@ def lib():
     L.pg_guide_gradient.argtypes = [vp, vp, vp, vp, vp, i64, vp, ctypes.POINTER(ctypes.c_float)]
     L.pg_guide_train.argtypes = [vp, vp, vp, vp, vp, i64, i32, i32, ctypes.POINTER(ctypes.c_float)]
     L.pg_guide_release.argtypes = [vp]
+    L.pg_render_guided.argtypes = [vp, vp, ctypes.POINTER(_GuidedOptions), vp]
     _lib = L
     return L
 
@@ -309,9 +319,16 @@ class Scene:
         check(lib().pg_scene_get_info(self.h, ctypes.byref(s)))
         return {k: getattr(s, k) for k, _ in _SceneInfo._fields_}
 
-    def render(self, spp=0, samples_per_batch=0):
-        o = _RenderOptions(spp, samples_per_batch)
+    def render(self, spp=0, samples_per_batch=0, sample_offset=0):
+        o = _RenderOptions(spp, samples_per_batch, sample_offset)
         check(lib().pg_render(self.h, ctypes.byref(o), _stream_handle()))
+        return self.info()
+
+    def render_guided(self, guide, train_passes=16, train_spp=8, steps_per_pass=0, batch=1024,
+                      record_depth=5, final_spp=0):
+        """Trains `guide` on this scene, then renders final_spp with it (pg_render_guided)."""
+        o = _GuidedOptions(train_passes, train_spp, steps_per_pass, batch, record_depth, final_spp)
+        check(lib().pg_render_guided(self.h, guide.h, ctypes.byref(o), _stream_handle()))
         return self.info()
 
     def film(self):
@@ -345,9 +362,9 @@ class Guide:
     """Owns a pg_guide (the path-guiding MLP).  Batch methods take and return
     device arrays (pine_amd.hip.DeviceArray)."""
 
-    def __init__(self, seed=1, learning_rate=1e-3):
+    def __init__(self, seed=1, learning_rate=1e-3, freq_offset=0):
         self.h = ctypes.c_void_p()
-        cfg = _GuideConfig(seed, learning_rate)
+        cfg = _GuideConfig(seed, learning_rate, freq_offset)
         check(lib().pg_guide_create(ctypes.byref(cfg), ctypes.byref(self.h)))
 
     def set_params(self, params):

@@ -45,6 +45,7 @@ struct pg_guide {
     int64_t step = 0;
     float lr = 1e-3f;
     uint64_t seed = 1;
+    int oct = 0;  // encoding octave offset
 
     void free_all() {
         for (void* p : {(void*)params, (void*)adam_m, (void*)adam_v, (void*)grad, (void*)partials,
@@ -65,9 +66,9 @@ __device__ __forceinline__ bool live_param(int p) {
     if (p >= OFF_W0 && p < OFF_B0)
         return (p - OFF_W0) % kFeat < kLiveFeat;
     if (p >= OFF_W3 && p < OFF_B3)
-        return (p - OFF_W3) / kH < kOut;
+        return live_out((p - OFF_W3) / kH);
     if (p >= OFF_B3)
-        return p - OFF_B3 < kOut;
+        return live_out(p - OFF_B3);
     return true;
 }
 
@@ -112,10 +113,16 @@ struct Batch {
     const float* dir;     // n x 3
     const float* weight;  // n
     const float* pbsdf;   // n
-    int64_t n;
-    int64_t first;        // first sample of this launch
+    int64_t n;            // slots in the arrays
+    int64_t first, end;   // positions [first, end) of the sample sequence in this launch
     float inv_n;          // 1 / samples in the loss mean
+    uint64_t perm_a;      // sample s of the sequence is slot (s * perm_a) mod n (1 = in order)
+    int oct;              // encoding octave offset
 };
+
+__device__ __forceinline__ int64_t slot_of(const Batch& B, int64_t s) {
+    return B.perm_a == 1 ? s : (int64_t)(((unsigned __int128)s * B.perm_a) % (uint64_t)B.n);
+}
 
 // Head gradient of one sample: d(-weight log q_mix)/d o, scaled by inv_n
 // (oracle/guide.py loss_and_grad).
@@ -132,31 +139,34 @@ __device__ float head_grad(const float* o, f3 w, float weight, float pb, float i
         d[k] = 0.0f;
     if (!(weight > 0.0f))
         return 0.0f;  // a sample that carried no radiance teaches nothing
-    const float sig = p[48] / kAlphaMax, alpha = p[48];
+    // Guide density by its own cross-entropy (defensive uniform share bounds the
+    // gradient); alpha by the mixture's, the guide density held constant.
+    const float sig = p[kAlphaAt] / kAlphaMax, alpha = p[kAlphaAt];
+    static_assert(out_unit(0, 0) == 0 && out_unit(7, 4) == 45 && live_out(kAlphaUnit), "output units");
     const float qm = fmaxf(alpha * q + (1.0f - alpha) * pb, 1e-30f);
-    const float g_qm = -weight / qm * inv_n;
-    const float g_q = g_qm * alpha;
-    d[40] = g_qm * (q - pb) * kAlphaMax * sig * (1.0f - sig);
+    const float qu = (1.0f - kQDefensive) * q + kQDefensive * kInv4PiF;
+    const float g_q = -weight * (1.0f - kQDefensive) / qu * inv_n;
+    d[kAlphaUnit] = -weight / qm * inv_n * (q - pb) * kAlphaMax * sig * (1.0f - sig);
     for (int j = 0; j < kLobes; j++) {
-        const float mx = o[5 * j], my = o[5 * j + 1], mz = o[5 * j + 2];
+        const float mx = o[out_unit(j, 0)], my = o[out_unit(j, 1)], mz = o[out_unit(j, 2)];
         const float rn = sqrtf(mx * mx + my * my + mz * mz), nrm = fmaxf(rn, 1e-6f);
         const float kap = p[6 * j + 3], pi = p[6 * j + 4];
         const f3 mu = mk(p[6 * j], p[6 * j + 1], p[6 * j + 2]);
         const float c = mu.x * w.x + mu.y * w.y + mu.z * w.z;
-        const float kr = o[5 * j + 3];
-        d[5 * j + 4] = g_q * pi * (v[j] - q);
+        const float kr = o[out_unit(j, 3)];
+        d[out_unit(j, 4)] = g_q * pi * (v[j] - q);
         const float em = expf(-2.0f * kap);
         const float dlogv = 1.0f / kap - 2.0f * em / (-expm1f(-2.0f * kap)) + (c - 1.0f);
-        d[5 * j + 3] = (kr > kKrMin && kr < kKrMax) ? g_q * pi * v[j] * dlogv * kap : 0.0f;
+        d[out_unit(j, 3)] = (kr > kKrMin && kr < kKrMax) ? g_q * pi * v[j] * dlogv * kap : 0.0f;
         const float s = g_q * pi * v[j] * kap;  // dq/dmu = s * w
         const f3 dmu = w * s;
         const float proj = dmu.x * mu.x + dmu.y * mu.y + dmu.z * mu.z;
         const float inv = rn > 1e-6f ? 1.0f / nrm : 0.0f;
-        d[5 * j + 0] = (dmu.x - proj * mu.x) * inv;
-        d[5 * j + 1] = (dmu.y - proj * mu.y) * inv;
-        d[5 * j + 2] = (dmu.z - proj * mu.z) * inv;
+        d[out_unit(j, 0)] = (dmu.x - proj * mu.x) * inv;
+        d[out_unit(j, 1)] = (dmu.y - proj * mu.y) * inv;
+        d[out_unit(j, 2)] = (dmu.z - proj * mu.z) * inv;
     }
-    return -weight * logf(qm) * inv_n;
+    return -weight * (logf(qu) + logf(qm)) * inv_n;
 }
 
 // One workgroup (4 waves) per 64 samples; partial gradient row per workgroup.
@@ -179,7 +189,7 @@ __global__ __launch_bounds__(256) void k_guide_grad(const float* __restrict__ pa
     float* part = partials + (size_t)blockIdx.x * kParams;
     for (int k = tid; k < kQ * kIn; k += 256) {
         const int64_t q = q0 + k / kIn;
-        sx[k] = q < B.n ? B.x[q * kIn + k % kIn] : 0.0f;
+        sx[k] = q < B.end ? B.x[slot_of(B, q) * kIn + k % kIn] : 0.0f;
     }
     __syncthreads();
 
@@ -188,7 +198,7 @@ __global__ __launch_bounds__(256) void k_guide_grad(const float* __restrict__ pa
         const int q = 16 * wave + r;
         f32x4 out[3];
         float h[3][16];
-        forward16(sx + q * kIn, wf + WF0, kFeat, wf + WF1, wf + WF2, wf + WF3, kH, params, out,
+        forward16(sx + q * kIn, wf + WF0, kFeat, wf + WF1, wf + WF2, wf + WF3, kH, params, B.oct, out,
                   h[0], h[1], h[2]);
         for (int L = 0; L < 3; L++)
             for (int t = 0; t < 4; t++)
@@ -207,9 +217,10 @@ __global__ __launch_bounds__(256) void k_guide_grad(const float* __restrict__ pa
         const int64_t gq = q0 + q;
         float d[kOutPad];
         float lo = 0.0f;
-        if (gq < B.n) {
-            const f3 w = mk(B.dir[3 * gq], B.dir[3 * gq + 1], B.dir[3 * gq + 2]);
-            lo = head_grad(so + q * (kOutPad + 1), w, B.weight[gq], B.pbsdf[gq], B.inv_n, d);
+        if (gq < B.end) {
+            const int64_t sq = slot_of(B, gq);
+            const f3 w = mk(B.dir[3 * sq], B.dir[3 * sq + 1], B.dir[3 * sq + 2]);
+            lo = head_grad(so + q * (kOutPad + 1), w, B.weight[sq], B.pbsdf[sq], B.inv_n, d);
         } else {
             for (int k = 0; k < kOutPad; k++)
                 d[k] = 0.0f;
@@ -297,11 +308,12 @@ __global__ __launch_bounds__(256) void k_guide_grad(const float* __restrict__ pa
             // lane needs Xt[f = 16 ft + r][q = k0 + 8g + i].
             for (int tI = wave; tI < 64; tI += 4) {
                 const int mt = tI / 16, ft = tI % 16, f = 16 * ft + r;
+                const FeatureSource fs = feature_source(f, B.oct);
                 f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
                 for (int k0 = 0; k0 < kQ; k0 += 32) {
                     bf16x8 b;
                     for (int i = 0; i < 8; i++)
-                        b[i] = (__bf16)feature(sx + (k0 + 8 * g + i) * kIn, f);
+                        b[i] = (__bf16)feature_value(fs, sx + (k0 + 8 * g + i) * kIn);
                     acc = mfma(load8(dt + (16 * mt + r) * kLdq + k0 + 8 * g), b, acc);
                 }
                 for (int i = 0; i < 4; i++)
@@ -345,7 +357,7 @@ __global__ __launch_bounds__(256) void k_guide_apply(const float* __restrict__ p
                                                      const float* __restrict__ x, int64_t n,
                                                      const float* __restrict__ aux,
                                                      const float* __restrict__ aux2,
-                                                     float* __restrict__ out) {
+                                                     float* __restrict__ out, int oct) {
     __shared__ __attribute__((aligned(16))) __bf16 sw[(2 * kH + kOutPad) * kLd];
     __shared__ float so[4][16 * (kOutPad + 1)];
     const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63, r = l & 15, g = l >> 4;
@@ -359,7 +371,7 @@ __global__ __launch_bounds__(256) void k_guide_apply(const float* __restrict__ p
         for (int c = 0; c < kIn; c++)
             xin[c] = q < n ? x[q * kIn + c] : 0.0f;
         f32x4 o[3];
-        forward16(xin, wf + WF0, kFeat, sw, sw + kH * kLd, sw + 2 * kH * kLd, kLd, params, o,
+        forward16(xin, wf + WF0, kFeat, sw, sw + kH * kLd, sw + 2 * kH * kLd, kLd, params, oct, o,
                   nullptr, nullptr, nullptr);
         float* s = so[wave];
         for (int t = 0; t < 3; t++)
@@ -378,8 +390,8 @@ __global__ __launch_bounds__(256) void k_guide_apply(const float* __restrict__ p
                     const f3 w = mk(aux[3 * q], aux[3 * q + 1], aux[3 * q + 2]);
                     const float qq = mix_pdf(p, w);
                     out[3 * q] = qq;
-                    out[3 * q + 1] = p[48];
-                    out[3 * q + 2] = p[48] * qq + (1.0f - p[48]) * aux2[q];
+                    out[3 * q + 1] = p[kAlphaAt];
+                    out[3 * q + 2] = p[kAlphaAt] * qq + (1.0f - p[kAlphaAt]) * aux2[q];
                 } else {
                     const f3 w = mix_sample(p, aux[3 * q], aux[3 * q + 1], aux[3 * q + 2]);
                     out[4 * q] = w.x;
@@ -405,19 +417,19 @@ std::vector<float> init_params(uint64_t seed) {
     std::vector<float> p(kParams, 0.0f);
     uint64_t s = seed;
     auto uni = [&]() { return (float)((splitmix(s) >> 40) * (1.0 / 16777216.0)) * 2.0f - 1.0f; };
-    auto fill = [&](int off, int rows, int cols, int live_rows, int live_cols, float scale) {
+    auto fill = [&](int off, int rows, int cols, bool out_rows, int live_cols, float scale) {
         const float lim = std::sqrt(6.0f / (float)cols) * scale;
         for (int m = 0; m < rows; m++)
             for (int k = 0; k < cols; k++) {
                 const float u = uni();
-                if (m < live_rows && k < live_cols)
+                if ((!out_rows || live_out(m)) && k < live_cols)
                     p[off + m * cols + k] = lim * u;
             }
     };
-    fill(OFF_W0, kH, kFeat, kH, kLiveFeat, 1.0f);
-    fill(OFF_W1, kH, kH, kH, kH, 1.0f);
-    fill(OFF_W2, kH, kH, kH, kH, 1.0f);
-    fill(OFF_W3, kOutPad, kH, kOut, kH, 0.1f);
+    fill(OFF_W0, kH, kFeat, false, kLiveFeat, 1.0f);
+    fill(OFF_W1, kH, kH, false, kH, 1.0f);
+    fill(OFF_W2, kH, kH, false, kH, 1.0f);
+    fill(OFF_W3, kOutPad, kH, true, kH, 0.1f);
     return p;
 }
 
@@ -464,15 +476,17 @@ int check_n(pg_guide* G, int64_t n, const void* a, const char* fn) {
 
 namespace pg {
 const float* guide_params_dev(const pg_guide* G) { return G->params; }
+int guide_freq_offset(const pg_guide* G) { return G->oct; }
 const __bf16* guide_wf_dev(const pg_guide* G) { return G->wf; }
 
 // One Adam step on samples [first, first + count) of the arrays.
 int guide_train_step(pg_guide* G, const float* x, const float* dir, const float* weight,
-                     const float* pb, int64_t n, int64_t first, int64_t count, hipStream_t st) {
+                     const float* pb, int64_t n, int64_t first, int64_t count, uint64_t perm_a,
+                     hipStream_t st) {
     const int blocks = (int)((count + kQ - 1) / kQ);
     if (int rc = ensure_partials(G, blocks))
         return rc;
-    Batch B{x, dir, weight, pb, std::min<int64_t>(n, first + count), first, 1.0f / (float)count};
+    Batch B{x, dir, weight, pb, n, first, first + count, 1.0f / (float)count, perm_a, G->oct};
     hipLaunchKernelGGL(k_guide_grad, dim3(blocks), dim3(256), 0, st, G->params, G->wf, G->wt, B,
                        G->partials, G->loss_part);
     G->last_blocks = blocks;
@@ -498,6 +512,11 @@ extern "C" int pg_guide_create(const pg_guide_config* cfg, pg_guide** out) {
         return pg_fail(PG_E_NOMEM, "pg_guide_create: out of host memory");
     G->seed = cfg ? cfg->seed : 1;
     G->lr = cfg && cfg->learning_rate > 0.0f ? cfg->learning_rate : 1e-3f;
+    G->oct = cfg ? cfg->freq_offset : 0;
+    if (G->oct < -16 || G->oct > 8) {
+        delete G;
+        return pg_fail(PG_E_ARG, "pg_guide_create: freq_offset out of range");
+    }
     hipError_t e = hipSuccess;
     for (float** p : {&G->params, &G->adam_m, &G->adam_v, &G->grad})
         if (e == hipSuccess)
@@ -543,7 +562,7 @@ extern "C" int pg_guide_forward(pg_guide* G, const float* d_inputs, int64_t n, f
     if (n == 0)
         return PG_OK;
     hipLaunchKernelGGL(k_guide_apply<0>, dim3(apply_grid(n)), dim3(256), 0, (hipStream_t)stream,
-                       G->params, G->wf, d_inputs, n, nullptr, nullptr, d_out);
+                       G->params, G->wf, d_inputs, n, nullptr, nullptr, d_out, G->oct);
     PG_HIP(hipGetLastError());
     return PG_OK;
 }
@@ -556,7 +575,7 @@ extern "C" int pg_guide_pdf(pg_guide* G, const float* d_inputs, const float* d_d
     if (n == 0)
         return PG_OK;
     hipLaunchKernelGGL(k_guide_apply<1>, dim3(apply_grid(n)), dim3(256), 0, (hipStream_t)stream,
-                       G->params, G->wf, d_inputs, n, d_dirs, d_pbsdf, d_out);
+                       G->params, G->wf, d_inputs, n, d_dirs, d_pbsdf, d_out, G->oct);
     PG_HIP(hipGetLastError());
     return PG_OK;
 }
@@ -569,7 +588,7 @@ extern "C" int pg_guide_sample(pg_guide* G, const float* d_inputs, const float* 
     if (n == 0)
         return PG_OK;
     hipLaunchKernelGGL(k_guide_apply<2>, dim3(apply_grid(n)), dim3(256), 0, (hipStream_t)stream,
-                       G->params, G->wf, d_inputs, n, d_u, nullptr, d_out);
+                       G->params, G->wf, d_inputs, n, d_u, nullptr, d_out, G->oct);
     PG_HIP(hipGetLastError());
     return PG_OK;
 }
@@ -585,7 +604,7 @@ extern "C" int pg_guide_gradient(pg_guide* G, const float* d_inputs, const float
     const int blocks = (int)((n + kQ - 1) / kQ);
     if (int rc = ensure_partials(G, blocks))
         return rc;
-    Batch B{d_inputs, d_dirs, d_weight, d_pbsdf, n, 0, 1.0f / (float)n};
+    Batch B{d_inputs, d_dirs, d_weight, d_pbsdf, n, 0, n, 1.0f / (float)n, 1, G->oct};
     hipLaunchKernelGGL(k_guide_grad, dim3(blocks), dim3(256), 0, 0, G->params, G->wf, G->wt, B,
                        G->partials, G->loss_part);
     hipLaunchKernelGGL(k_guide_adam, dim3((kParams + 255) / 256), dim3(256), 0, 0, G->params,
@@ -615,7 +634,7 @@ extern "C" int pg_guide_train(pg_guide* G, const float* d_inputs, const float* d
     int64_t first = 0;
     for (int s = 0; s < steps; s++) {
         const int64_t count = std::min<int64_t>(batch, n - first);
-        if (int rc = guide_train_step(G, d_inputs, d_dirs, d_weight, d_pbsdf, n, first, count, 0))
+        if (int rc = guide_train_step(G, d_inputs, d_dirs, d_weight, d_pbsdf, n, first, count, 1, 0))
             return rc;
         first += count;
         if (first >= n)

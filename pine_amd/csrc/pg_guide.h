@@ -22,6 +22,7 @@
 #pragma once
 
 #include "pg_vec.h"
+#include "../../include/pinehip.h"
 
 #include <hip/hip_runtime.h>
 
@@ -43,23 +44,88 @@ constexpr int WF0 = 0, WF1 = WF0 + kH * kFeat, WF2 = WF1 + kH * kH, WF3 = WF2 + 
 constexpr int kWf = WF3 + kOutPad * kH;  // 27648
 constexpr int WT1 = 0, WT2 = kH * kH, WT3 = 2 * kH * kH, kWt = 3 * kH * kH;
 constexpr float kAlphaMax = 0.95f, kKrMin = -4.0f, kKrMax = 7.0f;
+// Output unit of lobe j = 2g + h, component c (mu.xyz, log kappa, logit):
+// slot s = 5h + c of lane group g's twelve output registers (units 16t + 4g + i,
+// s = 4t + i), so each lane holds its two lobes.  Alpha's logit takes lane
+// group 0's slot 10.  The other units (s = 10, 11 elsewhere) are padding.
+__host__ __device__ constexpr int out_unit(int j, int c) {
+    return 16 * ((5 * (j & 1) + c) >> 2) + 4 * (j >> 1) + ((5 * (j & 1) + c) & 3);
+}
+constexpr int kAlphaUnit = 34;
+__host__ __device__ constexpr bool live_out(int u) {
+    const int s = 4 * (u >> 4) + (u & 3), g = (u >> 2) & 3;
+    return u < kOutPad && (s < 10 || (g == 0 && s == 10));
+}
+constexpr float kQDefensive = 0.1f;  // uniform share in the guide density's own objective
+constexpr float kInv4PiF = 0.0795774715459476679f;
 constexpr float kTwoPiF = 6.28318530717958647692f;
 
 __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// Feature f of the encoding of input x[9]: x itself, then sin / cos of
-// pi 2^k x_c as sin / cos(2 pi frac(2^(k-1) x_c)) (exact range reduction).
-__device__ __forceinline__ float feature(const float* x, int f) {
-    if (f < kIn)
-        return x[f];
-    if (f >= kLiveFeat)
-        return 0.0f;
-    const int j = f - kIn, k = j / 18, rem = j - 18 * k, s = rem / 9, c = rem - 9 * s;
-    float r = ldexpf(x[c], k - 1);
-    r = r - floorf(r);
-    return s == 0 ? __sinf(kTwoPiF * r) : __cosf(kTwoPiF * r);
+// Encoding: column f = 8b + i of W0.  Blocks b < 24 hold sin (b even) or cos
+// (b odd) of pi 2^(b/2 + oct) x_i for inputs i < 8; blocks 24..26 the 24
+// (frequency, sin/cos) rows rho = 8(b - 24) + i of input 8; block 27 inputs
+// 0..7 raw; column 224 input 8 raw; columns 225.. are zero.  A lane's B
+// operand of k-step s is block 4s + g, so its inputs are compile-time
+// registers and only the octave and the phase depend on the lane.
+//
+// sin / cos(pi 2^e' x) = sin(2 pi (frac(2^(e'-1) x) + 0 / 0.25)): exact range
+// reduction, then the hardware sine, which takes revolutions.
+__device__ __forceinline__ float periodic(float x, int e, float phase) {
+    return __builtin_amdgcn_sinf(__builtin_amdgcn_fractf(ldexpf(x, e)) + phase);
+}
+
+// Where feature f comes from: kind 0 zero, 1 raw input c, 2 periodic of input
+// c with exponent e and phase.
+struct FeatureSource {
+    int kind, c, e;
+    float phase;
+};
+__device__ __forceinline__ FeatureSource feature_source(int f, int oct) {
+    const int b = f >> 3, i = f & 7;
+    if (b < 24)
+        return {2, i, (b >> 1) + oct - 1, (b & 1) ? 0.25f : 0.0f};
+    if (b < 27) {
+        const int rho = 8 * (b - 24) + i;
+        return {2, 8, (rho >> 1) + oct - 1, (rho & 1) ? 0.25f : 0.0f};
+    }
+    if (b == 27)
+        return {1, i, 0, 0.0f};
+    if (b == 28 && i == 0)
+        return {1, 8, 0, 0.0f};
+    return {0, 0, 0, 0.0f};
+}
+__device__ __forceinline__ float feature_value(const FeatureSource& fs, const float* x) {
+    if (fs.kind == 2)
+        return periodic(x[fs.c], fs.e, fs.phase);
+    return fs.kind == 1 ? x[fs.c] : 0.0f;
+}
+
+// The B operand of layer 0's k-step s for this lane's query (inputs x[9] in
+// registers): features 32s + 8g + i.
+template <int s>
+__device__ __forceinline__ bf16x8 encode8(const float* x, int g, int oct) {
+    bf16x8 b;
+    if constexpr (s < 6) {
+        const int blk = 4 * s + g, e = (blk >> 1) + oct - 1;
+        const float ph = (blk & 1) ? 0.25f : 0.0f;
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+            b[i] = (__bf16)periodic(x[i], e, ph);
+    } else if constexpr (s == 6) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const float v = periodic(x[8], 4 * g + (i >> 1) + oct - 1, (i & 1) ? 0.25f : 0.0f);
+            b[i] = (__bf16)(g < 3 ? v : x[i]);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+            b[i] = (__bf16)((g == 0 && i == 0) ? x[8] : 0.0f);
+    }
+    return b;
 }
 
 // Input k of a hidden layer -> unit of the previous layer (PHI), and back.
@@ -111,22 +177,26 @@ __device__ __forceinline__ void hidden64(const __bf16* W, int ldw, const bf16x8 
 // 16t + 4g + i of each hidden layer.
 __device__ __forceinline__ void forward16(const float* x, const __bf16* W0, int ld0,
                                           const __bf16* W1, const __bf16* W2, const __bf16* W3,
-                                          int ld, const float* params, f32x4 out[3], float* h1,
-                                          float* h2, float* h3) {
+                                          int ld, const float* params, int oct, f32x4 out[3],
+                                          float* h1, float* h2, float* h3) {
     const int l = threadIdx.x & 63, r = l & 15, g = l >> 4;
     f32x4 acc[4];
 #pragma unroll
     for (int t = 0; t < 4; t++)
         acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int s = 0; s < kFeat / 32; s++) {
-        bf16x8 b;
-#pragma unroll
-        for (int i = 0; i < 8; i++)
-            b[i] = (__bf16)feature(x, 32 * s + 8 * g + i);
+    auto step = [&](const bf16x8 b, int s) {
 #pragma unroll
         for (int t = 0; t < 4; t++)
             acc[t] = mfma(load8(W0 + (16 * t + r) * ld0 + 32 * s + 8 * g), b, acc[t]);
-    }
+    };
+    step(encode8<0>(x, g, oct), 0);
+    step(encode8<1>(x, g, oct), 1);
+    step(encode8<2>(x, g, oct), 2);
+    step(encode8<3>(x, g, oct), 3);
+    step(encode8<4>(x, g, oct), 4);
+    step(encode8<5>(x, g, oct), 5);
+    step(encode8<6>(x, g, oct), 6);
+    step(encode8<7>(x, g, oct), 7);
     float h[16];
     bf16x8 bop[2];
     relu_pack(acc, params + OFF_B0, g, h, bop);
@@ -158,21 +228,21 @@ __device__ __forceinline__ void forward16(const float* x, const __bf16* W0, int 
 
 // Mixture parameters of one query from its raw outputs o[48]:
 // p[6j..6j+5] = (mu.xyz, kappa, pi, vMF normaliser kappa/(2 pi (1 - e^-2kappa))),
-// p[48] = alpha (guide share).
-constexpr int kParamStride = 49;
+// p[48] = alpha (guide share); records are 52 floats (16-byte aligned).
+constexpr int kParamStride = 52, kAlphaAt = 48;
 __device__ __forceinline__ void head(const float* o, float* p) {
     float lmax = -3.0e38f;
     for (int j = 0; j < kLobes; j++)
-        lmax = fmaxf(lmax, o[5 * j + 4]);
+        lmax = fmaxf(lmax, o[out_unit(j, 4)]);
     float esum = 0.0f, e[kLobes];
     for (int j = 0; j < kLobes; j++) {
-        e[j] = expf(o[5 * j + 4] - lmax);
+        e[j] = expf(o[out_unit(j, 4)] - lmax);
         esum += e[j];
     }
     for (int j = 0; j < kLobes; j++) {
-        float mx = o[5 * j], my = o[5 * j + 1], mz = o[5 * j + 2];
+        float mx = o[out_unit(j, 0)], my = o[out_unit(j, 1)], mz = o[out_unit(j, 2)];
         const float n = fmaxf(sqrtf(mx * mx + my * my + mz * mz), 1e-6f);
-        const float kap = expf(fminf(fmaxf(o[5 * j + 3], kKrMin), kKrMax));
+        const float kap = expf(fminf(fmaxf(o[out_unit(j, 3)], kKrMin), kKrMax));
         p[6 * j + 0] = mx / n;
         p[6 * j + 1] = my / n;
         p[6 * j + 2] = mz / n;
@@ -180,12 +250,55 @@ __device__ __forceinline__ void head(const float* o, float* p) {
         p[6 * j + 4] = e[j] / esum;
         p[6 * j + 5] = kap / (kTwoPiF * -expm1f(-2.0f * kap));
     }
-    p[48] = kAlphaMax / (1.0f + expf(-o[40]));
+    p[kAlphaAt] = kAlphaMax / (1.0f + expf(-o[kAlphaUnit]));
+}
+
+// The same parameters computed by the four lanes of a query straight from the
+// output registers of forward16 (lane group g holds lobes 2g and 2g + 1 in
+// slots 0..9 and, for g = 0, alpha in slot 10; the softmax reduces across
+// lanes l, l ^ 16, l ^ 32), written to the 16-byte aligned record p.  Every
+// lane of the wave must call it (cross-lane reductions); `store` selects the
+// lanes that write.
+__device__ __forceinline__ void head4(const f32x4 out[3], int g, bool store, float* p) {
+    float v[12];
+#pragma unroll
+    for (int sl = 0; sl < 12; sl++)
+        v[sl] = out[sl >> 2][sl & 3];
+    float m = fmaxf(v[4], v[9]);
+    m = fmaxf(m, __shfl_xor(m, 16));
+    m = fmaxf(m, __shfl_xor(m, 32));
+    const float e0 = expf(v[4] - m), e1 = expf(v[9] - m);
+    float es = e0 + e1;
+    es += __shfl_xor(es, 16);
+    es += __shfl_xor(es, 32);
+    const float inv = 1.0f / es;
+    float r[12];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const float mx = v[5 * h], my = v[5 * h + 1], mz = v[5 * h + 2];
+        const float n = fmaxf(sqrtf(mx * mx + my * my + mz * mz), 1e-6f);
+        const float kap = expf(fminf(fmaxf(v[5 * h + 3], kKrMin), kKrMax));
+        r[6 * h + 0] = mx / n;
+        r[6 * h + 1] = my / n;
+        r[6 * h + 2] = mz / n;
+        r[6 * h + 3] = kap;
+        r[6 * h + 4] = (h ? e1 : e0) * inv;
+        r[6 * h + 5] = kap / (kTwoPiF * -expm1f(-2.0f * kap));
+    }
+    if (store) {
+        float4* d = reinterpret_cast<float4*>(p + 12 * g);
+        d[0] = make_float4(r[0], r[1], r[2], r[3]);
+        d[1] = make_float4(r[4], r[5], r[6], r[7]);
+        d[2] = make_float4(r[8], r[9], r[10], r[11]);
+        if (g == 0)
+            p[kAlphaAt] = kAlphaMax / (1.0f + expf(-v[10]));
+    }
 }
 
 // Guide density q(w) of a direction under mixture parameters p.
 __device__ __forceinline__ float mix_pdf(const float* p, f3 w) {
     float q = 0.0f;
+#pragma unroll
     for (int j = 0; j < kLobes; j++) {
         const float c = p[6 * j] * w.x + p[6 * j + 1] * w.y + p[6 * j + 2] * w.z;
         q += p[6 * j + 4] * p[6 * j + 5] * expf(p[6 * j + 3] * (c - 1.0f));
@@ -217,4 +330,14 @@ __device__ __forceinline__ f3 mix_sample(const float* p, float u0, float u1, flo
 }
 
 }  // namespace guide
+
+// Host interface of the network for the path tracer (guide.hip).
+const float* guide_params_dev(const pg_guide* G);
+int guide_freq_offset(const pg_guide* G);
+const __bf16* guide_wf_dev(const pg_guide* G);
+// One Adam step on sequence positions [first, first + count) of n slots, slot
+// of position s = (s * perm_a) mod n (perm_a coprime to n).
+int guide_train_step(pg_guide* G, const float* x, const float* dir, const float* weight,
+                     const float* pb, int64_t n, int64_t first, int64_t count, uint64_t perm_a,
+                     hipStream_t st);
 }  // namespace pg

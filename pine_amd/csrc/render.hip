@@ -16,6 +16,7 @@
 // path sees exactly the sample values pine's would.
 #include "bvh_build.h"
 #include "pg_api.h"
+#include "pg_guide.h"
 #include "pg_isect.h"
 #include "pg_medium.h"
 #include "pg_shade.h"
@@ -93,6 +94,12 @@ struct SceneDev {
     int32_t camera_medium;  // ThinLenCamera medium (camera.cpp:98-105), -1 = none
     int32_t has_env;        // a Sky environment light is present
     f3 sky_dir, sky_color;  // Sky sunDirection (normalised) and sunColor
+    // Learned path guiding (general mode only).
+    int32_t guided;
+    const float* g_params;  // FP32 parameters (biases used in FP32)
+    const __bf16* g_wf;     // BF16 forward weights
+    f3 g_bmin, g_binv;      // scene bounds: network position input = (p - bmin) * binv
+    int32_t g_oct;          // encoding octave offset
 };
 
 // Path state, structure of arrays (one entry per path of the batch).
@@ -128,7 +135,7 @@ struct PathsDev {
     float* vtx;            // 9 per path: vertex p, n, wi
     int32_t* vinfo;        // 4 per path: kind (0 surface, 1 medium), material, medium in, out
     int32_t* queue_b;      // paths that sample a direction after their shadow ray
-    float* sh_rec;         // 12 per shadow record: f.xyz, w, Le.xyz, pdf, p2.xyz, delta
+    float* sh_rec;         // 16 per shadow record: f.xyz, w, Le.xyz, pdf, p2.xyz, delta, wo.xyz, p_bsdf
     float* sh_tr;          // 3 per shadow record: transmittance so far
     pg_ray* seg_rays[2];   // shadow segments (IntersectTr loop), ping-pong
     int32_t* seg_rec[2];
@@ -136,6 +143,30 @@ struct PathsDev {
     pg_hit* seg_hits;
     pg_hit_uv* seg_uvs;
     unsigned long long* lookups;  // density lookups of the tracking kernels (8 taps each)
+    // Guiding: mixture of each path's current vertex, and training records
+    // (slot = path * rec_depth + depth; rec_pdf 0 = empty slot).
+    float* gparams;   // guide::kParamStride per path
+    int32_t* gflag;   // 1 if the vertex samples from the guide
+    int32_t* shadow_rec;  // fused bounce: record slot of each shadow ray's vertex, -1 if none
+    float* rec_x;     // 9 per slot: network input
+    float* rec_dir;   // 3 per slot: sampled direction
+    float* rec_w;     // radiance / pdf, filled after the batch
+    float* rec_pb;    // BSDF pdf of the direction
+    float* rec_pdf;   // mixture pdf it was drawn with
+    float* rec_lb;    // 3 per slot: path radiance before the direction
+    float* rec_beta;  // 3 per slot: throughput after the direction
+    float* rec_acc;   // 1: mean nonzero weight of a batch
+    // Nonzero-weight records compacted in slot order (the training set of a
+    // batch) and the per-256-slot counts, sums and offsets that place them.
+    float* c_x;
+    float* c_dir;
+    float* c_w;
+    float* c_pb;
+    int32_t* rec_bcnt;
+    float* rec_bsum;
+    int32_t* rec_boff;
+    int32_t* rec_n;
+    int32_t rec_depth;
 };
 
 }  // namespace pg
@@ -360,6 +391,87 @@ __device__ LightSampleD sample_light_point(const SceneDev& s, const LightDev& L,
     return ls;
 }
 
+// Network input of a surface vertex: position in the scene bounds, normal,
+// outgoing direction.
+__device__ __forceinline__ void guide_input(const SceneDev& s, f3 p, f3 n, f3 wo, float* x) {
+    x[0] = (p.x - s.g_bmin.x) * s.g_binv.x;
+    x[1] = (p.y - s.g_bmin.y) * s.g_binv.y;
+    x[2] = (p.z - s.g_bmin.z) * s.g_binv.z;
+    x[3] = n.x, x[4] = n.y, x[5] = n.z;
+    x[6] = wo.x, x[7] = wo.y, x[8] = wo.z;
+}
+
+// Vertices the guide serves: single-layer diffuse surfaces, whose BSDF
+// sampling density is exactly material_pdf (the mixture pdf needs it).
+__device__ __forceinline__ bool guidable(const SceneDev& s, int4 info) {
+    if (info.x != 0)
+        return false;
+    const MaterialDev& m = s.materials[info.y];
+    return m.kind == kMatLayered && m.nlayers == 1 && m.layers[0].type == kDiffuse;
+}
+
+// Direction sampling at a guided vertex: one-sample MIS of the guide (share
+// alpha) and the BSDF; the estimator divides by the mixture pdf, whichever
+// strategy drew wo, so it stays unbiased for any guide.  With training on,
+// the vertex is recorded (network input, direction, pdfs, path radiance so far
+// and throughput after the direction).  False when the sample carries nothing.
+__device__ bool guided_direction(const SceneDev& s, PathsDev& P, SamplerState& ss, int32_t i, int depth,
+                                 const MaterialDev& m, f3 p, f3 n, f3 wi, f3 L_before, f3& beta, f3& wo_out) {
+    const M3 n2w = frame_from(n);
+    const M3 w2n = inverse(n2w);
+    const f3 wiL = mul(w2n, wi);
+    const float* gp = P.gparams + guide::kParamStride * (int64_t)i;
+    const float alpha = gp[guide::kAlphaAt];
+    const float usel = get_1d(s.halton, ss);
+    const float u1 = get_1d(s.halton, ss);
+    float u2a, u2b;
+    get_2d(s.halton, ss, u2a, u2b);
+    f3 wo = mk(0.0f, 0.0f, 1.0f), f = mk(0.0f, 0.0f, 0.0f);
+    float pb = 0.0f;
+    if (usel < alpha) {
+        wo = guide::mix_sample(gp, u1, u2a, u2b);
+        const f3 woL = mul(w2n, wo);
+        f = material_f(m, wiL, woL);
+        pb = material_pdf(m, wiL, woL);
+    } else {
+        BSDFSample bs;
+        if (material_sample(m, wiL, u1, u2a, u2b, bs)) {
+            wo = mul(n2w, bs.wo);
+            f = bs.f;
+            pb = bs.pdf;
+        }
+    }
+    const float pmix = alpha * guide::mix_pdf(gp, wo) + (1.0f - alpha) * pb;
+    if (is_black(f) || !(pmix > 0.0f))
+        return false;
+    beta = beta * (abs_dot(wo, n) * f / pmix);
+    P.bsdf_pdf[i] = pmix;
+    wo_out = wo;
+    if (depth < P.rec_depth) {
+        const int64_t slot = (int64_t)i * P.rec_depth + depth;
+        float x[guide::kIn];
+        guide_input(s, p, n, wi, x);
+        for (int c = 0; c < guide::kIn; c++)
+            P.rec_x[guide::kIn * slot + c] = x[c];
+        P.rec_dir[3 * slot + 0] = wo.x;
+        P.rec_dir[3 * slot + 1] = wo.y;
+        P.rec_dir[3 * slot + 2] = wo.z;
+        P.rec_pb[slot] = pb;
+        P.rec_pdf[slot] = pmix;
+        P.rec_lb[3 * slot + 0] = L_before.x;
+        P.rec_lb[3 * slot + 1] = L_before.y;
+        P.rec_lb[3 * slot + 2] = L_before.z;
+        P.rec_beta[3 * slot + 0] = beta.x;
+        P.rec_beta[3 * slot + 1] = beta.y;
+        P.rec_beta[3 * slot + 2] = beta.z;
+    }
+    return true;
+}
+
+// kGuided: directions at the vertices the guide serves (gflag, set for the
+// bounce by k_guide_select / k_guide_infer) come from the guide/BSDF mixture,
+// and the light sample's MIS weight uses the mixture pdf.
+template <bool kGuided>
 __global__ __launch_bounds__(256) void k_shade(SceneDev s, PathsDev P, int32_t n_active) {
     const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n_active)
@@ -411,6 +523,8 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev s, PathsDev P, int32_t n
                 const M3 n2w = frame_from(h.n);
                 const M3 w2n = inverse(n2w);
                 const f3 wiL = mul(w2n, wi);
+                const bool guided = kGuided && P.gflag[i];
+                const float* gp = P.gparams + guide::kParamStride * (int64_t)i;
                 // EstimateDirect (integrator.cpp:88-119)
                 float lpdf;
                 const float ul = get_1d(s.halton, ss);
@@ -423,8 +537,13 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev s, PathsDev P, int32_t n
                     const f3 woL = mul(w2n, ls.wo);
                     f3 f = material_f(m, wiL, woL) * abs_dot(ls.wo, h.n);
                     float scatPdf = 0.0f;
-                    if (!ls.delta)
+                    if (!ls.delta) {
                         scatPdf = material_pdf(m, wiL, woL);
+                        if (guided) {
+                            const float alpha = gp[guide::kAlphaAt];
+                            scatPdf = alpha * guide::mix_pdf(gp, ls.wo) + (1.0f - alpha) * scatPdf;
+                        }
+                    }
                     float w = power_heuristic(ls.pdf, scatPdf);
                     f3 ld = ls.delta ? f * ls.Le / ls.pdf : f * w * ls.Le / ls.pdf;
                     f3 contrib = beta * ld;
@@ -437,19 +556,14 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev s, PathsDev P, int32_t n
                         P.shadow_contrib[3 * slot + 0] = contrib.x;
                         P.shadow_contrib[3 * slot + 1] = contrib.y;
                         P.shadow_contrib[3 * slot + 2] = contrib.z;
+                        if (kGuided && P.rec_depth > 0)
+                            P.shadow_rec[slot] = guided && depth < P.rec_depth ? i * P.rec_depth + depth : -1;
                     }
                 }
-                // BSDF sample (path.cpp:66-86)
-                const float u1 = get_1d(s.halton, ss);
-                float u2a, u2b;
-                get_2d(s.halton, ss, u2a, u2b);
-                BSDFSample bs;
-                if (material_sample(m, wiL, u1, u2a, u2b, bs)) {
-                    bs.wo = mul(n2w, bs.wo);
-                    if (!is_black(bs.f) && bs.pdf != 0.0f) {
-                        beta = beta * (abs_dot(bs.wo, h.n) * bs.f / bs.pdf);
-                        P.bsdf_pdf[i] = bs.pdf;
-                        nextD = bs.wo;
+                if (guided) {
+                    f3 wo;
+                    if (guided_direction(s, P, ss, i, depth, m, h.p, h.n, wi, L, beta, wo)) {
+                        nextD = wo;
                         nextO = offset_ray_origin(h.p, face_same_hemisphere(h.n, nextD));
                         alive = true;
                         if (depth > 2) {
@@ -458,6 +572,29 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev s, PathsDev P, int32_t n
                                 alive = false;
                             else
                                 beta = beta / (1.0f - q);
+                        }
+                    }
+                } else {
+                    // BSDF sample (path.cpp:66-86)
+                    const float u1 = get_1d(s.halton, ss);
+                    float u2a, u2b;
+                    get_2d(s.halton, ss, u2a, u2b);
+                    BSDFSample bs;
+                    if (material_sample(m, wiL, u1, u2a, u2b, bs)) {
+                        bs.wo = mul(n2w, bs.wo);
+                        if (!is_black(bs.f) && bs.pdf != 0.0f) {
+                            beta = beta * (abs_dot(bs.wo, h.n) * bs.f / bs.pdf);
+                            P.bsdf_pdf[i] = bs.pdf;
+                            nextD = bs.wo;
+                            nextO = offset_ray_origin(h.p, face_same_hemisphere(h.n, nextD));
+                            alive = true;
+                            if (depth > 2) {
+                                float q = pstd::clamp(1.0f - luminance(beta), 0.05f, 1.0f);
+                                if (get_1d(s.halton, ss) < q)
+                                    alive = false;
+                                else
+                                    beta = beta / (1.0f - q);
+                            }
                         }
                     }
                 }
@@ -489,6 +626,16 @@ __global__ __launch_bounds__(256) void k_resolve_shadow(PathsDev P, int32_t n_sh
     P.L[3 * i + 0] += P.shadow_contrib[3 * k + 0];
     P.L[3 * i + 1] += P.shadow_contrib[3 * k + 1];
     P.L[3 * i + 2] += P.shadow_contrib[3 * k + 2];
+    // A recorded guided vertex's own light sample is not radiance along its
+    // sampled direction: it joins the record's path radiance before it.
+    if (P.rec_depth > 0 && P.shadow_rec) {
+        const int32_t r = P.shadow_rec[k];
+        if (r >= 0) {
+            P.rec_lb[3 * (int64_t)r + 0] += P.shadow_contrib[3 * k + 0];
+            P.rec_lb[3 * (int64_t)r + 1] += P.shadow_contrib[3 * k + 1];
+            P.rec_lb[3 * (int64_t)r + 2] += P.shadow_contrib[3 * k + 2];
+        }
+    }
 }
 
 
@@ -548,13 +695,14 @@ __device__ void light_record(const SceneDev& s, PathsDev& P, SamplerState& ss, i
     LightSampleD ls = sample_light_point(s, s.lights[li], p, la, lb);
     ls.pdf *= lpdf;
     f3 f;
-    float scatPdf = 0.0f;
+    float scatPdf = 0.0f, pb = 0.0f;
     if (mat >= 0) {
         const MaterialDev& m = s.materials[mat];
         const M3 w2n = inverse(frame_from(n));
         f = material_f(m, mul(w2n, wi), mul(w2n, ls.wo)) * abs_dot(ls.wo, n);
+        pb = material_pdf(m, mul(w2n, wi), mul(w2n, ls.wo));
         if (!ls.delta)
-            scatPdf = material_pdf(m, mul(w2n, wi), mul(w2n, ls.wo));
+            scatPdf = pb;
     } else {
         const float ph = phase_p(g, wi, ls.wo);
         f = mk(ph, ph, ph);
@@ -565,11 +713,12 @@ __device__ void light_record(const SceneDev& s, PathsDev& P, SamplerState& ss, i
     const f3 so = offset_ray_origin(p, face_same_hemisphere(n, ls.wo));
     const float stmax = ls.distance * (1.0f - 1e-3f);
     const int r = atomicAdd(&P.counters[1], 1);
-    float4* rec = reinterpret_cast<float4*>(P.sh_rec + 12 * (int64_t)r);
+    float4* rec = reinterpret_cast<float4*>(P.sh_rec + 16 * (int64_t)r);
     const f3 p2 = so + stmax * ls.wo;  // IntersectTr's target ray(ray.tmax)
     rec[0] = make_float4(f.x, f.y, f.z, w);
     rec[1] = make_float4(ls.Le.x, ls.Le.y, ls.Le.z, ls.pdf);
     rec[2] = make_float4(p2.x, p2.y, p2.z, ls.delta ? 1.0f : 0.0f);
+    rec[3] = make_float4(ls.wo.x, ls.wo.y, ls.wo.z, pb);
     P.sh_tr[3 * r + 0] = P.sh_tr[3 * r + 1] = P.sh_tr[3 * r + 2] = 1.0f;
     P.shadow_path[r] = i;
     store_ray(&P.seg_rays[0][r], so, ls.wo, 0.0f, stmax);
@@ -715,7 +864,7 @@ __global__ __launch_bounds__(256) void k_shadow_tr(SceneDev s, PathsDev P, int32
                 const int min_ = transition ? sh.mi_inside : med;
                 const int mout = transition ? sh.mi_outside : med;
                 // Interaction::SpawnRayTo(p2) (interaction.h:27-35)
-                const float* rec = P.sh_rec + 12 * (int64_t)r;
+                const float* rec = P.sh_rec + 16 * (int64_t)r;
                 const f3 p2 = mk(rec[8], rec[9], rec[10]);
                 float len;
                 const f3 d2 = normalize(p2 - h.p, len);
@@ -734,7 +883,9 @@ __global__ __launch_bounds__(256) void k_shadow_tr(SceneDev s, PathsDev P, int32
 }
 
 // EstimateDirect's tail (integrator.cpp:101-118) and `L += beta * ...`.
-__global__ __launch_bounds__(256) void k_resolve_gen(PathsDev P, int32_t n_rec) {
+// At a guided vertex the directional strategy is the guide/BSDF mixture, so
+// the light sample's MIS weight uses the mixture pdf of the light direction.
+__global__ __launch_bounds__(256) void k_resolve_gen(SceneDev s, PathsDev P, int32_t n_rec) {
     const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n_rec)
         return;
@@ -742,10 +893,18 @@ __global__ __launch_bounds__(256) void k_resolve_gen(PathsDev P, int32_t n_rec) 
     if (is_black(tr))
         return;
     const int32_t i = P.shadow_path[r];
-    const float4* rec = reinterpret_cast<const float4*>(P.sh_rec + 12 * (int64_t)r);
+    const float4* rec = reinterpret_cast<const float4*>(P.sh_rec + 16 * (int64_t)r);
     const float4 a = rec[0], b = rec[1], c = rec[2];
     const f3 f = mk(a.x, a.y, a.z), le = mk(b.x, b.y, b.z);
-    const float w = a.w, pdf = b.w;
+    const float pdf = b.w;
+    float w = a.w;
+    if (s.guided && P.gflag[i] && c.w == 0.0f) {
+        const float4 d = rec[3];
+        const float* gp = P.gparams + guide::kParamStride * (int64_t)i;
+        const float alpha = gp[guide::kAlphaAt];
+        const float pmix = alpha * guide::mix_pdf(gp, mk(d.x, d.y, d.z)) + (1.0f - alpha) * d.w;
+        w = power_heuristic(pdf, pmix);
+    }
     const f3 ld = c.w != 0.0f ? tr * f * le / pdf : tr * f * w * le / pdf;
     const f3 beta = mk(P.beta[3 * i], P.beta[3 * i + 1], P.beta[3 * i + 2]);
     const f3 add = beta * ld;
@@ -782,6 +941,22 @@ __global__ __launch_bounds__(256) void k_shade_gen_b(SceneDev s, PathsDev P, int
         nextO = offset_ray_origin(p, face_same_hemisphere(n, wo));
         next_med = info.z;
         alive = true;
+    } else if (s.guided && P.gflag[i]) {
+        f3 wo;
+        if (guided_direction(s, P, ss, i, depth, s.materials[info.y], p, n, wi,
+                             mk(P.L[3 * i], P.L[3 * i + 1], P.L[3 * i + 2]), beta, wo)) {
+            nextD = wo;
+            nextO = offset_ray_origin(p, face_same_hemisphere(n, nextD));
+            next_med = medium_for(nextD, n, info.z, info.w);
+            alive = true;
+            if (depth > 2) {
+                const float q = pstd::clamp(1.0f - luminance(beta), 0.05f, 1.0f);
+                if (get_1d(s.halton, ss) < q)
+                    alive = false;
+                else
+                    beta = beta / (1.0f - q);
+            }
+        }
     } else {
         const MaterialDev& m = s.materials[info.y];
         const M3 n2w = frame_from(n);
@@ -822,6 +997,206 @@ __global__ __launch_bounds__(256) void k_shade_gen_b(SceneDev s, PathsDev P, int
         P.medium[i] = next_med;
         store_ray(&P.rays_out[slot], nextO, nextD, 0.0f, kFloatMax);
     }
+}
+
+
+// --- learned path guiding -----------------------------------------------------
+
+// Mixture parameters of every guided vertex: 16 vertices per wave through the
+// register-chained MFMA network, all weights staged in LDS once per workgroup
+// (59 KB: two 8-wave workgroups per CU), the head computed from the output
+// registers.  kSrc 0: the split bounce's direction-sampling queue (queue_b,
+// vtx, vinfo); kSrc 1: the fused bounce's hits in queue order, the vertex
+// evaluated here.  Vertices the guide does not serve (misses, emitters,
+// non-diffuse materials, the last depth) get gflag 0.  Waves are independent
+// after the staging barrier: a grid-stride loop over tiles of 16 per wave.
+constexpr int kInferThreads = 512;
+template <int kSrc>
+__global__ __launch_bounds__(kInferThreads) void k_guide_infer(SceneDev s, PathsDev P, int32_t n) {
+    using namespace guide;
+    constexpr int kLd = kH + 8, kLd0 = kFeat + 8;
+    __shared__ __attribute__((aligned(16))) __bf16 sw[(2 * kH + kOutPad) * kLd];
+    __shared__ __attribute__((aligned(16))) __bf16 sw0[kH * kLd0];
+    const int tid = threadIdx.x, l = tid & 63, r = l & 15, g = l >> 4;
+    for (int k = tid; k < (2 * kH * kH + kOutPad * kH) / 8; k += kInferThreads) {
+        const int row = k / (kH / 8), c8 = k % (kH / 8);
+        *reinterpret_cast<bf16x8*>(sw + row * kLd + 8 * c8) =
+            *reinterpret_cast<const bf16x8*>(s.g_wf + WF1 + row * kH + 8 * c8);
+    }
+    for (int k = tid; k < kH * kFeat / 8; k += kInferThreads) {
+        const int row = k / (kFeat / 8), c8 = k % (kFeat / 8);
+        *reinterpret_cast<bf16x8*>(sw0 + row * kLd0 + 8 * c8) =
+            *reinterpret_cast<const bf16x8*>(s.g_wf + WF0 + row * kFeat + 8 * c8);
+    }
+    __syncthreads();
+    const int32_t waves = gridDim.x * (kInferThreads / 64);
+    for (int32_t tile0 = 16 * (blockIdx.x * (kInferThreads / 64) + (tid >> 6)); tile0 < n; tile0 += 16 * waves) {
+        // The weight operands' LDS addresses do not depend on the tile: keep
+        // the compiler from hoisting all 56 of them into registers (224 VGPRs,
+        // then spills) by making each iteration's loads opaque.
+        asm volatile("" ::: "memory");
+        const int32_t k = tile0 + r;
+        int32_t i = -1;
+        bool use = false;
+        float x[kIn] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (k < n) {
+            if constexpr (kSrc == 1) {
+                i = P.queue_in[k];
+                const pg_hit hit = P.hits[k];
+                if (hit.prim >= 0 && P.depth[i] + 1 < s.max_depth) {
+                    const float4* r4 = reinterpret_cast<const float4*>(&P.rays_in[k]);
+                    const float4 ra = r4[0], rb = r4[1];
+                    const f3 ro = mk(ra.x, ra.y, ra.z), rd = mk(rb.x, rb.y, rb.z);
+                    const pg_hit_uv uv = P.uvs[k];
+                    const Hit h = surface(s, hit.prim, hit.t, uv.u, uv.v, ro, rd);
+                    const int mat = s.shapes[h.shape].material;
+                    use = mat >= 0 && guidable(s, make_int4(0, mat, 0, 0));
+                    if (use)
+                        guide_input(s, h.p, h.n, -rd, x);
+                }
+            } else {
+                i = P.queue_b[k];
+                const int4 info = *reinterpret_cast<const int4*>(P.vinfo + 4 * (int64_t)i);
+                use = guidable(s, info);
+                if (use) {
+                    const float* v = P.vtx + 9 * (int64_t)i;
+                    guide_input(s, mk(v[0], v[1], v[2]), mk(v[3], v[4], v[5]), mk(v[6], v[7], v[8]), x);
+                }
+            }
+        }
+        f32x4 o[3];
+        forward16(x, sw0, kLd0, sw, sw + kH * kLd, sw + 2 * kH * kLd, kLd, s.g_params, s.g_oct, o, nullptr,
+                  nullptr, nullptr);
+        if (g == 0 && i >= 0)
+            P.gflag[i] = use ? 1 : 0;
+        head4(o, g, use, P.gparams + kParamStride * (int64_t)(i >= 0 ? i : 0));
+    }
+}
+
+// Training weights of the recorded vertices once their paths are complete:
+// incident radiance along the sampled direction, (L_final - L_before) /
+// beta_after in luminance, times the BSDF pdf (for the diffuse vertices the
+// guide serves, proportional to f cos: the guide learns the product), over the
+// pdf the direction was sampled with.
+__global__ void k_guide_finalize(PathsDev P, int64_t n_slots) {
+    const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= n_slots)
+        return;
+    float w = 0.0f;
+    const float pdf = P.rec_pdf[slot];
+    if (pdf > 0.0f) {
+        const int64_t i = slot / P.rec_depth;
+        const f3 dl = mk(P.L[3 * i] - P.rec_lb[3 * slot], P.L[3 * i + 1] - P.rec_lb[3 * slot + 1],
+                         P.L[3 * i + 2] - P.rec_lb[3 * slot + 2]);
+        const float b = luminance(mk(P.rec_beta[3 * slot], P.rec_beta[3 * slot + 1],
+                                     P.rec_beta[3 * slot + 2]));
+        const float lin = b > 0.0f ? luminance(dl) / b : 0.0f;
+        const float v = lin * P.rec_pb[slot] / pdf;
+        w = lin > 0.0f && isfinite(v) ? v : 0.0f;
+    }
+    P.rec_w[slot] = w;
+}
+
+// Training-set compaction: the records with a nonzero weight, in slot order,
+// with weights relative to their batch mean and clamped at kWeightClamp (a
+// handful of high-radiance paths must not decide a whole step).  Every sum
+// runs in a fixed order, so a batch trains to the same bits on every run.
+constexpr float kWeightClamp = 20.0f;
+
+// Per block of 256 slots: count and sum of the nonzero weights.
+__global__ __launch_bounds__(256) void k_rec_count(const float* w, int64_t n, int32_t* bcnt, float* bsum) {
+    __shared__ float ss[4];
+    __shared__ int32_t sc[4];
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const float v = k < n ? w[k] : 0.0f;
+    float s = v > 0.0f ? v : 0.0f;
+    int32_t c = v > 0.0f ? 1 : 0;
+    for (int off = 32; off > 0; off >>= 1) {
+        s += __shfl_down(s, off, 64);
+        c += __shfl_down(c, off, 64);
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        ss[wave] = s;
+        sc[wave] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        bsum[blockIdx.x] = ((ss[0] + ss[1]) + ss[2]) + ss[3];
+        bcnt[blockIdx.x] = sc[0] + sc[1] + sc[2] + sc[3];
+    }
+}
+
+// One workgroup: exclusive scan of the block counts into boff, the total
+// count into n_out and the mean weight into acc[0].
+__global__ __launch_bounds__(1024) void k_rec_scan(const int32_t* bcnt, const float* bsum, int32_t nblk,
+                                                   int32_t* boff, float* acc, int32_t* n_out) {
+    __shared__ int32_t sc[1024];
+    __shared__ float ss[1024];
+    const int t = threadIdx.x;
+    const int32_t chunk = (nblk + 1023) / 1024;
+    const int32_t b0 = t * chunk, b1 = min(nblk, b0 + chunk);
+    int32_t c = 0;
+    float s = 0.0f;
+    for (int32_t b = b0; b < b1; b++) {
+        c += bcnt[b];
+        s += bsum[b];
+    }
+    sc[t] = c;
+    ss[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {  // inclusive scan of the counts
+        const int32_t add = t >= off ? sc[t - off] : 0;
+        __syncthreads();
+        sc[t] += add;
+        __syncthreads();
+    }
+    for (int off = 512; off > 0; off >>= 1) {  // tree sum of the weights
+        if (t < off)
+            ss[t] += ss[t + off];
+        __syncthreads();
+    }
+    int32_t run = sc[t] - c;
+    for (int32_t b = b0; b < b1; b++) {
+        boff[b] = run;
+        run += bcnt[b];
+    }
+    if (t == 0) {
+        const int32_t total = sc[1023];
+        n_out[0] = total;
+        acc[0] = total > 0 ? ss[0] / (float)total : 0.0f;
+    }
+}
+
+// Writes each nonzero record to its place in the compacted training set.
+__global__ __launch_bounds__(256) void k_rec_scatter(PathsDev P, int64_t n) {
+    __shared__ int32_t wc[4];
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const float v = k < n ? P.rec_w[k] : 0.0f;
+    const bool keep = v > 0.0f;
+    const unsigned long long m = __ballot(keep);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        wc[wave] = __popcll(m);
+    __syncthreads();
+    if (!keep)
+        return;
+    int32_t o = P.rec_boff[blockIdx.x] + __popcll(m & ((1ull << lane) - 1ull));
+    for (int w = 0; w < wave; w++)
+        o += wc[w];
+    const float r = v / P.rec_acc[0];
+    P.c_w[o] = r < kWeightClamp ? r : kWeightClamp;
+    P.c_pb[o] = P.rec_pb[k];
+    for (int c = 0; c < guide::kIn; c++)
+        P.c_x[guide::kIn * (int64_t)o + c] = P.rec_x[guide::kIn * k + c];
+    for (int c = 0; c < 3; c++)
+        P.c_dir[3 * (int64_t)o + c] = P.rec_dir[3 * k + c];
+}
+
+__global__ void k_fill_f32(float* p, int64_t n, float v) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n)
+        p[k] = v;
 }
 
 // Film::AddSample (film.h:26-45) for every path of a pixel, in sample order.
@@ -922,6 +1297,10 @@ struct pg_scene {
     int lds_medium = -1;  // medium whose brick majorants the tracking kernels stage in LDS
     int64_t last_lookups = 0;
     bool general = false;
+    float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};  // bounds of the finite shapes
+    int64_t guide_capacity = 0, record_capacity = 0;
+    double last_guide_ms = 0.0;
+    int64_t guide_steps = 0, guide_records = 0;
 
     ~pg_scene() {
         for (void* p : allocations)
@@ -1400,6 +1779,35 @@ void build_scene(pg_scene& S) {
         }
         S.shapes.push_back(sh);
     }
+    // Scene bounds over the finite shapes (the guide's position input).
+    {
+        float lo[3] = {kFloatMax, kFloatMax, kFloatMax}, hi[3] = {-kFloatMax, -kFloatMax, -kFloatMax};
+        auto grow = [&](f3 q) {
+            const float c[3] = {q.x, q.y, q.z};
+            for (int k = 0; k < 3; k++) {
+                lo[k] = std::min(lo[k], c[k]);
+                hi[k] = std::max(hi[k], c[k]);
+            }
+        };
+        for (size_t k = 0; k + 2 < S.tri_verts.size(); k += 3)
+            grow(mk(S.tri_verts[k], S.tri_verts[k + 1], S.tri_verts[k + 2]));
+        for (const ShapeDev& sh : S.shapes) {
+            if (sh.type == kShapeSphere) {
+                grow(sh.p0 - mk(sh.r, sh.r, sh.r));
+                grow(sh.p0 + mk(sh.r, sh.r, sh.r));
+            } else if (sh.type == kShapeRect) {
+                for (int a = -1; a <= 1; a += 2)
+                    for (int b = -1; b <= 1; b += 2)
+                        grow(sh.p0 + sh.e0 * (0.5f * a * sh.lx) + sh.e1 * (0.5f * b * sh.ly));
+            }
+        }
+        for (int k = 0; k < 3; k++) {
+            if (!(lo[k] <= hi[k]))
+                lo[k] = -1.0f, hi[k] = 1.0f;
+            S.bmin[k] = lo[k];
+            S.bmax[k] = hi[k];
+        }
+    }
     // Area lights follow the Light blocks (fileio.cpp:269-271).
     for (int si : area_light_shapes) {
         LightDev l{};
@@ -1680,7 +2088,7 @@ int ensure_paths(pg_scene& S, int64_t n) {
     if (S.general) {
         if ((rc = dalloc(S, n, &P.medium)) || (rc = dalloc(S, 9 * n, &P.vtx)) ||
             (rc = dalloc(S, 4 * n, &P.vinfo)) || (rc = dalloc(S, n, &P.queue_b)) ||
-            (rc = dalloc(S, 12 * n, &P.sh_rec)) || (rc = dalloc(S, 3 * n, &P.sh_tr)) ||
+            (rc = dalloc(S, 16 * n, &P.sh_rec)) || (rc = dalloc(S, 3 * n, &P.sh_tr)) ||
             (rc = dalloc(S, n, &P.seg_rays[0])) || (rc = dalloc(S, n, &P.seg_rays[1])) ||
             (rc = dalloc(S, n, &P.seg_rec[0])) || (rc = dalloc(S, n, &P.seg_rec[1])) ||
             (rc = dalloc(S, n, &P.seg_med[0])) || (rc = dalloc(S, n, &P.seg_med[1])) ||
@@ -1689,6 +2097,31 @@ int ensure_paths(pg_scene& S, int64_t n) {
             return rc;
     }
     S.path_capacity = n;
+    return PG_OK;
+}
+
+int ensure_guide(pg_scene& S, int64_t n, int rec_depth) {
+    PathsDev& P = S.paths;
+    int rc;
+    if (n > S.guide_capacity) {
+        if ((rc = dalloc(S, (int64_t)guide::kParamStride * n, &P.gparams)) || (rc = dalloc(S, n, &P.gflag)) ||
+            (rc = dalloc(S, n, &P.shadow_rec)))
+            return rc;
+        S.guide_capacity = n;
+    }
+    const int64_t slots = n * rec_depth;
+    if (slots > S.record_capacity) {
+        if ((rc = dalloc(S, (int64_t)guide::kIn * slots, &P.rec_x)) || (rc = dalloc(S, 3 * slots, &P.rec_dir)) ||
+            (rc = dalloc(S, slots, &P.rec_w)) || (rc = dalloc(S, slots, &P.rec_pb)) ||
+            (rc = dalloc(S, slots, &P.rec_pdf)) || (rc = dalloc(S, 3 * slots, &P.rec_lb)) ||
+            (rc = dalloc(S, 3 * slots, &P.rec_beta)) || (rc = dalloc(S, 2, &P.rec_acc)) ||
+            (rc = dalloc(S, (int64_t)guide::kIn * slots, &P.c_x)) || (rc = dalloc(S, 3 * slots, &P.c_dir)) ||
+            (rc = dalloc(S, slots, &P.c_w)) || (rc = dalloc(S, slots, &P.c_pb)) ||
+            (rc = dalloc(S, (slots + 255) / 256, &P.rec_bcnt)) || (rc = dalloc(S, (slots + 255) / 256, &P.rec_bsum)) ||
+            (rc = dalloc(S, (slots + 255) / 256, &P.rec_boff)) || (rc = dalloc(S, 1, &P.rec_n)))
+            return rc;
+        S.record_capacity = slots;
+    }
     return PG_OK;
 }
 
@@ -1755,11 +2188,19 @@ extern "C" int pg_scene_get_info(const pg_scene* S, pg_scene_info* info) {
     info->n_media = (int32_t)S->media.size();
     info->general_mode = S->general ? 1 : 0;
     info->medium_steps = S->last_lookups;
+    info->last_guided_ms = S->last_guide_ms;
+    info->guide_steps = S->guide_steps;
+    info->guide_records = S->guide_records;
     return PG_OK;
 }
 
 constexpr int kMaxShadowSegments = 1024;
 constexpr int kMaxBounces = 1 << 16;
+// Guide inference: one workgroup per 128 vertices up to two per CU (each
+// stages 59 KB of weights, then its waves stride over the rest).
+int infer_blocks(int32_t n) {
+    return (int)std::min<int64_t>((n + 127) / 128, 512);
+}
 
 // Grid of a grid-stride tracking kernel: enough workgroups to fill the chip,
 // each staging the brick majorants once.
@@ -1797,8 +2238,10 @@ static int bounce_general(pg_scene* S, PathsDev& Q, int32_t& active, hipStream_t
         n_seg = next;
         src = 1 - src;
     }
+    if (S->dev.guided && n_b > 0)
+        hipLaunchKernelGGL(k_guide_infer<0>, dim3(infer_blocks(n_b)), dim3(kInferThreads), 0, st, S->dev, Q, n_b);
     if (n_rec > 0)
-        hipLaunchKernelGGL(k_resolve_gen, dim3((n_rec + B - 1) / B), dim3(B), 0, st, Q, n_rec);
+        hipLaunchKernelGGL(k_resolve_gen, dim3((n_rec + B - 1) / B), dim3(B), 0, st, S->dev, Q, n_rec);
     if (n_b > 0)
         hipLaunchKernelGGL(k_shade_gen_b, dim3((n_b + B - 1) / B), dim3(B), 0, st, S->dev, Q, n_b);
     PG_HIP(hipMemcpyAsync(counts, Q.counters, sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -1807,12 +2250,32 @@ static int bounce_general(pg_scene* S, PathsDev& Q, int32_t& active, hipStream_t
     return PG_OK;
 }
 
-extern "C" int pg_render(pg_scene* S, const pg_render_options* opts, void* stream) {
-    if (!S)
-        return pg_fail(PG_E_ARG, "pg_render: null scene");
-    const int spp = opts && opts->spp > 0 ? opts->spp : S->spp;
+// Per-batch training of the guide during a training pass.
+constexpr int64_t kAutoSteps = 32;
+struct GuideTraining {
+    pg_guide* G;
+    int rec_depth;
+    int steps_per_batch;  // 0: one epoch over each batch's records
+    int batch;
+    int64_t steps;         // Adam steps taken
+    int64_t last_records;  // nonzero-weight records trained on
+};
+
+static uint64_t gcd64(uint64_t a, uint64_t b) {
+    while (b) {
+        uint64_t t = a % b;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+
+// Renders spp samples per pixel into the (cleared) film; with `tr`, records
+// the guided vertices of every batch and trains the guide after it.
+static int render_impl(pg_scene* S, int spp, int spb_req, hipStream_t st, GuideTraining* tr,
+                       double* ms_out, int64_t* rays_out, int sample_offset = 0) {
     const int64_t npix = (int64_t)S->dev.width * S->dev.height;
-    int spb = opts && opts->samples_per_batch > 0 ? opts->samples_per_batch : 0;
+    int spb = spb_req;
     if (spb <= 0) {
         // About 4M paths per batch: enough lanes for 256 CUs, bounded state.
         spb = (int)std::max<int64_t>(1, std::min<int64_t>(spp, (4 << 20) / std::max<int64_t>(npix, 1)));
@@ -1823,13 +2286,17 @@ extern "C" int pg_render(pg_scene* S, const pg_render_options* opts, void* strea
         return pg_fail(PG_E_ARG, "pg_render: batch too large");
     if (int rc = ensure_paths(*S, n_paths))
         return rc;
-    hipStream_t st = (hipStream_t)stream;
+    if (S->dev.guided) {
+        if (int rc = ensure_guide(*S, n_paths, tr ? tr->rec_depth : 0))
+            return rc;
+    }
     hipEvent_t e0, e1;
     PG_HIP(hipEventCreate(&e0));
     PG_HIP(hipEventCreate(&e1));
     PG_HIP(hipEventRecord(e0, st));
     PG_HIP(hipMemsetAsync(S->d_film, 0, (size_t)npix * 4 * sizeof(float), st));
     PathsDev P = S->paths;
+    P.rec_depth = tr ? tr->rec_depth : 0;
     if (P.lookups)
         PG_HIP(hipMemsetAsync(P.lookups, 0, sizeof(unsigned long long), st));
     int64_t rays = 0;
@@ -1837,7 +2304,12 @@ extern "C" int pg_render(pg_scene* S, const pg_render_options* opts, void* strea
     for (int s0 = 0; s0 < spp; s0 += spb) {
         const int nb = std::min(spb, spp - s0);
         const int32_t np = (int32_t)(npix * nb);
-        hipLaunchKernelGGL(k_raygen, dim3((np + B - 1) / B), dim3(B), 0, st, S->dev, P, np, nb, s0);
+        const int64_t n_slots = tr ? (int64_t)np * tr->rec_depth : 0;
+        if (tr)
+            hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((n_slots + B - 1) / B)), dim3(B), 0, st,
+                               P.rec_pdf, n_slots, 0.0f);
+        hipLaunchKernelGGL(k_raygen, dim3((np + B - 1) / B), dim3(B), 0, st, S->dev, P, np, nb,
+                           s0 + sample_offset);
         int32_t active = np;
         PathsDev Q = P;
         // Depth bounds every path except those crossing material-less surfaces,
@@ -1847,7 +2319,15 @@ extern "C" int pg_render(pg_scene* S, const pg_render_options* opts, void* strea
             if (int rc = pg_trace_closest(S->accel, Q.rays_in, active, Q.hits, Q.uvs, st))
                 return rc;
             if (!S->general) {
-                hipLaunchKernelGGL(k_shade, dim3((active + B - 1) / B), dim3(B), 0, st, S->dev, Q, active);
+                if (S->dev.guided) {
+                    hipLaunchKernelGGL(k_guide_infer<1>, dim3(infer_blocks(active)), dim3(kInferThreads), 0, st,
+                                       S->dev, Q, active);
+                    hipLaunchKernelGGL(k_shade<true>, dim3((active + B - 1) / B), dim3(B), 0, st, S->dev, Q,
+                                       active);
+                } else {
+                    hipLaunchKernelGGL(k_shade<false>, dim3((active + B - 1) / B), dim3(B), 0, st, S->dev, Q,
+                                       active);
+                }
                 int32_t counts[2];
                 PG_HIP(hipMemcpyAsync(counts, Q.counters, sizeof(counts), hipMemcpyDeviceToHost, st));
                 PG_HIP(hipStreamSynchronize(st));
@@ -1866,6 +2346,46 @@ extern "C" int pg_render(pg_scene* S, const pg_render_options* opts, void* strea
             std::swap(Q.queue_in, Q.queue_out);
             std::swap(Q.rays_in, Q.rays_out);
         }
+        if (tr) {
+            hipLaunchKernelGGL(k_guide_finalize, dim3((unsigned)((n_slots + B - 1) / B)), dim3(B), 0, st,
+                               P, n_slots);
+            const int32_t nblk = (int32_t)((n_slots + 255) / 256);
+            hipLaunchKernelGGL(k_rec_count, dim3(nblk), dim3(256), 0, st, P.rec_w, n_slots, P.rec_bcnt,
+                               P.rec_bsum);
+            hipLaunchKernelGGL(k_rec_scan, dim3(1), dim3(1024), 0, st, P.rec_bcnt, P.rec_bsum, nblk,
+                               P.rec_boff, P.rec_acc, P.rec_n);
+            hipLaunchKernelGGL(k_rec_scatter, dim3(nblk), dim3(256), 0, st, P, n_slots);
+            int32_t n_train = 0;
+            PG_HIP(hipMemcpyAsync(&n_train, P.rec_n, sizeof(n_train), hipMemcpyDeviceToHost, st));
+            PG_HIP(hipStreamSynchronize(st));
+            tr->last_records += n_train;
+            if (n_train > 0) {
+                // Minibatches walk the records in a fixed pseudo-random order;
+                // by default one epoch, so every step sees fresh paths.
+                uint64_t a = 2654435761ull % (uint64_t)n_train;
+                if (a < 2)
+                    a = 1;
+                while (a > 1 && gcd64(a, (uint64_t)n_train) != 1)
+                    a++;
+                // Default: one epoch in at most kAutoSteps steps of at least
+                // `batch` records (larger films train on larger batches).
+                int64_t count = std::min<int64_t>(tr->batch, n_train), steps = tr->steps_per_batch;
+                if (steps <= 0) {
+                    steps = std::min<int64_t>(kAutoSteps, (n_train + count - 1) / count);
+                    count = (n_train + steps - 1) / steps;
+                }
+                int64_t first = 0;
+                for (int64_t k = 0; k < steps; k++) {
+                    const int64_t take = std::min<int64_t>(count, n_train - first);
+                    if (int rc = guide_train_step(tr->G, P.c_x, P.c_dir, P.c_w, P.c_pb, n_train, first, take, a, st))
+                        return rc;
+                    first += take;
+                    if (first >= n_train)
+                        first = 0;
+                }
+                tr->steps += steps;
+            }
+        }
         PG_HIP(hipMemsetAsync(P.spill_count, 0, sizeof(int32_t), st));
         hipLaunchKernelGGL(k_film, dim3((npix + B - 1) / B), dim3(B), 0, st, S->dev, P, (int32_t)npix,
                            nb, S->d_film);
@@ -1878,14 +2398,76 @@ extern "C" int pg_render(pg_scene* S, const pg_render_options* opts, void* strea
     PG_HIP(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    S->last_render_ms = ms;
-    S->last_rays = rays;
+    *ms_out = ms;
+    *rays_out = rays;
     S->last_lookups = 0;
     if (P.lookups) {
         unsigned long long n = 0;
         PG_HIP(hipMemcpy(&n, P.lookups, sizeof(n), hipMemcpyDeviceToHost));
         S->last_lookups = (int64_t)n;
     }
+    return PG_OK;
+}
+
+extern "C" int pg_render(pg_scene* S, const pg_render_options* opts, void* stream) {
+    if (!S)
+        return pg_fail(PG_E_ARG, "pg_render: null scene");
+    const int spp = opts && opts->spp > 0 ? opts->spp : S->spp;
+    const int spb = opts && opts->samples_per_batch > 0 ? opts->samples_per_batch : 0;
+    const int offset = opts && opts->sample_offset > 0 ? opts->sample_offset : 0;
+    S->dev.guided = 0;
+    double ms = 0.0;
+    int64_t rays = 0;
+    if (int rc = render_impl(S, spp, spb, (hipStream_t)stream, nullptr, &ms, &rays, offset))
+        return rc;
+    S->last_render_ms = ms;
+    S->last_rays = rays;
+    S->last_guide_ms = 0.0;
+    return PG_OK;
+}
+
+extern "C" int pg_render_guided(pg_scene* S, pg_guide* G, const pg_guided_options* o,
+                                void* stream) {
+    if (!S || !G)
+        return pg_fail(PG_E_ARG, "pg_render_guided: null scene or guide");
+    const int passes = o && o->train_passes > 0 ? o->train_passes : 16;
+    const int train_spp = o && o->train_spp > 0 ? o->train_spp : 8;
+    const int steps = o && o->steps_per_pass > 0 ? o->steps_per_pass : 0;
+    const int batch = o && o->batch > 0 ? o->batch : 1024;
+    const int depth = o && o->record_depth > 0 ? o->record_depth : 5;
+    const int final_spp = o && o->final_spp > 0 ? o->final_spp : S->spp;
+    SceneDev& d = S->dev;
+    d.g_params = guide_params_dev(G);
+    d.g_wf = guide_wf_dev(G);
+    d.g_oct = guide_freq_offset(G);
+    d.g_bmin = mk(S->bmin[0], S->bmin[1], S->bmin[2]);
+    d.g_binv = mk(1.0f / std::max(S->bmax[0] - S->bmin[0], 1e-6f),
+                  1.0f / std::max(S->bmax[1] - S->bmin[1], 1e-6f),
+                  1.0f / std::max(S->bmax[2] - S->bmin[2], 1e-6f));
+    d.guided = 1;
+    hipStream_t st = (hipStream_t)stream;
+    double train_ms = 0.0, ms = 0.0;
+    int64_t rays = 0, all_rays = 0;
+    const int64_t npix = (int64_t)d.width * d.height;
+    const int64_t per_batch = std::max<int64_t>(1, std::min<int64_t>(train_spp, (4 << 20) / std::max<int64_t>(npix, 1)));
+    const int nbatches = (int)((train_spp + per_batch - 1) / per_batch);
+    GuideTraining tr{G, depth, steps > 0 ? std::max(1, (steps + nbatches - 1) / nbatches) : 0, batch, 0, 0};
+    int rc = PG_OK;
+    for (int p = 0; p < passes && rc == PG_OK; p++) {
+        rc = render_impl(S, train_spp, 0, st, &tr, &ms, &rays);
+        train_ms += ms;
+        all_rays += rays;
+    }
+    if (rc == PG_OK)
+        rc = render_impl(S, final_spp, 0, st, nullptr, &ms, &rays);
+    d.guided = 0;
+    if (rc != PG_OK)
+        return rc;
+    S->last_render_ms = ms;
+    S->last_rays = rays;
+    S->last_guide_ms = train_ms + ms;
+    S->guide_steps = tr.steps;
+    S->guide_records = tr.last_records;
     return PG_OK;
 }
 

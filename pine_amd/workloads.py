@@ -13,6 +13,8 @@ mesh, origins offset by pine's OffsetRayOrigin.
 """
 from dataclasses import dataclass
 
+import numpy as np
+
 
 @dataclass(frozen=True)
 class MeshSpec:
@@ -421,3 +423,64 @@ Shape: Rect{{
     material: lamp
 }}
 """
+
+
+# --- occluded-light box (learned path guiding) ----------------------------------
+
+def occluded_box_clutter(n_objects=100, freq=16, seed=21):
+    """Seeded clutter for the guiding scene: n_objects displaced icospheres
+    (20 freq^2 triangles each; 100 x 5,120 = 512,000 by default) resting on the
+    floor of the main room.  Returns (vertices, indices) of one mesh."""
+    rng = np.random.default_rng(seed)
+    verts, idx, base = [], [], 0
+    for k in range(n_objects):
+        v, i = make_mesh(MeshSpec(freq, 0.2, seed + k, 0))
+        r = rng.uniform(0.04, 0.13)
+        c = np.array([rng.uniform(-0.85, 0.85), r, rng.uniform(-0.25, 0.85)])
+        verts.append((v * r + c).astype(np.float32))
+        idx.append(i + base)
+        base += len(v)
+    return np.concatenate(verts), np.concatenate(idx).astype(np.uint32)
+
+
+def occluded_box_scene_text(obj_file, width=1024, height=1024, spp=64, max_depth=8,
+                            output="box.bmp"):
+    """A closed five-wall box (open towards the camera) split by a partition
+    with a 0.3 x 0.3 gap; the only light hangs behind the partition, so most
+    radiance reaches the room indirectly through the gap."""
+    def rect(pos, ex, ey, mat=None):
+        out = ["Shape: Rect{", f"    position: {_v3(pos)}", f"    ex: {_v3(ex)}",
+               f"    ey: {_v3(ey)}"]
+        if mat:
+            out.append(f"    material: {mat}")
+        return out + ["}"]
+    lines = [
+        "Integrator: Path{", f"    maxDepth: {max_depth}", "    sampler: Halton{",
+        f"        samplesPerPixel: {spp}", "    }", "}",
+        "Camera: ThinLen{", "    film{", f"        outputFileName: {output}",
+        f"        size: {width} {height}", "        filter: LanczosSinc{", "            radius: 0.5",
+        "        }", "    }", "    from: 0 1 3.3", "    to: 0 0.9 0", "    fov: 0.75", "}",
+        "Material white: Layered{", "    layer0: Diffuse{", "        albedo: 0.7 0.7 0.7", "    }", "}",
+        "Material red: Layered{", "    layer0: Diffuse{", "        albedo: 0.7 0.15 0.1", "    }", "}",
+        "Material green: Layered{", "    layer0: Diffuse{", "        albedo: 0.15 0.6 0.15", "    }", "}",
+        "Material clutter: Layered{", "    layer0: Diffuse{", "        albedo: 0.55 0.5 0.4", "    }", "}",
+        "Material lamp: Emissive{", "    color: 30 28 25", "}",
+    ]
+    # Walls: floor, ceiling, back, left, right (centre positions, full extents).
+    lines += rect((0, 0, 0), (0, 0, 2), (2, 0, 0), "white")
+    lines += rect((0, 2, 0), (2, 0, 0), (0, 0, 2), "white")
+    lines += rect((0, 1, -1), (2, 0, 0), (0, 2, 0), "white")
+    lines += rect((-1, 1, 0), (0, 2, 0), (0, 0, 2), "red")
+    lines += rect((1, 1, 0), (0, 0, 2), (0, 2, 0), "green")
+    # Partition at z = -0.4 with a 0.3 x 0.3 gap centred at (0, 1.0).
+    z, gy = -0.4, 1.0
+    lines += rect((-0.575, 1, z), (0.85, 0, 0), (0, 2, 0), "white")
+    lines += rect((0.575, 1, z), (0.85, 0, 0), (0, 2, 0), "white")
+    lines += rect((0, (gy - 0.15) / 2, z), (0.3, 0, 0), (0, gy - 0.15, 0), "white")
+    lines += rect((0, (gy + 0.15 + 2) / 2, z), (0.3, 0, 0), (0, 2 - gy - 0.15, 0), "white")
+    # The lamp under the ceiling of the compartment behind the partition,
+    # facing down: no point of the room sees it, light reaches the room only
+    # after bouncing off the compartment, through the gap.
+    lines += rect((0, 1.95, -0.72), (0.5, 0, 0), (0, 0, 0.4), "lamp")
+    lines += ["Shape: TriangleMesh{", f"    file: {obj_file}", "    material: clutter", "}"]
+    return "\n".join(lines) + "\n"

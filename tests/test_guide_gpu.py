@@ -172,8 +172,9 @@ def test_training_learns_a_target_distribution(pg, cuda, G):
     _, l0 = g.gradient(*d)
     l1 = g.train(*d, steps=400, batch=4096)
     _, l2 = g.gradient(*d)
-    # Target entropy of vMF(kappa=20): log(2 pi (1 - e^-40) / 20) + 1 - ... ~ -0.16
-    print(f"cross-entropy: start {l0:.3f}, last batch {l1:.3f}, end {l2:.3f}")
-    assert l2 < l0 - 1.5
-    assert l2 < 0.6
+    # Two cross-entropy terms (guide with a defensive uniform share, mixture);
+    # each starts near log(4 pi) = 2.53 for a broad guide.
+    print(f"loss: start {l0:.3f}, last batch {l1:.3f}, end {l2:.3f}")
+    assert l2 < l0 - 3.0
+    assert l2 < 1.2
     g.release()

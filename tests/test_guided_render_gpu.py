@@ -1,0 +1,107 @@
+"""Learned path guiding inside the GPU path tracer (pg_render_guided) on the
+occluded-light box: a five-wall room lit only through a 0.3 x 0.3 gap in a
+partition, behind which the lamp faces away from the room.
+
+What is checked:
+  - unbiasedness: the guided estimator converges to the plain tracer's image
+    (block means over 8 x 8 pixels), whatever the guide learned;
+  - the gain: with the trained guide the image error at equal samples per
+    pixel is lower than the plain tracer's (the plain tracer is the judge:
+    the reference is its own robust high-spp image, pine_amd.metrics);
+  - what the network learned: at points of the room, the trained mixture's
+    density towards the gap exceeds the cosine density several-fold.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def box(tmp_path_factory, pg):
+    from pine_amd import workloads as W
+    d = tmp_path_factory.mktemp("box")
+    v, i = W.occluded_box_clutter()
+    W.write_obj(str(d / "clutter.obj"), v, i)
+    return str(d), W
+
+
+def scene(pg, box, res, spp=64):
+    d, W = box
+    return pg.Scene(text=W.occluded_box_scene_text("clutter.obj", width=res, height=res, spp=spp,
+                                                   max_depth=8), directory=d)
+
+
+def test_guided_estimator_is_unbiased(pg, cuda, tmp_path):
+    """The spheres scene (diffuse spheres are guided, dielectric and conductor
+    ones are not): the guided image converges to the plain tracer's, block by
+    block, whatever the guide learned."""
+    from conftest import golden
+    from pine_amd import metrics as M
+    text = golden("render_spheres")["scene"].item().decode()
+    s = pg.Scene(text=text, directory=str(tmp_path))
+    s.render(spp=2048)
+    ref = M.linear(s.film())
+    g = pg.Guide(seed=3, learning_rate=3e-3)
+    inf = s.render_guided(g, train_passes=4, train_spp=4, final_spp=2048)
+    img = M.linear(s.film())
+    ok = np.isfinite(img).all(-1) & np.isfinite(ref).all(-1)
+    assert ok.all()
+    def blocks(a):
+        return a.reshape(8, 12, 8, 12, 3).mean(axis=(1, 3))
+    bg, br = blocks(img), blocks(ref)
+    rel = np.abs(bg - br).mean() / br.mean()
+    print(f"guided {inf['last_render_ms']:.1f} ms; block mean rel. difference {rel:.4f}, "
+          f"image means {bg.mean():.5f} vs {br.mean():.5f}")
+    assert abs(bg.mean() - br.mean()) < 0.01 * br.mean()
+    assert rel < 0.03
+    g.release()
+    s.release()
+
+
+def test_trained_guide_lowers_error_at_equal_spp(pg, cuda, box):
+    from pine_amd import metrics as M
+    s = scene(pg, box, 128)
+    ref = M.robust_reference(s, 1024, 8)
+    s.render(spp=128, sample_offset=1 << 20)
+    plain = M.relmse(M.linear(s.film()), ref)
+    g = pg.Guide(seed=3, learning_rate=3e-3)
+    s.render_guided(g, final_spp=128)
+    guided = M.relmse(M.linear(s.film()), ref)
+    print(f"relMSE at 128 spp: plain {plain:.3e}, guided {guided:.3e} ({plain / guided:.2f}x)")
+    assert guided < 0.9 * plain
+    g.release()
+    s.release()
+
+
+def test_guide_learns_the_gap(pg, cuda, box):
+    from pine_amd import hip
+    s = scene(pg, box, 128)
+    g = pg.Guide(seed=3, learning_rate=3e-3)
+    s.render_guided(g, final_spp=4)
+    # Floor and wall points of the room, the direction to the gap centre.
+    bmin, bmax = np.array([-1.0, 0.0, -1.0]), np.array([1.0, 2.0, 1.0])
+    pts = np.array([[0.0, 0.001, 0.4], [0.4, 0.001, 0.6], [-0.4, 0.001, 0.2], [-0.999, 1.0, 0.3],
+                    [0.999, 0.8, 0.5]])
+    nrm = np.array([[0, 1, 0], [0, 1, 0], [0, 1, 0], [1, 0, 0], [-1, 0, 0]], float)
+    gap = np.array([0.0, 1.0, -0.4])
+    x, dirs = [], []
+    for p, n in zip(pts, nrm):
+        wo = np.array([0.0, 1.0, 3.3]) - p
+        wo /= np.linalg.norm(wo)
+        x.append(np.concatenate([(p - bmin) / (bmax - bmin), n, wo]))
+        w = gap - p
+        dirs.append(w / np.linalg.norm(w))
+    x = np.array(x, np.float32)
+    dirs = np.array(dirs, np.float32)
+    cosine = np.maximum((dirs * nrm).sum(1), 0) / np.pi
+    out = g.pdf(cuda.DeviceArray.from_numpy(x), cuda.DeviceArray.from_numpy(dirs),
+                cuda.DeviceArray.from_numpy(cosine.astype(np.float32))).numpy()
+    print("guide density towards the gap:", np.round(out[:, 0], 3), "cosine:", np.round(cosine, 3),
+          "alpha:", np.round(out[:, 1], 3))
+    assert np.median(out[:, 0] / cosine) > 3.0
+    assert (out[:, 1] > 0.2).all()
+    g.release()
+    s.release()

@@ -194,6 +194,30 @@ def test_halton_is_a_permuted_van_der_corput(oracle):
     assert np.bincount(strata, minlength=25).tolist() == [5] * 25
 
 
+def test_guide_oracle_encoding_is_the_frequency_basis():
+    """Every (input, octave, sin/cos) pair and every raw input appears exactly
+    once, in the first 225 columns; each column equals sin / cos(pi 2^(k+oct) x_c)
+    evaluated without range reduction."""
+    from oracle import guide as G
+    lay = G.feature_layout()
+    per = sorted((c, k, s) for kind, c, k, s in lay if kind == 2)
+    assert per == sorted((c, k, s) for c in range(9) for k in range(12) for s in (0, 1))
+    assert sorted(c for kind, c, _, _ in lay if kind == 1) == list(range(9))
+    assert all(kind == 0 for kind, *_ in lay[225:]) and all(kind != 0 for kind, *_ in lay[:225])
+    rng = np.random.default_rng(0)
+    x = rng.uniform(-1, 1, (50, 9))
+    for oct in (0, -2):
+        f = G.encode(x, oct)
+        for col, (kind, c, k, s) in enumerate(lay):
+            if kind == 2:
+                a = np.pi * 2.0 ** (k + oct) * x[:, c]
+                np.testing.assert_allclose(f[:, col], np.cos(a) if s else np.sin(a), atol=1e-9)
+            elif kind == 1:
+                assert np.array_equal(f[:, col], x[:, c])
+            else:
+                assert not f[:, col].any()
+
+
 def test_guide_oracle_backward_pinned_by_finite_differences():
     """The guide network's float64 oracle: its analytic gradient (the GPU's
     checker) against central differences of its own loss."""
@@ -206,6 +230,8 @@ def test_guide_oracle_backward_pinned_by_finite_differences():
     w /= np.linalg.norm(w, axis=1, keepdims=True)
     wt, pb = rng.uniform(0.1, 2, n), rng.uniform(0.05, 0.5, n)
     _, g = G.loss_and_grad(p, x, w, wt, pb)
+    # The gradient holds q constant inside q_mix: difference that surrogate.
+    q0, _, _ = G.pdf(p, x, w, pb)
     live = np.nonzero(G.live_mask())[0]
     sel = np.concatenate([rng.choice(live, 150, replace=False),
                           np.arange(G.N_PARAMS - 48 - 48 * 64, G.N_PARAMS - 48 * 64 + 64, 7),
@@ -214,7 +240,8 @@ def test_guide_oracle_backward_pinned_by_finite_differences():
     for k in sel:
         d = np.zeros_like(p)
         d[k] = eps
-        num = (G.loss_and_grad(p + d, x, w, wt, pb)[0] - G.loss_and_grad(p - d, x, w, wt, pb)[0]) / (2 * eps)
+        num = (G.loss_and_grad(p + d, x, w, wt, pb, q_fixed=q0)[0] -
+               G.loss_and_grad(p - d, x, w, wt, pb, q_fixed=q0)[0]) / (2 * eps)
         assert abs(num - g[k]) <= 1e-4 * (abs(num) + abs(g[k])) + 1e-9, (k, num, g[k])
     assert np.all(g[G.live_mask() == 0] == 0)
 

@@ -96,6 +96,10 @@ typedef struct pg_build_options {
  * BVH::Initialize (bvh.cpp:612-650).  opts may be NULL for defaults. */
 int pg_accel_build(pg_accel* a, const pg_build_options* opts);
 
+/* Builds the hierarchy again (other options, e.g. node width) from the same
+ * primitives; prim ids do not change.  No pine counterpart. */
+int pg_accel_rebuild(pg_accel* a, const pg_build_options* opts);
+
 /* Closest hit for n rays: replaces BVH::Intersect (bvh.cpp:669-709).  Hit rule:
  * the smallest t in (tmin, tmax) over all prims, using pine's per-shape tests;
  * on an exact tie in t the smaller prim id wins (pine keeps the first found in
@@ -121,6 +125,16 @@ int pg_trace_closest_counts(pg_accel* a, const pg_ray* d_rays, int64_t n, pg_hit
                             int32_t* d_counts, void* stream);
 
 /* Statistics of the uploaded hierarchy (for reporting and roofline accounting). */
+/* pg_trace_closest / pg_trace_occluded for a ray count held in device memory
+ * (*d_n <= capacity, read when the kernel runs): the launch is sized for
+ * capacity rays and does not depend on the count, so it can be recorded in a
+ * HIP graph whose earlier nodes produce the rays.  No pine counterpart (pine
+ * traces one ray per call, bvh.cpp:652-709). */
+int pg_trace_closest_dev(pg_accel* a, const pg_ray* d_rays, const int32_t* d_n, int64_t capacity,
+                         pg_hit* d_hits, pg_hit_uv* d_uv, void* stream);
+int pg_trace_occluded_dev(pg_accel* a, const pg_ray* d_rays, const int32_t* d_n, int64_t capacity,
+                          int32_t* d_occluded, void* stream);
+
 typedef struct pg_accel_stats {
     int64_t n_prims;
     int64_t n_nodes;
@@ -129,6 +143,8 @@ typedef struct pg_accel_stats {
     int32_t width;
     int32_t max_depth;
     double build_ms;
+    int32_t block;         /* traversal workgroup size (pg_accel_set_launch) */
+    int32_t launch_flags;  /* PG_LAUNCH_* bits */
 } pg_accel_stats;
 int pg_accel_get_stats(const pg_accel* a, pg_accel_stats* out);
 
@@ -176,19 +192,66 @@ typedef struct pg_scene_info {
     double last_guided_ms;   /* last pg_render_guided: training passes + final render */
     int64_t guide_steps;     /* last pg_render_guided: Adam steps taken */
     int64_t guide_records;   /* last pg_render_guided: nonzero-weight records trained on */
+    int64_t graph_captures;  /* bounce loops recorded as HIP graphs so far */
+    int64_t paths_in_flight; /* paths per batch when samples_per_batch is automatic */
+    int32_t trace_block, trace_flags, bvh_width;  /* traversal launch and node width */
 } pg_scene_info;
 int pg_scene_get_info(const pg_scene* s, pg_scene_info* info);
 
 typedef struct pg_render_options {
     int32_t spp;                /* 0 = the scene's samplesPerPixel */
-    int32_t samples_per_batch;  /* 0 = automatic (about 4M paths per batch) */
+    int32_t samples_per_batch;  /* 0 = automatic (the scene's paths in flight, 4M unless tuned) */
     int32_t sample_offset;      /* first sample index per pixel (progressive rendering) */
+    int32_t flags;              /* PG_RENDER_* bits */
 } pg_render_options;
+/* Run the fused pipeline's bounce loop launch by launch, counts read back per
+ * bounce, instead of replaying it as a recorded HIP graph (the default; both
+ * give the same image bit for bit). */
+#define PG_RENDER_NO_GRAPH 1
 
 /* Renders the whole film: PixelIntegrator::Render + RadianceIntegrator::Compute
  * + PathIntegrator::Li (integrator.cpp:121-158, path.cpp:6-87) for every pixel
  * and sample, pine's Halton sample sequence per pixel.  Synchronous. */
 int pg_render(pg_scene* s, const pg_render_options* opts, void* stream);
+
+/* Renders n scenes (each its own film; spp[i], or the scene's samplesPerPixel
+ * when spp is NULL) as one batch: the scenes are assigned, longest first, to
+ * `streams` concurrent streams (default 4) and enqueued without host round
+ * trips (fused scenes replay their recorded bounce loops), so small and large
+ * scenes share the GPU.  `stream` waits for the whole batch.  Every scene's
+ * info then holds its own time on its stream and its rays; out gets the batch
+ * wall time on `stream`, camera samples and rays.  A batch-level counterpart
+ * of PixelIntegrator::Render (integrator.cpp:121-150) over many scenes; pine
+ * renders one scene per process.  Synchronous. */
+typedef struct pg_batch_options {
+    int32_t streams;  /* concurrent streams, 1..16 (0 = 4) */
+    int32_t flags;    /* PG_RENDER_* bits for every scene */
+} pg_batch_options;
+typedef struct pg_batch_result {
+    double ms;        /* wall time of the batch on `stream` */
+    int64_t samples;  /* camera samples: sum of width * height * spp */
+    int64_t rays;     /* rays traced (extension + shadow) */
+} pg_batch_result;
+int pg_render_batch(pg_scene* const* scenes, int32_t n, const int32_t* spp, const pg_batch_options* o,
+                    void* stream, pg_batch_result* out);
+
+/* Chooses the scene's launch configuration by timing probe renders of the
+ * scene itself: traversal workgroup size (64 / 128 / 256, i.e. waves per CU)
+ * and leaf postponing, paths in flight per batch (queue size: 1M..8M), and,
+ * with try_width, the BVH node width (2 / 4; the hierarchy is rebuilt).  The
+ * best configuration stays set on the scene.  No pine counterpart. */
+typedef struct pg_tune_options {
+    int32_t probe_spp;  /* samples per pixel of a launch-configuration probe (0 = 1) */
+    int32_t try_width;  /* 1: also try the other node width */
+} pg_tune_options;
+typedef struct pg_tune_result {
+    int32_t trace_block, trace_flags, bvh_width;
+    int32_t probes;           /* probe renders made */
+    int64_t paths_in_flight;
+    double default_ms;        /* probe time of the starting configuration */
+    double best_ms;           /* probe time of the chosen configuration */
+} pg_tune_result;
+int pg_scene_autotune(pg_scene* s, const pg_tune_options* o, void* stream, pg_tune_result* out);
 
 /* Film accumulators of the last render, h*w*4 floats (sum w*r, sum w*g, sum w*b,
  * sum w) in pine's pixel order (film.h:60-62: row 0 is the top of the image).
@@ -269,9 +332,11 @@ int pg_guide_release(pg_guide* g);
  * records that brought radiance (`steps_per_pass` Adam steps of `batch`
  * records; 0 = one epoch, every record once, in at most 32 steps of at least
  * `batch` records); then the film is cleared and `final_spp` samples per
- * pixel are rendered with the trained guide.  The estimator stays unbiased
- * for any guide.  Zero fields take the defaults 16 / 8 / one epoch / 1024 /
- * 5 / the scene's spp.  Synchronous. */
+ * pixel are rendered with the trained guide.  Every pass and the final render
+ * take their own range of each pixel's sample sequence, from `sample_offset`
+ * on, so the guide never estimates with the samples it learned from.  The
+ * estimator stays unbiased for any guide.  Zero fields take the defaults
+ * 16 / 8 / one epoch / 1024 / 5 / the scene's spp / 0.  Synchronous. */
 typedef struct pg_guided_options {
     int32_t train_passes;
     int32_t train_spp;
@@ -279,6 +344,7 @@ typedef struct pg_guided_options {
     int32_t batch;
     int32_t record_depth;  /* path vertices recorded per path */
     int32_t final_spp;
+    int32_t sample_offset;  /* first sample index of the first training pass */
 } pg_guided_options;
 int pg_render_guided(pg_scene* s, pg_guide* g, const pg_guided_options* o, void* stream);
 

@@ -445,6 +445,8 @@ int ModeRender(int argc, char** argv) {
     if (argc < 4)
         return 2;
     int threads = Threads(argc, argv, 4);
+    // Optional first sample index per pixel (disjoint sample ranges).
+    const int64_t sample0 = argc > 5 ? atoll(argv[5]) : 0;
     auto scene = pstd::make_shared<Scene>();
     Parameters params = LoadScene(argv[2], scene.get());
     auto* ri = dynamic_cast<RadianceIntegrator*>(scene->integrator.get());
@@ -476,7 +478,7 @@ int ModeRender(int argc, char** argv) {
                 if (index >= (int64_t)size.x * size.y)
                     break;
                 vec2i p = {int(index % size.x), int(index / size.x)};
-                sampler.StartPixel(p, 0);
+                sampler.StartPixel(p, sample0);
                 for (int s = 0; s < spp; s++) {
                     // RadianceIntegrator::Compute (integrator.cpp:152-158)
                     vec2 pFilm = (p + sampler.Get2D()) / film.Size();

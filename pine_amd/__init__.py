@@ -30,8 +30,10 @@ EXPORTED = [
     "pg_scene_get_info", "pg_render", "pg_scene_film", "pg_scene_output_name", "pg_scene_release",
     "pg_gen_density_grid", "pg_guide_create", "pg_guide_set_params", "pg_guide_get_params",
     "pg_guide_forward", "pg_guide_pdf", "pg_guide_sample", "pg_guide_gradient", "pg_guide_train",
-    "pg_guide_release", "pg_render_guided",
+    "pg_guide_release", "pg_render_guided", "pg_trace_closest_dev", "pg_trace_occluded_dev",
+    "pg_accel_rebuild", "pg_render_batch", "pg_scene_autotune",
 ]
+RENDER_NO_GRAPH = 1
 GUIDE_N_PARAMS = 27888
 
 
@@ -48,7 +50,8 @@ class _Stats(ctypes.Structure):
     _fields_ = [("n_prims", ctypes.c_int64), ("n_nodes", ctypes.c_int64),
                 ("node_bytes", ctypes.c_int64), ("prim_bytes", ctypes.c_int64),
                 ("width", ctypes.c_int32), ("max_depth", ctypes.c_int32),
-                ("build_ms", ctypes.c_double)]
+                ("build_ms", ctypes.c_double), ("block", ctypes.c_int32),
+                ("launch_flags", ctypes.c_int32)]
 
 
 class _GuideConfig(ctypes.Structure):
@@ -63,18 +66,41 @@ class _SceneInfo(ctypes.Structure):
                 ("last_render_ms", ctypes.c_double), ("last_rays", ctypes.c_int64),
                 ("n_media", ctypes.c_int32), ("general_mode", ctypes.c_int32),
                 ("medium_steps", ctypes.c_int64), ("last_guided_ms", ctypes.c_double),
-                ("guide_steps", ctypes.c_int64), ("guide_records", ctypes.c_int64)]
+                ("guide_steps", ctypes.c_int64), ("guide_records", ctypes.c_int64),
+                ("graph_captures", ctypes.c_int64), ("paths_in_flight", ctypes.c_int64),
+                ("trace_block", ctypes.c_int32), ("trace_flags", ctypes.c_int32),
+                ("bvh_width", ctypes.c_int32)]
 
 
 class _GuidedOptions(ctypes.Structure):
     _fields_ = [("train_passes", ctypes.c_int32), ("train_spp", ctypes.c_int32),
                 ("steps_per_pass", ctypes.c_int32), ("batch", ctypes.c_int32),
-                ("record_depth", ctypes.c_int32), ("final_spp", ctypes.c_int32)]
+                ("record_depth", ctypes.c_int32), ("final_spp", ctypes.c_int32),
+                ("sample_offset", ctypes.c_int32)]
 
 
 class _RenderOptions(ctypes.Structure):
     _fields_ = [("spp", ctypes.c_int32), ("samples_per_batch", ctypes.c_int32),
-                ("sample_offset", ctypes.c_int32)]
+                ("sample_offset", ctypes.c_int32), ("flags", ctypes.c_int32)]
+
+
+class _BatchOptions(ctypes.Structure):
+    _fields_ = [("streams", ctypes.c_int32), ("flags", ctypes.c_int32)]
+
+
+class _BatchResult(ctypes.Structure):
+    _fields_ = [("ms", ctypes.c_double), ("samples", ctypes.c_int64), ("rays", ctypes.c_int64)]
+
+
+class _TuneOptions(ctypes.Structure):
+    _fields_ = [("probe_spp", ctypes.c_int32), ("try_width", ctypes.c_int32)]
+
+
+class _TuneResult(ctypes.Structure):
+    _fields_ = [("trace_block", ctypes.c_int32), ("trace_flags", ctypes.c_int32),
+                ("bvh_width", ctypes.c_int32), ("probes", ctypes.c_int32),
+                ("paths_in_flight", ctypes.c_int64), ("default_ms", ctypes.c_double),
+                ("best_ms", ctypes.c_double)]
 
 
 _lib = None
@@ -130,6 +156,12 @@ def lib():
     L.pg_guide_train.argtypes = [vp, vp, vp, vp, vp, i64, i32, i32, ctypes.POINTER(ctypes.c_float)]
     L.pg_guide_release.argtypes = [vp]
     L.pg_render_guided.argtypes = [vp, vp, ctypes.POINTER(_GuidedOptions), vp]
+    L.pg_trace_closest_dev.argtypes = [vp, vp, vp, i64, vp, vp, vp]
+    L.pg_trace_occluded_dev.argtypes = [vp, vp, vp, i64, vp, vp]
+    L.pg_accel_rebuild.argtypes = [vp, ctypes.POINTER(_BuildOptions)]
+    L.pg_render_batch.argtypes = [vp, i32, vp, ctypes.POINTER(_BatchOptions), vp,
+                                  ctypes.POINTER(_BatchResult)]
+    L.pg_scene_autotune.argtypes = [vp, ctypes.POINTER(_TuneOptions), vp, ctypes.POINTER(_TuneResult)]
     _lib = L
     return L
 
@@ -319,15 +351,22 @@ class Scene:
         check(lib().pg_scene_get_info(self.h, ctypes.byref(s)))
         return {k: getattr(s, k) for k, _ in _SceneInfo._fields_}
 
-    def render(self, spp=0, samples_per_batch=0, sample_offset=0):
-        o = _RenderOptions(spp, samples_per_batch, sample_offset)
+    def render(self, spp=0, samples_per_batch=0, sample_offset=0, graph=True):
+        o = _RenderOptions(spp, samples_per_batch, sample_offset, 0 if graph else RENDER_NO_GRAPH)
         check(lib().pg_render(self.h, ctypes.byref(o), _stream_handle()))
         return self.info()
 
+    def autotune(self, probe_spp=1, try_width=False):
+        """Picks the launch configuration by timing probe renders (pg_scene_autotune)."""
+        o, r = _TuneOptions(probe_spp, 1 if try_width else 0), _TuneResult()
+        check(lib().pg_scene_autotune(self.h, ctypes.byref(o), _stream_handle(), ctypes.byref(r)))
+        return {k: getattr(r, k) for k, _ in _TuneResult._fields_}
+
     def render_guided(self, guide, train_passes=16, train_spp=8, steps_per_pass=0, batch=1024,
-                      record_depth=5, final_spp=0):
+                      record_depth=5, final_spp=0, sample_offset=0):
         """Trains `guide` on this scene, then renders final_spp with it (pg_render_guided)."""
-        o = _GuidedOptions(train_passes, train_spp, steps_per_pass, batch, record_depth, final_spp)
+        o = _GuidedOptions(train_passes, train_spp, steps_per_pass, batch, record_depth, final_spp,
+                           sample_offset)
         check(lib().pg_render_guided(self.h, guide.h, ctypes.byref(o), _stream_handle()))
         return self.info()
 
@@ -354,6 +393,20 @@ class Scene:
             self.release()
         except Exception:
             pass
+
+
+def render_batch(scenes, spp=None, streams=4, graph=True):
+    """Renders a list of Scene objects as one batch (pg_render_batch); returns
+    the batch wall time, camera samples and rays.  Every scene's film and info
+    hold its own result afterwards."""
+    n = len(scenes)
+    arr = (ctypes.c_void_p * max(n, 1))(*[s.h for s in scenes])
+    sp = None
+    if spp is not None:
+        sp = (ctypes.c_int32 * max(n, 1))(*[int(x) for x in spp])
+    o, r = _BatchOptions(streams, 0 if graph else RENDER_NO_GRAPH), _BatchResult()
+    check(lib().pg_render_batch(arr, n, sp, ctypes.byref(o), _stream_handle(), ctypes.byref(r)))
+    return {k: getattr(r, k) for k, _ in _BatchResult._fields_}
 
 
 # --- learned path guiding ----------------------------------------------------------

@@ -68,6 +68,7 @@ struct pg_accel {
     int block = 128;
     int32_t launch_flags = PG_LAUNCH_DEFAULT;
     int stack_entries = 1;  // LDS stack entries per lane, from the built depth
+    int64_t generation = 0;  // bumped by every build and launch change (graph reuse key)
 
     void free_device() {
         if (d_nodes)
@@ -299,11 +300,14 @@ __global__ __launch_bounds__(kBlock) void k_trace(const float4* __restrict__ nod
                                                   pg_hit* __restrict__ hits,
                                                   pg_hit_uv* __restrict__ uvs,
                                                   int32_t* __restrict__ occluded,
-                                                  int2* __restrict__ counts, int xcd_remap) {
+                                                  int2* __restrict__ counts, int xcd_remap,
+                                                  const int32_t* __restrict__ d_n) {
     extern __shared__ int32_t lds_stack[];
     const int lane = threadIdx.x;
     const int64_t i = logical_block(xcd_remap != 0) * kBlock + lane;
-    if (i >= n)
+    // d_n: the ray count in device memory (a launch sized for a queue's
+    // capacity, as in a captured bounce loop); the blocks past it exit.
+    if (i >= (d_n ? (int64_t)*d_n : n))
         return;
     const RayIn r = load_ray(rays, i);
     float best_t = r.tmax, bu = 0.0f, bv = 0.0f;
@@ -372,12 +376,13 @@ __global__ __launch_bounds__(kBlock) void k_trace(const float4* __restrict__ nod
 
 template <int W, int kMode, bool kStats, bool kPostpone>
 void launch_w(const pg_accel* a, int64_t grid, size_t lds, const pg_ray* rays, int64_t n,
-              pg_hit* hits, pg_hit_uv* uvs, int32_t* occ, int2* counts, hipStream_t s) {
+              pg_hit* hits, pg_hit_uv* uvs, int32_t* occ, int2* counts, hipStream_t s,
+              const int32_t* d_n) {
     const int remap = (a->launch_flags & PG_LAUNCH_XCD_REMAP) ? 1 : 0;
 #define PG_LAUNCH(B)                                                                           \
     hipLaunchKernelGGL((k_trace<W, B, kMode, kStats, kPostpone>), dim3((unsigned)grid), dim3(B), \
                        lds, s, a->d_nodes, a->d_tris, a->d_ana, rays, n, hits, uvs, occ,         \
-                       counts, remap)
+                       counts, remap, d_n)
     if (a->block == 64)
         PG_LAUNCH(64);
     else if (a->block == 256)
@@ -389,20 +394,21 @@ void launch_w(const pg_accel* a, int64_t grid, size_t lds, const pg_ray* rays, i
 
 template <int kMode, bool kStats>
 hipError_t launch_trace(const pg_accel* a, const pg_ray* rays, int64_t n, pg_hit* hits,
-                        pg_hit_uv* uvs, int32_t* occ, int2* counts, hipStream_t s) {
+                        pg_hit_uv* uvs, int32_t* occ, int2* counts, hipStream_t s,
+                        const int32_t* d_n = nullptr) {
     const int64_t grid = (n + a->block - 1) / a->block;
     const size_t lds = (size_t)a->stack_entries * a->block * sizeof(int32_t);
     const bool postpone = (a->launch_flags & PG_LAUNCH_POSTPONE_LEAVES) != 0;
     if (a->width == 2) {
         if (postpone)
-            launch_w<2, kMode, kStats, true>(a, grid, lds, rays, n, hits, uvs, occ, counts, s);
+            launch_w<2, kMode, kStats, true>(a, grid, lds, rays, n, hits, uvs, occ, counts, s, d_n);
         else
-            launch_w<2, kMode, kStats, false>(a, grid, lds, rays, n, hits, uvs, occ, counts, s);
+            launch_w<2, kMode, kStats, false>(a, grid, lds, rays, n, hits, uvs, occ, counts, s, d_n);
     } else {
         if (postpone)
-            launch_w<4, kMode, kStats, true>(a, grid, lds, rays, n, hits, uvs, occ, counts, s);
+            launch_w<4, kMode, kStats, true>(a, grid, lds, rays, n, hits, uvs, occ, counts, s, d_n);
         else
-            launch_w<4, kMode, kStats, false>(a, grid, lds, rays, n, hits, uvs, occ, counts, s);
+            launch_w<4, kMode, kStats, false>(a, grid, lds, rays, n, hits, uvs, occ, counts, s, d_n);
     }
     return hipGetLastError();
 }
@@ -533,6 +539,13 @@ extern "C" int pg_accel_add_plane(pg_accel* a, const float pos[3], const float n
     return add_analytic(a, rec, box, prim_out);
 }
 
+extern "C" int pg_accel_rebuild(pg_accel* a, const pg_build_options* opts) {
+    if (!a)
+        return pg_fail(PG_E_ARG, "pg_accel_rebuild: null accel");
+    a->built = false;
+    return pg_accel_build(a, opts);
+}
+
 extern "C" int pg_accel_build(pg_accel* a, const pg_build_options* opts) {
     if (int rc = check_open(a, "pg_accel_build"))
         return rc;
@@ -622,6 +635,7 @@ extern "C" int pg_accel_build(pg_accel* a, const pg_build_options* opts) {
     a->n_tri_slots = (int64_t)flat.tri_order.size();
     a->n_ana_slots = (int64_t)flat.ana_order.size();
     a->built = true;
+    a->generation++;
     a->build_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return PG_OK;
@@ -660,6 +674,30 @@ extern "C" int pg_trace_any(pg_accel* a, const pg_ray* d_rays, int64_t n, int32_
     return PG_OK;
 }
 
+extern "C" int pg_trace_closest_dev(pg_accel* a, const pg_ray* d_rays, const int32_t* d_n,
+                                    int64_t capacity, pg_hit* d_hits, pg_hit_uv* d_uv, void* stream) {
+    if (int rc = check_trace(a, d_rays, capacity, d_hits, "pg_trace_closest_dev"))
+        return rc;
+    if (!d_n)
+        return pg_fail(PG_E_ARG, "pg_trace_closest_dev: null count");
+    if (capacity == 0)
+        return PG_OK;
+    PG_HIP((launch_trace<kClosest, false>(a, d_rays, capacity, d_hits, d_uv, nullptr, nullptr,
+                                       (hipStream_t)stream, d_n)));
+    return PG_OK;
+}
+extern "C" int pg_trace_occluded_dev(pg_accel* a, const pg_ray* d_rays, const int32_t* d_n,
+                                     int64_t capacity, int32_t* d_occluded, void* stream) {
+    if (int rc = check_trace(a, d_rays, capacity, d_occluded, "pg_trace_occluded_dev"))
+        return rc;
+    if (!d_n)
+        return pg_fail(PG_E_ARG, "pg_trace_occluded_dev: null count");
+    if (capacity == 0)
+        return PG_OK;
+    PG_HIP((launch_trace<kOccluded, false>(a, d_rays, capacity, nullptr, nullptr, d_occluded, nullptr,
+                                          (hipStream_t)stream, d_n)));
+    return PG_OK;
+}
 extern "C" int pg_trace_occluded(pg_accel* a, const pg_ray* d_rays, int64_t n,
                                   int32_t* d_occluded, void* stream) {
     if (int rc = check_trace(a, d_rays, n, d_occluded, "pg_trace_occluded"))
@@ -694,6 +732,8 @@ extern "C" int pg_accel_get_stats(const pg_accel* a, pg_accel_stats* out) {
     out->width = a->width;
     out->max_depth = a->max_depth;
     out->build_ms = a->build_ms;
+    out->block = a->block;
+    out->launch_flags = a->launch_flags;
     return PG_OK;
 }
 
@@ -719,8 +759,11 @@ extern "C" int pg_accel_set_launch(pg_accel* a, int32_t block, int32_t flags) {
         a->block = block;
     if (flags >= 0)
         a->launch_flags = flags;
+    a->generation++;
     return PG_OK;
 }
+
+int64_t pg::accel_generation(const pg_accel* a) { return a ? a->generation : 0; }
 
 extern "C" int pg_accel_release(pg_accel* a) {
     if (!a)

@@ -197,8 +197,13 @@ __device__ __forceinline__ void store_ray(pg_ray* r, f3 o, f3 d, float tmin, flo
     p[1] = make_float4(d.x, d.y, d.z, tmax);
 }
 
-__global__ __launch_bounds__(256) void k_raygen(SceneDev s, PathsDev P, int32_t n_paths, int32_t spb, int32_t sample0) {
+// d_np (optional): receives n_paths, the first bounce's path count for a
+// bounce loop that reads its counts from device memory.
+__global__ __launch_bounds__(256) void k_raygen(SceneDev s, PathsDev P, int32_t n_paths, int32_t spb, int32_t sample0,
+                                                int32_t* d_np) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0 && d_np)
+        *d_np = n_paths;
     if (i >= n_paths)
         return;
     const int32_t pixel = i / spb, sl = i - pixel * spb;
@@ -471,10 +476,12 @@ __device__ bool guided_direction(const SceneDev& s, PathsDev& P, SamplerState& s
 // kGuided: directions at the vertices the guide serves (gflag, set for the
 // bounce by k_guide_select / k_guide_infer) come from the guide/BSDF mixture,
 // and the light sample's MIS weight uses the mixture pdf.
+// d_active (optional): the active count in device memory, read in place of
+// n_active (launches sized for the queue's capacity).
 template <bool kGuided>
-__global__ __launch_bounds__(256) void k_shade(SceneDev s, PathsDev P, int32_t n_active) {
+__global__ __launch_bounds__(256) void k_shade(SceneDev s, PathsDev P, int32_t n_active, const int32_t* d_active) {
     const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n_active)
+    if (k >= (d_active ? *d_active : n_active))
         return;
     const int32_t i = P.queue_in[k];
     const float4* r4 = reinterpret_cast<const float4*>(&P.rays_in[k]);
@@ -617,9 +624,9 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev s, PathsDev P, int32_t n
     }
 }
 
-__global__ __launch_bounds__(256) void k_resolve_shadow(PathsDev P, int32_t n_shadow) {
+__global__ __launch_bounds__(256) void k_resolve_shadow(PathsDev P, int32_t n_shadow, const int32_t* d_shadow) {
     const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n_shadow || P.occluded[k])
+    if (k >= (d_shadow ? *d_shadow : n_shadow) || P.occluded[k])
         return;
     const int32_t i = P.shadow_path[k];
     // One shadow ray per path per bounce: no two lanes share a path here.
@@ -1012,8 +1019,12 @@ __global__ __launch_bounds__(256) void k_shade_gen_b(SceneDev s, PathsDev P, int
 // after the staging barrier: a grid-stride loop over tiles of 16 per wave.
 constexpr int kInferThreads = 512;
 template <int kSrc>
-__global__ __launch_bounds__(kInferThreads) void k_guide_infer(SceneDev s, PathsDev P, int32_t n) {
+__global__ __launch_bounds__(kInferThreads) void k_guide_infer(SceneDev s, PathsDev P, int32_t n_in,
+                                                               const int32_t* d_n) {
     using namespace guide;
+    const int32_t n = d_n ? *d_n : n_in;
+    if ((int64_t)blockIdx.x * (kInferThreads / 4) >= n)
+        return;  // whole workgroup: no tile of it is in range, no barrier reached
     constexpr int kLd = kH + 8, kLd0 = kFeat + 8;
     __shared__ __attribute__((aligned(16))) __bf16 sw[(2 * kH + kOutPad) * kLd];
     __shared__ __attribute__((aligned(16))) __bf16 sw0[kH * kLd0];
@@ -1301,8 +1312,35 @@ struct pg_scene {
     int64_t guide_capacity = 0, record_capacity = 0;
     double last_guide_ms = 0.0;
     int64_t guide_steps = 0, guide_records = 0;
+    // The fused pipeline's bounce loop recorded as a HIP graph (counts stay in
+    // device memory), replayed for every batch while its key holds.
+    struct GraphKey {
+        int64_t capacity = -1, accel_gen = -1, paths_gen = -1;
+        int guided = 0, oct = 0, max_depth = 0;
+        const void* g_params = nullptr;
+        const void* g_wf = nullptr;
+        bool operator==(const GraphKey& o) const {
+            return capacity == o.capacity && accel_gen == o.accel_gen && paths_gen == o.paths_gen &&
+                   guided == o.guided && oct == o.oct && max_depth == o.max_depth &&
+                   g_params == o.g_params && g_wf == o.g_wf;
+        }
+    };
+    bool use_graph = true;
+    GraphKey graph_key;
+    hipGraphExec_t graph_exec = nullptr;
+    hipStream_t capture_stream = nullptr;
+    int32_t* g_counts = nullptr;         // 4 per bounce after the batch's path count
+    unsigned long long* g_rays = nullptr;  // rays traced (extension + shadow)
+    int64_t paths_gen = 0;               // bumped when path or guide state is reallocated
+    int64_t graph_captures = 0;
+    int64_t batch_paths = 4 << 20;       // paths per batch when not given (tunable)
+    pg_build_options build_opts{};       // the accel's build options (rebuilds keep them)
 
     ~pg_scene() {
+        if (graph_exec)
+            (void)hipGraphExecDestroy(graph_exec);
+        if (capture_stream)
+            (void)hipStreamDestroy(capture_stream);
         for (void* p : allocations)
             (void)hipFree(p);
         if (accel)
@@ -2030,6 +2068,8 @@ int dalloc(pg_scene& S, int64_t n, T** out) {
 }
 
 int finish_scene(pg_scene& S, const pg_build_options* opts) {
+    if (opts)
+        S.build_opts = *opts;
     if (int rc = pg_accel_build(S.accel, opts))
         return rc;
     std::vector<int> primes, sums;
@@ -2082,6 +2122,7 @@ int ensure_paths(pg_scene& S, int64_t n) {
         (rc = dalloc(S, 3 * n, &P.shadow_contrib)) || (rc = dalloc(S, n, &P.occluded)) ||
         (rc = dalloc(S, 1, &P.spill_count)))
         return rc;
+    S.paths_gen++;
     P.spill_capacity = (int32_t)std::min<int64_t>(n, 1 << 20);
     if ((rc = dalloc(S, 5 * (int64_t)P.spill_capacity, &P.spill)))
         return rc;
@@ -2108,9 +2149,11 @@ int ensure_guide(pg_scene& S, int64_t n, int rec_depth) {
             (rc = dalloc(S, n, &P.shadow_rec)))
             return rc;
         S.guide_capacity = n;
+        S.paths_gen++;
     }
     const int64_t slots = n * rec_depth;
     if (slots > S.record_capacity) {
+        S.paths_gen++;
         if ((rc = dalloc(S, (int64_t)guide::kIn * slots, &P.rec_x)) || (rc = dalloc(S, 3 * slots, &P.rec_dir)) ||
             (rc = dalloc(S, slots, &P.rec_w)) || (rc = dalloc(S, slots, &P.rec_pb)) ||
             (rc = dalloc(S, slots, &P.rec_pdf)) || (rc = dalloc(S, 3 * slots, &P.rec_lb)) ||
@@ -2191,6 +2234,14 @@ extern "C" int pg_scene_get_info(const pg_scene* S, pg_scene_info* info) {
     info->last_guided_ms = S->last_guide_ms;
     info->guide_steps = S->guide_steps;
     info->guide_records = S->guide_records;
+    info->graph_captures = S->graph_captures;
+    info->paths_in_flight = S->batch_paths;
+    pg_accel_stats as{};
+    if (pg_accel_get_stats(S->accel, &as) == PG_OK) {
+        info->bvh_width = as.width;
+        info->trace_block = as.block;
+        info->trace_flags = as.launch_flags;
+    }
     return PG_OK;
 }
 
@@ -2239,7 +2290,8 @@ static int bounce_general(pg_scene* S, PathsDev& Q, int32_t& active, hipStream_t
         src = 1 - src;
     }
     if (S->dev.guided && n_b > 0)
-        hipLaunchKernelGGL(k_guide_infer<0>, dim3(infer_blocks(n_b)), dim3(kInferThreads), 0, st, S->dev, Q, n_b);
+        hipLaunchKernelGGL(k_guide_infer<0>, dim3(infer_blocks(n_b)), dim3(kInferThreads), 0, st, S->dev, Q, n_b,
+                           nullptr);
     if (n_rec > 0)
         hipLaunchKernelGGL(k_resolve_gen, dim3((n_rec + B - 1) / B), dim3(B), 0, st, S->dev, Q, n_rec);
     if (n_b > 0)
@@ -2251,6 +2303,100 @@ static int bounce_general(pg_scene* S, PathsDev& Q, int32_t& active, hipStream_t
 }
 
 // Per-batch training of the guide during a training pass.
+__global__ void k_fill_i32(int32_t* p, int32_t n, int32_t v) {
+    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n)
+        p[k] = v;
+}
+
+// Rays of one bounce: extension rays traced plus shadow rays traced.
+__global__ void k_count_rays(const int32_t* d_active, const int32_t* d_shadow, unsigned long long* total) {
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        *total += (unsigned long long)*d_active + (unsigned long long)*d_shadow;
+}
+
+// The fused pipeline's bounce loop for up to `cap` paths with every count in
+// device memory: bounce b reads its path count from the previous bounce's
+// counters (bounce 0 from g_counts[0], written by k_raygen) and writes its
+// own (g_counts + 4(b + 1): next paths, shadow rays); every launch is sized
+// for cap.  A fused vertex always adds one to the path depth, so max_depth
+// bounces finish every path.  Counters are cleared by a fill kernel, not a
+// memset node.
+static int record_bounces(pg_scene* S, const PathsDev& P, int64_t cap, hipStream_t st) {
+    const int B = 256;
+    const int nb = S->max_depth;
+    const unsigned grid = (unsigned)((cap + B - 1) / B);
+    hipLaunchKernelGGL(k_fill_i32, dim3((4 * nb + B - 1) / B), dim3(B), 0, st, S->g_counts + 4, 4 * nb, 0);
+    PathsDev Q = P;
+    for (int b = 0; b < nb; b++) {
+        const int32_t* d_active = S->g_counts + 4 * b;
+        Q.counters = S->g_counts + 4 * (b + 1);
+        if (int rc = pg_trace_closest_dev(S->accel, Q.rays_in, d_active, cap, Q.hits, Q.uvs, st))
+            return rc;
+        if (S->dev.guided) {
+            hipLaunchKernelGGL(k_guide_infer<1>, dim3(infer_blocks((int32_t)cap)), dim3(kInferThreads), 0, st,
+                               S->dev, Q, 0, d_active);
+            hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(B), 0, st, S->dev, Q, 0, d_active);
+        } else {
+            hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(B), 0, st, S->dev, Q, 0, d_active);
+        }
+        if (int rc = pg_trace_occluded_dev(S->accel, Q.shadow_rays, Q.counters + 1, cap, Q.occluded, st))
+            return rc;
+        hipLaunchKernelGGL(k_resolve_shadow, dim3(grid), dim3(B), 0, st, Q, 0, Q.counters + 1);
+        hipLaunchKernelGGL(k_count_rays, dim3(1), dim3(64), 0, st, d_active, Q.counters + 1, S->g_rays);
+        std::swap(Q.queue_in, Q.queue_out);
+        std::swap(Q.rays_in, Q.rays_out);
+    }
+    PG_HIP(hipGetLastError());
+    return PG_OK;
+}
+
+// Records the bounce loop for `cap` paths into the scene's graph unless the
+// one it holds was recorded for the same state.
+static int ensure_graph(pg_scene* S, const PathsDev& P, int64_t cap) {
+    pg_scene::GraphKey key;
+    key.capacity = cap;
+    key.accel_gen = accel_generation(S->accel);
+    key.paths_gen = S->paths_gen;
+    key.guided = S->dev.guided;
+    key.oct = S->dev.g_oct;
+    key.max_depth = S->max_depth;
+    key.g_params = S->dev.g_params;
+    key.g_wf = S->dev.g_wf;
+    if (S->graph_exec && key == S->graph_key)
+        return PG_OK;
+    if (S->graph_exec) {
+        (void)hipGraphExecDestroy(S->graph_exec);
+        S->graph_exec = nullptr;
+    }
+    if (!S->g_counts) {
+        if (int rc = dalloc(*S, 4 * (S->max_depth + 1), &S->g_counts))
+            return rc;
+        if (int rc = dalloc(*S, 1, &S->g_rays))
+            return rc;
+    }
+    if (!S->capture_stream)
+        PG_HIP(hipStreamCreateWithFlags(&S->capture_stream, hipStreamNonBlocking));
+    PG_HIP(hipStreamBeginCapture(S->capture_stream, hipStreamCaptureModeThreadLocal));
+    const int rc = record_bounces(S, P, cap, S->capture_stream);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(S->capture_stream, &g);
+    if (rc != PG_OK || e != hipSuccess) {
+        if (g)
+            (void)hipGraphDestroy(g);
+        return rc != PG_OK ? rc : pg_fail(PG_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    }
+    const hipError_t ei = hipGraphInstantiate(&S->graph_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        S->graph_exec = nullptr;
+        return pg_fail(PG_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
+    }
+    S->graph_key = key;
+    S->graph_captures++;
+    return PG_OK;
+}
+
 constexpr int64_t kAutoSteps = 32;
 struct GuideTraining {
     pg_guide* G;
@@ -2272,13 +2418,31 @@ static uint64_t gcd64(uint64_t a, uint64_t b) {
 
 // Renders spp samples per pixel into the (cleared) film; with `tr`, records
 // the guided vertices of every batch and trains the guide after it.
-static int render_impl(pg_scene* S, int spp, int spb_req, hipStream_t st, GuideTraining* tr,
-                       double* ms_out, int64_t* rays_out, int sample_offset = 0) {
+// A render enqueued on a stream: its bracketing events and what the host
+// reads once it is done.
+struct RenderTicket {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool graph = false;
+    int64_t rays = 0;
+    void release() {
+        if (e0)
+            (void)hipEventDestroy(e0);
+        if (e1)
+            (void)hipEventDestroy(e1);
+        e0 = e1 = nullptr;
+    }
+};
+
+// Enqueues a whole render on st.  Without training and in the fused pipeline
+// nothing waits on the host (graph replay per batch); training passes and the
+// split bounce read counts back per bounce or per batch.
+static int render_enqueue(pg_scene* S, int spp, int spb_req, hipStream_t st, GuideTraining* tr,
+                          int sample_offset, RenderTicket* t) {
     const int64_t npix = (int64_t)S->dev.width * S->dev.height;
     int spb = spb_req;
     if (spb <= 0) {
-        // About 4M paths per batch: enough lanes for 256 CUs, bounded state.
-        spb = (int)std::max<int64_t>(1, std::min<int64_t>(spp, (4 << 20) / std::max<int64_t>(npix, 1)));
+        // About 4M paths per batch (unless tuned): enough lanes for 256 CUs, bounded state.
+        spb = (int)std::max<int64_t>(1, std::min<int64_t>(spp, S->batch_paths / std::max<int64_t>(npix, 1)));
     }
     spb = std::min(spb, spp);
     const int64_t n_paths = npix * spb;
@@ -2290,15 +2454,22 @@ static int render_impl(pg_scene* S, int spp, int spb_req, hipStream_t st, GuideT
         if (int rc = ensure_guide(*S, n_paths, tr ? tr->rec_depth : 0))
             return rc;
     }
-    hipEvent_t e0, e1;
-    PG_HIP(hipEventCreate(&e0));
-    PG_HIP(hipEventCreate(&e1));
-    PG_HIP(hipEventRecord(e0, st));
+    PG_HIP(hipEventCreate(&t->e0));
+    PG_HIP(hipEventCreate(&t->e1));
+    PG_HIP(hipEventRecord(t->e0, st));
     PG_HIP(hipMemsetAsync(S->d_film, 0, (size_t)npix * 4 * sizeof(float), st));
     PathsDev P = S->paths;
     P.rec_depth = tr ? tr->rec_depth : 0;
     if (P.lookups)
         PG_HIP(hipMemsetAsync(P.lookups, 0, sizeof(unsigned long long), st));
+    // Fused pipeline without training: the bounce loop is a graph replay per
+    // batch (no host round trip per bounce).
+    const bool graph = S->use_graph && !S->general && !tr;
+    if (graph) {
+        if (int rc = ensure_graph(S, P, n_paths))
+            return rc;
+        PG_HIP(hipMemsetAsync(S->g_rays, 0, sizeof(unsigned long long), st));
+    }
     int64_t rays = 0;
     const int B = 256;
     for (int s0 = 0; s0 < spp; s0 += spb) {
@@ -2309,8 +2480,10 @@ static int render_impl(pg_scene* S, int spp, int spb_req, hipStream_t st, GuideT
             hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((n_slots + B - 1) / B)), dim3(B), 0, st,
                                P.rec_pdf, n_slots, 0.0f);
         hipLaunchKernelGGL(k_raygen, dim3((np + B - 1) / B), dim3(B), 0, st, S->dev, P, np, nb,
-                           s0 + sample_offset);
-        int32_t active = np;
+                           s0 + sample_offset, graph ? S->g_counts : nullptr);
+        int32_t active = graph ? 0 : np;
+        if (graph)
+            PG_HIP(hipGraphLaunch(S->graph_exec, st));
         PathsDev Q = P;
         // Depth bounds every path except those crossing material-less surfaces,
         // which pine does not count (path.cpp:42-46); the bound guards those.
@@ -2321,12 +2494,12 @@ static int render_impl(pg_scene* S, int spp, int spb_req, hipStream_t st, GuideT
             if (!S->general) {
                 if (S->dev.guided) {
                     hipLaunchKernelGGL(k_guide_infer<1>, dim3(infer_blocks(active)), dim3(kInferThreads), 0, st,
-                                       S->dev, Q, active);
+                                       S->dev, Q, active, nullptr);
                     hipLaunchKernelGGL(k_shade<true>, dim3((active + B - 1) / B), dim3(B), 0, st, S->dev, Q,
-                                       active);
+                                       active, nullptr);
                 } else {
                     hipLaunchKernelGGL(k_shade<false>, dim3((active + B - 1) / B), dim3(B), 0, st, S->dev, Q,
-                                       active);
+                                       active, nullptr);
                 }
                 int32_t counts[2];
                 PG_HIP(hipMemcpyAsync(counts, Q.counters, sizeof(counts), hipMemcpyDeviceToHost, st));
@@ -2336,7 +2509,7 @@ static int render_impl(pg_scene* S, int spp, int spb_req, hipStream_t st, GuideT
                     if (int rc = pg_trace_occluded(S->accel, Q.shadow_rays, counts[1], Q.occluded, st))
                         return rc;
                     hipLaunchKernelGGL(k_resolve_shadow, dim3((counts[1] + B - 1) / B), dim3(B), 0, st,
-                                       Q, counts[1]);
+                                       Q, counts[1], nullptr);
                 }
                 active = counts[0];
             } else {
@@ -2392,21 +2565,44 @@ static int render_impl(pg_scene* S, int spp, int spb_req, hipStream_t st, GuideT
         hipLaunchKernelGGL(k_film_spill, dim3(64), dim3(B), 0, st, P, S->d_film);
     }
     PG_HIP(hipGetLastError());
-    PG_HIP(hipEventRecord(e1, st));
-    PG_HIP(hipEventSynchronize(e1));
+    PG_HIP(hipEventRecord(t->e1, st));
+    t->graph = graph;
+    t->rays = rays;
+    return PG_OK;
+}
+
+// Waits for an enqueued render; its time on its stream and rays traced.
+static int render_finish(pg_scene* S, RenderTicket* t, double* ms_out, int64_t* rays_out) {
+    PG_HIP(hipEventSynchronize(t->e1));
     float ms = 0.0f;
-    PG_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    PG_HIP(hipEventElapsedTime(&ms, t->e0, t->e1));
+    t->release();
+    int64_t rays = t->rays;
+    if (t->graph) {
+        unsigned long long n = 0;
+        PG_HIP(hipMemcpy(&n, S->g_rays, sizeof(n), hipMemcpyDeviceToHost));
+        rays = (int64_t)n;
+    }
     *ms_out = ms;
     *rays_out = rays;
     S->last_lookups = 0;
-    if (P.lookups) {
+    if (S->paths.lookups) {
         unsigned long long n = 0;
-        PG_HIP(hipMemcpy(&n, P.lookups, sizeof(n), hipMemcpyDeviceToHost));
+        PG_HIP(hipMemcpy(&n, S->paths.lookups, sizeof(n), hipMemcpyDeviceToHost));
         S->last_lookups = (int64_t)n;
     }
     return PG_OK;
+}
+
+static int render_impl(pg_scene* S, int spp, int spb_req, hipStream_t st, GuideTraining* tr,
+                       double* ms_out, int64_t* rays_out, int sample_offset = 0) {
+    RenderTicket t;
+    if (int rc = render_enqueue(S, spp, spb_req, st, tr, sample_offset, &t)) {
+        (void)hipStreamSynchronize(st);
+        t.release();
+        return rc;
+    }
+    return render_finish(S, &t, ms_out, rays_out);
 }
 
 extern "C" int pg_render(pg_scene* S, const pg_render_options* opts, void* stream) {
@@ -2416,13 +2612,213 @@ extern "C" int pg_render(pg_scene* S, const pg_render_options* opts, void* strea
     const int spb = opts && opts->samples_per_batch > 0 ? opts->samples_per_batch : 0;
     const int offset = opts && opts->sample_offset > 0 ? opts->sample_offset : 0;
     S->dev.guided = 0;
+    S->use_graph = !(opts && (opts->flags & PG_RENDER_NO_GRAPH));
     double ms = 0.0;
     int64_t rays = 0;
-    if (int rc = render_impl(S, spp, spb, (hipStream_t)stream, nullptr, &ms, &rays, offset))
+    const int rc = render_impl(S, spp, spb, (hipStream_t)stream, nullptr, &ms, &rays, offset);
+    S->use_graph = true;
+    if (rc)
         return rc;
     S->last_render_ms = ms;
     S->last_rays = rays;
     S->last_guide_ms = 0.0;
+    return PG_OK;
+}
+
+extern "C" int pg_render_batch(pg_scene* const* scenes, int32_t n, const int32_t* spp,
+                               const pg_batch_options* o, void* stream, pg_batch_result* out) {
+    if (n < 0 || (n > 0 && !scenes))
+        return pg_fail(PG_E_ARG, "pg_render_batch: bad scene list");
+    for (int32_t i = 0; i < n; i++)
+        if (!scenes[i])
+            return pg_fail(PG_E_ARG, "pg_render_batch: null scene");
+    const int K = o && o->streams > 0 ? std::min(o->streams, 16) : 4;
+    const int flags = o ? o->flags : 0;
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<hipStream_t> ws(K, nullptr);
+    std::vector<hipEvent_t> done(K, nullptr);
+    hipEvent_t start = nullptr, stop = nullptr;
+    std::vector<RenderTicket> tickets(n);
+    auto cleanup = [&]() {
+        for (auto& t : tickets)
+            t.release();
+        for (int k = 0; k < K; k++) {
+            if (ws[k])
+                (void)hipStreamDestroy(ws[k]);
+            if (done[k])
+                (void)hipEventDestroy(done[k]);
+        }
+        if (start)
+            (void)hipEventDestroy(start);
+        if (stop)
+            (void)hipEventDestroy(stop);
+    };
+    int rc = PG_OK;
+    auto hip = [&](hipError_t e, const char* what) {
+        if (e != hipSuccess && rc == PG_OK)
+            rc = pg_fail(PG_E_HIP, std::string("pg_render_batch: ") + what + ": " + hipGetErrorString(e));
+        return rc == PG_OK;
+    };
+    hip(hipEventCreate(&start), "event") && hip(hipEventCreate(&stop), "event");
+    for (int k = 0; k < K && rc == PG_OK; k++)
+        hip(hipStreamCreateWithFlags(&ws[k], hipStreamNonBlocking), "stream") && hip(hipEventCreate(&done[k]), "event");
+    if (rc == PG_OK)
+        hip(hipEventRecord(start, st), "record");
+    for (int k = 0; k < K && rc == PG_OK; k++)
+        hip(hipStreamWaitEvent(ws[k], start, 0), "wait");
+    // Longest first onto the least loaded stream (cost ~ camera samples).
+    std::vector<int32_t> order(n);
+    std::vector<int64_t> cost(n);
+    for (int32_t i = 0; i < n; i++) {
+        order[i] = i;
+        const int sp = spp && spp[i] > 0 ? spp[i] : scenes[i]->spp;
+        cost[i] = (int64_t)scenes[i]->dev.width * scenes[i]->dev.height * sp;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return cost[a] > cost[b]; });
+    std::vector<int64_t> load(K, 0);
+    for (int32_t j = 0; j < n && rc == PG_OK; j++) {
+        const int32_t i = order[j];
+        const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[k] += cost[i];
+        pg_scene* S = scenes[i];
+        const int sp = spp && spp[i] > 0 ? spp[i] : S->spp;
+        S->dev.guided = 0;
+        S->use_graph = !(flags & PG_RENDER_NO_GRAPH);
+        rc = render_enqueue(S, sp, 0, ws[k], nullptr, 0, &tickets[i]);
+        S->use_graph = true;
+    }
+    for (int k = 0; k < K && rc == PG_OK; k++)
+        hip(hipEventRecord(done[k], ws[k]), "record") && hip(hipStreamWaitEvent(st, done[k], 0), "wait");
+    if (rc == PG_OK)
+        hip(hipEventRecord(stop, st), "record");
+    if (rc != PG_OK) {
+        for (int k = 0; k < K; k++)
+            if (ws[k])
+                (void)hipStreamSynchronize(ws[k]);
+        cleanup();
+        return rc;
+    }
+    pg_batch_result r{};
+    for (int32_t i = 0; i < n && rc == PG_OK; i++) {
+        double ms = 0.0;
+        int64_t rays = 0;
+        rc = render_finish(scenes[i], &tickets[i], &ms, &rays);
+        scenes[i]->last_render_ms = ms;
+        scenes[i]->last_rays = rays;
+        scenes[i]->last_guide_ms = 0.0;
+        const int sp = spp && spp[i] > 0 ? spp[i] : scenes[i]->spp;
+        r.samples += (int64_t)scenes[i]->dev.width * scenes[i]->dev.height * sp;
+        r.rays += rays;
+    }
+    if (rc == PG_OK && hip(hipEventSynchronize(stop), "sync")) {
+        float ms = 0.0f;
+        if (hip(hipEventElapsedTime(&ms, start, stop), "elapsed"))
+            r.ms = ms;
+    }
+    cleanup();
+    if (rc == PG_OK && out)
+        *out = r;
+    return rc;
+}
+
+// One probe render of the scene as configured; its time on the stream.
+static int probe(pg_scene* S, int spp, hipStream_t st, double* ms) {
+    S->dev.guided = 0;
+    int64_t rays = 0;
+    return render_impl(S, spp, 0, st, nullptr, ms, &rays, 0);
+}
+
+extern "C" int pg_scene_autotune(pg_scene* S, const pg_tune_options* o, void* stream, pg_tune_result* out) {
+    if (!S)
+        return pg_fail(PG_E_ARG, "pg_scene_autotune: null scene");
+    hipStream_t st = (hipStream_t)stream;
+    const int pspp = std::max(1, std::min(o && o->probe_spp > 0 ? o->probe_spp : 1, S->spp));
+    pg_tune_result r{};
+    int rc;
+    double ms = 0.0, best = 0.0;
+    auto measure = [&](int spp, double* t) -> int {
+        double a = 0.0, b = 0.0;
+        if (int e = probe(S, spp, st, &a))  // first run records the graph
+            return e;
+        if (int e = probe(S, spp, st, &b))
+            return e;
+        r.probes += 2;
+        *t = std::min(a, b);
+        return PG_OK;
+    };
+    pg_accel_stats as{};
+    if ((rc = pg_accel_get_stats(S->accel, &as)))
+        return rc;
+    if ((rc = measure(pspp, &best)))
+        return rc;
+    r.default_ms = best;
+    // Node width: rebuild with the other width, keep it if faster.
+    if (o && o->try_width) {
+        pg_build_options bo = S->build_opts;
+        const int other = as.width == 2 ? 4 : 2;
+        bo.width = other;
+        if ((rc = pg_accel_rebuild(S->accel, &bo)))
+            return rc;
+        pg_accel_set_launch(S->accel, as.block, as.launch_flags);
+        if ((rc = measure(pspp, &ms)))
+            return rc;
+        if (ms < best) {
+            best = ms;
+            S->build_opts = bo;
+        } else {
+            bo.width = as.width;
+            if ((rc = pg_accel_rebuild(S->accel, &bo)))
+                return rc;
+            pg_accel_set_launch(S->accel, as.block, as.launch_flags);
+        }
+    }
+    // Traversal workgroup size x leaf postponing.
+    int best_block = as.block, best_flags = as.launch_flags;
+    for (int block : {64, 128, 256})
+        for (int flags : {0, (int)PG_LAUNCH_POSTPONE_LEAVES}) {
+            if (block == best_block && flags == best_flags)
+                continue;
+            if ((rc = pg_accel_set_launch(S->accel, block, flags)) || (rc = measure(pspp, &ms)))
+                return rc;
+            if (ms < best) {
+                best = ms;
+                best_block = block;
+                best_flags = flags;
+            }
+        }
+    pg_accel_set_launch(S->accel, best_block, best_flags);
+    // Paths in flight: probes large enough to fill the largest candidate.
+    const int64_t npix = (int64_t)S->dev.width * S->dev.height;
+    const int64_t cands[4] = {1 << 20, 2 << 20, 4 << 20, 8 << 20};
+    const int qspp = (int)std::min<int64_t>(S->spp, std::max<int64_t>(pspp, (cands[3] + npix - 1) / npix));
+    if (qspp * npix > cands[0]) {
+        const int64_t start = S->batch_paths;
+        double qbest = 0.0;
+        if ((rc = measure(qspp, &qbest)))
+            return rc;
+        int64_t qpaths = start;
+        for (int64_t c : cands) {
+            if (c == start)
+                continue;
+            S->batch_paths = c;
+            if ((rc = measure(qspp, &ms)))
+                return rc;
+            if (ms < qbest) {
+                qbest = ms;
+                qpaths = c;
+            }
+        }
+        S->batch_paths = qpaths;
+    }
+    if ((rc = pg_accel_get_stats(S->accel, &as)))
+        return rc;
+    r.trace_block = as.block;
+    r.trace_flags = as.launch_flags;
+    r.bvh_width = as.width;
+    r.paths_in_flight = S->batch_paths;
+    r.best_ms = best;
+    if (out)
+        *out = r;
     return PG_OK;
 }
 
@@ -2436,6 +2832,7 @@ extern "C" int pg_render_guided(pg_scene* S, pg_guide* G, const pg_guided_option
     const int batch = o && o->batch > 0 ? o->batch : 1024;
     const int depth = o && o->record_depth > 0 ? o->record_depth : 5;
     const int final_spp = o && o->final_spp > 0 ? o->final_spp : S->spp;
+    const int offset = o && o->sample_offset > 0 ? o->sample_offset : 0;
     SceneDev& d = S->dev;
     d.g_params = guide_params_dev(G);
     d.g_wf = guide_wf_dev(G);
@@ -2454,12 +2851,12 @@ extern "C" int pg_render_guided(pg_scene* S, pg_guide* G, const pg_guided_option
     GuideTraining tr{G, depth, steps > 0 ? std::max(1, (steps + nbatches - 1) / nbatches) : 0, batch, 0, 0};
     int rc = PG_OK;
     for (int p = 0; p < passes && rc == PG_OK; p++) {
-        rc = render_impl(S, train_spp, 0, st, &tr, &ms, &rays);
+        rc = render_impl(S, train_spp, 0, st, &tr, &ms, &rays, offset + p * train_spp);
         train_ms += ms;
         all_rays += rays;
     }
     if (rc == PG_OK)
-        rc = render_impl(S, final_spp, 0, st, nullptr, &ms, &rays);
+        rc = render_impl(S, final_spp, 0, st, nullptr, &ms, &rays, offset + passes * train_spp);
     d.guided = 0;
     if (rc != PG_OK)
         return rc;

@@ -232,6 +232,20 @@ def case_render_mixed():
     print("render_mixed:", line, flush=True)
 
 
+def case_render_box():
+    """The occluded-light box with its 512,000-triangle clutter at 128^2, 2048
+    spp, depth 8: pine's film, non-finite pixels included (pine's estimator
+    produces them; the GPU must reproduce them at the same pixels)."""
+    v, i = W.occluded_box_clutter()
+    with tempfile.TemporaryDirectory() as d:
+        W.write_obj(os.path.join(d, "clutter.obj"), v, i)
+        text = W.occluded_box_scene_text("clutter.obj", width=128, height=128, spp=2048, max_depth=8)
+        film, line = run_render(text, d)
+    np.savez_compressed(os.path.join(OUT, "render_box.npz"), scene=np.bytes_(text), film=film,
+                        harness=np.bytes_(line))
+    print("render_box:", line, flush=True)
+
+
 CLOUD_GRID_SMALL = dict(n=32, base_freq=2.0, threshold=0.3, scale=40.0, seed=3)
 
 
@@ -276,6 +290,7 @@ CASES = {
     "render_spheres": case_render_spheres,
     "render_mixed": case_render_mixed,
     "render_cloud": case_render_cloud,
+    "render_box": case_render_box,
     "slab_through": lambda: case_slab("through"),
     "slab_shadow": lambda: case_slab("shadow"),
 }

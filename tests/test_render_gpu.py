@@ -65,6 +65,30 @@ def test_mixed_scene_matches_pine(pg, cuda, tmp_path):
     print(f"mixed relMSE {e:.3e}")
 
 
+def test_occluded_box_matches_pine(pg, cuda, tmp_path):
+    """The path-guiding scene (512,000 clutter triangles, light behind a
+    partition) rendered plain: pine's film at 2048 spp, depth 8.  pine's
+    estimator produces non-finite pixels here; they must be the same pixels."""
+    from pine_amd import workloads as W
+    fx = golden("render_box")
+    v, i = W.occluded_box_clutter()
+    W.write_obj(str(tmp_path / "clutter.obj"), v, i)
+    film, info = render_text(pg, fx["scene"].item().decode(), str(tmp_path))
+    ref = fx["film"]
+    bad_ref = ~np.isfinite(ref).all(-1)
+    bad = ~np.isfinite(film).all(-1)
+    print(f"non-finite pixels: pine {np.argwhere(bad_ref).tolist()}, gpu {np.argwhere(bad).tolist()}")
+    assert np.array_equal(bad, bad_ref)
+    np.testing.assert_allclose(film[..., 3], ref[..., 3], rtol=1e-5, atol=1e-6)
+    ok = ~bad
+    a, b = linear(film)[ok], linear(ref)[ok]
+    e = float(np.mean((a - b) ** 2 / (b ** 2 + 1e-2)))
+    close = float((np.abs(a - b) <= 1e-4 * np.abs(b) + 1e-7).all(-1).mean())
+    print(f"box relMSE {e:.3e} over finite pixels, {close:.4f} within 1e-4, "
+          f"{info['last_render_ms']:.1f} ms")
+    assert e < 1e-3 and close > 0.99
+
+
 def test_batching_does_not_change_the_image(pg, cuda, tmp_path):
     # Samples are assigned to pixels and Halton indices the same way whatever
     # the batch size, and the film adds them in sample order.

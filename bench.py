@@ -23,6 +23,12 @@ relMSE (the GPU reproduces pine's per-pixel sample sequence):
   mesh_path_tracing  the 2^20-triangle icosphere in a lit scene, 512x512
   volume             a 256^3 fBm density grid, 1024x1024, 256 spp, maxDepth 64,
                      16^3-voxel majorant bricks, HG g = 0.85, albedo 0.99
+and, GPU only:
+  batch              64 scenes of uneven size (10k..2M triangles, 256^2..1024^2,
+                     16..128 spp) autotuned, then rendered one by one and as one
+                     batch over concurrent streams with graph-replayed bounce loops
+  guided             the occluded-light box: plain vs learned-guided path tracing,
+                     time to reach the plain tracer's 512-spp relMSE
 """
 import argparse
 import json
@@ -202,6 +208,114 @@ def render_gpu(pg, text, directory, warm_spp=2):
     return inf
 
 
+def batch_leg(args, pg, W, work):
+    """The batched workload: 64 scenes of uneven size (W.batch_workload)
+    rendered one by one (pg_render, graph replay), as one batch over concurrent
+    streams (pg_render_batch), and as a batch with the launch-by-launch loop;
+    each scene first autotuned (pg_scene_autotune)."""
+    t0 = time.time()
+    items = W.batch_workload(args.batch_scenes)
+    scenes = []
+    for it in items:
+        v, i = W.make_mesh(it["mesh"])
+        name = f"batch{it['index']}.obj"
+        pg.write_obj_fixed(os.path.join(work, name), v, i)
+        scenes.append(pg.Scene(text=W.batch_scene_text(it, name), directory=work))
+        os.remove(os.path.join(work, name))
+    setup_s = time.time() - t0
+    log(f"[bench] batch: {len(scenes)} scenes, {sum(it['triangles'] for it in items)} triangles "
+        f"loaded and built in {setup_s:.1f} s")
+    t0 = time.time()
+    tuned = [s.autotune(probe_spp=1) for s in scenes] if not args.no_tune else []
+    tune_s = time.time() - t0
+    samples = sum(it["width"] * it["height"] * it["spp"] for it in items)
+    for s in scenes:  # warm: record every scene's bounce loop
+        s.render(spp=1)
+    pg.hip.synchronize()
+    t0 = time.perf_counter()
+    for s in scenes:
+        s.render()
+    seq_s = time.perf_counter() - t0
+    seq_rays = sum(s.info()["last_rays"] for s in scenes)
+    res = pg.render_batch(scenes, streams=args.batch_streams)
+    res_ng = pg.render_batch(scenes, streams=args.batch_streams, graph=False)
+    for s in scenes:
+        s.release()
+    out = {
+        "workload": f"{len(items)} scenes: displaced icosphere of log-uniform 10k..2M triangles "
+                    f"in a seeded 16-sphere scene, 256^2..1024^2, 16..128 spp, maxDepth 8",
+        "triangles": int(sum(it["triangles"] for it in items)), "camera_samples": int(samples),
+        "samples_per_s": round(samples / (res["ms"] * 1e-3), 1),
+        "mrays_per_s": round(res["rays"] / (res["ms"] * 1e-3) * 1e-6, 2),
+        "batch_ms": round(res["ms"], 2), "streams": args.batch_streams,
+        "sequential_ms": round(seq_s * 1e3, 2),
+        "sequential_samples_per_s": round(samples / seq_s, 1),
+        "batch_launch_loop_ms": round(res_ng["ms"], 2),
+        "speedup_vs_sequential": round(seq_s * 1e3 / res["ms"], 3),
+        "graph_speedup": round(res_ng["ms"] / res["ms"], 3),
+        "rays_match_sequential": bool(res["rays"] == seq_rays),
+        "setup_s": round(setup_s, 1), "autotune_s": round(tune_s, 1),
+        "tuned": {"trace_block": sorted({t["trace_block"] for t in tuned}),
+                  "postpone_leaves": sum(1 for t in tuned if t["trace_flags"] & 1),
+                  "paths_in_flight": sorted({t["paths_in_flight"] for t in tuned}),
+                  "probe_gain": round(float(np.mean([t["default_ms"] / t["best_ms"] for t in tuned])), 3)
+                  if tuned else None},
+    }
+    log(f"[bench] batch: {out['samples_per_s'] / 1e6:.1f} M samples/s in {res['ms']:.0f} ms "
+        f"({out['speedup_vs_sequential']}x one by one, {out['graph_speedup']}x the launch loop)")
+    return out
+
+
+def guided_leg(args, pg, W, work):
+    """Learned path guiding on the occluded-light box at equal time: the plain
+    tracer and the guided one (8 training passes of 4 spp, then the final
+    render) at several spp each, relMSE against a reference of the plain
+    tracer (pine_amd.metrics: mean of 1024-spp ranges without singular ones);
+    time for each to reach the plain tracer's error at 512 spp."""
+    from pine_amd import metrics as M
+    res = args.guide_res
+    v, i = W.occluded_box_clutter()
+    W.write_obj(os.path.join(work, "clutter.obj"), v, i)
+    s = pg.Scene(text=W.occluded_box_scene_text("clutter.obj", width=res, height=res, spp=64,
+                                                 max_depth=8), directory=work)
+    t0 = time.time()
+    ref = M.robust_reference(s, 1024, args.guide_ref_chunks)
+    ref_s = time.time() - t0
+    pt, pe, gt, ge = [], [], [], []
+    for spp in (64, 128, 256, 512, 1024):
+        s.render(spp=spp, sample_offset=1 << 20)
+        pt.append(s.info()["last_render_ms"])
+        pe.append(M.relmse(M.linear(s.film()), ref))
+    train_ms = []
+    for spp in (32, 64, 128, 256, 512):
+        g = pg.Guide(seed=3, learning_rate=3e-3)
+        inf = s.render_guided(g, train_passes=8, train_spp=4, final_spp=spp, sample_offset=1 << 21)
+        gt.append(inf["last_guided_ms"])
+        train_ms.append(inf["last_guided_ms"] - inf["last_render_ms"])
+        ge.append(M.relmse(M.linear(s.film()), ref))
+        g.release()
+    s.release()
+    target = pe[3]
+    tp, tg = M.time_to_error(pt, pe, target), M.time_to_error(gt, ge, target)
+    out = {
+        "workload": f"occluded-light box (512,000 clutter triangles), {res}x{res}, maxDepth 8; "
+                    f"guide trained 8 passes x 4 spp",
+        "reference": f"plain tracer, {args.guide_ref_chunks} x 1024 spp ({ref_s:.0f} s)",
+        "plain": {"spp": [64, 128, 256, 512, 1024], "ms": [round(x, 2) for x in pt],
+                  "relmse": [float(f"{x:.4g}") for x in pe]},
+        "guided": {"final_spp": [32, 64, 128, 256, 512], "ms_incl_training": [round(x, 2) for x in gt],
+                   "relmse": [float(f"{x:.4g}") for x in ge],
+                   "training_ms": round(float(np.mean(train_ms)), 2)},
+        "target_relmse": float(f"{target:.4g}"),
+        "plain_ms_to_target": round(tp, 2) if tp else None,
+        "guided_ms_to_target": round(tg, 2) if tg else None,
+        "equal_time_speedup": round(tp / tg, 3) if tp and tg else None,
+        "equal_spp_error_ratio_64": round(pe[0] / ge[1], 3),
+    }
+    log(f"[bench] guided: plain {tp} ms vs guided {tg} ms to relMSE {target:.3g}")
+    return out
+
+
 def measured_traffic(width, leaf, block, flags):
     """HBM bytes per launch of the trace kernel for this launch configuration,
     as measured by rocprofv3 FETCH_SIZE + WRITE_SIZE (scaled by a known-size
@@ -233,6 +347,14 @@ def main():
     ap.add_argument("--no-pt", action="store_true", help="skip the path-tracing workloads")
     ap.add_argument("--pt-spp", type=int, default=64)
     ap.add_argument("--volume-spp", type=int, default=256)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-scene leg")
+    ap.add_argument("--batch-scenes", type=int, default=64)
+    ap.add_argument("--batch-streams", type=int, default=4)
+    ap.add_argument("--no-tune", action="store_true", help="batch leg without the autotuner")
+    ap.add_argument("--no-guide", action="store_true", help="skip the path-guiding leg")
+    ap.add_argument("--guide-res", type=int, default=512,
+                    help="occluded-box resolution of the guiding leg (1024 is the full workload)")
+    ap.add_argument("--guide-ref-chunks", type=int, default=32)
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one device-to-device copy of the ray buffer (512 MiB), the "
                          "known-size kernel that scales rocprof's FETCH_SIZE/WRITE_SIZE")
@@ -307,6 +429,12 @@ def main():
                                      "whole render time"}}
         log(f"[bench] volume: {vol['samples_per_s'] / 1e6:.1f} M samples/s, {lookups} lookups")
 
+    batch = guided = None
+    if not args.no_batch:
+        batch = batch_leg(args, pg, W, work.name)
+    if not args.no_guide:
+        guided = guided_leg(args, pg, W, work.name)
+
     cpu = None
     if not args.no_cpu:
         import oracle
@@ -340,6 +468,8 @@ def main():
         "path_tracing": pt,
         "mesh_path_tracing": mesh_pt,
         "volume": vol,
+        "batch": batch,
+        "guided": guided,
     }
     print(json.dumps(out), flush=True)
 

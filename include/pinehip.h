@@ -287,6 +287,12 @@ int pg_gen_rays(const float* h_vertices, int64_t n_vertices, const uint32_t* h_i
 int pg_gen_density_grid(int32_t nx, int32_t ny, int32_t nz, float base_freq, float threshold,
                         float scale, uint64_t seed, float* h_density);
 
+/* Writes a triangle mesh as OBJ text (`v x y z` with six decimals, no
+ * exponent, as pine's stof reads them; `f a b c` 1-based), formatted on all
+ * host cores: workload files of tens of millions of triangles in seconds. */
+int pg_write_obj(const char* path, const float* h_vertices, int64_t n_vertices,
+                 const uint32_t* h_indices, int64_t n_triangles);
+
 /* --- Learned path guiding (no counterpart in pine) ---------------------------
  * A 225-64-64-64-41 MLP (BF16 weights, FP32 accumulation on the matrix cores)
  * mapping a surface point to an 8-lobe von Mises-Fisher mixture over incident
@@ -295,7 +301,8 @@ int pg_gen_density_grid(int32_t nx, int32_t ny, int32_t nz, float base_freq, flo
  * position normalised to the scene bounds, shading normal, outgoing direction.
  * Parameters: PG_GUIDE_N_PARAMS floats, the padded layout
  *   W0[64][256] b0[64] W1[64][64] b1[64] W2[64][64] b2[64] W3[48][64] b3[48]
- * (W0 columns >= 225 and W3/b3 rows >= 41 are padding, kept at zero). */
+ * (W0 columns >= 225 and the 7 W3/b3 rows no lobe or alpha uses are padding,
+ * kept at zero). */
 #define PG_GUIDE_N_PARAMS 27888
 typedef struct pg_guide pg_guide;
 typedef struct pg_guide_config {

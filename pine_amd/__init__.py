@@ -31,7 +31,7 @@ EXPORTED = [
     "pg_gen_density_grid", "pg_guide_create", "pg_guide_set_params", "pg_guide_get_params",
     "pg_guide_forward", "pg_guide_pdf", "pg_guide_sample", "pg_guide_gradient", "pg_guide_train",
     "pg_guide_release", "pg_render_guided", "pg_trace_closest_dev", "pg_trace_occluded_dev",
-    "pg_accel_rebuild", "pg_render_batch", "pg_scene_autotune",
+    "pg_accel_rebuild", "pg_render_batch", "pg_scene_autotune", "pg_write_obj",
 ]
 RENDER_NO_GRAPH = 1
 GUIDE_N_PARAMS = 27888
@@ -162,6 +162,7 @@ def lib():
     L.pg_render_batch.argtypes = [vp, i32, vp, ctypes.POINTER(_BatchOptions), vp,
                                   ctypes.POINTER(_BatchResult)]
     L.pg_scene_autotune.argtypes = [vp, ctypes.POINTER(_TuneOptions), vp, ctypes.POINTER(_TuneResult)]
+    L.pg_write_obj.argtypes = [ctypes.c_char_p, vp, i64, vp, i64]
     _lib = L
     return L
 
@@ -220,6 +221,15 @@ def gen_density_grid(n, base_freq=4.0, threshold=0.3, scale=40.0, seed=3):
     out = np.zeros((n, n, n), np.float32)
     check(lib().pg_gen_density_grid(n, n, n, base_freq, threshold, scale, seed, _ptr(out)))
     return out
+
+
+def write_obj_fixed(path, vertices, indices):
+    """Writes a mesh as OBJ text with six-decimal coordinates (pg_write_obj, all
+    host cores).  For workload files; the scenes compared with pine use
+    workloads.write_obj, whose decimals reproduce each float exactly."""
+    v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+    i = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
+    check(lib().pg_write_obj(path.encode(), _ptr(v), v.shape[0], _ptr(i), i.shape[0]))
 
 
 def write_volume(path, density):

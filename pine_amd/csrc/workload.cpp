@@ -5,6 +5,8 @@
 #include "pg_api.h"
 
 #include <algorithm>
+#include <cstdio>
+#include <string>
 #include <cmath>
 #include <thread>
 #include <unordered_map>
@@ -264,4 +266,95 @@ extern "C" int pg_gen_density_grid(int32_t nx, int32_t ny, int32_t nz, float bas
     for (auto& th : pool)
         th.join();
     return PG_OK;
+}
+
+namespace {
+
+// Fixed-point text of x with six decimals (rounded half away from zero).
+char* put_fixed6(char* p, float x) {
+    double d = x;
+    if (d < 0) {
+        *p++ = '-';
+        d = -d;
+    }
+    uint64_t q = (uint64_t)(d * 1e6 + 0.5);
+    uint64_t ip = q / 1000000, fp = q % 1000000;
+    char tmp[24];
+    int n = 0;
+    do {
+        tmp[n++] = (char)('0' + ip % 10);
+        ip /= 10;
+    } while (ip);
+    while (n)
+        *p++ = tmp[--n];
+    *p++ = '.';
+    for (int k = 5; k >= 0; k--) {
+        p[k] = (char)('0' + fp % 10);
+        fp /= 10;
+    }
+    return p + 6;
+}
+
+char* put_uint(char* p, uint64_t v) {
+    char tmp[24];
+    int n = 0;
+    do {
+        tmp[n++] = (char)('0' + v % 10);
+        v /= 10;
+    } while (v);
+    while (n)
+        *p++ = tmp[--n];
+    return p;
+}
+
+}  // namespace
+
+extern "C" int pg_write_obj(const char* path, const float* v, int64_t nv, const uint32_t* idx, int64_t nt) {
+    if (!path || nv < 0 || nt < 0 || (nv && !v) || (nt && !idx))
+        return pg_fail(PG_E_ARG, "pg_write_obj: bad arguments");
+    for (int64_t k = 0; k < 3 * nt; k++)
+        if (idx[k] >= (uint64_t)nv)
+            return pg_fail(PG_E_ARG, "pg_write_obj: index out of range");
+    for (int64_t k = 0; k < 3 * nv; k++)
+        if (!(std::fabs(v[k]) < 1e12f))
+            return pg_fail(PG_E_ARG, "pg_write_obj: vertex coordinate not finite or too large");
+    const int nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const int64_t lines = nv + nt;
+    std::vector<std::string> parts(nth);
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nth; t++)
+        pool.emplace_back([&, t]() {
+            const int64_t a = lines * t / nth, b = lines * (t + 1) / nth;
+            std::string& out = parts[t];
+            out.resize((size_t)(b - a) * 80);  // longest line: "v" + 3 x (" " + 21 chars) + "\n"
+            char* p = &out[0];
+            for (int64_t l = a; l < b; l++) {
+                if (l < nv) {
+                    *p++ = 'v';
+                    for (int c = 0; c < 3; c++) {
+                        *p++ = ' ';
+                        p = put_fixed6(p, v[3 * l + c]);
+                    }
+                } else {
+                    const int64_t f = l - nv;
+                    *p++ = 'f';
+                    for (int c = 0; c < 3; c++) {
+                        *p++ = ' ';
+                        p = put_uint(p, (uint64_t)idx[3 * f + c] + 1);
+                    }
+                }
+                *p++ = '\n';
+            }
+            out.resize((size_t)(p - &out[0]));
+        });
+    for (auto& th : pool)
+        th.join();
+    FILE* fp = std::fopen(path, "wb");
+    if (!fp)
+        return pg_fail(PG_E_ARG, std::string("pg_write_obj: cannot open ") + path);
+    bool ok = true;
+    for (auto& s : parts)
+        ok = ok && std::fwrite(s.data(), 1, s.size(), fp) == s.size();
+    ok = (std::fclose(fp) == 0) && ok;
+    return ok ? PG_OK : pg_fail(PG_E_ARG, std::string("pg_write_obj: write failed: ") + path);
 }

@@ -156,6 +156,39 @@ def spheres_scene_text(seed=0, width=512, height=512, spp=64, max_depth=8, n_sph
     return "\n".join(out) + "\n"
 
 
+BATCH_SCENES = 64
+
+
+def batch_workload(n=BATCH_SCENES, seed=7, min_triangles=10_000, max_triangles=2_000_000):
+    """The batched-rendering workload: n scenes of uneven size.  Scene k is the
+    seeded spheres scene (materials and light as the spheres workload, 16
+    spheres) with a displaced icosphere at its centre whose triangle count is
+    log-uniform in [min_triangles, max_triangles]; square resolution uniform in
+    {256, 384, ..., 1024}, spp uniform in {16, 24, ..., 128}, maxDepth 8.
+    Returns one dict per scene; batch_scene_text writes its scene file."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        tris = int(round(float(np.exp(rng.uniform(np.log(min_triangles), np.log(max_triangles))))))
+        res = int(rng.choice(np.arange(256, 1025, 128)))
+        spp = int(rng.choice(np.arange(16, 129, 8)))
+        freq = int(np.ceil(np.sqrt(tris / 20.0)))
+        out.append(dict(index=k, triangles=tris, mesh=MeshSpec(freq, 0.1, seed * 1000 + k, tris),
+                        width=res, height=res, spp=spp, scene_seed=seed * 1000 + k))
+    return out
+
+
+def batch_scene_text(item, obj_file):
+    """Scene file of one batch_workload item (its mesh in obj_file)."""
+    text = spheres_scene_text(seed=item["scene_seed"], width=item["width"], height=item["height"],
+                              spp=item["spp"], max_depth=8, n_spheres=16,
+                              output=f"batch{item['index']}.bmp")
+    return text + "\n".join([
+        "Material batch_mesh: Layered{", "    layer0: Diffuse{", "        albedo: 0.6 0.55 0.5",
+        "    }", "}", "Shape: TriangleMesh{", f"    file: {obj_file}", "    scale: 3 3 3",
+        "    position: 0 3 0", "    material: batch_mesh", "}"]) + "\n"
+
+
 def write_obj(path, vertices, indices):
     """Writes a mesh as OBJ text with plain decimals (pine's stof has no exponent)."""
     with open(path, "w") as f:

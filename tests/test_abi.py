@@ -122,3 +122,30 @@ def test_volume_file_layout(pg, tmp_path):
     assert np.frombuffer(raw[:12], np.int32).tolist() == [2, 3, 4]
     vals = np.frombuffer(raw[12:], np.float32)
     assert vals[1] == d[0, 0, 1] and vals[2] == d[0, 1, 0] and vals[6] == d[1, 0, 0]
+
+
+def test_write_obj_fixed_round_trips_to_six_decimals(pg, tmp_path):
+    v = np.array([[0.0, -1.25, 3.0000004], [123456.5, -0.0000004, 1e-7], [-2.5, 0.1, 7.0]], np.float32)
+    i = np.array([[0, 1, 2], [2, 1, 0]], np.uint32)
+    p = str(tmp_path / "m.obj")
+    pg.write_obj_fixed(p, v, i)
+    lines = open(p).read().splitlines()
+    assert len(lines) == 5 and all("e" not in l for l in lines)
+    got = np.array([[float(x) for x in l.split()[1:]] for l in lines[:3]])
+    assert np.abs(got - v).max() <= 5e-7 + 1e-7 * np.abs(v).max()
+    assert lines[3] == "f 1 2 3" and lines[4] == "f 3 2 1"
+    with pytest.raises(pg.PineError):
+        pg.write_obj_fixed(p, v, np.array([[0, 1, 3]], np.uint32))
+
+
+def test_batch_workload_is_seeded_and_spans_its_ranges():
+    from pine_amd import workloads as W
+    a, b = W.batch_workload(), W.batch_workload()
+    assert [x["triangles"] for x in a] == [x["triangles"] for x in b] and len(a) == 64
+    tr = np.array([x["triangles"] for x in a])
+    assert tr.min() >= 10_000 and tr.max() <= 2_000_000
+    assert {x["width"] for x in a} <= set(range(256, 1025, 128))
+    assert {x["spp"] for x in a} <= set(range(16, 129, 8))
+    v, i = W.make_mesh(a[0]["mesh"])
+    assert len(i.reshape(-1, 3)) == a[0]["triangles"]
+    assert "TriangleMesh" in W.batch_scene_text(a[0], "x.obj")

@@ -1330,7 +1330,8 @@ struct pg_scene {
     hipGraphExec_t graph_exec = nullptr;
     hipStream_t capture_stream = nullptr;
     int32_t* g_counts = nullptr;         // 4 per bounce after the batch's path count
-    unsigned long long* g_rays = nullptr;  // rays traced (extension + shadow)
+    unsigned long long* g_rays = nullptr;  // rays traced: [0] extension + shadow, [1] extension
+    int64_t last_ext_rays = 0;           // extension rays (path vertices) of the last render
     int64_t paths_gen = 0;               // bumped when path or guide state is reallocated
     int64_t graph_captures = 0;
     int64_t batch_paths = 4 << 20;       // paths per batch when not given (tunable)
@@ -2235,6 +2236,7 @@ extern "C" int pg_scene_get_info(const pg_scene* S, pg_scene_info* info) {
     info->guide_steps = S->guide_steps;
     info->guide_records = S->guide_records;
     info->graph_captures = S->graph_captures;
+    info->last_extension_rays = S->last_ext_rays;
     info->paths_in_flight = S->batch_paths;
     pg_accel_stats as{};
     if (pg_accel_get_stats(S->accel, &as) == PG_OK) {
@@ -2309,10 +2311,12 @@ __global__ void k_fill_i32(int32_t* p, int32_t n, int32_t v) {
         p[k] = v;
 }
 
-// Rays of one bounce: extension rays traced plus shadow rays traced.
+// Rays of one bounce: total[0] += extension + shadow rays, total[1] += extension rays.
 __global__ void k_count_rays(const int32_t* d_active, const int32_t* d_shadow, unsigned long long* total) {
-    if (threadIdx.x == 0 && blockIdx.x == 0)
-        *total += (unsigned long long)*d_active + (unsigned long long)*d_shadow;
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        total[0] += (unsigned long long)*d_active + (unsigned long long)*d_shadow;
+        total[1] += (unsigned long long)*d_active;
+    }
 }
 
 // The fused pipeline's bounce loop for up to `cap` paths with every count in
@@ -2372,7 +2376,7 @@ static int ensure_graph(pg_scene* S, const PathsDev& P, int64_t cap) {
     if (!S->g_counts) {
         if (int rc = dalloc(*S, 4 * (S->max_depth + 1), &S->g_counts))
             return rc;
-        if (int rc = dalloc(*S, 1, &S->g_rays))
+        if (int rc = dalloc(*S, 2, &S->g_rays))
             return rc;
     }
     if (!S->capture_stream)
@@ -2423,7 +2427,7 @@ static uint64_t gcd64(uint64_t a, uint64_t b) {
 struct RenderTicket {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     bool graph = false;
-    int64_t rays = 0;
+    int64_t rays = 0, ext_rays = 0;
     void release() {
         if (e0)
             (void)hipEventDestroy(e0);
@@ -2468,9 +2472,9 @@ static int render_enqueue(pg_scene* S, int spp, int spb_req, hipStream_t st, Gui
     if (graph) {
         if (int rc = ensure_graph(S, P, n_paths))
             return rc;
-        PG_HIP(hipMemsetAsync(S->g_rays, 0, sizeof(unsigned long long), st));
+        PG_HIP(hipMemsetAsync(S->g_rays, 0, 2 * sizeof(unsigned long long), st));
     }
-    int64_t rays = 0;
+    int64_t rays = 0, ext_rays = 0;
     const int B = 256;
     for (int s0 = 0; s0 < spp; s0 += spb) {
         const int nb = std::min(spb, spp - s0);
@@ -2488,6 +2492,7 @@ static int render_enqueue(pg_scene* S, int spp, int spb_req, hipStream_t st, Gui
         // Depth bounds every path except those crossing material-less surfaces,
         // which pine does not count (path.cpp:42-46); the bound guards those.
         for (int bounce = 0; active > 0 && bounce < kMaxBounces; bounce++) {
+            ext_rays += active;
             PG_HIP(hipMemsetAsync(Q.counters, 0, 4 * sizeof(int32_t), st));
             if (int rc = pg_trace_closest(S->accel, Q.rays_in, active, Q.hits, Q.uvs, st))
                 return rc;
@@ -2568,6 +2573,7 @@ static int render_enqueue(pg_scene* S, int spp, int spb_req, hipStream_t st, Gui
     PG_HIP(hipEventRecord(t->e1, st));
     t->graph = graph;
     t->rays = rays;
+    t->ext_rays = ext_rays;
     return PG_OK;
 }
 
@@ -2578,10 +2584,12 @@ static int render_finish(pg_scene* S, RenderTicket* t, double* ms_out, int64_t* 
     PG_HIP(hipEventElapsedTime(&ms, t->e0, t->e1));
     t->release();
     int64_t rays = t->rays;
+    S->last_ext_rays = t->ext_rays;
     if (t->graph) {
-        unsigned long long n = 0;
-        PG_HIP(hipMemcpy(&n, S->g_rays, sizeof(n), hipMemcpyDeviceToHost));
-        rays = (int64_t)n;
+        unsigned long long n[2] = {0, 0};
+        PG_HIP(hipMemcpy(n, S->g_rays, sizeof(n), hipMemcpyDeviceToHost));
+        rays = (int64_t)n[0];
+        S->last_ext_rays = (int64_t)n[1];
     }
     *ms_out = ms;
     *rays_out = rays;

@@ -353,6 +353,8 @@ typedef struct pg_guided_options {
     int32_t record_depth;  /* path vertices recorded per path */
     int32_t final_spp;
     int32_t sample_offset;  /* first sample index of the first training pass */
+    int32_t guide_depth;    /* vertices of depth < guide_depth use the guide (0 = all); deeper
+                               ones sample their BSDF and cost no network evaluation */
 } pg_guided_options;
 int pg_render_guided(pg_scene* s, pg_guide* g, const pg_guided_options* o, void* stream);
 

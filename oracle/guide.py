@@ -18,7 +18,7 @@ matrix-core output registers feed the next layer).  Head, per lobe j < 8, with
 m_j = o[OUT_UNIT[j]] (the output units of lobe j; OUT_UNIT places lobes 2g and
 2g + 1 in the output registers of the GPU's lane group g):
   mu_j = normalize(m_j[0..2]), kappa_j = exp(clamp(m_j[3], -4, 7)),
-  pi = softmax(m_j[4]); alpha = 0.95 * sigmoid(o[ALPHA_UNIT]) (guide share);
+  pi = softmax(m_j[4]); alpha = 0.5 * sigmoid(o[ALPHA_UNIT]) (guide share);
   vMF_j(w) = kappa / (2 pi (1 - exp(-2 kappa))) exp(kappa (mu.w - 1)),
   q(w) = sum_j pi_j vMF_j(w), q_mix = alpha q + (1 - alpha) p_bsdf.
 Loss over a batch, mean over samples of
@@ -32,7 +32,7 @@ the guide of gradient when it drops.
 import numpy as np
 
 N_IN, N_FEAT, N_FREQ, H, N_OUT, N_OUT_PAD, N_LOBES = 9, 256, 12, 64, 41, 48, 8
-ALPHA_MAX = 0.95
+ALPHA_MAX = 0.5
 Q_DEFENSIVE = 0.1  # uniform share inside the guide's own objective
 KR_MIN, KR_MAX = -4.0, 7.0
 

@@ -43,7 +43,10 @@ constexpr int kParams = OFF_B3 + kOutPad;  // 27888
 constexpr int WF0 = 0, WF1 = WF0 + kH * kFeat, WF2 = WF1 + kH * kH, WF3 = WF2 + kH * kH;
 constexpr int kWf = WF3 + kOutPad * kH;  // 27648
 constexpr int WT1 = 0, WT2 = kH * kH, WT3 = 2 * kH * kH, kWt = 3 * kH * kH;
-constexpr float kAlphaMax = 0.95f, kKrMin = -4.0f, kKrMax = 7.0f;
+// alpha <= 0.5: a direction the guide misses keeps at least half the BSDF's
+// density, so one vertex scales the throughput by at most 2x the albedo (a
+// larger share made the estimator's error tail heavier than it gained).
+constexpr float kAlphaMax = 0.5f, kKrMin = -4.0f, kKrMax = 7.0f;
 // Output unit of lobe j = 2g + h, component c (mu.xyz, log kappa, logit):
 // slot s = 5h + c of lane group g's twelve output registers (units 16t + 4g + i,
 // s = 4t + i), so each lane holds its two lobes.  Alpha's logit takes lane

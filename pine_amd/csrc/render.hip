@@ -96,6 +96,7 @@ struct SceneDev {
     f3 sky_dir, sky_color;  // Sky sunDirection (normalised) and sunColor
     // Learned path guiding (general mode only).
     int32_t guided;
+    int32_t g_depth;  // vertices of depth < g_depth sample from the guide
     const float* g_params;  // FP32 parameters (biases used in FP32)
     const __bf16* g_wf;     // BF16 forward weights
     f3 g_bmin, g_binv;      // scene bounds: network position input = (p - bmin) * binv
@@ -145,8 +146,9 @@ struct PathsDev {
     unsigned long long* lookups;  // density lookups of the tracking kernels (8 taps each)
     // Guiding: mixture of each path's current vertex, and training records
     // (slot = path * rec_depth + depth; rec_pdf 0 = empty slot).
-    float* gparams;   // guide::kParamStride per path
-    int32_t* gflag;   // 1 if the vertex samples from the guide
+    float* gparams;   // guide::kParamStride per vertex: per path (split bounce) or per queue slot (fused)
+    int32_t* gflag;   // 1 if the vertex samples from the guide (same indexing)
+    float* gx;        // fused bounce: network input of queue slot k (9 floats)
     int32_t* shadow_rec;  // fused bounce: record slot of each shadow ray's vertex, -1 if none
     float* rec_x;     // 9 per slot: network input
     float* rec_dir;   // 3 per slot: sampled direction
@@ -421,11 +423,11 @@ __device__ __forceinline__ bool guidable(const SceneDev& s, int4 info) {
 // the vertex is recorded (network input, direction, pdfs, path radiance so far
 // and throughput after the direction).  False when the sample carries nothing.
 __device__ bool guided_direction(const SceneDev& s, PathsDev& P, SamplerState& ss, int32_t i, int depth,
-                                 const MaterialDev& m, f3 p, f3 n, f3 wi, f3 L_before, f3& beta, f3& wo_out) {
+                                 const float* gp, const MaterialDev& m, f3 p, f3 n, f3 wi, f3 L_before,
+                                 f3& beta, f3& wo_out) {
     const M3 n2w = frame_from(n);
     const M3 w2n = inverse(n2w);
     const f3 wiL = mul(w2n, wi);
-    const float* gp = P.gparams + guide::kParamStride * (int64_t)i;
     const float alpha = gp[guide::kAlphaAt];
     const float usel = get_1d(s.halton, ss);
     const float u1 = get_1d(s.halton, ss);
@@ -530,8 +532,9 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev s, PathsDev P, int32_t n
                 const M3 n2w = frame_from(h.n);
                 const M3 w2n = inverse(n2w);
                 const f3 wiL = mul(w2n, wi);
-                const bool guided = kGuided && P.gflag[i];
-                const float* gp = P.gparams + guide::kParamStride * (int64_t)i;
+                // The fused bounce's mixture records are in queue order.
+                const bool guided = kGuided && P.gflag[k];
+                const float* gp = P.gparams + guide::kParamStride * (int64_t)k;
                 // EstimateDirect (integrator.cpp:88-119)
                 float lpdf;
                 const float ul = get_1d(s.halton, ss);
@@ -569,7 +572,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev s, PathsDev P, int32_t n
                 }
                 if (guided) {
                     f3 wo;
-                    if (guided_direction(s, P, ss, i, depth, m, h.p, h.n, wi, L, beta, wo)) {
+                    if (guided_direction(s, P, ss, i, depth, gp, m, h.p, h.n, wi, L, beta, wo)) {
                         nextD = wo;
                         nextO = offset_ray_origin(h.p, face_same_hemisphere(h.n, nextD));
                         alive = true;
@@ -950,7 +953,8 @@ __global__ __launch_bounds__(256) void k_shade_gen_b(SceneDev s, PathsDev P, int
         alive = true;
     } else if (s.guided && P.gflag[i]) {
         f3 wo;
-        if (guided_direction(s, P, ss, i, depth, s.materials[info.y], p, n, wi,
+        if (guided_direction(s, P, ss, i, depth, P.gparams + guide::kParamStride * (int64_t)i,
+                             s.materials[info.y], p, n, wi,
                              mk(P.L[3 * i], P.L[3 * i + 1], P.L[3 * i + 2]), beta, wo)) {
             nextD = wo;
             nextO = offset_ray_origin(p, face_same_hemisphere(n, nextD));
@@ -1013,10 +1017,37 @@ __global__ __launch_bounds__(256) void k_shade_gen_b(SceneDev s, PathsDev P, int
 // register-chained MFMA network, all weights staged in LDS once per workgroup
 // (59 KB: two 8-wave workgroups per CU), the head computed from the output
 // registers.  kSrc 0: the split bounce's direction-sampling queue (queue_b,
-// vtx, vinfo); kSrc 1: the fused bounce's hits in queue order, the vertex
-// evaluated here.  Vertices the guide does not serve (misses, emitters,
-// non-diffuse materials, the last depth) get gflag 0.  Waves are independent
-// after the staging barrier: a grid-stride loop over tiles of 16 per wave.
+// vtx, vinfo; records and gflag per path); kSrc 1: the fused bounce's queue
+// (inputs and flags from k_guide_gather; records per queue slot).  Vertices
+// the guide does not serve (misses, emitters, non-diffuse materials, the last
+// depth, depth >= g_depth) have gflag 0.  Waves are independent after the
+// staging barrier: a grid-stride loop over tiles of 16 per wave.
+// Fused bounce, before inference: the network input of every hit that will
+// sample a direction at a vertex the guide serves, in queue order (gx, gflag
+// per queue slot).  Its dependent loads (queue, hit, shape, triangle,
+// material) run here at full occupancy rather than in the network kernel.
+__global__ __launch_bounds__(256) void k_guide_gather(SceneDev s, PathsDev P, int32_t n_in, const int32_t* d_n) {
+    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (d_n ? *d_n : n_in))
+        return;
+    const int32_t i = P.queue_in[k];
+    const pg_hit hit = P.hits[k];
+    int use = 0;
+    if (hit.prim >= 0 && P.depth[i] + 1 < s.max_depth && P.depth[i] < s.g_depth) {
+        const float4* r4 = reinterpret_cast<const float4*>(&P.rays_in[k]);
+        const float4 ra = r4[0], rb = r4[1];
+        const f3 ro = mk(ra.x, ra.y, ra.z), rd = mk(rb.x, rb.y, rb.z);
+        const pg_hit_uv uv = P.uvs[k];
+        const Hit h = surface(s, hit.prim, hit.t, uv.u, uv.v, ro, rd);
+        const int mat = s.shapes[h.shape].material;
+        if (mat >= 0 && guidable(s, make_int4(0, mat, 0, 0))) {
+            use = 1;
+            guide_input(s, h.p, h.n, -rd, P.gx + guide::kIn * (int64_t)k);
+        }
+    }
+    P.gflag[k] = use;
+}
+
 constexpr int kInferThreads = 512;
 template <int kSrc>
 __global__ __launch_bounds__(kInferThreads) void k_guide_infer(SceneDev s, PathsDev P, int32_t n_in,
@@ -1052,23 +1083,15 @@ __global__ __launch_bounds__(kInferThreads) void k_guide_infer(SceneDev s, Paths
         float x[kIn] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         if (k < n) {
             if constexpr (kSrc == 1) {
-                i = P.queue_in[k];
-                const pg_hit hit = P.hits[k];
-                if (hit.prim >= 0 && P.depth[i] + 1 < s.max_depth) {
-                    const float4* r4 = reinterpret_cast<const float4*>(&P.rays_in[k]);
-                    const float4 ra = r4[0], rb = r4[1];
-                    const f3 ro = mk(ra.x, ra.y, ra.z), rd = mk(rb.x, rb.y, rb.z);
-                    const pg_hit_uv uv = P.uvs[k];
-                    const Hit h = surface(s, hit.prim, hit.t, uv.u, uv.v, ro, rd);
-                    const int mat = s.shapes[h.shape].material;
-                    use = mat >= 0 && guidable(s, make_int4(0, mat, 0, 0));
-                    if (use)
-                        guide_input(s, h.p, h.n, -rd, x);
-                }
+                i = k;
+                use = P.gflag[k] != 0;
+                if (use)
+                    for (int c = 0; c < kIn; c++)
+                        x[c] = P.gx[kIn * (int64_t)k + c];
             } else {
                 i = P.queue_b[k];
                 const int4 info = *reinterpret_cast<const int4*>(P.vinfo + 4 * (int64_t)i);
-                use = guidable(s, info);
+                use = P.depth[i] < s.g_depth && guidable(s, info);
                 if (use) {
                     const float* v = P.vtx + 9 * (int64_t)i;
                     guide_input(s, mk(v[0], v[1], v[2]), mk(v[3], v[4], v[5]), mk(v[6], v[7], v[8]), x);
@@ -1078,7 +1101,7 @@ __global__ __launch_bounds__(kInferThreads) void k_guide_infer(SceneDev s, Paths
         f32x4 o[3];
         forward16(x, sw0, kLd0, sw, sw + kH * kLd, sw + 2 * kH * kLd, kLd, s.g_params, s.g_oct, o, nullptr,
                   nullptr, nullptr);
-        if (g == 0 && i >= 0)
+        if (kSrc == 0 && g == 0 && i >= 0)
             P.gflag[i] = use ? 1 : 0;
         head4(o, g, use, P.gparams + kParamStride * (int64_t)(i >= 0 ? i : 0));
     }
@@ -1316,12 +1339,12 @@ struct pg_scene {
     // device memory), replayed for every batch while its key holds.
     struct GraphKey {
         int64_t capacity = -1, accel_gen = -1, paths_gen = -1;
-        int guided = 0, oct = 0, max_depth = 0;
+        int guided = 0, oct = 0, max_depth = 0, g_depth = 0;
         const void* g_params = nullptr;
         const void* g_wf = nullptr;
         bool operator==(const GraphKey& o) const {
             return capacity == o.capacity && accel_gen == o.accel_gen && paths_gen == o.paths_gen &&
-                   guided == o.guided && oct == o.oct && max_depth == o.max_depth &&
+                   guided == o.guided && oct == o.oct && max_depth == o.max_depth && g_depth == o.g_depth &&
                    g_params == o.g_params && g_wf == o.g_wf;
         }
     };
@@ -2147,7 +2170,7 @@ int ensure_guide(pg_scene& S, int64_t n, int rec_depth) {
     int rc;
     if (n > S.guide_capacity) {
         if ((rc = dalloc(S, (int64_t)guide::kParamStride * n, &P.gparams)) || (rc = dalloc(S, n, &P.gflag)) ||
-            (rc = dalloc(S, n, &P.shadow_rec)))
+            (rc = dalloc(S, n, &P.shadow_rec)) || (rc = dalloc(S, (int64_t)guide::kIn * n, &P.gx)))
             return rc;
         S.guide_capacity = n;
         S.paths_gen++;
@@ -2337,7 +2360,8 @@ static int record_bounces(pg_scene* S, const PathsDev& P, int64_t cap, hipStream
         Q.counters = S->g_counts + 4 * (b + 1);
         if (int rc = pg_trace_closest_dev(S->accel, Q.rays_in, d_active, cap, Q.hits, Q.uvs, st))
             return rc;
-        if (S->dev.guided) {
+        if (S->dev.guided && b < S->dev.g_depth) {
+            hipLaunchKernelGGL(k_guide_gather, dim3(grid), dim3(B), 0, st, S->dev, Q, 0, d_active);
             hipLaunchKernelGGL(k_guide_infer<1>, dim3(infer_blocks((int32_t)cap)), dim3(kInferThreads), 0, st,
                                S->dev, Q, 0, d_active);
             hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(B), 0, st, S->dev, Q, 0, d_active);
@@ -2363,6 +2387,7 @@ static int ensure_graph(pg_scene* S, const PathsDev& P, int64_t cap) {
     key.accel_gen = accel_generation(S->accel);
     key.paths_gen = S->paths_gen;
     key.guided = S->dev.guided;
+    key.g_depth = S->dev.g_depth;
     key.oct = S->dev.g_oct;
     key.max_depth = S->max_depth;
     key.g_params = S->dev.g_params;
@@ -2497,7 +2522,9 @@ static int render_enqueue(pg_scene* S, int spp, int spb_req, hipStream_t st, Gui
             if (int rc = pg_trace_closest(S->accel, Q.rays_in, active, Q.hits, Q.uvs, st))
                 return rc;
             if (!S->general) {
-                if (S->dev.guided) {
+                if (S->dev.guided && bounce < S->dev.g_depth) {
+                    hipLaunchKernelGGL(k_guide_gather, dim3((active + B - 1) / B), dim3(B), 0, st, S->dev, Q,
+                                       active, nullptr);
                     hipLaunchKernelGGL(k_guide_infer<1>, dim3(infer_blocks(active)), dim3(kInferThreads), 0, st,
                                        S->dev, Q, active, nullptr);
                     hipLaunchKernelGGL(k_shade<true>, dim3((active + B - 1) / B), dim3(B), 0, st, S->dev, Q,
@@ -2842,6 +2869,7 @@ extern "C" int pg_render_guided(pg_scene* S, pg_guide* G, const pg_guided_option
     const int final_spp = o && o->final_spp > 0 ? o->final_spp : S->spp;
     const int offset = o && o->sample_offset > 0 ? o->sample_offset : 0;
     SceneDev& d = S->dev;
+    d.g_depth = o && o->guide_depth > 0 ? o->guide_depth : (1 << 30);
     d.g_params = guide_params_dev(G);
     d.g_wf = guide_wf_dev(G);
     d.g_oct = guide_freq_offset(G);

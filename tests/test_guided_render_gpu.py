@@ -77,31 +77,43 @@ def test_trained_guide_lowers_error_at_equal_spp(pg, cuda, box):
 
 
 def test_guide_learns_the_gap(pg, cuda, box):
-    from pine_amd import hip
+    """From floor points of the room, the trained guide's probability of a
+    direction through the 0.3 x 0.3 gap is several times the cosine lobe's
+    (both integrated over the gap's solid angle by quadrature on its area)."""
     s = scene(pg, box, 128)
     g = pg.Guide(seed=3, learning_rate=3e-3)
     s.render_guided(g, final_spp=4)
-    # Floor and wall points of the room, the direction to the gap centre.
     bmin, bmax = np.array([-1.0, 0.0, -1.0]), np.array([1.0, 2.0, 1.0])
-    pts = np.array([[0.0, 0.001, 0.4], [0.4, 0.001, 0.6], [-0.4, 0.001, 0.2], [-0.999, 1.0, 0.3],
-                    [0.999, 0.8, 0.5]])
-    nrm = np.array([[0, 1, 0], [0, 1, 0], [0, 1, 0], [1, 0, 0], [-1, 0, 0]], float)
-    gap = np.array([0.0, 1.0, -0.4])
-    x, dirs = [], []
-    for p, n in zip(pts, nrm):
-        wo = np.array([0.0, 1.0, 3.3]) - p
-        wo /= np.linalg.norm(wo)
-        x.append(np.concatenate([(p - bmin) / (bmax - bmin), n, wo]))
-        w = gap - p
-        dirs.append(w / np.linalg.norm(w))
-    x = np.array(x, np.float32)
-    dirs = np.array(dirs, np.float32)
-    cosine = np.maximum((dirs * nrm).sum(1), 0) / np.pi
-    out = g.pdf(cuda.DeviceArray.from_numpy(x), cuda.DeviceArray.from_numpy(dirs),
-                cuda.DeviceArray.from_numpy(cosine.astype(np.float32))).numpy()
-    print("guide density towards the gap:", np.round(out[:, 0], 3), "cosine:", np.round(cosine, 3),
-          "alpha:", np.round(out[:, 1], 3))
-    assert np.median(out[:, 0] / cosine) > 3.0
+    cam = np.array([0.0, 1.0, 3.3])
+    n = np.array([0.0, 1.0, 0.0])
+    # Gap quadrature: 8 x 8 cell centres of the opening (x in [-0.15, 0.15],
+    # y in [0.85, 1.15], z = -0.4, normal +z).
+    u = (np.arange(8) + 0.5) / 8
+    gx, gy = np.meshgrid(-0.15 + 0.3 * u, 0.85 + 0.3 * u)
+    gap = np.stack([gx.ravel(), gy.ravel(), np.full(64, -0.4)], 1)
+    ratios = []
+    pts = [np.array([x, 0.001, z]) for x in (-0.6, -0.3, 0.0, 0.3, 0.6) for z in (0.0, 0.3, 0.6)]
+    xs, ws, jac = [], [], []
+    for p in pts:
+        wo = (cam - p) / np.linalg.norm(cam - p)
+        d = gap - p
+        dist = np.linalg.norm(d, axis=1)
+        w = d / dist[:, None]
+        xs.append(np.repeat(np.concatenate([(p - bmin) / (bmax - bmin), n, wo])[None], 64, 0))
+        ws.append(w)
+        jac.append(np.abs(w[:, 2]) / dist ** 2)  # d(solid angle) / d(area)
+    x = np.concatenate(xs).astype(np.float32)
+    w = np.concatenate(ws).astype(np.float32)
+    cosine = (np.maximum(w @ n, 0) / np.pi).astype(np.float32)
+    out = g.pdf(cuda.DeviceArray.from_numpy(x), cuda.DeviceArray.from_numpy(w),
+                cuda.DeviceArray.from_numpy(cosine)).numpy()
+    q = out[:, 0].reshape(len(pts), 64)
+    c = cosine.reshape(len(pts), 64)
+    j = np.stack(jac)
+    ratios = (q * j).sum(1) / (c * j).sum(1)
+    print("guide / cosine probability through the gap:", np.round(ratios, 2),
+          "alpha:", np.round(out[::64, 1], 3))
+    assert np.median(ratios) > 3.0
     assert (out[:, 1] > 0.2).all()
     g.release()
     s.release()

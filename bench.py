@@ -19,7 +19,8 @@ launch stream.  cpu_baseline times the oracle's CPU port of pine's traversal
 Secondary workloads, each next to pine itself (oracle/_ref/ref_harness, pine's
 own PathIntegrator built from its sources) on the host cores, at matched
 relMSE (the GPU reproduces pine's per-pixel sample sequence):
-  path_tracing       the seeded 64-spheres scene, 512x512, 64 spp, maxDepth 8
+  path_tracing       the seeded 64-spheres scene, 512x512, 64 spp, maxDepth 8 (also
+                     in spectral mode, four wavelengths per path; GPU only)
   mesh_path_tracing  the 2^20-triangle icosphere in a lit scene, 512x512
   volume             a 256^3 fBm density grid, 1024x1024, 256 spp, maxDepth 64,
                      16^3-voxel majorant bricks, HG g = 0.85, albedo 0.99
@@ -352,9 +353,10 @@ def main():
     ap.add_argument("--batch-streams", type=int, default=4)
     ap.add_argument("--no-tune", action="store_true", help="batch leg without the autotuner")
     ap.add_argument("--no-guide", action="store_true", help="skip the path-guiding leg")
-    ap.add_argument("--guide-res", type=int, default=512,
-                    help="occluded-box resolution of the guiding leg (1024 is the full workload)")
-    ap.add_argument("--guide-ref-chunks", type=int, default=32)
+    ap.add_argument("--guide-res", type=int, default=1024,
+                    help="occluded-box resolution of the guiding leg")
+    ap.add_argument("--guide-ref-chunks", type=int, default=16,
+                    help="1024-spp ranges in the guiding leg's reference")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one device-to-device copy of the ray buffer (512 MiB), the "
                          "known-size kernel that scales rocprof's FETCH_SIZE/WRITE_SIZE")
@@ -400,6 +402,15 @@ def main():
               "render_ms": round(ms, 2),
               "mrays_per_s": round(inf["last_rays"] / (ms * 1e-3) * 1e-6, 2)}
         log(f"[bench] spheres: {pt['samples_per_s'] / 1e6:.1f} M samples/s")
+        # The same scene with four wavelengths per path (spectral mode).
+        sc = pg.Scene(text=text, directory=work.name)
+        sc.render(spp=2, spectral=True)
+        sc.render(spectral=True)
+        sms = sc.info()["last_render_ms"]
+        sc.release()
+        pt["spectral"] = {"samples_per_s": round(512 * 512 * args.pt_spp / (sms * 1e-3), 1),
+                          "render_ms": round(sms, 2), "wavelengths_per_path": 4}
+        log(f"[bench] spheres, spectral: {pt['spectral']['samples_per_s'] / 1e6:.1f} M samples/s")
         # 2^20-triangle mesh scene.
         mesh_spp = 64
         text = W.mesh_scene_text("ico1m.obj", width=512, height=512, spp=mesh_spp)

@@ -209,6 +209,17 @@ typedef struct pg_render_options {
  * bounce, instead of replaying it as a recorded HIP graph (the default; both
  * give the same image bit for bit). */
 #define PG_RENDER_NO_GRAPH 1
+/* Spectral transport (an extension; pine renders RGB): four wavelengths per
+ * path by hero-wavelength sampling, the scene's RGB reflectances, emission and
+ * sky upsampled to smooth spectra, radiance projected to RGB through the CIE
+ * 1931 matching functions.  Gray scenes render as in RGB in expectation.  The
+ * fused pipeline only (scenes without media or material-less shapes). */
+#define PG_RENDER_SPECTRAL 2
+
+/* The spectral mode's tables: Q (RGB -> spectral basis coefficients), M (XYZ ->
+ * RGB), both row-major 3x3, and the film weight scale 470 / (4 * integral of
+ * ybar).  For tests; no pine counterpart. */
+int pg_spectral_tables(float* Q, float* M, float* w_scale);
 
 /* Renders the whole film: PixelIntegrator::Render + RadianceIntegrator::Compute
  * + PathIntegrator::Li (integrator.cpp:121-158, path.cpp:6-87) for every pixel
@@ -355,6 +366,7 @@ typedef struct pg_guided_options {
     int32_t sample_offset;  /* first sample index of the first training pass */
     int32_t guide_depth;    /* vertices of depth < guide_depth use the guide (0 = all); deeper
                                ones sample their BSDF and cost no network evaluation */
+    int32_t flags;          /* PG_RENDER_* bits for the final render */
 } pg_guided_options;
 int pg_render_guided(pg_scene* s, pg_guide* g, const pg_guided_options* o, void* stream);
 

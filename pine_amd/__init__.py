@@ -32,8 +32,10 @@ EXPORTED = [
     "pg_guide_forward", "pg_guide_pdf", "pg_guide_sample", "pg_guide_gradient", "pg_guide_train",
     "pg_guide_release", "pg_render_guided", "pg_trace_closest_dev", "pg_trace_occluded_dev",
     "pg_accel_rebuild", "pg_render_batch", "pg_scene_autotune", "pg_write_obj",
+    "pg_spectral_tables",
 ]
 RENDER_NO_GRAPH = 1
+RENDER_SPECTRAL = 2
 GUIDE_N_PARAMS = 27888
 
 
@@ -76,7 +78,8 @@ class _GuidedOptions(ctypes.Structure):
     _fields_ = [("train_passes", ctypes.c_int32), ("train_spp", ctypes.c_int32),
                 ("steps_per_pass", ctypes.c_int32), ("batch", ctypes.c_int32),
                 ("record_depth", ctypes.c_int32), ("final_spp", ctypes.c_int32),
-                ("sample_offset", ctypes.c_int32), ("guide_depth", ctypes.c_int32)]
+                ("sample_offset", ctypes.c_int32), ("guide_depth", ctypes.c_int32),
+                ("flags", ctypes.c_int32)]
 
 
 class _RenderOptions(ctypes.Structure):
@@ -163,6 +166,7 @@ def lib():
                                   ctypes.POINTER(_BatchResult)]
     L.pg_scene_autotune.argtypes = [vp, ctypes.POINTER(_TuneOptions), vp, ctypes.POINTER(_TuneResult)]
     L.pg_write_obj.argtypes = [ctypes.c_char_p, vp, i64, vp, i64]
+    L.pg_spectral_tables.argtypes = [vp, vp, vp]
     _lib = L
     return L
 
@@ -230,6 +234,13 @@ def write_obj_fixed(path, vertices, indices):
     v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
     i = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
     check(lib().pg_write_obj(path.encode(), _ptr(v), v.shape[0], _ptr(i), i.shape[0]))
+
+
+def spectral_tables():
+    """(Q, M, w_scale) of the spectral mode (pg_spectral_tables)."""
+    Q, M, w = np.zeros(9, np.float32), np.zeros(9, np.float32), np.zeros(1, np.float32)
+    check(lib().pg_spectral_tables(_ptr(Q), _ptr(M), _ptr(w)))
+    return Q.reshape(3, 3), M.reshape(3, 3), float(w[0])
 
 
 def write_volume(path, density):
@@ -361,8 +372,9 @@ class Scene:
         check(lib().pg_scene_get_info(self.h, ctypes.byref(s)))
         return {k: getattr(s, k) for k, _ in _SceneInfo._fields_}
 
-    def render(self, spp=0, samples_per_batch=0, sample_offset=0, graph=True):
-        o = _RenderOptions(spp, samples_per_batch, sample_offset, 0 if graph else RENDER_NO_GRAPH)
+    def render(self, spp=0, samples_per_batch=0, sample_offset=0, graph=True, spectral=False):
+        flags = (0 if graph else RENDER_NO_GRAPH) | (RENDER_SPECTRAL if spectral else 0)
+        o = _RenderOptions(spp, samples_per_batch, sample_offset, flags)
         check(lib().pg_render(self.h, ctypes.byref(o), _stream_handle()))
         return self.info()
 
@@ -373,10 +385,10 @@ class Scene:
         return {k: getattr(r, k) for k, _ in _TuneResult._fields_}
 
     def render_guided(self, guide, train_passes=16, train_spp=8, steps_per_pass=0, batch=1024,
-                      record_depth=5, final_spp=0, sample_offset=0, guide_depth=0):
+                      record_depth=5, final_spp=0, sample_offset=0, guide_depth=0, graph=True):
         """Trains `guide` on this scene, then renders final_spp with it (pg_render_guided)."""
         o = _GuidedOptions(train_passes, train_spp, steps_per_pass, batch, record_depth, final_spp,
-                           sample_offset, guide_depth)
+                           sample_offset, guide_depth, 0 if graph else RENDER_NO_GRAPH)
         check(lib().pg_render_guided(self.h, guide.h, ctypes.byref(o), _stream_handle()))
         return self.info()
 

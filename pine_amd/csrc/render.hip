@@ -19,6 +19,7 @@
 #include "pg_guide.h"
 #include "pg_isect.h"
 #include "pg_medium.h"
+#include "pg_spectral.h"
 #include "pg_shade.h"
 #include "scene_parse.h"
 
@@ -97,6 +98,7 @@ struct SceneDev {
     // Learned path guiding (general mode only).
     int32_t guided;
     int32_t g_depth;  // vertices of depth < g_depth sample from the guide
+    spec::Tables spec;  // spectral mode: RGB <-> spectrum tables
     const float* g_params;  // FP32 parameters (biases used in FP32)
     const __bf16* g_wf;     // BF16 forward weights
     f3 g_bmin, g_binv;      // scene bounds: network position input = (p - bmin) * binv
@@ -123,6 +125,12 @@ struct PathsDev {
     pg_ray* shadow_rays;
     int32_t* shadow_path;
     float* shadow_contrib;  // 3 per shadow ray
+    // Spectral mode: hero-wavelength variable per path and 4-wavelength
+    // radiance, throughput and shadow contributions.
+    float* lam;
+    float* L4;
+    float* beta4;
+    float* sh4;
     int32_t* occluded;
     // Film contributions a sample makes to a neighbouring pixel (a sample on a
     // pixel border): (pixel, w*r, w*g, w*b, w), applied after the main pass.
@@ -233,6 +241,21 @@ __global__ __launch_bounds__(256) void k_raygen(SceneDev s, PathsDev P, int32_t 
     store_ray(&P.rays_in[i], o, d, 0.0f, kFloatMax);
     if (P.medium)
         P.medium[i] = s.camera_medium;  // camera.h:61 ray.medium = medium
+    if (P.lam) {
+        // Spectral mode: the hero-wavelength variable from a hash of (pixel,
+        // sample index), so a render is deterministic and batch-independent.
+        uint32_t h = (uint32_t)pixel * 0x9E3779B1u ^ ((uint32_t)(sample0 + sl) + 0x7F4A7C15u) * 0x85EBCA77u;
+        h ^= h >> 15;
+        h *= 0x2C1B3C6Du;
+        h ^= h >> 12;
+        h *= 0x297A2D39u;
+        h ^= h >> 15;
+        P.lam[i] = (float)(h >> 8) * (1.0f / 16777216.0f);
+        for (int j = 0; j < 4; j++) {
+            P.L4[4 * (int64_t)i + j] = 0.0f;
+            P.beta4[4 * (int64_t)i + j] = 1.0f;
+        }
+    }
 }
 
 // Sky::Color / SkyColor (light.cpp:74-76, color.cpp:91-99): a gradient in the
@@ -648,6 +671,170 @@ __global__ __launch_bounds__(256) void k_resolve_shadow(PathsDev P, int32_t n_sh
     }
 }
 
+
+// --- spectral mode (fused bounce, no guide) -----------------------------------
+
+struct Spec4 {
+    float v[4];
+};
+__device__ __forceinline__ Spec4 spec_load(const float* p, int64_t i) {
+    const float4 a = reinterpret_cast<const float4*>(p)[i];
+    return {{a.x, a.y, a.z, a.w}};
+}
+__device__ __forceinline__ void spec_store(float* p, int64_t i, const Spec4& s) {
+    reinterpret_cast<float4*>(p)[i] = make_float4(s.v[0], s.v[1], s.v[2], s.v[3]);
+}
+
+// k_shade's fused bounce with every colour a 4-wavelength spectrum: RGB
+// quantities from the scene (albedo-driven BSDF values, emission, sky) are
+// upsampled at the path's wavelengths (spec::upsample_row), products are taken
+// per wavelength, Russian roulette uses the mean throughput.  Geometry,
+// sampling and the sampler's dimension order are those of k_shade.
+__global__ __launch_bounds__(256) void k_shade_spec(SceneDev s, PathsDev P, int32_t n_active) {
+    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_active)
+        return;
+    const int32_t i = P.queue_in[k];
+    const float4* r4 = reinterpret_cast<const float4*>(&P.rays_in[k]);
+    const float4 ra = r4[0], rb = r4[1];
+    const f3 ro = mk(ra.x, ra.y, ra.z), rd = mk(rb.x, rb.y, rb.z);
+    const pg_hit hit = P.hits[k];
+    int depth = P.depth[i];
+    Spec4 L = spec_load(P.L4, i), beta = spec_load(P.beta4, i);
+    f3 up[4];
+    {
+        const float u = P.lam[i];
+        for (int j = 0; j < 4; j++)
+            up[j] = spec::upsample_row(s.spec, spec::wavelength(u, j));
+    }
+    auto lift = [&](f3 c, float scale) {
+        Spec4 r;
+        for (int j = 0; j < 4; j++)
+            r.v[j] = fmaxf(0.0f, dot(up[j], c)) * scale;
+        return r;
+    };
+    const float bsdfPdf = P.bsdf_pdf[i];
+    SamplerState ss;
+    ss.index = P.sidx[i];
+    ss.dim = P.sdim[i];
+    bool alive = false;
+    f3 nextO, nextD;
+    if (hit.prim < 0) {
+        if (s.has_env) {
+            const float w = depth == 0 ? 1.0f : power_heuristic(bsdfPdf, 0.0f);
+            const Spec4 le = lift(sky_color(s, rd), w);
+            for (int j = 0; j < 4; j++)
+                L.v[j] += beta.v[j] * le.v[j];
+        }
+    } else {
+        const pg_hit_uv uv = P.uvs[k];
+        Hit h = surface(s, hit.prim, hit.t, uv.u, uv.v, ro, rd);
+        const ShapeDev& sh = s.shapes[h.shape];
+        const MaterialDev& m = s.materials[sh.material];
+        const f3 wi = -rd;
+        if (m.kind == kMatEmissive) {
+            const float w = depth == 0 ? 1.0f
+                                       : power_heuristic(bsdfPdf, hit.t * hit.t / (abs_dot(-rd, h.n) * sh.area));
+            const Spec4 le = lift(emitted(m, h.n, wi), w);
+            for (int j = 0; j < 4; j++)
+                L.v[j] += beta.v[j] * le.v[j];
+        } else if (depth + 1 < s.max_depth) {
+            const M3 n2w = frame_from(h.n);
+            const M3 w2n = inverse(n2w);
+            const f3 wiL = mul(w2n, wi);
+            float lpdf;
+            const float ul = get_1d(s.halton, ss);
+            float la, lb;
+            if (s.n_lights > 0) {
+                int li = sample_light(s, ul, lpdf);
+                get_2d(s.halton, ss, la, lb);
+                LightSampleD ls = sample_light_point(s, s.lights[li], h.p, la, lb);
+                ls.pdf *= lpdf;
+                const f3 woL = mul(w2n, ls.wo);
+                const f3 f = material_f(m, wiL, woL) * abs_dot(ls.wo, h.n);
+                const float scatPdf = ls.delta ? 0.0f : material_pdf(m, wiL, woL);
+                const float w = ls.delta ? 1.0f : power_heuristic(ls.pdf, scatPdf);
+                const Spec4 fs = lift(f, 1.0f), le = lift(ls.Le, w / ls.pdf);
+                Spec4 contrib;
+                bool any = false;
+                for (int j = 0; j < 4; j++) {
+                    contrib.v[j] = beta.v[j] * fs.v[j] * le.v[j];
+                    any = any || contrib.v[j] != 0.0f;
+                }
+                if (any) {
+                    int slot = atomicAdd(&P.counters[1], 1);
+                    f3 so = offset_ray_origin(h.p, face_same_hemisphere(h.n, ls.wo));
+                    store_ray(&P.shadow_rays[slot], so, ls.wo, 0.0f, ls.distance * (1.0f - 1e-3f));
+                    P.shadow_path[slot] = i;
+                    spec_store(P.sh4, slot, contrib);
+                }
+            }
+            const float u1 = get_1d(s.halton, ss);
+            float u2a, u2b;
+            get_2d(s.halton, ss, u2a, u2b);
+            BSDFSample bs;
+            if (material_sample(m, wiL, u1, u2a, u2b, bs)) {
+                bs.wo = mul(n2w, bs.wo);
+                if (!is_black(bs.f) && bs.pdf != 0.0f) {
+                    const Spec4 fs = lift(bs.f, abs_dot(bs.wo, h.n) / bs.pdf);
+                    for (int j = 0; j < 4; j++)
+                        beta.v[j] *= fs.v[j];
+                    P.bsdf_pdf[i] = bs.pdf;
+                    nextD = bs.wo;
+                    nextO = offset_ray_origin(h.p, face_same_hemisphere(h.n, nextD));
+                    alive = true;
+                    if (depth > 2) {
+                        const float mean = 0.25f * (beta.v[0] + beta.v[1] + beta.v[2] + beta.v[3]);
+                        const float q = pstd::clamp(1.0f - mean, 0.05f, 1.0f);
+                        if (get_1d(s.halton, ss) < q) {
+                            alive = false;
+                        } else {
+                            for (int j = 0; j < 4; j++)
+                                beta.v[j] /= (1.0f - q);
+                        }
+                    }
+                }
+            }
+            depth++;
+        }
+    }
+    spec_store(P.L4, i, L);
+    spec_store(P.beta4, i, beta);
+    P.sdim[i] = ss.dim;
+    P.depth[i] = depth;
+    if (alive) {
+        int slot = atomicAdd(&P.counters[0], 1);
+        P.queue_out[slot] = i;
+        store_ray(&P.rays_out[slot], nextO, nextD, 0.0f, kFloatMax);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_resolve_shadow_spec(PathsDev P, int32_t n_shadow) {
+    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_shadow || P.occluded[k])
+        return;
+    const int32_t i = P.shadow_path[k];
+    Spec4 L = spec_load(P.L4, i);
+    const Spec4 c = spec_load(P.sh4, k);
+    for (int j = 0; j < 4; j++)
+        L.v[j] += c.v[j];
+    spec_store(P.L4, i, L);
+}
+
+// Each path's 4-wavelength radiance to the film's RGB (spec::film_weight).
+__global__ __launch_bounds__(256) void k_spec_to_rgb(SceneDev s, PathsDev P, int32_t n_paths) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_paths)
+        return;
+    const Spec4 L = spec_load(P.L4, i);
+    const float u = P.lam[i];
+    f3 rgb = mk(0.0f, 0.0f, 0.0f);
+    for (int j = 0; j < 4; j++)
+        rgb = rgb + spec::film_weight(s.spec, spec::wavelength(u, j)) * L.v[j];
+    P.L[3 * i + 0] = rgb.x;
+    P.L[3 * i + 1] = rgb.y;
+    P.L[3 * i + 2] = rgb.z;
+}
 
 // ---------------------------------------------------------------------------
 // General mode: media and material-less shapes.  One bounce of
@@ -1349,6 +1536,8 @@ struct pg_scene {
         }
     };
     bool use_graph = true;
+    bool spectral = false;               // this render in spectral mode (PG_RENDER_SPECTRAL)
+    int64_t spectral_capacity = 0;
     GraphKey graph_key;
     hipGraphExec_t graph_exec = nullptr;
     hipStream_t capture_stream = nullptr;
@@ -2094,6 +2283,7 @@ int dalloc(pg_scene& S, int64_t n, T** out) {
 int finish_scene(pg_scene& S, const pg_build_options* opts) {
     if (opts)
         S.build_opts = *opts;
+    spec::spectral_tables(S.dev.spec);
     if (int rc = pg_accel_build(S.accel, opts))
         return rc;
     std::vector<int> primes, sums;
@@ -2162,6 +2352,19 @@ int ensure_paths(pg_scene& S, int64_t n) {
             return rc;
     }
     S.path_capacity = n;
+    return PG_OK;
+}
+
+int ensure_spectral(pg_scene& S, int64_t n) {
+    if (n <= S.spectral_capacity)
+        return PG_OK;
+    PathsDev& P = S.paths;
+    int rc;
+    if ((rc = dalloc(S, n, &P.lam)) || (rc = dalloc(S, 4 * n, &P.L4)) || (rc = dalloc(S, 4 * n, &P.beta4)) ||
+        (rc = dalloc(S, 4 * n, &P.sh4)))
+        return rc;
+    S.spectral_capacity = n;
+    S.paths_gen++;
     return PG_OK;
 }
 
@@ -2483,17 +2686,26 @@ static int render_enqueue(pg_scene* S, int spp, int spb_req, hipStream_t st, Gui
         if (int rc = ensure_guide(*S, n_paths, tr ? tr->rec_depth : 0))
             return rc;
     }
+    if (S->spectral) {
+        if (S->general || S->dev.guided)
+            return pg_fail(PG_E_ARG, "spectral rendering needs the fused pipeline: no media, no material-less "
+                                     "shapes, no guide");
+        if (int rc = ensure_spectral(*S, n_paths))
+            return rc;
+    }
     PG_HIP(hipEventCreate(&t->e0));
     PG_HIP(hipEventCreate(&t->e1));
     PG_HIP(hipEventRecord(t->e0, st));
     PG_HIP(hipMemsetAsync(S->d_film, 0, (size_t)npix * 4 * sizeof(float), st));
     PathsDev P = S->paths;
     P.rec_depth = tr ? tr->rec_depth : 0;
+    if (!S->spectral)
+        P.lam = nullptr;  // k_raygen initialises spectral state only in spectral mode
     if (P.lookups)
         PG_HIP(hipMemsetAsync(P.lookups, 0, sizeof(unsigned long long), st));
     // Fused pipeline without training: the bounce loop is a graph replay per
     // batch (no host round trip per bounce).
-    const bool graph = S->use_graph && !S->general && !tr;
+    const bool graph = S->use_graph && !S->general && !tr && !S->spectral;
     if (graph) {
         if (int rc = ensure_graph(S, P, n_paths))
             return rc;
@@ -2521,7 +2733,20 @@ static int render_enqueue(pg_scene* S, int spp, int spb_req, hipStream_t st, Gui
             PG_HIP(hipMemsetAsync(Q.counters, 0, 4 * sizeof(int32_t), st));
             if (int rc = pg_trace_closest(S->accel, Q.rays_in, active, Q.hits, Q.uvs, st))
                 return rc;
-            if (!S->general) {
+            if (S->spectral) {
+                hipLaunchKernelGGL(k_shade_spec, dim3((active + B - 1) / B), dim3(B), 0, st, S->dev, Q, active);
+                int32_t counts[2];
+                PG_HIP(hipMemcpyAsync(counts, Q.counters, sizeof(counts), hipMemcpyDeviceToHost, st));
+                PG_HIP(hipStreamSynchronize(st));
+                rays += active + counts[1];
+                if (counts[1] > 0) {
+                    if (int rc = pg_trace_occluded(S->accel, Q.shadow_rays, counts[1], Q.occluded, st))
+                        return rc;
+                    hipLaunchKernelGGL(k_resolve_shadow_spec, dim3((counts[1] + B - 1) / B), dim3(B), 0, st, Q,
+                                       counts[1]);
+                }
+                active = counts[0];
+            } else if (!S->general) {
                 if (S->dev.guided && bounce < S->dev.g_depth) {
                     hipLaunchKernelGGL(k_guide_gather, dim3((active + B - 1) / B), dim3(B), 0, st, S->dev, Q,
                                        active, nullptr);
@@ -2591,6 +2816,8 @@ static int render_enqueue(pg_scene* S, int spp, int spb_req, hipStream_t st, Gui
                 tr->steps += steps;
             }
         }
+        if (S->spectral)
+            hipLaunchKernelGGL(k_spec_to_rgb, dim3((np + B - 1) / B), dim3(B), 0, st, S->dev, P, np);
         PG_HIP(hipMemsetAsync(P.spill_count, 0, sizeof(int32_t), st));
         hipLaunchKernelGGL(k_film, dim3((npix + B - 1) / B), dim3(B), 0, st, S->dev, P, (int32_t)npix,
                            nb, S->d_film);
@@ -2648,15 +2875,30 @@ extern "C" int pg_render(pg_scene* S, const pg_render_options* opts, void* strea
     const int offset = opts && opts->sample_offset > 0 ? opts->sample_offset : 0;
     S->dev.guided = 0;
     S->use_graph = !(opts && (opts->flags & PG_RENDER_NO_GRAPH));
+    S->spectral = opts && (opts->flags & PG_RENDER_SPECTRAL);
     double ms = 0.0;
     int64_t rays = 0;
     const int rc = render_impl(S, spp, spb, (hipStream_t)stream, nullptr, &ms, &rays, offset);
     S->use_graph = true;
+    S->spectral = false;
     if (rc)
         return rc;
     S->last_render_ms = ms;
     S->last_rays = rays;
     S->last_guide_ms = 0.0;
+    return PG_OK;
+}
+
+extern "C" int pg_spectral_tables(float* Q, float* M, float* w_scale) {
+    if (!Q || !M || !w_scale)
+        return pg_fail(PG_E_ARG, "pg_spectral_tables: null argument");
+    spec::Tables t;
+    spec::spectral_tables(t);
+    for (int k = 0; k < 9; k++) {
+        Q[k] = t.Q[k];
+        M[k] = t.M[k];
+    }
+    *w_scale = t.w_scale;
     return PG_OK;
 }
 
@@ -2891,8 +3133,11 @@ extern "C" int pg_render_guided(pg_scene* S, pg_guide* G, const pg_guided_option
         train_ms += ms;
         all_rays += rays;
     }
-    if (rc == PG_OK)
+    if (rc == PG_OK) {
+        S->use_graph = !(o && (o->flags & PG_RENDER_NO_GRAPH));
         rc = render_impl(S, final_spp, 0, st, nullptr, &ms, &rays, offset + passes * train_spp);
+        S->use_graph = true;
+    }
     d.guided = 0;
     if (rc != PG_OK)
         return rc;

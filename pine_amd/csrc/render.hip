@@ -99,6 +99,7 @@ struct SceneDev {
     int32_t guided;
     int32_t g_depth;  // vertices of depth < g_depth sample from the guide
     spec::Tables spec;  // spectral mode: RGB <-> spectrum tables
+    int32_t viz_type;   // Viz integrator: 0 Position, 1 Normal
     const float* g_params;  // FP32 parameters (biases used in FP32)
     const __bf16* g_wf;     // BF16 forward weights
     f3 g_bmin, g_binv;      // scene bounds: network position input = (p - bmin) * binv
@@ -671,6 +672,139 @@ __global__ __launch_bounds__(256) void k_resolve_shadow(PathsDev P, int32_t n_sh
     }
 }
 
+
+// --- pine's AO, Viz and RandomWalk integrators --------------------------------
+
+// AOIntegrator::Li (ao.cpp:9-19): at the camera ray's hit, one direction
+// uniform over the sphere (UniformHemisphereMampling, sampling.h:49-51, with
+// pine's acos/sin/cos) turned into the hemisphere of the face-forward normal;
+// radiance 1 when BVH::Hit finds nothing along it.  Misses give 0.
+__global__ __launch_bounds__(256) void k_shade_ao(SceneDev s, PathsDev P, int32_t n_active) {
+    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_active)
+        return;
+    const int32_t i = P.queue_in[k];
+    P.L[3 * i + 0] = P.L[3 * i + 1] = P.L[3 * i + 2] = 0.0f;
+    const pg_hit hit = P.hits[k];
+    if (hit.prim < 0)
+        return;
+    const float4* r4 = reinterpret_cast<const float4*>(&P.rays_in[k]);
+    const float4 ra = r4[0], rb = r4[1];
+    const f3 ro = mk(ra.x, ra.y, ra.z), rd = mk(rb.x, rb.y, rb.z);
+    const pg_hit_uv uv = P.uvs[k];
+    const Hit h = surface(s, hit.prim, hit.t, uv.u, uv.v, ro, rd);
+    const f3 n = face_same_hemisphere(h.n, -rd);
+    SamplerState ss;
+    ss.index = P.sidx[i];
+    ss.dim = P.sdim[i];
+    float u0, u1;
+    get_2d(s.halton, ss, u0, u1);
+    const float phi = u0 * pstd::Pi * 2.0f, theta = pstd::acos(u1);
+    const float st = pstd::sin(theta);
+    const f3 w = face_same_hemisphere(mk(st * pstd::cos(phi), st * pstd::sin(phi), pstd::cos(theta)), n);
+    const int32_t slot = atomicAdd(&P.counters[1], 1);
+    store_ray(&P.shadow_rays[slot], offset_ray_origin(h.p, face_same_hemisphere(n, w)), w, 0.0f, kFloatMax);
+    P.shadow_path[slot] = i;
+}
+
+__global__ __launch_bounds__(256) void k_resolve_ao(PathsDev P, int32_t n_shadow) {
+    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_shadow || P.occluded[k])
+        return;
+    const int32_t i = P.shadow_path[k];
+    P.L[3 * i + 0] = P.L[3 * i + 1] = P.L[3 * i + 2] = 1.0f;
+}
+
+// VizIntegrator::Li (viz.cpp:27-41), Position and Normal: the camera ray's
+// hit point or shading normal as the colour, 0 for a miss.
+__global__ __launch_bounds__(256) void k_shade_viz(SceneDev s, PathsDev P, int32_t n_active) {
+    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_active)
+        return;
+    const int32_t i = P.queue_in[k];
+    f3 c = mk(0.0f, 0.0f, 0.0f);
+    const pg_hit hit = P.hits[k];
+    if (hit.prim >= 0) {
+        const float4* r4 = reinterpret_cast<const float4*>(&P.rays_in[k]);
+        const float4 ra = r4[0], rb = r4[1];
+        const pg_hit_uv uv = P.uvs[k];
+        const Hit h = surface(s, hit.prim, hit.t, uv.u, uv.v, mk(ra.x, ra.y, ra.z), mk(rb.x, rb.y, rb.z));
+        c = s.viz_type == 0 ? h.p : h.n;
+    }
+    P.L[3 * i + 0] = c.x;
+    P.L[3 * i + 1] = c.y;
+    P.L[3 * i + 2] = c.z;
+}
+
+// RandomWalkIntegrator::Li (randomwalk.cpp:6-62) without media: emission and
+// the environment added with throughput and no light sampling; Material::Sample
+// per vertex (Get1D, Get2D); no Russian roulette; material-less surfaces
+// continue straight without counting depth.
+__global__ __launch_bounds__(256) void k_shade_rw(SceneDev s, PathsDev P, int32_t n_active) {
+    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_active)
+        return;
+    const int32_t i = P.queue_in[k];
+    const float4* r4 = reinterpret_cast<const float4*>(&P.rays_in[k]);
+    const float4 ra = r4[0], rb = r4[1];
+    const f3 ro = mk(ra.x, ra.y, ra.z), rd = mk(rb.x, rb.y, rb.z);
+    const pg_hit hit = P.hits[k];
+    int depth = P.depth[i];
+    f3 L = mk(P.L[3 * i], P.L[3 * i + 1], P.L[3 * i + 2]);
+    f3 beta = mk(P.beta[3 * i], P.beta[3 * i + 1], P.beta[3 * i + 2]);
+    SamplerState ss;
+    ss.index = P.sidx[i];
+    ss.dim = P.sdim[i];
+    bool alive = false;
+    f3 nextO, nextD;
+    if (hit.prim < 0) {
+        if (s.has_env)
+            L = L + beta * sky_color(s, rd);
+    } else {
+        const pg_hit_uv uv = P.uvs[k];
+        const Hit h = surface(s, hit.prim, hit.t, uv.u, uv.v, ro, rd);
+        const ShapeDev& sh = s.shapes[h.shape];
+        if (sh.material < 0) {
+            nextD = rd;
+            nextO = offset_ray_origin(h.p, face_same_hemisphere(h.n, rd));
+            alive = true;
+        } else {
+            const MaterialDev& m = s.materials[sh.material];
+            const f3 wi = -rd;
+            if (m.kind == kMatEmissive) {
+                L = L + beta * emitted(m, h.n, wi);
+            } else if (depth + 1 < s.max_depth) {
+                const M3 n2w = frame_from(h.n);
+                const M3 w2n = inverse(n2w);
+                const float u1 = get_1d(s.halton, ss);
+                float u2a, u2b;
+                get_2d(s.halton, ss, u2a, u2b);
+                BSDFSample bs;
+                if (material_sample(m, mul(w2n, wi), u1, u2a, u2b, bs)) {
+                    bs.wo = mul(n2w, bs.wo);
+                    beta = beta * (abs_dot(bs.wo, h.n) * bs.f / bs.pdf);
+                    nextD = bs.wo;
+                    nextO = offset_ray_origin(h.p, face_same_hemisphere(h.n, nextD));
+                    alive = true;
+                }
+                depth++;
+            }
+        }
+    }
+    P.L[3 * i + 0] = L.x;
+    P.L[3 * i + 1] = L.y;
+    P.L[3 * i + 2] = L.z;
+    P.beta[3 * i + 0] = beta.x;
+    P.beta[3 * i + 1] = beta.y;
+    P.beta[3 * i + 2] = beta.z;
+    P.sdim[i] = ss.dim;
+    P.depth[i] = depth;
+    if (alive) {
+        const int slot = atomicAdd(&P.counters[0], 1);
+        P.queue_out[slot] = i;
+        store_ray(&P.rays_out[slot], nextO, nextD, 0.0f, kFloatMax);
+    }
+}
 
 // --- spectral mode (fused bounce, no guide) -----------------------------------
 
@@ -1488,6 +1622,8 @@ __global__ __launch_bounds__(256) void k_film_spill(PathsDev P, float* film) {
 // ----------------------------------------------------------------------------
 // Host: scene loading (fileio.cpp:251-281 and the Create* factories)
 
+enum { kIntPath = 0, kIntAO = 1, kIntViz = 2, kIntRandomWalk = 3 };
+
 struct pg_scene {
     std::string dir;
     Params root;
@@ -1536,6 +1672,7 @@ struct pg_scene {
         }
     };
     bool use_graph = true;
+    int integrator = 0;                  // kIntPath / kIntAO / kIntViz / kIntRandomWalk
     bool spectral = false;               // this render in spectral mode (PG_RENDER_SPECTRAL)
     int64_t spectral_capacity = 0;
     GraphKey graph_key;
@@ -1838,11 +1975,31 @@ void make_medium(pg_scene& S, const Params& p) {
 
 void build_scene(pg_scene& S) {
     const Params& root = S.root;
-    // Integrator block (integrator.cpp:20-56)
+    // Integrator block (integrator.cpp:20-56): Path, and AO, Viz (Position,
+    // Normal) and RandomWalk on the same kernels.
     const Params& ip = root.sub("Integrator");
     std::string itype = ip.get_string("type");
-    if (itype != "Path")
-        throw unsupported("integrator \"" + itype + "\" (the GPU path is pine's Path integrator)");
+    if (itype == "Path") {
+        S.integrator = kIntPath;
+    } else if (itype == "AO") {
+        S.integrator = kIntAO;
+    } else if (itype == "RandomWalk") {
+        S.integrator = kIntRandomWalk;
+    } else if (itype == "Viz") {
+        S.integrator = kIntViz;
+        const std::string vt = ip.get_string("viztype", "");
+        if (vt == "Position") {
+            S.dev.viz_type = 0;
+        } else if (vt == "Bvh" || vt == "Texcoord" || vt == "DpDu" || vt == "DpDv") {
+            throw unsupported("Viz type \"" + vt + "\" (Position and Normal are supported)");
+        } else {
+            if (vt != "Normal")
+                fprintf(stderr, "Unknown viztype %s\n", vt.c_str());  // viz.cpp:18-21
+            S.dev.viz_type = 1;
+        }
+    } else {
+        throw unsupported("integrator \"" + itype + "\" (Path, AO, Viz and RandomWalk are supported)");
+    }
     S.max_depth = ip.get_int("maxDepth", 4);
     const Params& sp = ip.sub("sampler");
     std::string stype = sp.get_string("type");
@@ -2101,6 +2258,8 @@ void build_scene(pg_scene& S) {
     }
     if (uses_media)
         S.general = true;
+    if (uses_media && S.integrator == kIntRandomWalk)
+        throw unsupported("RandomWalk with participating media");
     const Params& fp = cp.sub("film");
     int size[2];
     const int defsize[2] = {720, 480};
@@ -2705,7 +2864,9 @@ static int render_enqueue(pg_scene* S, int spp, int spb_req, hipStream_t st, Gui
         PG_HIP(hipMemsetAsync(P.lookups, 0, sizeof(unsigned long long), st));
     // Fused pipeline without training: the bounce loop is a graph replay per
     // batch (no host round trip per bounce).
-    const bool graph = S->use_graph && !S->general && !tr && !S->spectral;
+    const bool graph = S->use_graph && !S->general && !tr && !S->spectral && S->integrator == kIntPath;
+    if (S->integrator != kIntPath && (S->spectral || S->dev.guided || tr))
+        return pg_fail(PG_E_ARG, "spectral and guided rendering need the Path integrator");
     if (graph) {
         if (int rc = ensure_graph(S, P, n_paths))
             return rc;
@@ -2733,7 +2894,30 @@ static int render_enqueue(pg_scene* S, int spp, int spb_req, hipStream_t st, Gui
             PG_HIP(hipMemsetAsync(Q.counters, 0, 4 * sizeof(int32_t), st));
             if (int rc = pg_trace_closest(S->accel, Q.rays_in, active, Q.hits, Q.uvs, st))
                 return rc;
-            if (S->spectral) {
+            if (S->integrator == kIntAO) {
+                hipLaunchKernelGGL(k_shade_ao, dim3((active + B - 1) / B), dim3(B), 0, st, S->dev, Q, active);
+                int32_t counts[2];
+                PG_HIP(hipMemcpyAsync(counts, Q.counters, sizeof(counts), hipMemcpyDeviceToHost, st));
+                PG_HIP(hipStreamSynchronize(st));
+                rays += active + counts[1];
+                if (counts[1] > 0) {
+                    if (int rc = pg_trace_any(S->accel, Q.shadow_rays, counts[1], Q.occluded, st))
+                        return rc;
+                    hipLaunchKernelGGL(k_resolve_ao, dim3((counts[1] + B - 1) / B), dim3(B), 0, st, Q, counts[1]);
+                }
+                active = 0;
+            } else if (S->integrator == kIntViz) {
+                hipLaunchKernelGGL(k_shade_viz, dim3((active + B - 1) / B), dim3(B), 0, st, S->dev, Q, active);
+                rays += active;
+                active = 0;
+            } else if (S->integrator == kIntRandomWalk) {
+                hipLaunchKernelGGL(k_shade_rw, dim3((active + B - 1) / B), dim3(B), 0, st, S->dev, Q, active);
+                int32_t next = 0;
+                PG_HIP(hipMemcpyAsync(&next, Q.counters, sizeof(next), hipMemcpyDeviceToHost, st));
+                PG_HIP(hipStreamSynchronize(st));
+                rays += active;
+                active = next;
+            } else if (S->spectral) {
                 hipLaunchKernelGGL(k_shade_spec, dim3((active + B - 1) / B), dim3(B), 0, st, S->dev, Q, active);
                 int32_t counts[2];
                 PG_HIP(hipMemcpyAsync(counts, Q.counters, sizeof(counts), hipMemcpyDeviceToHost, st));

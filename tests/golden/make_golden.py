@@ -246,6 +246,35 @@ def case_render_box():
     print("render_box:", line, flush=True)
 
 
+def with_integrator(text, kind, extra=""):
+    """The scene text with its Path integrator replaced by pine's `kind`."""
+    assert text.startswith("Integrator: Path{")
+    return text.replace("Integrator: Path{", f"Integrator: {kind}{{" + extra, 1)
+
+
+def case_other_integrators():
+    """pine's AO and Viz (Normal, Position) on the mixed scene and RandomWalk on
+    the spheres scene."""
+    v, i = mixed_mesh()
+    with tempfile.TemporaryDirectory() as d:
+        W.write_obj(os.path.join(d, "blob.obj"), v, i)
+        mixed = W.mixed_scene_text("blob.obj", spp=64)
+        for name, text in (("render_ao", with_integrator(mixed, "AO")),
+                           ("render_viz_normal", with_integrator(mixed, "Viz", "\n    viztype: Normal")),
+                           ("render_viz_position", with_integrator(mixed, "Viz", "\n    viztype: Position"))):
+            film, line = run_render(text, d)
+            np.savez_compressed(os.path.join(OUT, name + ".npz"), scene=np.bytes_(text), film=film,
+                                harness=np.bytes_(line))
+            print(f"{name}:", line, flush=True)
+    text = with_integrator(W.spheres_scene_text(seed=0, width=96, height=96, spp=256, max_depth=8),
+                           "RandomWalk")
+    with tempfile.TemporaryDirectory() as d:
+        film, line = run_render(text, d)
+    np.savez_compressed(os.path.join(OUT, "render_randomwalk.npz"), scene=np.bytes_(text), film=film,
+                        harness=np.bytes_(line))
+    print("render_randomwalk:", line, flush=True)
+
+
 CLOUD_GRID_SMALL = dict(n=32, base_freq=2.0, threshold=0.3, scale=40.0, seed=3)
 
 
@@ -291,6 +320,7 @@ CASES = {
     "render_mixed": case_render_mixed,
     "render_cloud": case_render_cloud,
     "render_box": case_render_box,
+    "other_integrators": case_other_integrators,
     "slab_through": lambda: case_slab("through"),
     "slab_shadow": lambda: case_slab("shadow"),
 }

@@ -118,3 +118,26 @@ Camera: ThinLen{
 """
     with pytest.raises(pg.PineError, match="integrator"):
         pg.Scene(text=text)
+
+
+@pytest.mark.parametrize("name", ["render_ao", "render_viz_normal", "render_viz_position"])
+def test_ao_and_viz_match_pine(pg, cuda, tmp_path, name):
+    """pine's AO and Viz integrators on the mixed scene (same scene file, same
+    Halton sequence): the images must match pine's films."""
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    import make_golden as mg
+    from pine_amd import workloads as W
+    fx = golden(name)
+    v, i = mg.mixed_mesh()
+    W.write_obj(str(tmp_path / "blob.obj"), v, i)
+    film, info = render_text(pg, fx["scene"].item().decode(), str(tmp_path))
+    e = check_film(film, fx["film"], 1e-3)
+    print(f"{name} relMSE {e:.3e}")
+
+
+def test_randomwalk_matches_pine(pg, cuda, tmp_path):
+    fx = golden("render_randomwalk")
+    film, info = render_text(pg, fx["scene"].item().decode(), str(tmp_path))
+    e = check_film(film, fx["film"], 1e-3)
+    print(f"randomwalk relMSE {e:.3e}")

@@ -170,10 +170,13 @@ typedef struct pg_scene pg_scene;
 /* Loads a pine scene file: the scene-file grammar of parser.cpp:45-110 with
  * pine's number parsing, the object order of LoadScene (fileio.cpp:251-281) and
  * the Create* factories, then builds the accel (opts may be NULL).  Supported
- * on the GPU: Integrator Path + Halton sampler, ThinLen camera, LanczosSinc/Box
- * filters, Layered (Diffuse/Conductor/Dielectric) and Emissive materials with
- * constant inputs, Sphere/Rect/Plane/Triangle/TriangleMesh shapes, Point,
- * Directional and area lights.  Anything else fails with PG_E_PARSE naming it. */
+ * on the GPU: integrators Path, AO, Viz (Position, Normal) and RandomWalk
+ * (without media) (CreateIntegrator, integrator.cpp:20-37) with the Halton
+ * sampler, ThinLen camera, LanczosSinc/Box filters, Layered
+ * (Diffuse/Conductor/Dielectric) and Emissive materials with constant inputs,
+ * Sphere/Rect/Plane/Triangle/TriangleMesh shapes, Point, Directional, area and
+ * Sky lights, Grid/Homogeneous media.  Anything else fails with PG_E_PARSE
+ * naming it. */
 int pg_scene_load(const char* path, const pg_build_options* opts, pg_scene** out);
 /* Same from scene text; `dir` resolves relative file names (may be NULL). */
 int pg_scene_load_text(const char* text, const char* dir, const pg_build_options* opts,

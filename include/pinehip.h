@@ -172,7 +172,7 @@ typedef struct pg_scene pg_scene;
  * the Create* factories, then builds the accel (opts may be NULL).  Supported
  * on the GPU: integrators Path, AO, Viz (Position, Normal) and RandomWalk
  * (without media) (CreateIntegrator, integrator.cpp:20-37) with the Halton
- * sampler, ThinLen camera, LanczosSinc/Box filters, Layered
+ * or Sobol sampler (CreateSampler, sampler.cpp:280-294), ThinLen camera, LanczosSinc/Box filters, Layered
  * (Diffuse/Conductor/Dielectric) and Emissive materials with constant inputs,
  * Sphere/Rect/Plane/Triangle/TriangleMesh shapes, Point, Directional, area and
  * Sky lights, Grid/Homogeneous media.  Anything else fails with PG_E_PARSE
@@ -226,7 +226,7 @@ int pg_spectral_tables(float* Q, float* M, float* w_scale);
 
 /* Renders the whole film: PixelIntegrator::Render + RadianceIntegrator::Compute
  * + PathIntegrator::Li (integrator.cpp:121-158, path.cpp:6-87) for every pixel
- * and sample, pine's Halton sample sequence per pixel.  Synchronous. */
+ * and sample, pine's Halton or Sobol sample sequence per pixel.  Synchronous. */
 int pg_render(pg_scene* s, const pg_render_options* opts, void* stream);
 
 /* Renders n scenes (each its own film; spp[i], or the scene's samplesPerPixel
@@ -272,6 +272,15 @@ int pg_scene_autotune(pg_scene* s, const pg_tune_options* o, void* stream, pg_tu
  * sum w) in pine's pixel order (film.h:60-62: row 0 is the top of the image).
  * Linear radiance = rgb / w, as Film::CopyToRGBArray divides (film.cpp:72-89). */
 int pg_scene_film(const pg_scene* s, float* h_rgbw);
+
+/* The scene's sampler evaluated on the device, for parity tests: for npix
+ * pixels spread as (py0 * width + px0 + 977 k) (wrapping over the film),
+ * samples 0..spp-1 each start at the pixel and draw ndims values.  Replaces
+ * Sampler::StartPixel / StartNextSample / Get1D (sampler.h:250-300) of pine's
+ * HaltonSampler (sampler.cpp:32-96) and SobolSampler (sampler.cpp:129-220).
+ * h_out: npix * spp * ndims floats, [pixel][sample][dimension].  Synchronous. */
+int pg_scene_sampler_values(const pg_scene* s, int32_t px0, int32_t py0, int32_t npix, int32_t spp,
+                            int32_t ndims, float* h_out);
 
 /* The film's outputFileName (film.cpp:9), NUL-terminated, truncated to len. */
 int pg_scene_output_name(const pg_scene* s, char* buf, int32_t len);

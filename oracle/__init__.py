@@ -53,6 +53,7 @@ def lib():
         L.po_bvh_trace.argtypes = [vp, vp, i64, vp, vp, ctypes.c_int]
         L.po_bvh_trace.restype = ctypes.c_double
         L.po_halton_values.argtypes = [ctypes.c_int] * 7 + [vp]
+        L.po_sobol_values.argtypes = [ctypes.c_int] * 8 + [vp]
         _lib = L
     return _lib
 
@@ -91,6 +92,14 @@ def halton_values(w, h, spp, ndims, px0, py0, npix):
     """pine's HaltonSampler values, layout [npix, spp, ndims] (see pinecpu.cpp)."""
     out = np.zeros(npix * spp * ndims, np.float32)
     lib().po_halton_values(w, h, spp, ndims, px0, py0, npix, _p(out))
+    return out.reshape(npix, spp, ndims)
+
+
+def sobol_values(w, h, spp, ndims, px0, py0, npix, strategy):
+    """pine's SobolSampler values (strategy 0 none, 1 binaryPermutate, 2 fastOwen,
+    3 Owen), layout [npix, spp, ndims] (see pinecpu.cpp)."""
+    out = np.zeros(npix * spp * ndims, np.float32)
+    lib().po_sobol_values(w, h, spp, ndims, px0, py0, npix, strategy, _p(out))
     return out.reshape(npix, spp, ndims)
 
 

@@ -10,6 +10,11 @@
 //                            rays, plus closed forms (sphere/plane distances)
 //   - po_bvh_*             : same answers as po_trace_brute (any valid tree gives
 //                            the same closest t)
+//   - po_halton_values     : golden vectors of pine's HaltonSampler (harness)
+//   - po_sobol_values      : golden vectors of pine's SobolSampler (harness), all
+//                            four scramblers; the per-pixel index is found by
+//                            search (closed form: its first two Sobol dimensions
+//                            fall in the pixel), not from pine's index tables
 //
 // Built by oracle/Makefile into oracle/libpinecpu.so with g++ -O2
 // -ffp-contract=off (no fast-math): x86-64 has no FMA by default, so each
@@ -22,6 +27,9 @@
 #include <cstring>
 #include <thread>
 #include <vector>
+
+// Joe & Kuo direction numbers (data only): {s, a, m_1..m_s} for dimensions 1..1023.
+#include "../pine_amd/csrc/pg_sobol_dirnums.h"
 
 namespace {
 
@@ -701,7 +709,148 @@ bool BoxHit(const float* lo, const float* hi, const V3& inv, const V3& nod, cons
 
 }  // namespace
 
+// ---- Sobol sampler (sampler.cpp:129-220, lowdiscrepancy.h:128-158) ---------------
+// Generator matrices: column k of dimension d holds direction number v_{k+1}
+// as a 32-bit fraction; dimension 0 is van der Corput, the others follow the
+// Bratley-Fox recurrence v_k = v_{k-s} ^ (v_{k-s} >> s) ^ XOR_j a_j v_{k-j}.
+struct SobolTables {
+    std::vector<uint32_t> C;  // [1024][32]
+    SobolTables() : C(1024 * 32) {
+        for (int k = 0; k < 32; k++)
+            C[k] = 1u << (31 - k);
+        const uint16_t* q = kSobolDirnums;
+        for (int d = 1; d < 1024; d++) {
+            const int s = *q++, a = *q++;
+            uint32_t* c = &C[d * 32];
+            for (int k = 1; k <= 32; k++) {
+                if (k <= s) {
+                    c[k - 1] = (uint32_t)q[k - 1] << (32 - k);
+                } else {
+                    uint32_t v = c[k - s - 1] ^ (c[k - s - 1] >> s);
+                    for (int j = 1; j < s; j++)
+                        if ((a >> (s - 1 - j)) & 1)
+                            v ^= c[k - j - 1];
+                    c[k - 1] = v;
+                }
+            }
+            q += s;
+        }
+    }
+    // SobolSample before scrambling: the matrix-vector product over GF(2).
+    uint32_t Raw(int dim, uint64_t a) const {
+        uint32_t v = 0;
+        for (int i = 0; a && i < 32; i++, a >>= 1)
+            if (a & 1)
+                v ^= C[dim * 32 + i];
+        return v;
+    }
+};
+
+uint32_t ReverseBits32(uint32_t x) {  // math.h:19-26
+    uint32_t r = 0;
+    for (int i = 0; i < 32; i++)
+        r |= ((x >> i) & 1u) << (31 - i);
+    return r;
+}
+uint64_t Hash64u(uint64_t s) {  // rng.h:10-15
+    s += 0x9E3779B97f4A7C15ULL;
+    s = (s ^ (s >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    s = (s ^ (s >> 27)) * 0x94D049BB133111EBULL;
+    return s ^ (s >> 31);
+}
+uint64_t MixBits(uint64_t v) {  // rng.h:43-50
+    v ^= (v >> 31);
+    v *= 0x7fb5d329728ea185ULL;
+    v ^= (v >> 27);
+    v *= 0x81dadef4bc2dd44dULL;
+    v ^= (v >> 33);
+    return v;
+}
+
+struct Sobol {
+    const SobolTables& T;
+    int m = 0, strategy, dim = 2;
+    uint32_t px = 0, py = 0;
+    uint64_t index = 0;
+    Sobol(const SobolTables& T, int w, int h, int strategy) : T(T), strategy(strategy) {
+        int scale = 1;  // roundup2(max(w, h)), log2int
+        while (scale < std::max(w, h))
+            scale <<= 1;
+        while ((1 << m) < scale)
+            m++;
+    }
+    // SobolIntervalToIndex: the index frame * 4^m + j whose dimension 0 and 1
+    // values, scaled by 2^m, floor to (px, py).  Dimension 0 is the bit reversal
+    // of the index, so the low m bits of j are px reversed; the high m bits are
+    // searched.  Pine keeps the index in 32 bits.
+    void Start(uint64_t frame) {
+        dim = 2;
+        if (m == 0) {
+            index = frame;
+            return;
+        }
+        uint64_t lo = 0;
+        for (int b = 0; b < m; b++)
+            if ((px >> (m - 1 - b)) & 1)
+                lo |= 1ull << b;
+        for (uint64_t hi = 0; hi < (1ull << m); hi++) {
+            const uint64_t i = (frame << (2 * m)) | (hi << m) | lo;
+            if ((T.Raw(0, i) >> (32 - m)) == px && (T.Raw(1, i) >> (32 - m)) == py) {
+                index = (uint32_t)i;
+                return;
+            }
+        }
+        index = ~0ull;  // unreachable: every cell holds exactly one index
+    }
+    float Sample(int d) const {  // SampleDimension (sampler.cpp:210-220)
+        uint32_t v = T.Raw(d, index);
+        const uint32_t seed = (uint32_t)Hash64u((uint64_t)(uint32_t)d);  // Hash(int)
+        if (strategy == 1) {
+            v ^= seed;
+        } else if (strategy == 2) {  // FastOwenScramber (sampler.cpp:173-188)
+            v = ReverseBits32(v);
+            v ^= v * 0x3d20adeau;
+            v += seed;
+            v *= (seed >> 16) | 1;
+            v ^= v * 0x05526c56u;
+            v ^= v * 0x53a22864u;
+            v = ReverseBits32(v);
+        } else if (strategy == 3) {  // OwenScramber (sampler.cpp:190-208)
+            if (seed & 1)
+                v ^= 1u << 31;
+            for (int b = 1; b < 32; b++) {
+                uint32_t mask = (~0u) << (32 - b);
+                if ((uint32_t)MixBits((v & mask) ^ seed) & (1u << b))
+                    v ^= 1u << (31 - b);
+            }
+        }
+        return std::min(v * 0x1p-32f, 0x1.fffffep-1f);
+    }
+    float Get1D() {  // sampler.cpp:147-151
+        if (dim >= 1024)
+            dim = 2;
+        return Sample(dim++);
+    }
+};
+
 extern "C" {
+
+// Sobol values in the harness's layout (as po_halton_values).
+void po_sobol_values(int w, int h, int spp, int ndims, int px0, int py0, int npix, int strategy,
+                     float* out) {
+    static const SobolTables T;
+    Sobol s(T, w, h, strategy);
+    for (int k = 0; k < npix; k++) {
+        int idx = py0 * w + px0 + k * 977;
+        s.px = (uint32_t)(idx % w);
+        s.py = (uint32_t)((idx / w) % h);
+        for (int si = 0; si < spp; si++) {
+            s.Start((uint64_t)si);
+            for (int d = 0; d < ndims; d++)
+                *out++ = s.Get1D();
+        }
+    }
+}
 
 // Halton values in the harness's layout: npix pixels spread as k*977 from
 // (px0, py0), spp samples each, ndims successive Get1D values per sample.

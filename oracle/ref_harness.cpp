@@ -18,7 +18,17 @@
 //       pstd libm outputs (pstd/math.h:141-351) on a fixed input grid.
 //   halton <out.bin> <filmW> <filmH> <spp> <ndims> <px0> <py0> <npix>
 //       HaltonSampler values (sampler.cpp:32-96) for a run of pixels.
-//   render <scene.txt> <out.bin> [threads]
+//   sobol  <out.bin> <filmW> <filmH> <spp> <ndims> <px0> <py0> <npix> <strategy>
+//       SobolSampler values (sampler.cpp:129-220) for a run of pixels; strategy
+//       0 none, 1 binaryPermutate, 2 fastOwen, 3 Owen.
+//   atmosphere <out.bin> <sx> <sy> <sz> <r> <g> <b> <sizeX> <sizeY>
+//       The Atmosphere environment light's colour table and sun colour as its
+//       constructor computes them (light.cpp:81-92): sizeX*sizeY*3 floats, then 3.
+//   render <scene.txt> <out.bin> [threads] [sample0] [atmosphere.bin]
+//       With atmosphere.bin (sizeX*sizeY*3 floats, then 3), every copy pine
+//       holds of the scene's Atmosphere light (scene->envLight, scene->lights,
+//       the integrator's light sampler) gets that colour table and sun colour
+//       in place of its own before rendering.
 //       Loads a scene with pine's own LoadScene (fileio.cpp:251-281) and renders
 //       it with the scene's integrator (PathIntegrator::Li, path.cpp:6-87) in
 //       pine's per-pixel sample order (integrator.cpp:121-158), accumulating
@@ -441,6 +451,55 @@ int ModeHalton(int argc, char** argv) {
     return 0;
 }
 
+int ModeSobol(int argc, char** argv) {
+    if (argc < 11)
+        return 2;
+    vec2i film(atoi(argv[3]), atoi(argv[4]));
+    int spp = atoi(argv[5]), ndims = atoi(argv[6]);
+    int px0 = atoi(argv[7]), py0 = atoi(argv[8]), npix = atoi(argv[9]);
+    const SobolSampler::RandomizeStrategy strategies[4] = {
+        SobolSampler::RandomizeStrategy::None, SobolSampler::RandomizeStrategy::BinaryPermutate,
+        SobolSampler::RandomizeStrategy::FastOwen, SobolSampler::RandomizeStrategy::Owen};
+    SobolSampler s(spp, film, strategies[atoi(argv[10]) & 3]);
+    std::vector<float> out;
+    for (int k = 0; k < npix; k++) {
+        int idx = py0 * film.x + px0 + k * 977;  // a spread of pixels
+        vec2i p(idx % film.x, (idx / film.x) % film.y);
+        s.StartPixel(p, 0);
+        for (int si = 0; si < spp; si++) {
+            for (int d = 0; d < ndims; d++)
+                out.push_back(s.Get1D());
+            s.StartNextSample();
+        }
+    }
+    FILE* f = fopen(argv[2], "wb");
+    fwrite(out.data(), sizeof(float), out.size(), f);
+    fclose(f);
+    printf("{\"floats\": %zu, \"log2scale\": %d}\n", out.size(), (int)s.log2Scale);
+    return 0;
+}
+
+int ModeAtmosphere(int argc, char** argv) {
+    if (argc < 11)
+        return 2;
+    vec3 sun(atof(argv[3]), atof(argv[4]), atof(argv[5]));
+    vec3 col(atof(argv[6]), atof(argv[7]), atof(argv[8]));
+    vec2i size(atoi(argv[9]), atoi(argv[10]));
+    Atmosphere a(sun, Spectrum(col), size, true);
+    std::vector<float> out;
+    for (const vec3& c : a.colors)
+        for (int j = 0; j < 3; j++)
+            out.push_back(c[j]);
+    vec3 sc = a.sunSampledColor.ToRGB();
+    for (int j = 0; j < 3; j++)
+        out.push_back(sc[j]);
+    FILE* f = fopen(argv[2], "wb");
+    fwrite(out.data(), sizeof(float), out.size(), f);
+    fclose(f);
+    printf("{\"floats\": %zu}\n", out.size());
+    return 0;
+}
+
 int ModeRender(int argc, char** argv) {
     if (argc < 4)
         return 2;
@@ -453,6 +512,38 @@ int ModeRender(int argc, char** argv) {
     if (!ri) {
         fprintf(stderr, "integrator is not a RadianceIntegrator\n");
         return 3;
+    }
+    if (argc > 6) {
+        std::vector<float> tab;
+        FILE* tf = fopen(argv[6], "rb");
+        if (!tf)
+            return 4;
+        float v;
+        while (fread(&v, 4, 1, tf) == 1)
+            tab.push_back(v);
+        fclose(tf);
+        auto patch = [&](Atmosphere& a) {
+            if (tab.size() != a.colors.size() * 3 + 3) {
+                fprintf(stderr, "atmosphere table size mismatch\n");
+                exit(5);
+            }
+            for (size_t i = 0; i < a.colors.size(); i++)
+                a.colors[i] = vec3(tab[3 * i], tab[3 * i + 1], tab[3 * i + 2]);
+            size_t n = a.colors.size() * 3;
+            a.sunSampledColor = Spectrum(vec3(tab[n], tab[n + 1], tab[n + 2]));
+        };
+        auto patch_lights = [&](pstd::vector<Light>& ls) {
+            for (auto& l : ls)
+                if (l.Is<EnvironmentLight>() && l.Be<EnvironmentLight>().Is<Atmosphere>())
+                    patch(l.Be<EnvironmentLight>().Be<Atmosphere>());
+        };
+        if (scene->envLight && scene->envLight->Is<Atmosphere>())
+            patch(scene->envLight->Be<Atmosphere>());
+        patch_lights(scene->lights);
+        if (ri->lightSampler.Is<PowerLightSampler>())
+            patch_lights(ri->lightSampler.Be<PowerLightSampler>().lights);
+        else
+            patch_lights(ri->lightSampler.Be<UniformLightSampler>().lights);
     }
     Film& film = scene->camera.GetFilm();
     vec2i size = film.Size();
@@ -531,7 +622,7 @@ int ModeRender(int argc, char** argv) {
 
 int main(int argc, char** argv) {
     if (argc < 2) {
-        fprintf(stderr, "usage: ref_harness trace|time|math|halton ...\n");
+        fprintf(stderr, "usage: ref_harness trace|time|math|halton|sobol|atmosphere|render ...\n");
         return 2;
     }
     std::string mode = argv[1];
@@ -543,6 +634,10 @@ int main(int argc, char** argv) {
         return ModeMath(argc, argv);
     if (mode == "halton")
         return ModeHalton(argc, argv);
+    if (mode == "sobol")
+        return ModeSobol(argc, argv);
+    if (mode == "atmosphere")
+        return ModeAtmosphere(argc, argv);
     if (mode == "render")
         return ModeRender(argc, argv);
     fprintf(stderr, "unknown mode\n");

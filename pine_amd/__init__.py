@@ -32,7 +32,7 @@ EXPORTED = [
     "pg_guide_forward", "pg_guide_pdf", "pg_guide_sample", "pg_guide_gradient", "pg_guide_train",
     "pg_guide_release", "pg_render_guided", "pg_trace_closest_dev", "pg_trace_occluded_dev",
     "pg_accel_rebuild", "pg_render_batch", "pg_scene_autotune", "pg_write_obj",
-    "pg_spectral_tables",
+    "pg_spectral_tables", "pg_scene_sampler_values",
 ]
 RENDER_NO_GRAPH = 1
 RENDER_SPECTRAL = 2
@@ -137,6 +137,7 @@ def lib():
     L.pg_scene_get_info.argtypes = [vp, ctypes.POINTER(_SceneInfo)]
     L.pg_render.argtypes = [vp, ctypes.POINTER(_RenderOptions), vp]
     L.pg_scene_film.argtypes = [vp, vp]
+    L.pg_scene_sampler_values.argtypes = [vp, i32, i32, i32, i32, i32, vp]
     L.pg_scene_output_name.argtypes = [vp, ctypes.c_char_p, i32]
     L.pg_scene_release.argtypes = [vp]
     L.pg_accel_get_stats.argtypes = [vp, ctypes.POINTER(_Stats)]
@@ -397,6 +398,12 @@ class Scene:
         inf = self.info()
         out = np.zeros((inf["height"], inf["width"], 4), np.float32)
         check(lib().pg_scene_film(self.h, _ptr(out)))
+        return out
+
+    def sampler_values(self, px0, py0, npix, spp, ndims):
+        """The scene's sampler on the device, [npix, spp, ndims] (pg_scene_sampler_values)."""
+        out = np.zeros((npix, spp, ndims), np.float32)
+        check(lib().pg_scene_sampler_values(self.h, px0, py0, npix, spp, ndims, _ptr(out)))
         return out
 
     def image(self):

@@ -240,7 +240,7 @@ struct MediumEvent {
 
 // Medium::Sample (medium.h:70-73) for a ray of the path; `maj` holds the brick
 // majorants of grid media in brick mode (LDS copy or global).
-PG_HD MediumEvent medium_sample(const MediumDev& m, const float* maj, const HaltonDev& hs,
+PG_HD MediumEvent medium_sample(const MediumDev& m, const float* maj, const SamplerDev& hs,
                                 SamplerState& ss, f3 o, f3 d, float tmax, int& lookups) {
     MediumEvent ev;
     ev.scattered = false;
@@ -340,7 +340,7 @@ PG_HD MediumEvent medium_sample(const MediumDev& m, const float* maj, const Halt
 }
 
 // Medium::Tr (medium.h:66-69): transmittance over [0, tmax] of the ray.
-PG_HD f3 medium_tr(const MediumDev& m, const float* maj, const HaltonDev& hs, SamplerState& ss,
+PG_HD f3 medium_tr(const MediumDev& m, const float* maj, const SamplerDev& hs, SamplerState& ss,
                    f3 o, f3 d, float tmax, int& lookups) {
     if (m.type == kMediumHomogeneous) {
         // HomogeneousMedium::Tr (medium.cpp:33-35): Exp(-ray.tmax * sigma_t)

@@ -1,4 +1,4 @@
-// Shading library of the path tracer, host+device: pine's Halton sampler,
+// Shading library of the path tracer, host+device: pine's Halton and Sobol samplers,
 // camera, layered BSDFs, lights and Path-integrator helpers, with pine's
 // evaluation order.  The rules pine's -ffast-math build applies and that this
 // file follows (each was seen bit for bit in pine's golden vectors):
@@ -8,6 +8,7 @@
 #pragma once
 
 #include "pg_pstd.h"
+#include "pg_sobol.h"
 
 namespace pg {
 
@@ -125,23 +126,58 @@ PG_HD float scrambled_radical_inverse(const HaltonDev& h, int dim, uint64_t a64)
     return v < kOneMinusEps ? v : kOneMinusEps;
 }
 
-// Per-path sampler state: Halton index and next dimension.
+// ---- the scene's sampler: Halton or Sobol (sampler.h:250-255) -------------------
+enum SamplerKind : int32_t { kSamplerHalton = 0, kSamplerSobol = 1 };
+struct SamplerDev {
+    int32_t kind;
+    HaltonDev halton;
+    SobolDev sobol;
+};
+
+// Per-path sampler state: the sample's index (Halton index, or the 32-bit Sobol
+// index) and the next dimension.
 struct SamplerState {
     uint64_t index;
     int dim;
 };
-PG_HD float get_1d(const HaltonDev& h, SamplerState& s) {  // sampler.cpp:72-76
-    if (s.dim >= 1000)
-        s.dim = 2;
-    return scrambled_radical_inverse(h, s.dim++, s.index);
+
+// StartPixel(p, sampleIndex): HaltonSampler::StartPixel (sampler.cpp:57-70) or
+// SobolSampler::StartPixel (sampler.cpp:136-141); both restart at dimension 2.
+PG_HD uint64_t sampler_start(const SamplerDev& h, int px, int py, int sampleIndex) {
+    if (h.kind == kSamplerSobol)
+        return sobol_interval_to_index(h.sobol, (uint32_t)sampleIndex, (uint32_t)px, (uint32_t)py);
+    return halton_index(h.halton, px, py, sampleIndex);
 }
-PG_HD void get_2d(const HaltonDev& h, SamplerState& s, float& u0, float& u1) {  // :77-83
-    if (s.dim + 1 >= 1000)
+
+PG_HD float get_1d(const SamplerDev& h, SamplerState& s) {
+    if (h.kind == kSamplerSobol) {  // sampler.cpp:147-151
+        if (s.dim >= kSobolDimensions)
+            s.dim = 2;
+        return sobol_sample(h.sobol, s.dim++, (uint32_t)s.index);
+    }
+    if (s.dim >= 1000)  // sampler.cpp:72-76
+        s.dim = 2;
+    return scrambled_radical_inverse(h.halton, s.dim++, s.index);
+}
+PG_HD void get_2d(const SamplerDev& h, SamplerState& s, float& u0, float& u1) {
+    if (h.kind == kSamplerSobol) {
+        // sampler.cpp:152-158 wraps only when the first dimension is past the
+        // table, so a pair starting at 1023 reads one column block past pine's
+        // table (undefined there); this reads dimension 2 for the second value.
+        if (s.dim >= kSobolDimensions)
+            s.dim = 2;
+        const int d = s.dim;
+        s.dim += 2;
+        u0 = sobol_sample(h.sobol, d, (uint32_t)s.index);
+        u1 = sobol_sample(h.sobol, d + 1 < kSobolDimensions ? d + 1 : 2, (uint32_t)s.index);
+        return;
+    }
+    if (s.dim + 1 >= 1000)  // sampler.cpp:77-83
         s.dim = 2;
     int d = s.dim;
     s.dim += 2;
-    u0 = scrambled_radical_inverse(h, d, s.index);
-    u1 = scrambled_radical_inverse(h, d + 1, s.index);
+    u0 = scrambled_radical_inverse(h.halton, d, s.index);
+    u1 = scrambled_radical_inverse(h.halton, d + 1, s.index);
 }
 
 // ---- sampling.h ----------------------------------------------------------------

@@ -21,6 +21,7 @@
 #include "pg_medium.h"
 #include "pg_spectral.h"
 #include "pg_shade.h"
+#include "pg_sobol_dirnums.h"
 #include "scene_parse.h"
 
 #include <hip/hip_runtime.h>
@@ -58,7 +59,10 @@ struct ShapeDev {
     int32_t mi_inside, mi_outside;  // MediumInterface (geometry.cpp:392-409), -1 = none
 };
 
-enum LightType : int32_t { kLightArea = 0, kLightPoint = 1, kLightDirectional = 2, kLightSky = 3 };
+enum LightType : int32_t { kLightArea = 0, kLightPoint = 1, kLightDirectional = 2, kLightSky = 3, kLightAtmosphere = 4 };
+enum EnvKind : int32_t { kEnvSky = 0, kEnvAtmosphere = 1 };
+// VizIntegrator::Type (viz.h:14)
+enum VizType : int32_t { kVizPosition = 0, kVizNormal = 1, kVizTexcoord = 2, kVizDpDu = 3, kVizDpDv = 4, kVizBvh = 5 };
 struct LightDev {
     int32_t type;
     int32_t shape;
@@ -84,7 +88,7 @@ struct SceneDev {
     int32_t light_uniform;    // UniformLightSampler
     float light_func_int;
     CameraDev cam;
-    HaltonDev halton;
+    SamplerDev sampler;
     float filter_table[256];
     float filter_radius[2];
     int32_t width, height;
@@ -93,13 +97,19 @@ struct SceneDev {
     const MediumDev* media;
     int32_t n_media;
     int32_t camera_medium;  // ThinLenCamera medium (camera.cpp:98-105), -1 = none
-    int32_t has_env;        // a Sky environment light is present
-    f3 sky_dir, sky_color;  // Sky sunDirection (normalised) and sunColor
+    int32_t has_env;        // an environment light is present (scene->envLight)
+    int32_t env_kind;       // EnvKind
+    f3 sky_dir, sky_color;  // sunDirection (normalised) and sunColor
+    // Atmosphere (light.cpp:81-139): colour table [size.y][size.x] of RGB, the
+    // colour along the sun, bilinear lookups when `interpolate`.
+    const float* atmo_colors;
+    int32_t atmo_w, atmo_h, atmo_interp;
+    f3 atmo_sun_le;
     // Learned path guiding (general mode only).
     int32_t guided;
     int32_t g_depth;  // vertices of depth < g_depth sample from the guide
     spec::Tables spec;  // spectral mode: RGB <-> spectrum tables
-    int32_t viz_type;   // Viz integrator: 0 Position, 1 Normal
+    int32_t viz_type;   // Viz integrator: VizType
     const float* g_params;  // FP32 parameters (biases used in FP32)
     const __bf16* g_wf;     // BF16 forward weights
     f3 g_bmin, g_binv;      // scene bounds: network position input = (p - bmin) * binv
@@ -111,8 +121,8 @@ struct PathsDev {
     float* L;       // 3 per path
     float* beta;    // 3 per path
     float* bsdf_pdf;
-    uint64_t* sidx; // Halton index
-    int32_t* sdim;  // next Halton dimension
+    uint64_t* sidx; // sampler index (Halton or Sobol)
+    int32_t* sdim;  // next sampler dimension
     int32_t* depth;
     float* pfilm;   // 2 per path
     // queues
@@ -220,14 +230,14 @@ __global__ __launch_bounds__(256) void k_raygen(SceneDev s, PathsDev P, int32_t 
     const int32_t pixel = i / spb, sl = i - pixel * spb;
     const int px = pixel % s.width, py = pixel / s.width;
     SamplerState ss;
-    ss.index = halton_index(s.halton, px, py, sample0 + sl);
+    ss.index = sampler_start(s.sampler, px, py, sample0 + sl);
     ss.dim = 2;
     float u0, u1, l0, l1;
-    get_2d(s.halton, ss, u0, u1);
+    get_2d(s.sampler, ss, u0, u1);
     // (p + Get2D()) / film->Size()  (integrator.cpp:153)
     float fx = ((float)px + u0) / (float)s.width;
     float fy = ((float)py + u1) / (float)s.height;
-    get_2d(s.halton, ss, l0, l1);
+    get_2d(s.sampler, ss, l0, l1);
     f3 o, d;
     gen_ray(s.cam, fx, fy, l0, l1, o, d);
     P.L[3 * i + 0] = P.L[3 * i + 1] = P.L[3 * i + 2] = 0.0f;
@@ -270,6 +280,53 @@ __device__ f3 sky_color(const SceneDev& s, f3 d) {
     const f3 l = lo * (1.0f - theta) + hi * theta;  // pstd::lerp (math.h:96-98)
     return s.sky_color * (l * l);
 }
+
+// Atmosphere::Color (light.cpp:106-135): the direction's (phi / 2pi,
+// (1 - cos theta) / 2) scaled to the table, nearest texel or a bilinear blend
+// of the four around it (cast-based floor/ceil, indices reduced by size with
+// C's signed remainder, the blend weights taken against those indices).
+// Pine's approximate atan2 diverges for x -> 0+ with y < 0 (phi near -1e5
+// instead of 3pi/2), so its x index can be negative there.  Pine then reads
+// colors[y * size.x + x]: another row of its table, or memory outside it (its
+// nearest-texel mode crashes on such scenes).  The flat index is formed the
+// same way here and wrapped into the table, so every read pine makes inside
+// its table is the same texel, and none leaves the table.
+__device__ f3 atmo_texel(const SceneDev& s, int x, int y) {
+    const int64_t n = (int64_t)s.atmo_w * s.atmo_h;
+    int64_t flat = ((int64_t)y * s.atmo_w + x) % n;
+    flat = flat < 0 ? flat + n : flat;
+    const float* c = s.atmo_colors + 3 * flat;
+    return mk(c[0], c[1], c[2]);
+}
+__device__ f3 atmosphere_color(const SceneDev& s, f3 d) {
+    // InverseUniformSphereMampling (sampling.h:44-47) via CartesianToSpherical
+    // (vecmath.h:1243-1250).
+    float phi = pstd::atan2(d.y, d.x);
+    phi = phi < 0.0f ? kPi * 2.0f + phi : phi;
+    const float theta = pstd::acos(d.z);
+    const float ux = phi * (1.0f / (kPi * 2.0f)), uy = (1.0f - pstd::cos(theta)) * 0.5f;
+    const float fx = ux * (float)s.atmo_w, fy = uy * (float)s.atmo_h;
+    if (!s.atmo_interp) {
+        const float mx = (float)s.atmo_w - 1.0f, my = (float)s.atmo_h - 1.0f;
+        return atmo_texel(s, (int)(fx < mx ? fx : mx), (int)(fy < my ? fy : my));
+    }
+    const int x0 = (int)pstd::floor(fx) % s.atmo_w, x1 = (int)pstd::ceil(fx) % s.atmo_w;
+    const int y0 = (int)pstd::floor(fy) % s.atmo_h, y1 = (int)pstd::ceil(fy) % s.atmo_h;
+    const float px = fx - (float)x0, py = fy - (float)y0;
+    const f3 c00 = atmo_texel(s, x0, y0), c01 = atmo_texel(s, x1, y0);
+    const f3 c10 = atmo_texel(s, x0, y1), c11 = atmo_texel(s, x1, y1);
+    const f3 c0 = c00 * (1.0f - px) + c01 * px;  // pstd::lerp (math.h:96-98)
+    const f3 c1 = c10 * (1.0f - px) + c11 * px;
+    return c0 * (1.0f - py) + c1 * py;
+}
+
+// EnvironmentLight::Color and ::Pdf (light.h:150-160): Sky's pdf is 0,
+// Atmosphere's 1 / 4pi (its sampling pdf is half that; pine uses this value in
+// the BSDF-side MIS weight).
+__device__ f3 env_color(const SceneDev& s, f3 d) {
+    return s.env_kind == kEnvAtmosphere ? atmosphere_color(s, d) : sky_color(s, d);
+}
+__device__ float env_pdf(const SceneDev& s) { return s.env_kind == kEnvAtmosphere ? 1.0f / (kPi * 4.0f) : 0.0f; }
 
 struct Hit {
     f3 p, n;
@@ -382,6 +439,28 @@ __device__ LightSampleD sample_light_point(const SceneDev& s, const LightDev& L,
         ls.pdf = 1.0f;
         return ls;
     }
+    if (L.type == kLightAtmosphere) {
+        // Atmosphere::Sample (light.cpp:94-111): half the time a uniform sphere
+        // direction with the nearest table colour, else the sun as a delta light.
+        ls.distance = 1e10f;
+        if (u0 < 0.5f) {
+            u0 *= 2.0f;
+            const float phi = u0 * kPi * 2.0f, theta = pstd::acos(1.0f - 2.0f * u1);
+            const float st = pstd::sin(theta);
+            ls.wo = mk(st * pstd::cos(phi), st * pstd::sin(phi), pstd::cos(theta));
+            ls.pdf = 1.0f / (kPi * 4.0f);
+            const float fx = u0 * (float)s.atmo_w, fy = u1 * (float)s.atmo_h;
+            const float mx = (float)s.atmo_w - 1.0f, my = (float)s.atmo_h - 1.0f;
+            ls.Le = atmo_texel(s, (int)(fx < mx ? fx : mx), (int)(fy < my ? fy : my));
+        } else {
+            ls.wo = s.sky_dir;
+            ls.pdf = 1.0f;
+            ls.Le = s.atmo_sun_le;
+            ls.delta = true;
+        }
+        ls.pdf = ls.pdf * 0.5f;
+        return ls;
+    }
     if (L.type == kLightDirectional) {
         ls.Le = L.color;
         ls.wo = L.v;
@@ -453,10 +532,10 @@ __device__ bool guided_direction(const SceneDev& s, PathsDev& P, SamplerState& s
     const M3 w2n = inverse(n2w);
     const f3 wiL = mul(w2n, wi);
     const float alpha = gp[guide::kAlphaAt];
-    const float usel = get_1d(s.halton, ss);
-    const float u1 = get_1d(s.halton, ss);
+    const float usel = get_1d(s.sampler, ss);
+    const float u1 = get_1d(s.sampler, ss);
     float u2a, u2b;
-    get_2d(s.halton, ss, u2a, u2b);
+    get_2d(s.sampler, ss, u2a, u2b);
     f3 wo = mk(0.0f, 0.0f, 1.0f), f = mk(0.0f, 0.0f, 0.0f);
     float pb = 0.0f;
     if (usel < alpha) {
@@ -526,10 +605,9 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev s, PathsDev P, int32_t n
 
     if (hit.prim < 0) {
         if (s.has_env) {
-            // path.cpp:29-39; Sky::Pdf is 0, so the MIS weight is the power
-            // heuristic against a zero light pdf.
-            const f3 le = sky_color(s, rd);
-            L = depth == 0 ? L + beta * le : L + beta * le * power_heuristic(bsdfPdf, 0.0f);
+            // path.cpp:29-39: the BSDF-side MIS weight against the environment's Pdf.
+            const f3 le = env_color(s, rd);
+            L = depth == 0 ? L + beta * le : L + beta * le * power_heuristic(bsdfPdf, env_pdf(s));
         }
     } else {
         const pg_hit_uv uv = P.uvs[k];
@@ -561,11 +639,11 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev s, PathsDev P, int32_t n
                 const float* gp = P.gparams + guide::kParamStride * (int64_t)k;
                 // EstimateDirect (integrator.cpp:88-119)
                 float lpdf;
-                const float ul = get_1d(s.halton, ss);
+                const float ul = get_1d(s.sampler, ss);
                 float la, lb;
                 if (s.n_lights > 0) {
                     int li = sample_light(s, ul, lpdf);
-                    get_2d(s.halton, ss, la, lb);
+                    get_2d(s.sampler, ss, la, lb);
                     LightSampleD ls = sample_light_point(s, s.lights[li], h.p, la, lb);
                     ls.pdf *= lpdf;
                     const f3 woL = mul(w2n, ls.wo);
@@ -602,7 +680,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev s, PathsDev P, int32_t n
                         alive = true;
                         if (depth > 2) {
                             float q = pstd::clamp(1.0f - luminance(beta), 0.05f, 1.0f);
-                            if (get_1d(s.halton, ss) < q)
+                            if (get_1d(s.sampler, ss) < q)
                                 alive = false;
                             else
                                 beta = beta / (1.0f - q);
@@ -610,9 +688,9 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev s, PathsDev P, int32_t n
                     }
                 } else {
                     // BSDF sample (path.cpp:66-86)
-                    const float u1 = get_1d(s.halton, ss);
+                    const float u1 = get_1d(s.sampler, ss);
                     float u2a, u2b;
-                    get_2d(s.halton, ss, u2a, u2b);
+                    get_2d(s.sampler, ss, u2a, u2b);
                     BSDFSample bs;
                     if (material_sample(m, wiL, u1, u2a, u2b, bs)) {
                         bs.wo = mul(n2w, bs.wo);
@@ -624,7 +702,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev s, PathsDev P, int32_t n
                             alive = true;
                             if (depth > 2) {
                                 float q = pstd::clamp(1.0f - luminance(beta), 0.05f, 1.0f);
-                                if (get_1d(s.halton, ss) < q)
+                                if (get_1d(s.sampler, ss) < q)
                                     alive = false;
                                 else
                                     beta = beta / (1.0f - q);
@@ -698,7 +776,7 @@ __global__ __launch_bounds__(256) void k_shade_ao(SceneDev s, PathsDev P, int32_
     ss.index = P.sidx[i];
     ss.dim = P.sdim[i];
     float u0, u1;
-    get_2d(s.halton, ss, u0, u1);
+    get_2d(s.sampler, ss, u0, u1);
     const float phi = u0 * pstd::Pi * 2.0f, theta = pstd::acos(u1);
     const float st = pstd::sin(theta);
     const f3 w = face_same_hemisphere(mk(st * pstd::cos(phi), st * pstd::sin(phi), pstd::cos(theta)), n);
@@ -717,19 +795,87 @@ __global__ __launch_bounds__(256) void k_resolve_ao(PathsDev P, int32_t n_shadow
 
 // VizIntegrator::Li (viz.cpp:27-41), Position and Normal: the camera ray's
 // hit point or shading normal as the colour, 0 for a miss.
-__global__ __launch_bounds__(256) void k_shade_viz(SceneDev s, PathsDev P, int32_t n_active) {
+// Surface parameterisation of a hit, as the shapes' Intersect members fill
+// Interaction::uv, dpdu, dpdv: triangles (geometry.h:292, 311-314 via
+// bvh.cpp:695-697) barycentric (u, v) and the edges v1 - v0, v2 - v0;
+// spheres (geometry.cpp:140-145) (phi, theta) of the normal and the spherical
+// tangents; rects (geometry.cpp:191-193) (u, v) + 1/2 and the unit edges;
+// planes (geometry.cpp:104-107) the hit point's coordinates along u, v.
+__device__ void surface_param(const SceneDev& s, int32_t prim, const Hit& h, float t, float u, float v,
+                              f3 o, f3 d, f3& uv, f3& dpdu, f3& dpdv) {
+    const ShapeDev& sh = s.shapes[h.shape];
+    if (sh.type == kShapeMesh) {
+        const float* tv = s.tri_verts + 9 * (int64_t)(sh.tri_offset + (prim - sh.first_prim));
+        const f3 v0 = mk(tv[0], tv[1], tv[2]), v1 = mk(tv[3], tv[4], tv[5]), v2 = mk(tv[6], tv[7], tv[8]);
+        uv = mk(u, v, 0.0f);
+        dpdu = v1 - v0;
+        dpdv = v2 - v0;
+    } else if (sh.type == kShapeSphere) {
+        // CartesianToSpherical (vecmath.h:1243-1250)
+        float phi = pstd::atan2(h.n.y, h.n.x);
+        phi = phi < 0.0f ? kPi * 2.0f + phi : phi;
+        const float theta = pstd::acos(h.n.z);
+        const float st = pstd::sin(theta), ct = pstd::cos(theta);
+        const float sp = pstd::sin(phi), cp = pstd::cos(phi);
+        dpdu = mk(st * -sp, st * cp, 0.0f);
+        dpdv = mk(ct * cp, ct * sp, -st);
+        uv = mk(phi, theta, 0.0f);
+    } else if (sh.type == kShapeRect) {
+        uv = mk(u, v, 0.0f);
+        dpdu = sh.e0;
+        dpdv = sh.e1;
+    } else {
+        const f3 p = o + t * d;
+        uv = mk(dot(p - sh.p0, sh.e0), dot(p - sh.p0, sh.e1), 0.0f);
+        dpdu = sh.e0;
+        dpdv = sh.e1;
+    }
+}
+
+// ColorMap (color.cpp:25-35): black -> dark blue -> green -> red.
+__device__ f3 color_map(float v) {
+    const f3 k0 = mk(0.0f, 0.0f, 0.0f), k1 = mk(0.0f, 0.0f, 0.5f), k2 = mk(0.0f, 1.0f, 0.0f), k3 = mk(1.0f, 0.0f, 0.0f);
+    auto lerp = [](float t, f3 a, f3 b) { return (1.0f - t) * a + t * b; };  // pstd::lerp
+    if (v < 1.0f / 3.0f)
+        return lerp(v * 3.0f, k0, k1);
+    if (v < 2.0f / 3.0f)
+        return lerp((v - 1.0f / 3.0f) * 3.0f, k1, k2);
+    if (v < 3.0f / 3.0f)
+        return lerp((v - 2.0f / 3.0f) * 3.0f, k2, k3);
+    return k3;
+}
+
+// VizIntegrator::Li (viz.cpp:24-41).  Bvh: pine adds 2 per hierarchy node its
+// traversal visits (bvh.cpp:553) and maps count / 200 through ColorMap, hits
+// and misses alike; here the count is of this library's hierarchy (`counts`,
+// nodes fetched per ray), so the heat map shows this tree, not pine's.
+__global__ __launch_bounds__(256) void k_shade_viz(SceneDev s, PathsDev P, int32_t n_active,
+                                                   const int32_t* counts) {
     const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n_active)
         return;
     const int32_t i = P.queue_in[k];
     f3 c = mk(0.0f, 0.0f, 0.0f);
     const pg_hit hit = P.hits[k];
-    if (hit.prim >= 0) {
+    if (s.viz_type == kVizBvh) {
+        c = color_map((2.0f * (float)counts[2 * k]) / 200.0f);
+    } else if (hit.prim >= 0) {
         const float4* r4 = reinterpret_cast<const float4*>(&P.rays_in[k]);
         const float4 ra = r4[0], rb = r4[1];
+        const f3 o = mk(ra.x, ra.y, ra.z), d = mk(rb.x, rb.y, rb.z);
         const pg_hit_uv uv = P.uvs[k];
-        const Hit h = surface(s, hit.prim, hit.t, uv.u, uv.v, mk(ra.x, ra.y, ra.z), mk(rb.x, rb.y, rb.z));
-        c = s.viz_type == 0 ? h.p : h.n;
+        const Hit h = surface(s, hit.prim, hit.t, uv.u, uv.v, o, d);
+        if (s.viz_type == kVizPosition) {
+            c = h.p;
+        } else if (s.viz_type == kVizNormal) {
+            c = h.n;
+        } else {
+            f3 tc, du, dv;
+            surface_param(s, hit.prim, h, hit.t, uv.u, uv.v, o, d, tc, du, dv);
+            c = s.viz_type == kVizTexcoord ? tc : s.viz_type == kVizDpDu ? du : dv;
+            if (s.viz_type != kVizTexcoord)  // Abs(dpdu), Abs(dpdv)
+                c = mk(fabsf(c.x), fabsf(c.y), fabsf(c.z));
+        }
     }
     P.L[3 * i + 0] = c.x;
     P.L[3 * i + 1] = c.y;
@@ -759,7 +905,7 @@ __global__ __launch_bounds__(256) void k_shade_rw(SceneDev s, PathsDev P, int32_
     f3 nextO, nextD;
     if (hit.prim < 0) {
         if (s.has_env)
-            L = L + beta * sky_color(s, rd);
+            L = L + beta * env_color(s, rd);
     } else {
         const pg_hit_uv uv = P.uvs[k];
         const Hit h = surface(s, hit.prim, hit.t, uv.u, uv.v, ro, rd);
@@ -776,9 +922,9 @@ __global__ __launch_bounds__(256) void k_shade_rw(SceneDev s, PathsDev P, int32_
             } else if (depth + 1 < s.max_depth) {
                 const M3 n2w = frame_from(h.n);
                 const M3 w2n = inverse(n2w);
-                const float u1 = get_1d(s.halton, ss);
+                const float u1 = get_1d(s.sampler, ss);
                 float u2a, u2b;
-                get_2d(s.halton, ss, u2a, u2b);
+                get_2d(s.sampler, ss, u2a, u2b);
                 BSDFSample bs;
                 if (material_sample(m, mul(w2n, wi), u1, u2a, u2b, bs)) {
                     bs.wo = mul(n2w, bs.wo);
@@ -855,8 +1001,8 @@ __global__ __launch_bounds__(256) void k_shade_spec(SceneDev s, PathsDev P, int3
     f3 nextO, nextD;
     if (hit.prim < 0) {
         if (s.has_env) {
-            const float w = depth == 0 ? 1.0f : power_heuristic(bsdfPdf, 0.0f);
-            const Spec4 le = lift(sky_color(s, rd), w);
+            const float w = depth == 0 ? 1.0f : power_heuristic(bsdfPdf, env_pdf(s));
+            const Spec4 le = lift(env_color(s, rd), w);
             for (int j = 0; j < 4; j++)
                 L.v[j] += beta.v[j] * le.v[j];
         }
@@ -877,11 +1023,11 @@ __global__ __launch_bounds__(256) void k_shade_spec(SceneDev s, PathsDev P, int3
             const M3 w2n = inverse(n2w);
             const f3 wiL = mul(w2n, wi);
             float lpdf;
-            const float ul = get_1d(s.halton, ss);
+            const float ul = get_1d(s.sampler, ss);
             float la, lb;
             if (s.n_lights > 0) {
                 int li = sample_light(s, ul, lpdf);
-                get_2d(s.halton, ss, la, lb);
+                get_2d(s.sampler, ss, la, lb);
                 LightSampleD ls = sample_light_point(s, s.lights[li], h.p, la, lb);
                 ls.pdf *= lpdf;
                 const f3 woL = mul(w2n, ls.wo);
@@ -903,9 +1049,9 @@ __global__ __launch_bounds__(256) void k_shade_spec(SceneDev s, PathsDev P, int3
                     spec_store(P.sh4, slot, contrib);
                 }
             }
-            const float u1 = get_1d(s.halton, ss);
+            const float u1 = get_1d(s.sampler, ss);
             float u2a, u2b;
-            get_2d(s.halton, ss, u2a, u2b);
+            get_2d(s.sampler, ss, u2a, u2b);
             BSDFSample bs;
             if (material_sample(m, wiL, u1, u2a, u2b, bs)) {
                 bs.wo = mul(n2w, bs.wo);
@@ -920,7 +1066,7 @@ __global__ __launch_bounds__(256) void k_shade_spec(SceneDev s, PathsDev P, int3
                     if (depth > 2) {
                         const float mean = 0.25f * (beta.v[0] + beta.v[1] + beta.v[2] + beta.v[3]);
                         const float q = pstd::clamp(1.0f - mean, 0.05f, 1.0f);
-                        if (get_1d(s.halton, ss) < q) {
+                        if (get_1d(s.sampler, ss) < q) {
                             alive = false;
                         } else {
                             for (int j = 0; j < 4; j++)
@@ -1017,12 +1163,12 @@ __device__ __forceinline__ int medium_for(f3 w, f3 n, int inside, int outside) {
 // media it consumes sampler dimensions.
 __device__ void light_record(const SceneDev& s, PathsDev& P, SamplerState& ss, int32_t i, f3 p,
                              f3 n, f3 wi, int mat, float g, int med_in, int med_out) {
-    const float ul = get_1d(s.halton, ss);  // SampleLight's Get1D runs even with no lights
+    const float ul = get_1d(s.sampler, ss);  // SampleLight's Get1D runs even with no lights
     if (s.n_lights <= 0)
         return;
     float lpdf, la, lb;
     const int li = sample_light(s, ul, lpdf);
-    get_2d(s.halton, ss, la, lb);
+    get_2d(s.sampler, ss, la, lb);
     LightSampleD ls = sample_light_point(s, s.lights[li], p, la, lb);
     ls.pdf *= lpdf;
     f3 f;
@@ -1095,7 +1241,7 @@ __global__ __launch_bounds__(256) void k_shade_gen(SceneDev s, PathsDev P, int32
         MediumEvent ev;
         ev.scattered = false;
         if (med >= 0) {
-            ev = medium_sample(s.media[med], majorants(s, med, lds_medium, lds), s.halton, ss, ro,
+            ev = medium_sample(s.media[med], majorants(s, med, lds_medium, lds), s.sampler, ss, ro,
                                rd, tmax, lookups);
             beta = beta * ev.weight;
         }
@@ -1107,8 +1253,8 @@ __global__ __launch_bounds__(256) void k_shade_gen(SceneDev s, PathsDev P, int32
             }
         } else if (hit.prim < 0) {
             if (s.has_env) {
-                const f3 le = sky_color(s, rd);
-                L = depth == 0 ? L + beta * le : L + beta * le * power_heuristic(bsdfPdf, 0.0f);
+                const f3 le = env_color(s, rd);
+                L = depth == 0 ? L + beta * le : L + beta * le * power_heuristic(bsdfPdf, env_pdf(s));
             }
         } else {
             const pg_hit_uv uv = P.uvs[k];
@@ -1157,6 +1303,113 @@ __global__ __launch_bounds__(256) void k_shade_gen(SceneDev s, PathsDev P, int32
     add_lookups(P.lookups, lookups);
 }
 
+// RandomWalkIntegrator::Li (randomwalk.cpp:6-62) with participating media:
+// the ray's medium samples an event first (Medium::Sample, as in the Path
+// integrator's split bounce; its weight multiplies beta either way); at a
+// medium event the phase function picks the next direction (Get2D) unless the
+// path is at maxDepth; emission and the environment are added with the
+// throughput alone; material-less surfaces pass the ray into the medium on
+// their far side without counting depth.
+__global__ __launch_bounds__(256) void k_shade_rw_gen(SceneDev s, PathsDev P, int32_t n_active, int32_t lds_medium) {
+    extern __shared__ float lds_maj[];
+    const float* lds = stage_majorants(s, lds_medium, lds_maj);
+    int lookups = 0;
+    for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n_active;
+         k += gridDim.x * blockDim.x) {
+        const int32_t i = P.queue_in[k];
+        const float4* r4 = reinterpret_cast<const float4*>(&P.rays_in[k]);
+        const float4 ra = r4[0], rb = r4[1];
+        const f3 ro = mk(ra.x, ra.y, ra.z), rd = mk(rb.x, rb.y, rb.z);
+        const pg_hit hit = P.hits[k];
+        int depth = P.depth[i];
+        f3 L = mk(P.L[3 * i], P.L[3 * i + 1], P.L[3 * i + 2]);
+        f3 beta = mk(P.beta[3 * i], P.beta[3 * i + 1], P.beta[3 * i + 2]);
+        SamplerState ss;
+        ss.index = P.sidx[i];
+        ss.dim = P.sdim[i];
+        const int med = P.medium[i];
+        bool alive = false;
+        f3 nextO, nextD;
+        int next_med = med;
+        const float tmax = hit.prim >= 0 ? hit.t : kFloatMax;
+        MediumEvent ev;
+        ev.scattered = false;
+        if (med >= 0) {
+            ev = medium_sample(s.media[med], majorants(s, med, lds_medium, lds), s.sampler, ss, ro, rd, tmax,
+                               lookups);
+            beta = beta * ev.weight;
+        }
+        if (ev.scattered) {
+            if (depth + 1 != s.max_depth) {
+                float u0, u1;
+                get_2d(s.sampler, ss, u0, u1);
+                f3 wo;
+                phase_sample(s.media[med].g, -rd, u0, u1, wo);
+                // MediumInteraction::SpawnRay: the normal is zero, no offset.
+                const f3 zero = mk(0.0f, 0.0f, 0.0f);
+                nextD = wo;
+                nextO = offset_ray_origin(ev.p, face_same_hemisphere(zero, wo));
+                next_med = med;
+                alive = true;
+                depth++;
+            }
+        } else if (hit.prim < 0) {
+            if (s.has_env)
+                L = L + beta * env_color(s, rd);
+        } else {
+            const pg_hit_uv uv = P.uvs[k];
+            const Hit h = surface(s, hit.prim, hit.t, uv.u, uv.v, ro, rd);
+            const ShapeDev& sh = s.shapes[h.shape];
+            const bool transition = sh.mi_inside != sh.mi_outside;
+            const int min_ = transition ? sh.mi_inside : med;
+            const int mout = transition ? sh.mi_outside : med;
+            if (sh.material < 0) {
+                nextD = rd;
+                nextO = offset_ray_origin(h.p, face_same_hemisphere(h.n, rd));
+                next_med = medium_for(rd, h.n, min_, mout);
+                alive = true;
+            } else {
+                const MaterialDev& m = s.materials[sh.material];
+                const f3 wi = -rd;
+                if (m.kind == kMatEmissive) {
+                    L = L + beta * emitted(m, h.n, wi);
+                } else if (depth + 1 < s.max_depth) {
+                    const M3 n2w = frame_from(h.n);
+                    const M3 w2n = inverse(n2w);
+                    const float u1 = get_1d(s.sampler, ss);
+                    float u2a, u2b;
+                    get_2d(s.sampler, ss, u2a, u2b);
+                    BSDFSample bs;
+                    if (material_sample(m, mul(w2n, wi), u1, u2a, u2b, bs)) {
+                        bs.wo = mul(n2w, bs.wo);
+                        beta = beta * (abs_dot(bs.wo, h.n) * bs.f / bs.pdf);
+                        nextD = bs.wo;
+                        nextO = offset_ray_origin(h.p, face_same_hemisphere(h.n, nextD));
+                        next_med = medium_for(nextD, h.n, min_, mout);
+                        alive = true;
+                    }
+                    depth++;
+                }
+            }
+        }
+        P.L[3 * i + 0] = L.x;
+        P.L[3 * i + 1] = L.y;
+        P.L[3 * i + 2] = L.z;
+        P.beta[3 * i + 0] = beta.x;
+        P.beta[3 * i + 1] = beta.y;
+        P.beta[3 * i + 2] = beta.z;
+        P.sdim[i] = ss.dim;
+        P.depth[i] = depth;
+        if (alive) {
+            const int slot = atomicAdd(&P.counters[0], 1);
+            P.queue_out[slot] = i;
+            P.medium[i] = next_med;
+            store_ray(&P.rays_out[slot], nextO, nextD, 0.0f, kFloatMax);
+        }
+    }
+    add_lookups(P.lookups, lookups);
+}
+
 // One segment of IntersectTr (integrator.cpp:68-87): transmittance of the
 // segment's medium up to the closest hit, then stop at a surface with a
 // material, or continue towards p2 through a material-less one.
@@ -1179,7 +1432,7 @@ __global__ __launch_bounds__(256) void k_shadow_tr(SceneDev s, PathsDev P, int32
             ss.index = P.sidx[i];
             ss.dim = P.sdim[i];
             const float tmax = hit.prim >= 0 ? hit.t : rb.w;
-            tr = tr * medium_tr(s.media[med], majorants(s, med, lds_medium, lds), s.halton, ss, ro,
+            tr = tr * medium_tr(s.media[med], majorants(s, med, lds_medium, lds), s.sampler, ss, ro,
                                 rd, tmax, lookups);
             P.sdim[i] = ss.dim;
         }
@@ -1265,7 +1518,7 @@ __global__ __launch_bounds__(256) void k_shade_gen_b(SceneDev s, PathsDev P, int
     int next_med = -1;
     if (info.x == 1) {
         float u0, u1;
-        get_2d(s.halton, ss, u0, u1);
+        get_2d(s.sampler, ss, u0, u1);
         f3 wo;
         P.bsdf_pdf[i] = phase_sample(s.media[info.z].g, wi, u0, u1, wo);
         nextD = wo;
@@ -1283,7 +1536,7 @@ __global__ __launch_bounds__(256) void k_shade_gen_b(SceneDev s, PathsDev P, int
             alive = true;
             if (depth > 2) {
                 const float q = pstd::clamp(1.0f - luminance(beta), 0.05f, 1.0f);
-                if (get_1d(s.halton, ss) < q)
+                if (get_1d(s.sampler, ss) < q)
                     alive = false;
                 else
                     beta = beta / (1.0f - q);
@@ -1294,9 +1547,9 @@ __global__ __launch_bounds__(256) void k_shade_gen_b(SceneDev s, PathsDev P, int
         const M3 n2w = frame_from(n);
         const M3 w2n = inverse(n2w);
         const f3 wiL = mul(w2n, wi);
-        const float u1 = get_1d(s.halton, ss);
+        const float u1 = get_1d(s.sampler, ss);
         float u2a, u2b;
-        get_2d(s.halton, ss, u2a, u2b);
+        get_2d(s.sampler, ss, u2a, u2b);
         BSDFSample bs;
         if (material_sample(m, wiL, u1, u2a, u2b, bs)) {
             bs.wo = mul(n2w, bs.wo);
@@ -1309,7 +1562,7 @@ __global__ __launch_bounds__(256) void k_shade_gen_b(SceneDev s, PathsDev P, int
                 alive = true;
                 if (depth > 2) {
                     const float q = pstd::clamp(1.0f - luminance(beta), 0.05f, 1.0f);
-                    if (get_1d(s.halton, ss) < q)
+                    if (get_1d(s.sampler, ss) < q)
                         alive = false;
                     else
                         beta = beta / (1.0f - q);
@@ -1638,6 +1891,11 @@ struct pg_scene {
     bool light_uniform = false;
     SceneDev dev{};
     int spp = 1, max_depth = 4;
+    int atmo_w = 0, atmo_h = 0;  // Atmosphere colour table size (0: no Atmosphere)
+    bool atmo_accurate = false;  // table with correctly rounded exp/sqrt/pow (extension)
+    int sampler_kind = kSamplerHalton;
+    bool halton_permute = true;            // HaltonSampler randomizeStrategy permutateDigits
+    int sobol_randomize = kSobolFastOwen;  // SobolSampler randomizeStrategy
     std::string output = "result.png";
     bool tone_map = true;
     pg_accel* accel = nullptr;
@@ -1646,6 +1904,8 @@ struct pg_scene {
     float* d_film = nullptr;
     PathsDev paths{};
     int64_t path_capacity = 0;
+    int32_t* d_viz_counts = nullptr;  // Viz Bvh: nodes and primitives per camera ray
+    int64_t viz_counts_capacity = 0;
     double last_render_ms = 0.0;
     int64_t last_rays = 0;
     // media
@@ -1987,24 +2247,39 @@ void build_scene(pg_scene& S) {
         S.integrator = kIntRandomWalk;
     } else if (itype == "Viz") {
         S.integrator = kIntViz;
+        // VizIntegrator's constructor (viz.cpp:7-22): an unknown type is Normal.
         const std::string vt = ip.get_string("viztype", "");
-        if (vt == "Position") {
-            S.dev.viz_type = 0;
-        } else if (vt == "Bvh" || vt == "Texcoord" || vt == "DpDu" || vt == "DpDv") {
-            throw unsupported("Viz type \"" + vt + "\" (Position and Normal are supported)");
-        } else {
-            if (vt != "Normal")
-                fprintf(stderr, "Unknown viztype %s\n", vt.c_str());  // viz.cpp:18-21
-            S.dev.viz_type = 1;
-        }
+        S.dev.viz_type = vt == "Position"   ? kVizPosition
+                         : vt == "Texcoord" ? kVizTexcoord
+                         : vt == "DpDu"     ? kVizDpDu
+                         : vt == "DpDv"     ? kVizDpDv
+                         : vt == "Bvh"      ? kVizBvh
+                                            : kVizNormal;
+        if (vt != "Normal" && S.dev.viz_type == kVizNormal)
+            fprintf(stderr, "Unknown viztype %s\n", vt.c_str());
     } else {
         throw unsupported("integrator \"" + itype + "\" (Path, AO, Viz and RandomWalk are supported)");
     }
     S.max_depth = ip.get_int("maxDepth", 4);
     const Params& sp = ip.sub("sampler");
     std::string stype = sp.get_string("type");
-    if (stype != "Halton")
-        throw unsupported("sampler \"" + stype + "\" (Halton only)");
+    if (stype == "Halton") {
+        // HaltonSampler::Create (sampler.cpp:253-260): an unknown strategy keeps the default.
+        S.sampler_kind = kSamplerHalton;
+        S.halton_permute = sp.get_string("randomizeStrategy", "permutateDigits") != "none";
+    } else if (stype == "Sobol") {
+        // SobolSampler::Create (sampler.cpp:268-278): an unknown strategy leaves
+        // the initial BinaryPermutate in place.
+        S.sampler_kind = kSamplerSobol;
+        const std::string rs = sp.get_string("randomizeStrategy", "fastOwen");
+        S.sobol_randomize = rs == "none"              ? kSobolNone
+                            : rs == "binaryPermutate" ? kSobolBinaryPermutate
+                            : rs == "Owen"            ? kSobolOwen
+                            : rs == "fastOwen"        ? kSobolFastOwen
+                                                      : kSobolBinaryPermutate;
+    } else {
+        throw unsupported("sampler \"" + stype + "\" (Halton and Sobol are supported)");
+    }
     S.spp = sp.get_int("samplesPerPixel");
     std::string lstype = ip.sub("lightSampler").get_string("type", "Power");
     S.light_uniform = lstype == "Uniform";
@@ -2047,14 +2322,39 @@ void build_scene(pg_scene& S) {
             p.get_vec3("position", a);
             l.v = mk(a[0], a[1], a[2]);
         } else if (t == "Environment") {
-            // EnvironmentLight::Create (light.cpp:157-168): Sky only.
+            // EnvironmentLight::Create (light.cpp:157-168): Sky, or Atmosphere,
+            // which is also what an unknown type falls back to.
             const Params& ep = p.sub("environment");
             std::string et = ep.get_string("type");
-            if (et != "Sky")
-                throw unsupported("environment light \"" + et + "\" (Sky only)");
-            ep.get_vec3("sunDirection", a);
-            ep.get_vec3("sunColor", c);
-            l.type = kLightSky;
+            if (et == "Sky") {
+                ep.get_vec3("sunDirection", a);
+                ep.get_vec3("sunColor", c);
+                l.type = kLightSky;
+                S.dev.env_kind = kEnvSky;
+                S.atmo_w = S.atmo_h = 0;
+            } else {
+                if (et != "Atmosphere")
+                    fprintf(stderr, "[EnvironmentLight][Create]Unknown type \"%s\"\n", et.c_str());
+                // Atmosphere::Create (light.cpp:152-156) defaults.  `accurate`
+                // is this library's extension (pine ignores unknown keys): the
+                // colour table with correctly rounded exp/sqrt/pow instead of
+                // pine's approximations, whose exp diverges here and makes every
+                // entry of pine's own table NaN.
+                const float dsun[3] = {2.0f, 6.0f, 3.0f}, dcol[3] = {1.0f, 1.0f, 1.0f};
+                const int dsize[2] = {512, 512};
+                int sz[2];
+                ep.get_vec3("sunDirection", a, dsun);
+                ep.get_vec3("sunColor", c, dcol);
+                ep.get_vec2i("size", sz, dsize);
+                if (sz[0] < 1 || sz[1] < 1 || (int64_t)sz[0] * sz[1] > (1 << 26))
+                    throw std::runtime_error("Atmosphere: size out of range");
+                l.type = kLightAtmosphere;
+                S.dev.env_kind = kEnvAtmosphere;
+                S.dev.atmo_interp = ep.get_bool("interpolate", true) ? 1 : 0;
+                S.atmo_w = sz[0];
+                S.atmo_h = sz[1];
+                S.atmo_accurate = ep.get_bool("accurate", false);
+            }
             l.v = normalize(mk(a[0], a[1], a[2]));
             l.color = mk(c[0], c[1], c[2]);
             // scene->envLight is the last environment light (fileio.cpp:272-274).
@@ -2258,8 +2558,6 @@ void build_scene(pg_scene& S) {
     }
     if (uses_media)
         S.general = true;
-    if (uses_media && S.integrator == kIntRandomWalk)
-        throw unsupported("RandomWalk with participating media");
     const Params& fp = cp.sub("film");
     int size[2];
     const int defsize[2] = {720, 480};
@@ -2416,6 +2714,88 @@ void halton_setup(HaltonDev& h, int w, int hgt) {
     h.stride = h.baseScales[0] * h.baseScales[1];
 }
 
+// Sobol generator matrices, dimension-major, 32 columns each: dimension 0 is
+// van der Corput, dimensions 1..1023 expand Joe & Kuo's direction numbers by
+// the Bratley-Fox recurrence in 32-bit fixed point (the numbers pine's table,
+// src/util/sobolmetrices.cpp, holds expanded).
+void sobol_matrices(std::vector<uint32_t>& C) {
+    C.assign((size_t)kSobolDimensions * kSobolColumns, 0u);
+    for (int k = 0; k < kSobolColumns; k++)
+        C[k] = 1u << (31 - k);
+    const uint16_t* q = kSobolDirnums;
+    for (int d = 1; d < kSobolDimensions; d++) {
+        const int deg = *q++, a = *q++;
+        uint32_t* c = &C[(size_t)d * kSobolColumns];
+        for (int k = 1; k <= kSobolColumns; k++) {
+            if (k <= deg) {
+                c[k - 1] = (uint32_t)q[k - 1] << (32 - k);
+                continue;
+            }
+            uint32_t v = c[k - deg - 1] ^ (c[k - deg - 1] >> deg);
+            for (int j = 1; j < deg; j++)
+                if ((a >> (deg - 1 - j)) & 1)
+                    v ^= c[k - j - 1];
+            c[k - 1] = v;
+        }
+        q += deg;
+    }
+}
+
+// The pixel -> Sobol index tables for a 2^m x 2^m pixel grid (the tables
+// pine's SobolIntervalToIndex reads as VdCSobolMatrices[m-1] and
+// VdCSobolMatricesInv[m-1]), derived here from the first two dimensions.
+// With index = frame * 4^m + j, the pixel code b = (x << m) | y of an index
+// (x, y: the top m bits of dimensions 0 and 1) is linear over GF(2) in the
+// index bits.  Dimension 0 is the bit reversal (index bit k < m gives x bit
+// m-1-k); dimension 1 is the Pascal matrix mod 2, whose column k has bit r
+// (from the top) set iff (k & r) == r (Lucas).  vdc[c] is the code of index
+// bit 2m + c (kept only where pine's 52-entry rows hold it, c < 52 - 2m), and
+// vdc_inv[c] the index bits j whose code is bit c, from the inverse of the
+// 2m x 2m map of j.
+void sobol_pixel_tables(int m, std::vector<uint64_t>& vdc, std::vector<uint64_t>& vdc_inv) {
+    vdc.assign(52, 0);
+    vdc_inv.assign(52, 0);
+    if (m == 0)
+        return;
+    auto code = [m](int k) -> uint64_t {
+        uint64_t x = k < m ? (1ull << (m - 1 - k)) : 0, y = 0;
+        for (int r = 0; r < m; r++)
+            if ((k & r) == r)
+                y |= 1ull << (m - 1 - r);
+        return (x << m) | y;
+    };
+    for (int c = 0; c < 52 - 2 * m; c++)
+        vdc[c] = code(2 * m + c);
+    // Gauss-Jordan over GF(2) on the rows of A (row r: code bit r as a mask over j).
+    const int n = 2 * m;
+    std::vector<uint64_t> rowA(n, 0), rowI(n, 0);
+    for (int r = 0; r < n; r++) {
+        for (int j = 0; j < n; j++)
+            if ((code(j) >> r) & 1)
+                rowA[r] |= 1ull << j;
+        rowI[r] = 1ull << r;
+    }
+    for (int col = 0; col < n; col++) {
+        int piv = col;
+        while (piv < n && !((rowA[piv] >> col) & 1))
+            piv++;
+        if (piv == n)
+            throw std::runtime_error("Sobol pixel map is singular");
+        std::swap(rowA[piv], rowA[col]);
+        std::swap(rowI[piv], rowI[col]);
+        for (int r = 0; r < n; r++)
+            if (r != col && ((rowA[r] >> col) & 1)) {
+                rowA[r] ^= rowA[col];
+                rowI[r] ^= rowI[col];
+            }
+    }
+    // rowI is now A^-1 by rows: bit j of the index for code bit c is bit c of rowI[j].
+    for (int c = 0; c < n; c++)
+        for (int j = 0; j < n; j++)
+            if ((rowI[j] >> c) & 1)
+                vdc_inv[c] |= 1ull << j;
+}
+
 template <typename T>
 int upload(pg_scene& S, const std::vector<T>& v, const T** out) {
     if (v.empty()) {
@@ -2439,6 +2819,161 @@ int dalloc(pg_scene& S, int64_t n, T** out) {
     return PG_OK;
 }
 
+// AtmosphereColor (color.cpp:37-89): single scattering along the view ray from
+// the ground (8 samples), each lit through 4 samples towards the sun, Rayleigh
+// and Mie coefficients and phase functions, x10 x sunColor.  M supplies the
+// scalar type and exp/sqrt/pow/acos/sin/cos: pine's float approximations
+// (pstd), or double precision with correctly rounded functions for the
+// `accurate` table (where exp's overflow threshold, not float's, decides which
+// texels below the horizon overflow).
+struct PstdMath {
+    using T = float;
+    static __device__ float exp(float x) { return pstd::exp(x); }
+    static __device__ float sqrt(float x) { return pstd::sqrt(x); }
+    static __device__ float pow(float x, float e) { return pstd::pow(x, e); }
+    static __device__ float acos(float x) { return pstd::acos(x); }
+    static __device__ float sin(float x) { return pstd::sin(x); }
+    static __device__ float cos(float x) { return pstd::cos(x); }
+};
+struct ExactMath {
+    using T = double;
+    static __device__ double exp(double x) { return ::exp(x); }
+    static __device__ double sqrt(double x) { return ::sqrt(x); }
+    static __device__ double pow(double x, double e) { return ::pow(x, e); }
+    static __device__ double acos(double x) { return ::acos(x); }
+    static __device__ double sin(double x) { return ::sin(x); }
+    static __device__ double cos(double x) { return ::cos(x); }
+};
+
+template <class T>
+struct AV3 {
+    T x, y, z;
+};
+template <class T>
+__device__ AV3<T> av3(T x, T y, T z) { return AV3<T>{x, y, z}; }
+template <class T>
+__device__ AV3<T> operator+(AV3<T> a, AV3<T> b) { return av3(a.x + b.x, a.y + b.y, a.z + b.z); }
+template <class T>
+__device__ AV3<T> operator*(AV3<T> a, AV3<T> b) { return av3(a.x * b.x, a.y * b.y, a.z * b.z); }
+template <class T>
+__device__ AV3<T> operator*(AV3<T> a, T s) { return av3(a.x * s, a.y * s, a.z * s); }
+template <class T>
+__device__ T adot(AV3<T> a, AV3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+// Sphere::ComputeT (geometry.cpp:114-126) about the origin, tmin 0.
+template <class M, class T = typename M::T>
+__device__ T atmo_sphere_t(AV3<T> ro, AV3<T> rd, T r) {
+    const T a = adot(rd, rd), b = T(2) * adot(ro, rd);
+    const T c = adot(ro, ro) - r * r;
+    T d = b * b - T(4) * a * c;
+    if (d <= T(0))
+        return T(-1);
+    d = M::sqrt(d);
+    T t = (-b - d) / (T(2) * a);
+    if (t < T(0))
+        t = (-b + d) / (T(2) * a);
+    return t;
+}
+
+template <class M, class T = typename M::T>
+__device__ AV3<T> atmosphere_sky(AV3<T> dir, AV3<T> sun, AV3<T> sun_color) {
+    const AV3<T> betaR = av3(T(3.8e-6f), T(13.5e-6f), T(33.1e-6f));
+    const T betaM = T(21e-6f);
+    const T atmosphereRadius = T(6420e3f), earthRadius = T(6360e3f);
+    const bool along_sun = dir.x == sun.x && dir.y == sun.y && dir.z == sun.z;
+    const T Hr = T(1) / T(7995), Hm = along_sun ? T(1) / T(10000) : T(1) / T(1200);
+    const T mu = adot(dir, sun), g = T(0.76f), pi = T(3.14159265358979323846);
+    const T phaseR = T(3) / (T(16) * pi) * (T(1) + mu * mu);
+    const T phaseM = T(3) / (T(8) * pi) * (T(1) - g * g) * (T(1) + mu * mu) /
+                     ((T(2) + g * g) * M::pow(T(1) + g * g - T(2) * g * mu, T(1.5)));
+    const AV3<T> o = av3(T(0), earthRadius, T(0));
+    const T seg = atmo_sphere_t<M>(o, dir, atmosphereRadius) / T(8);
+    T tcur = T(0), odR = T(0), odM = T(0);
+    AV3<T> sumR = av3(T(0), T(0), T(0)), sumM = sumR;
+    for (int i = 0; i < 8; i++) {
+        const AV3<T> sp = o + dir * (tcur + seg * T(0.5));
+        const T height = M::sqrt(adot(sp, sp)) - earthRadius;
+        const T hr = M::exp(-height * Hr) * seg, hm = M::exp(-height * Hm) * seg;
+        odR += hr;
+        odM += hm;
+        const T segL = atmo_sphere_t<M>(sp, sun, atmosphereRadius) / T(4);
+        T odRL = T(0), odML = T(0);
+        const T lo2 = adot(sp, sp), ld2 = adot(sun, sun), lod = adot(sp, sun);
+        T lt = segL * T(0.5);
+        int j = 0;
+        for (; j < 4; j++, lt += segL) {
+            // color.cpp:66-72, as written (a squared distance minus a radius)
+            const T hl = lo2 + lt * lt * ld2 + T(2) * lt * lod - earthRadius;
+            if (hl < T(0))
+                break;
+            odRL += M::exp(-hl * Hr) * segL;
+            odML += M::exp(-hl * Hm) * segL;
+        }
+        if (j == 4) {
+            const AV3<T> tau = betaR * (odR + odRL) + av3(betaM, betaM, betaM) * (odM + odML);
+            const AV3<T> tr = av3(M::exp(-tau.x), M::exp(-tau.y), M::exp(-tau.z));
+            sumR = sumR + tr * hr;
+            sumM = sumM + tr * hm;
+        }
+        tcur += seg;
+    }
+    return (sumR * betaR * phaseR + sumM * av3(betaM, betaM, betaM) * phaseM) * T(10) * sun_color;
+}
+
+// The Atmosphere constructor's table (light.cpp:81-92): texel (x, y) holds the
+// colour towards UniformSphereSampling(x / size.x, y / size.y); one more
+// thread computes the colour along the sun.  With pine's arithmetic, NaN
+// entries are kept: pine's HasNaN() test folds to false under its -ffast-math
+// build.  The accurate table applies the test as written (NaN -> 0).
+template <class M>
+__global__ void k_atmosphere_table(f3 sunf, f3 colf, int w, int h, float* out) {
+    using T = typename M::T;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)w * h;
+    if (t > n)
+        return;
+    const AV3<T> sun = av3(T(sunf.x), T(sunf.y), T(sunf.z)), col = av3(T(colf.x), T(colf.y), T(colf.z));
+    AV3<T> c;
+    if (t == n) {
+        c = atmosphere_sky<M>(sun, sun, col);
+    } else {
+        const int y = (int)(t / w), x = (int)(t - (int64_t)y * w);
+        const T u0 = T((float)x / (float)w), u1 = T((float)y / (float)h);
+        const T phi = u0 * T(3.14159265358979323846) * T(2), theta = M::acos(T(1) - T(2) * u1);
+        const T st = M::sin(theta);
+        const AV3<T> d = av3(st * M::cos(phi), st * M::sin(phi), M::cos(theta));
+        c = atmosphere_sky<M>(d, sun, col);
+    }
+    float r = (float)c.x, g = (float)c.y, b = (float)c.z;
+    if (sizeof(T) == 8 && (isnan(r) || isnan(g) || isnan(b)))
+        r = g = b = 0.0f;
+    out[3 * t + 0] = r;
+    out[3 * t + 1] = g;
+    out[3 * t + 2] = b;
+}
+
+int atmosphere_setup(pg_scene& S) {
+    const int64_t n = (int64_t)S.atmo_w * S.atmo_h;
+    float* d = nullptr;
+    PG_HIP(hipMalloc(&d, (n + 1) * 3 * sizeof(float)));
+    S.allocations.push_back(d);
+    const dim3 grid((unsigned)((n + 1 + 255) / 256));
+    if (S.atmo_accurate)
+        hipLaunchKernelGGL(k_atmosphere_table<ExactMath>, grid, dim3(256), 0, 0, S.dev.sky_dir, S.dev.sky_color,
+                           S.atmo_w, S.atmo_h, d);
+    else
+        hipLaunchKernelGGL(k_atmosphere_table<PstdMath>, grid, dim3(256), 0, 0, S.dev.sky_dir, S.dev.sky_color,
+                           S.atmo_w, S.atmo_h, d);
+    PG_HIP(hipGetLastError());
+    float sun[3];
+    PG_HIP(hipMemcpy(sun, d + 3 * n, sizeof(sun), hipMemcpyDeviceToHost));
+    S.dev.atmo_colors = d;
+    S.dev.atmo_w = S.atmo_w;
+    S.dev.atmo_h = S.atmo_h;
+    S.dev.atmo_sun_le = mk(sun[0], sun[1], sun[2]);
+    return PG_OK;
+}
+
 int finish_scene(pg_scene& S, const pg_build_options* opts) {
     if (opts)
         S.build_opts = *opts;
@@ -2447,11 +2982,41 @@ int finish_scene(pg_scene& S, const pg_build_options* opts) {
         return rc;
     std::vector<int> primes, sums;
     std::vector<uint16_t> perms;
+    if (S.atmo_w > 0)
+        if (int rc = atmosphere_setup(S))
+            return rc;
     halton_tables(primes, sums, perms);
-    halton_setup(S.dev.halton, S.dev.width, S.dev.height);
+    if (!S.halton_permute) {
+        // randomizeStrategy none: RadicalInverse (lowdiscrepancy.h:13-26) is the
+        // scrambled form with the identity permutation.
+        for (size_t d = 0, o = 0; d < primes.size(); o += primes[d], d++)
+            for (int j = 0; j < primes[d]; j++)
+                perms[o + j] = (uint16_t)j;
+    }
+    SamplerDev& sd = S.dev.sampler;
+    sd.kind = S.sampler_kind;
+    halton_setup(sd.halton, S.dev.width, S.dev.height);
     int rc;
-    if ((rc = upload(S, perms, &S.dev.halton.perms)) || (rc = upload(S, primes, &S.dev.halton.primes)) ||
-        (rc = upload(S, sums, &S.dev.halton.primeSums)) || (rc = upload(S, S.prim_shape, &S.dev.prim_shape)) ||
+    if (S.sampler_kind == kSamplerSobol) {
+        // SobolSampler (sampler.cpp:129-134): m = log2 of the larger film side
+        // rounded up to a power of two.
+        int m = 0;
+        while ((1 << m) < std::max(S.dev.width, S.dev.height))
+            m++;
+        if (m > 25)
+            return pg_fail(PG_E_PARSE, "Sobol sampler: film larger than 2^25 pixels a side");
+        std::vector<uint32_t> C;
+        std::vector<uint64_t> vdc, vdc_inv;
+        sobol_matrices(C);
+        sobol_pixel_tables(m, vdc, vdc_inv);
+        sd.sobol.log2Scale = m;
+        sd.sobol.randomize = S.sobol_randomize;
+        if ((rc = upload(S, C, &sd.sobol.C)) || (rc = upload(S, vdc, &sd.sobol.vdc)) ||
+            (rc = upload(S, vdc_inv, &sd.sobol.vdc_inv)))
+            return rc;
+    }
+    if ((rc = upload(S, perms, &sd.halton.perms)) || (rc = upload(S, primes, &sd.halton.primes)) ||
+        (rc = upload(S, sums, &sd.halton.primeSums)) || (rc = upload(S, S.prim_shape, &S.dev.prim_shape)) ||
         (rc = upload(S, S.tri_verts, &S.dev.tri_verts)) || (rc = upload(S, S.shapes, &S.dev.shapes)) ||
         (rc = upload(S, S.materials, &S.dev.materials)) || (rc = upload(S, S.lights, &S.dev.lights)) ||
         (rc = upload(S, S.light_func, &S.dev.light_func)) || (rc = upload(S, S.light_cdf, &S.dev.light_cdf)))
@@ -2892,8 +3457,18 @@ static int render_enqueue(pg_scene* S, int spp, int spb_req, hipStream_t st, Gui
         for (int bounce = 0; active > 0 && bounce < kMaxBounces; bounce++) {
             ext_rays += active;
             PG_HIP(hipMemsetAsync(Q.counters, 0, 4 * sizeof(int32_t), st));
-            if (int rc = pg_trace_closest(S->accel, Q.rays_in, active, Q.hits, Q.uvs, st))
+            const bool viz_bvh = S->integrator == kIntViz && S->dev.viz_type == kVizBvh;
+            if (viz_bvh) {
+                if (S->viz_counts_capacity < active) {
+                    if (int rc = dalloc(*S, 2 * (int64_t)active, &S->d_viz_counts))
+                        return rc;
+                    S->viz_counts_capacity = active;
+                }
+                if (int rc = pg_trace_closest_counts(S->accel, Q.rays_in, active, Q.hits, S->d_viz_counts, st))
+                    return rc;
+            } else if (int rc = pg_trace_closest(S->accel, Q.rays_in, active, Q.hits, Q.uvs, st)) {
                 return rc;
+            }
             if (S->integrator == kIntAO) {
                 hipLaunchKernelGGL(k_shade_ao, dim3((active + B - 1) / B), dim3(B), 0, st, S->dev, Q, active);
                 int32_t counts[2];
@@ -2907,11 +3482,18 @@ static int render_enqueue(pg_scene* S, int spp, int spb_req, hipStream_t st, Gui
                 }
                 active = 0;
             } else if (S->integrator == kIntViz) {
-                hipLaunchKernelGGL(k_shade_viz, dim3((active + B - 1) / B), dim3(B), 0, st, S->dev, Q, active);
+                hipLaunchKernelGGL(k_shade_viz, dim3((active + B - 1) / B), dim3(B), 0, st, S->dev, Q, active,
+                                   viz_bvh ? S->d_viz_counts : nullptr);
                 rays += active;
                 active = 0;
             } else if (S->integrator == kIntRandomWalk) {
-                hipLaunchKernelGGL(k_shade_rw, dim3((active + B - 1) / B), dim3(B), 0, st, S->dev, Q, active);
+                if (S->general) {
+                    const size_t lds = S->lds_medium >= 0 ? S->med_majorant[S->lds_medium].size() * sizeof(float) : 0;
+                    hipLaunchKernelGGL(k_shade_rw_gen, dim3(tracking_grid(active, B)), dim3(B), lds, st, S->dev, Q,
+                                       active, S->lds_medium);
+                } else {
+                    hipLaunchKernelGGL(k_shade_rw, dim3((active + B - 1) / B), dim3(B), 0, st, S->dev, Q, active);
+                }
                 int32_t next = 0;
                 PG_HIP(hipMemcpyAsync(&next, Q.counters, sizeof(next), hipMemcpyDeviceToHost, st));
                 PG_HIP(hipStreamSynchronize(st));
@@ -3338,6 +3920,43 @@ extern "C" int pg_scene_film(const pg_scene* S, float* h_rgbw) {
         return pg_fail(PG_E_ARG, "pg_scene_film: null argument");
     PG_HIP(hipMemcpy(h_rgbw, S->d_film, (size_t)S->dev.width * S->dev.height * 4 * sizeof(float),
                      hipMemcpyDeviceToHost));
+    return PG_OK;
+}
+
+// One thread per (pixel, sample): StartPixel(p, sample) and ndims Get1D values
+// in the harness's pixel spread (pixel k is (py0 * w + px0 + 977 k) mod film).
+__global__ void k_sampler_values(SamplerDev sd, int w, int h, int px0, int py0, int npix, int spp,
+                                 int ndims, float* out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)npix * spp)
+        return;
+    const int k = (int)(t / spp), si = (int)(t - (int64_t)k * spp);
+    const int64_t idx = (int64_t)py0 * w + px0 + (int64_t)k * 977;
+    const int px = (int)(idx % w), py = (int)((idx / w) % h);
+    SamplerState ss;
+    ss.index = sampler_start(sd, px, py, si);
+    ss.dim = 2;
+    float* o = out + t * ndims;
+    for (int d = 0; d < ndims; d++)
+        o[d] = get_1d(sd, ss);
+}
+
+extern "C" int pg_scene_sampler_values(const pg_scene* S, int32_t px0, int32_t py0, int32_t npix,
+                                       int32_t spp, int32_t ndims, float* h_out) {
+    if (!S || !h_out || px0 < 0 || py0 < 0 || npix <= 0 || spp <= 0 || ndims <= 0 ||
+        (int64_t)npix * spp * ndims > (1ll << 28))
+        return pg_fail(PG_E_ARG, "pg_scene_sampler_values: bad argument");
+    const int64_t n = (int64_t)npix * spp, bytes = n * ndims * (int64_t)sizeof(float);
+    float* d = nullptr;
+    PG_HIP(hipMalloc(&d, bytes));
+    hipLaunchKernelGGL(k_sampler_values, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, S->dev.sampler,
+                       S->dev.width, S->dev.height, px0, py0, npix, spp, ndims, d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpy(h_out, d, bytes, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess)
+        return pg_fail(PG_E_HIP, std::string("pg_scene_sampler_values: ") + hipGetErrorString(e));
     return PG_OK;
 }
 

@@ -101,15 +101,20 @@ def _v3(v):
 
 
 def spheres_scene_text(seed=0, width=512, height=512, spp=64, max_depth=8, n_spheres=64,
-                       output="spheres.bmp"):
-    """The seeded spheres scene as pine scene-file text (parser.cpp grammar)."""
+                       output="spheres.bmp", sampler="Halton", randomize=None, environment=None,
+                       area_light=True):
+    """The seeded spheres scene as pine scene-file text (parser.cpp grammar);
+    `sampler` names pine's sampler type, `randomize` its randomizeStrategy;
+    `environment` (a dict: "type" plus parameters) adds an Environment light,
+    `area_light` False drops the emissive rect."""
     sc = spheres_scene(seed, n_spheres)
     cam = sc["camera"]
     out = [
         "Integrator: Path{",
         f"    maxDepth: {max_depth}",
-        "    sampler: Halton{",
+        f"    sampler: {sampler}{{",
         f"        samplesPerPixel: {spp}",
+    ] + ([f"        randomizeStrategy: {randomize}"] if randomize else []) + [
         "    }",
         "}",
         "Camera: ThinLen{",
@@ -148,8 +153,13 @@ def spheres_scene_text(seed=0, width=512, height=512, spp=64, max_depth=8, n_sph
     g, li = sc["ground"], sc["light"]
     out += ["Shape: Rect{", f"    position: {_v3(g['position'])}", f"    ex: {_v3(g['ex'])}",
             f"    ey: {_v3(g['ey'])}", "    material: ground", "}"]
-    out += ["Shape: Rect{", f"    position: {_v3(li['position'])}", f"    ex: {_v3(li['ex'])}",
-            f"    ey: {_v3(li['ey'])}", "    material: light", "}"]
+    if area_light:
+        out += ["Shape: Rect{", f"    position: {_v3(li['position'])}", f"    ex: {_v3(li['ex'])}",
+                f"    ey: {_v3(li['ey'])}", "    material: light", "}"]
+    if environment:
+        out += ["Light: Environment{", f"    environment: {environment['type']}{{"]
+        out += [f"        {k}: {v}" for k, v in environment.items() if k != "type"]
+        out += ["    }", "}"]
     for k, s in enumerate(sc["spheres"]):
         out += ["Shape: Sphere{", f"    position: {_v3(s['c'])}", f"    radius: {_fmt(s['r'])}",
                 f"    material: m{k}", "}"]

@@ -15,14 +15,23 @@ Cases
               12,288 rays
   pstd        pine's libm on fixed grids
   halton      pine's Halton sampler values for a spread of pixels
+  sobol       pine's Sobol sampler values (four scramblers, several film sizes,
+              one run past dimension 1024)
+  atmosphere  pine's Atmosphere colour table and sun colour for two suns
   render_spheres  the seeded spheres scene (64 spheres, area light) at
               96x96, 256 spp, maxDepth 8, rendered by pine's PathIntegrator
+  render_sobol    the spheres scene under the Sobol sampler (fastOwen 96x96 256 spp,
+              Owen 40x24 128 spp)
+  render_atmosphere  the spheres scene under pine's Atmosphere light: two cases
+              with the accurate colour table swapped into pine's light, one
+              with pine's own (NaN) table
   render_mixed    point + directional + area lights, layered materials, a
               rough dielectric, an OBJ mesh, thin lens, Box filter (96x72, 256 spp)
   render_cloud    a 32^3 fBm density grid (GridMedium, delta + ratio tracking,
               HG g = 0.85), camera in the medium, Sky environment light, a
               material-less rect crossed by paths and shadow rays
               (64x64, 512 spp, maxDepth 16)
+  render_randomwalk_cloud  pine's RandomWalk integrator in the cloud scene (48x48, 256 spp)
   slab_through, slab_shadow   constant-density grid scenes whose expectation
               has a closed form (transmittance exp(-sigma d L)), 48x48, 1024 spp
 
@@ -193,6 +202,38 @@ def case_halton():
     print("halton:", ", ".join(out), flush=True)
 
 
+def case_sobol():
+    out = {}
+    cases = [(512, 512, 16, 24, 32, 2), (64, 48, 8, 12, 32, 0), (64, 48, 8, 12, 32, 1),
+             (64, 48, 8, 12, 32, 2), (64, 48, 8, 12, 32, 3), (1024, 96, 4, 8, 16, 2),
+             (1, 1, 8, 6, 1, 2), (33, 17, 2, 1100, 2, 2)]
+    for (w, h, spp, nd, npix, strat) in cases:
+        with tempfile.TemporaryDirectory() as d:
+            p = os.path.join(d, "s.bin")
+            subprocess.run([HARNESS, "sobol", p, str(w), str(h), str(spp), str(nd), "3", "1",
+                            str(npix), str(strat)], check=True, capture_output=True)
+            out[f"{w}x{h}_spp{spp}_d{nd}_n{npix}_s{strat}"] = np.fromfile(
+                p, dtype=np.float32).reshape(npix, spp, nd)
+    np.savez_compressed(os.path.join(OUT, "sobol.npz"), **out)
+    print("sobol:", ", ".join(out), flush=True)
+
+
+def case_atmosphere():
+    out = {}
+    for (sun, col, size) in [((2, 6, 3), (1, 1, 1), (64, 32)), ((-1, 0.3, 0.5), (1.5, 1.2, 0.9), (48, 40))]:
+        with tempfile.TemporaryDirectory() as d:
+            p = os.path.join(d, "a.bin")
+            subprocess.run([HARNESS, "atmosphere", p, *map(str, sun), *map(str, col),
+                            *map(str, size)], check=True, capture_output=True)
+            a = np.fromfile(p, dtype=np.float32)
+        key = "sun_" + "_".join(f"{x:g}" for x in sun)
+        out[key + "_table"] = a[:-3].reshape(size[1], size[0], 3)
+        out[key + "_sun"] = a[-3:]
+        out[key + "_args"] = np.array([*sun, *col], np.float32)
+    np.savez_compressed(os.path.join(OUT, "atmosphere.npz"), **out)
+    print("atmosphere:", ", ".join(out), flush=True)
+
+
 def run_render(scene_text, directory):
     path = os.path.join(directory, "scene.txt")
     with open(path, "w") as f:
@@ -215,6 +256,79 @@ def case_render_spheres():
     np.savez_compressed(os.path.join(OUT, "render_spheres.npz"), scene=np.bytes_(text), film=film,
                         harness=np.bytes_(line))
     print("render_spheres:", line, flush=True)
+
+
+def case_render_sobol():
+    """The spheres scene under pine's Sobol sampler: fastOwen (the default) at
+    96x96 (m = 7) and Owen on a 40x24 film (m = 6), maxDepth 8."""
+    out = {}
+    for name, w, h, spp, rs in [("fastowen", 96, 96, 256, None), ("owen", 40, 24, 128, "Owen")]:
+        text = W.spheres_scene_text(seed=0, width=w, height=h, spp=spp, max_depth=8,
+                                    sampler="Sobol", randomize=rs)
+        with tempfile.TemporaryDirectory() as d:
+            film, line = run_render(text, d)
+        out[name + "_scene"] = np.bytes_(text)
+        out[name + "_film"] = film
+        out[name + "_harness"] = np.bytes_(line)
+        print(f"render_sobol {name}:", line, flush=True)
+    np.savez_compressed(os.path.join(OUT, "render_sobol.npz"), **out)
+
+
+ATMOSPHERE_CASES = {
+    # name: (film w, h, spp, environment block, area light)
+    "accurate_interp": (64, 64, 128, {"type": "Atmosphere", "sunDirection": "2 6 3",
+                                      "sunColor": "1 0.9 0.8", "size": "128 64",
+                                      "interpolate": "true", "accurate": "true"}, False),
+    # with the area light: both lights have zero power, so each is picked with pdf 1/2
+    "accurate_area": (48, 40, 128, {"type": "Atmosphere", "sunDirection": "-1 3 0.5",
+                                    "size": "96 48", "accurate": "true"}, True),
+    "pine_table": (32, 32, 16, {"type": "Atmosphere", "size": "32 16"}, False),
+}
+
+
+def atmosphere_scene_text(name):
+    w, h, spp, env, area = ATMOSPHERE_CASES[name]
+    return W.spheres_scene_text(seed=3, width=w, height=h, spp=spp, max_depth=6, n_spheres=12,
+                                environment=env, area_light=area)
+
+
+def case_render_atmosphere():
+    """The spheres scene lit by pine's Atmosphere environment light.  Pine's own
+    table is NaN in every entry (its approximate exp diverges), so for the
+    `accurate` cases the harness renders with pine's Atmosphere light fed the
+    float64 table of oracle/atmosphere.py; `pine_table` keeps pine's own.
+    (`interpolate: false` is left out: pine's nearest-texel lookup crashes on
+    these scenes.)"""
+    import oracle.atmosphere as A
+    out = {}
+    for name, (w, h, spp, env, area) in ATMOSPHERE_CASES.items():
+        text = atmosphere_scene_text(name)
+        with tempfile.TemporaryDirectory() as d:
+            extra = []
+            if env.get("accurate") == "true":
+                sx, sy = (int(x) for x in env["size"].split())
+                sun = [float(x) for x in env["sunDirection"].split()]
+                col = [float(x) for x in env.get("sunColor", "1 1 1").split()]
+                table, sun_le = A.atmosphere_table(sun, col, sx, sy)
+                tp = os.path.join(d, "atmosphere.bin")
+                np.concatenate([table.ravel(), sun_le]).astype(np.float32).tofile(tp)
+                extra = ["0", tp]
+            path = os.path.join(d, "scene.txt")
+            with open(path, "w") as f:
+                f.write(text)
+            fp = os.path.join(d, "film.bin")
+            p = subprocess.run([HARNESS, "render", "./scene.txt", fp, "8", *extra],
+                               capture_output=True, text=True, cwd=d)
+            if p.returncode != 0:
+                raise RuntimeError(f"harness render failed: {p.stderr[-2000:]}")
+            line = [l for l in p.stdout.splitlines() if l.startswith("{")][-1]
+            a = np.fromfile(fp, dtype=np.float32)
+            fw, fh = a[:2].view(np.int32)
+        out[name + "_scene"] = np.bytes_(text)
+        out[name + "_film"] = a[2:].reshape(fh, fw, 4)
+        out[name + "_harness"] = np.bytes_(line)
+        print(f"render_atmosphere {name}:", line, flush=True)
+    np.savez_compressed(os.path.join(OUT, "render_atmosphere.npz"), **out)
 
 
 def mixed_mesh():
@@ -253,15 +367,18 @@ def with_integrator(text, kind, extra=""):
 
 
 def case_other_integrators():
-    """pine's AO and Viz (Normal, Position) on the mixed scene and RandomWalk on
-    the spheres scene."""
+    """pine's AO and Viz (Normal, Position, Texcoord, DpDu, DpDv) on the mixed
+    scene and RandomWalk on the spheres scene."""
     v, i = mixed_mesh()
     with tempfile.TemporaryDirectory() as d:
         W.write_obj(os.path.join(d, "blob.obj"), v, i)
         mixed = W.mixed_scene_text("blob.obj", spp=64)
         for name, text in (("render_ao", with_integrator(mixed, "AO")),
                            ("render_viz_normal", with_integrator(mixed, "Viz", "\n    viztype: Normal")),
-                           ("render_viz_position", with_integrator(mixed, "Viz", "\n    viztype: Position"))):
+                           ("render_viz_position", with_integrator(mixed, "Viz", "\n    viztype: Position")),
+                           ("render_viz_texcoord", with_integrator(mixed, "Viz", "\n    viztype: Texcoord")),
+                           ("render_viz_dpdu", with_integrator(mixed, "Viz", "\n    viztype: DpDu")),
+                           ("render_viz_dpdv", with_integrator(mixed, "Viz", "\n    viztype: DpDv"))):
             film, line = run_render(text, d)
             np.savez_compressed(os.path.join(OUT, name + ".npz"), scene=np.bytes_(text), film=film,
                                 harness=np.bytes_(line))
@@ -285,6 +402,19 @@ def cloud_volume(directory):
 
 def slab_volume(directory, density=1.0):
     pa.write_volume(os.path.join(directory, "const8.vol"), np.full((8, 8, 8), density, np.float32))
+
+
+def case_render_randomwalk_cloud():
+    """pine's RandomWalk integrator through the cloud scene's participating
+    medium (camera inside the grid medium, material-less rect, Sky)."""
+    text = with_integrator(W.cloud_scene_text("cloud32.vol", width=48, height=48, spp=256,
+                                              max_depth=16, passthrough=True), "RandomWalk")
+    with tempfile.TemporaryDirectory() as d:
+        cloud_volume(d)
+        film, line = run_render(text, d)
+    np.savez_compressed(os.path.join(OUT, "render_randomwalk_cloud.npz"), scene=np.bytes_(text),
+                        film=film, harness=np.bytes_(line))
+    print("render_randomwalk_cloud:", line, flush=True)
 
 
 def case_render_cloud():
@@ -316,9 +446,14 @@ CASES = {
     "shapes": case_shapes,
     "pstd": case_pstd,
     "halton": case_halton,
+    "sobol": case_sobol,
+    "atmosphere": case_atmosphere,
     "render_spheres": case_render_spheres,
+    "render_sobol": case_render_sobol,
+    "render_atmosphere": case_render_atmosphere,
     "render_mixed": case_render_mixed,
     "render_cloud": case_render_cloud,
+    "render_randomwalk_cloud": case_render_randomwalk_cloud,
     "render_box": case_render_box,
     "other_integrators": case_other_integrators,
     "slab_through": lambda: case_slab("through"),

@@ -71,6 +71,24 @@ def test_cloud_matches_pine(pg, cuda, tmp_path):
     assert e < 1e-3
 
 
+def test_randomwalk_through_media_matches_pine(pg, cuda, tmp_path):
+    """pine's RandomWalk integrator (randomwalk.cpp:6-62) in the cloud scene:
+    delta tracking in the grid medium, phase-function scattering, the
+    material-less rect crossed, the Sky seen with the path throughput alone."""
+    fx = golden("render_randomwalk_cloud")
+    film, info = render(pg, fx["scene"].item().decode(), cloud_dir(tmp_path))
+    assert info["general_mode"] == 1 and info["medium_steps"] > 0
+    ref = fx["film"]
+    np.testing.assert_allclose(film[..., 3], ref[..., 3], rtol=1e-5, atol=1e-6)
+    ours, theirs = linear(film), linear(ref)
+    both = np.isfinite(ours).all(-1) & np.isfinite(theirs).all(-1)
+    e = rel_mse(ours[both], theirs[both])
+    exact = float((film[..., :3] == ref[..., :3]).all(-1).mean())
+    print(f"RandomWalk cloud relMSE {e:.3e}, bit-exact pixels {exact:.4f}")
+    assert both.mean() > 0.99
+    assert e < 1e-3
+
+
 def test_cloud_bricks_agree_with_pine(pg, cuda, tmp_path):
     """8-voxel majorant bricks on the 32^3 grid: a different sample sequence,
     the same expectation."""

@@ -120,7 +120,8 @@ Camera: ThinLen{
         pg.Scene(text=text)
 
 
-@pytest.mark.parametrize("name", ["render_ao", "render_viz_normal", "render_viz_position"])
+@pytest.mark.parametrize("name", ["render_ao", "render_viz_normal", "render_viz_position",
+                                  "render_viz_texcoord", "render_viz_dpdu", "render_viz_dpdv"])
 def test_ao_and_viz_match_pine(pg, cuda, tmp_path, name):
     """pine's AO and Viz integrators on the mixed scene (same scene file, same
     Halton sequence): the images must match pine's films."""
@@ -141,3 +142,32 @@ def test_randomwalk_matches_pine(pg, cuda, tmp_path):
     film, info = render_text(pg, fx["scene"].item().decode(), str(tmp_path))
     e = check_film(film, fx["film"], 1e-3)
     print(f"randomwalk relMSE {e:.3e}")
+
+
+def test_viz_bvh_heat_map_is_on_the_colour_map(pg, cuda, tmp_path):
+    """Viz Bvh (viz.cpp:33): ColorMap(2 * nodes visited / 200) per camera ray,
+    misses included.  At one sample per pixel every pixel is a point of the
+    colour map's polyline black -> (0,0,0.5) -> green -> red, and below the red
+    end it decodes to a whole number of nodes.  The counts are of this
+    library's hierarchy, so pine's own heat map (a different tree) is no
+    reference.  (The spheres scene: an infinite Plane, as in the mixed scene,
+    puts a 2e6-wide box on the path of every ray and saturates most pixels.)"""
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    import make_golden as mg
+    from pine_amd import workloads as W
+    text = mg.with_integrator(W.spheres_scene_text(seed=0, width=96, height=72, spp=1), "Viz",
+                              "\n    viztype: Bvh")
+    film, _ = render_text(pg, text, str(tmp_path))
+    c = linear(film).reshape(-1, 3).astype(np.float64)
+    r, g, b = c[:, 0], c[:, 1], c[:, 2]
+    seg1 = (r == 0) & (g == 0) & (b <= 0.5 + 1e-6)          # [0, 1/3): (0, 0, 1.5 v)
+    seg2 = (r == 0) & np.isclose(b, 0.5 * (1 - g), atol=1e-5)  # [1/3, 2/3): green rises
+    seg3 = (b == 0) & np.isclose(r + g, 1, atol=1e-5)          # [2/3, 1]: red rises
+    assert (seg1 | seg2 | seg3).all()
+    heat = np.where(seg1, b / 1.5, np.where(seg3 & (r > 0), 2 / 3 + r / 3, 1 / 3 + g / 3))
+    below = heat < 1 - 1e-6
+    assert below.mean() > 0.5 and heat.max() > 0.1
+    nodes = heat[below] * 100
+    assert np.abs(nodes - np.round(nodes)).max() < 2e-3
+    print(f"Viz Bvh: mean nodes {np.mean(np.minimum(heat, 1)) * 100:.1f}, saturated {1 - below.mean():.3f}")

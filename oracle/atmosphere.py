@@ -6,7 +6,7 @@ generator; never called by the library).
 Pine computes the same table (Atmosphere::Atmosphere, light.cpp:81-92, and
 AtmosphereColor, color.cpp:37-89) with its own approximate libm; there its
 8-term Taylor exp of large negative arguments diverges and every entry comes
-out NaN, which is why the accurate table exists.  Pins: the closed forms in
+out NaN or +inf, which is why the accurate table exists.  Pins: the closed forms in
 tests/test_atmosphere.py (the Rayleigh phase integrates to 1 over the sphere,
 the per-channel ratio of an optically thin zenith path is betaR*phaseR*...,
 the table's texel directions are UniformSphereSampling's) and the harness

@@ -2338,8 +2338,8 @@ void build_scene(pg_scene& S) {
                 // Atmosphere::Create (light.cpp:152-156) defaults.  `accurate`
                 // is this library's extension (pine ignores unknown keys): the
                 // colour table with correctly rounded exp/sqrt/pow instead of
-                // pine's approximations, whose exp diverges here and makes every
-                // entry of pine's own table NaN.
+                // pine's approximations, whose exp diverges here and leaves no
+                // finite entry in pine's own table (NaN or +inf).
                 const float dsun[3] = {2.0f, 6.0f, 3.0f}, dcol[3] = {1.0f, 1.0f, 1.0f};
                 const int dsize[2] = {512, 512};
                 int sz[2];

@@ -24,7 +24,7 @@ Cases
               Owen 40x24 128 spp)
   render_atmosphere  the spheres scene under pine's Atmosphere light: two cases
               with the accurate colour table swapped into pine's light, one
-              with pine's own (NaN) table
+              with pine's own (non-finite) table
   render_mixed    point + directional + area lights, layered materials, a
               rough dielectric, an OBJ mesh, thin lens, Box filter (96x72, 256 spp)
   render_cloud    a 32^3 fBm density grid (GridMedium, delta + ratio tracking,
@@ -294,7 +294,7 @@ def atmosphere_scene_text(name):
 
 def case_render_atmosphere():
     """The spheres scene lit by pine's Atmosphere environment light.  Pine's own
-    table is NaN in every entry (its approximate exp diverges), so for the
+    table has no finite entry (its approximate exp diverges), so for the
     `accurate` cases the harness renders with pine's Atmosphere light fed the
     float64 table of oracle/atmosphere.py; `pine_table` keeps pine's own.
     (`interpolate: false` is left out: pine's nearest-texel lookup crashes on

@@ -1,7 +1,7 @@
 """Pine's Atmosphere environment light on the GPU against pine's renders of the
 same scene text (tests/golden/render_atmosphere.npz).
 
-Pine's own colour table is NaN in every entry (its approximate exp diverges),
+Pine's own colour table has no finite entry (NaN or +inf: its approximate exp diverges),
 so pine's film is NaN wherever the environment contributes; the GPU reproduces
 that with pine's arithmetic.  With `accurate: true` the GPU computes the table
 in double precision with correctly rounded functions; the goldens for those

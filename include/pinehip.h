@@ -379,6 +379,9 @@ typedef struct pg_guided_options {
     int32_t guide_depth;    /* vertices of depth < guide_depth use the guide (0 = all); deeper
                                ones sample their BSDF and cost no network evaluation */
     int32_t flags;          /* PG_RENDER_* bits for the final render */
+    int32_t keep_passes;    /* the last keep_passes training passes stay in the final film (0 =
+                               none): each is an unbiased guided render of its own sample range,
+                               so the film holds train_spp * keep_passes + final_spp samples */
 } pg_guided_options;
 int pg_render_guided(pg_scene* s, pg_guide* g, const pg_guided_options* o, void* stream);
 

@@ -79,7 +79,7 @@ class _GuidedOptions(ctypes.Structure):
                 ("steps_per_pass", ctypes.c_int32), ("batch", ctypes.c_int32),
                 ("record_depth", ctypes.c_int32), ("final_spp", ctypes.c_int32),
                 ("sample_offset", ctypes.c_int32), ("guide_depth", ctypes.c_int32),
-                ("flags", ctypes.c_int32)]
+                ("flags", ctypes.c_int32), ("keep_passes", ctypes.c_int32)]
 
 
 class _RenderOptions(ctypes.Structure):
@@ -386,10 +386,12 @@ class Scene:
         return {k: getattr(r, k) for k, _ in _TuneResult._fields_}
 
     def render_guided(self, guide, train_passes=16, train_spp=8, steps_per_pass=0, batch=1024,
-                      record_depth=5, final_spp=0, sample_offset=0, guide_depth=0, graph=True):
-        """Trains `guide` on this scene, then renders final_spp with it (pg_render_guided)."""
+                      record_depth=5, final_spp=0, sample_offset=0, guide_depth=0, graph=True,
+                      keep_passes=0):
+        """Trains `guide` on this scene, then renders final_spp with it (pg_render_guided);
+        the last keep_passes training passes stay in the film."""
         o = _GuidedOptions(train_passes, train_spp, steps_per_pass, batch, record_depth, final_spp,
-                           sample_offset, guide_depth, 0 if graph else RENDER_NO_GRAPH)
+                           sample_offset, guide_depth, 0 if graph else RENDER_NO_GRAPH, keep_passes)
         check(lib().pg_render_guided(self.h, guide.h, ctypes.byref(o), _stream_handle()))
         return self.info()
 

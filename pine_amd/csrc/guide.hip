@@ -325,15 +325,27 @@ __global__ __launch_bounds__(256) void k_guide_grad(const float* __restrict__ pa
 }
 
 // Sums the partial rows (fixed order), then Adam on live parameters.
-__global__ void k_guide_adam(float* params, float* m, float* v, float* grad, const float* partials,
-                             int nblocks, float lr, float bc1, float bc2, int apply, __bf16* wf,
-                             __bf16* wt) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= kParams)
+// Sums the per-workgroup partial gradients and takes one Adam step.  A block
+// holds 16 parameters x 16 row groups: group g sums rows g, g + 16, ... of its
+// parameter (reads of 16 consecutive parameters per row), then the 16 group
+// sums are added in group order, so the result is the same on every run.
+__global__ __launch_bounds__(256) void k_guide_adam(float* params, float* m, float* v, float* grad,
+                                                    const float* partials, int nblocks, float lr, float bc1,
+                                                    float bc2, int apply, __bf16* wf, __bf16* wt) {
+    __shared__ float part[16][17];
+    const int pl = threadIdx.x & 15, rg = threadIdx.x >> 4;
+    const int p = blockIdx.x * 16 + pl;
+    float s = 0.0f;
+    if (p < kParams)
+        for (int b = rg; b < nblocks; b += 16)
+            s += partials[(size_t)b * kParams + p];
+    part[rg][pl] = s;
+    __syncthreads();
+    if (rg != 0 || p >= kParams)
         return;
     float gsum = 0.0f;
-    for (int b = 0; b < nblocks; b++)
-        gsum += partials[(size_t)b * kParams + p];
+    for (int g = 0; g < 16; g++)
+        gsum += part[g][pl];
     if (!live_param(p))
         gsum = 0.0f;
     grad[p] = gsum;
@@ -493,7 +505,7 @@ int guide_train_step(pg_guide* G, const float* x, const float* dir, const float*
     G->step++;
     const float bc1 = 1.0f - std::pow(0.9f, (float)G->step);
     const float bc2 = 1.0f - std::pow(0.999f, (float)G->step);
-    hipLaunchKernelGGL(k_guide_adam, dim3((kParams + 255) / 256), dim3(256), 0, st, G->params,
+    hipLaunchKernelGGL(k_guide_adam, dim3((kParams + 15) / 16), dim3(256), 0, st, G->params,
                        G->adam_m, G->adam_v, G->grad, G->partials, blocks, G->lr, bc1, bc2, 1,
                        G->wf, G->wt);
     PG_HIP(hipGetLastError());
@@ -607,7 +619,7 @@ extern "C" int pg_guide_gradient(pg_guide* G, const float* d_inputs, const float
     Batch B{d_inputs, d_dirs, d_weight, d_pbsdf, n, 0, n, 1.0f / (float)n, 1, G->oct};
     hipLaunchKernelGGL(k_guide_grad, dim3(blocks), dim3(256), 0, 0, G->params, G->wf, G->wt, B,
                        G->partials, G->loss_part);
-    hipLaunchKernelGGL(k_guide_adam, dim3((kParams + 255) / 256), dim3(256), 0, 0, G->params,
+    hipLaunchKernelGGL(k_guide_adam, dim3((kParams + 15) / 16), dim3(256), 0, 0, G->params,
                        G->adam_m, G->adam_v, G->grad, G->partials, blocks, 0.0f, 1.0f, 1.0f, 0,
                        G->wf, G->wt);
     PG_HIP(hipGetLastError());

@@ -1932,6 +1932,7 @@ struct pg_scene {
         }
     };
     bool use_graph = true;
+    bool keep_film = false;  // the next render adds to the film instead of clearing it
     int integrator = 0;                  // kIntPath / kIntAO / kIntViz / kIntRandomWalk
     bool spectral = false;               // this render in spectral mode (PG_RENDER_SPECTRAL)
     int64_t spectral_capacity = 0;
@@ -3420,7 +3421,8 @@ static int render_enqueue(pg_scene* S, int spp, int spb_req, hipStream_t st, Gui
     PG_HIP(hipEventCreate(&t->e0));
     PG_HIP(hipEventCreate(&t->e1));
     PG_HIP(hipEventRecord(t->e0, st));
-    PG_HIP(hipMemsetAsync(S->d_film, 0, (size_t)npix * 4 * sizeof(float), st));
+    if (!S->keep_film)
+        PG_HIP(hipMemsetAsync(S->d_film, 0, (size_t)npix * 4 * sizeof(float), st));
     PathsDev P = S->paths;
     P.rec_depth = tr ? tr->rec_depth : 0;
     if (!S->spectral)
@@ -3876,6 +3878,7 @@ extern "C" int pg_render_guided(pg_scene* S, pg_guide* G, const pg_guided_option
     const int depth = o && o->record_depth > 0 ? o->record_depth : 5;
     const int final_spp = o && o->final_spp > 0 ? o->final_spp : S->spp;
     const int offset = o && o->sample_offset > 0 ? o->sample_offset : 0;
+    const int keep = o && o->keep_passes > 0 ? std::min(o->keep_passes, passes) : 0;
     SceneDev& d = S->dev;
     d.g_depth = o && o->guide_depth > 0 ? o->guide_depth : (1 << 30);
     d.g_params = guide_params_dev(G);
@@ -3894,16 +3897,22 @@ extern "C" int pg_render_guided(pg_scene* S, pg_guide* G, const pg_guided_option
     const int nbatches = (int)((train_spp + per_batch - 1) / per_batch);
     GuideTraining tr{G, depth, steps > 0 ? std::max(1, (steps + nbatches - 1) / nbatches) : 0, batch, 0, 0};
     int rc = PG_OK;
+    // The last `keep` training passes stay in the film: each is an unbiased
+    // guided render of its own sample range, and with the final render they
+    // cover one contiguous range of sample indices.
     for (int p = 0; p < passes && rc == PG_OK; p++) {
+        S->keep_film = p > passes - keep;
         rc = render_impl(S, train_spp, 0, st, &tr, &ms, &rays, offset + p * train_spp);
         train_ms += ms;
         all_rays += rays;
     }
     if (rc == PG_OK) {
         S->use_graph = !(o && (o->flags & PG_RENDER_NO_GRAPH));
+        S->keep_film = keep > 0;
         rc = render_impl(S, final_spp, 0, st, nullptr, &ms, &rays, offset + passes * train_spp);
         S->use_graph = true;
     }
+    S->keep_film = false;
     d.guided = 0;
     if (rc != PG_OK)
         return rc;

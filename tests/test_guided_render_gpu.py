@@ -117,3 +117,35 @@ def test_guide_learns_the_gap(pg, cuda, box):
     assert (out[:, 1] > 0.2).all()
     g.release()
     s.release()
+
+
+def test_kept_training_passes_are_the_same_samples(pg, cuda, box):
+    """keep_passes: the training passes' samples stay in the final film.  Film
+    weights depend only on each sample's film position (pine's filter table at
+    the jittered point, from sampler dimensions 2 and 3), so a guided render
+    keeping every pass holds exactly the weights of a plain render over the same
+    contiguous sample range; and its image is an unbiased guided estimate, close
+    to the one without the kept passes."""
+    from pine_amd import metrics as M
+    s = scene(pg, box, 128)
+    passes, tspp, fspp, off = 4, 2, 16, 1000
+    s.render(spp=passes * tspp + fspp, sample_offset=off)
+    plain = s.film()
+    g = pg.Guide(seed=3, learning_rate=3e-3)
+    s.render_guided(g, train_passes=passes, train_spp=tspp, final_spp=fspp, sample_offset=off,
+                    keep_passes=passes)
+    kept = s.film()
+    g.release()
+    g = pg.Guide(seed=3, learning_rate=3e-3)
+    s.render_guided(g, train_passes=passes, train_spp=tspp, final_spp=fspp, sample_offset=off)
+    final_only = s.film()
+    g.release()
+    s.release()
+    assert np.array_equal(kept[..., 3], plain[..., 3])
+    assert not np.array_equal(final_only[..., 3], plain[..., 3])
+    a, b = M.linear(kept), M.linear(final_only)
+    ok = np.isfinite(a).all(-1) & np.isfinite(b).all(-1)
+    assert ok.mean() > 0.999
+    ma, mb = a[ok].mean(), b[ok].mean()
+    print(f"image means: kept passes {ma:.5f}, final only {mb:.5f}")
+    assert abs(ma - mb) < 0.1 * mb

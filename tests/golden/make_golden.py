@@ -20,6 +20,7 @@ Cases
   atmosphere  pine's Atmosphere colour table and sun colour for two suns
   render_spheres  the seeded spheres scene (64 spheres, area light) at
               96x96, 256 spp, maxDepth 8, rendered by pine's PathIntegrator
+  render_4096     pine at 4096 spp: the spheres scene (64x64) and the mixed scene (96x72)
   render_sobol    the spheres scene under the Sobol sampler (fastOwen 96x96 256 spp,
               Owen 40x24 128 spp)
   render_atmosphere  the spheres scene under pine's Atmosphere light: two cases
@@ -346,6 +347,26 @@ def case_render_mixed():
     print("render_mixed:", line, flush=True)
 
 
+def case_render_4096():
+    """pine's PathIntegrator at 4096 spp: the spheres scene at 64x64 and the
+    mixed scene (point, directional and area lights, layered materials, a rough
+    dielectric, an OBJ mesh, thin lens, Box filter) at 96x72."""
+    out = {}
+    text = W.spheres_scene_text(seed=0, width=64, height=64, spp=4096, max_depth=8)
+    with tempfile.TemporaryDirectory() as d:
+        film, line = run_render(text, d)
+    out.update(spheres_scene=np.bytes_(text), spheres_film=film, spheres_harness=np.bytes_(line))
+    print("render_4096 spheres:", line, flush=True)
+    v, i = mixed_mesh()
+    with tempfile.TemporaryDirectory() as d:
+        W.write_obj(os.path.join(d, "blob.obj"), v, i)
+        text = W.mixed_scene_text("blob.obj", spp=4096)
+        film, line = run_render(text, d)
+    out.update(mixed_scene=np.bytes_(text), mixed_film=film, mixed_harness=np.bytes_(line))
+    print("render_4096 mixed:", line, flush=True)
+    np.savez_compressed(os.path.join(OUT, "render_4096.npz"), **out)
+
+
 def case_render_box():
     """The occluded-light box with its 512,000-triangle clutter at 128^2, 2048
     spp, depth 8: pine's film, non-finite pixels included (pine's estimator
@@ -450,6 +471,7 @@ CASES = {
     "atmosphere": case_atmosphere,
     "render_spheres": case_render_spheres,
     "render_sobol": case_render_sobol,
+    "render_4096": case_render_4096,
     "render_atmosphere": case_render_atmosphere,
     "render_mixed": case_render_mixed,
     "render_cloud": case_render_cloud,

@@ -65,6 +65,23 @@ def test_mixed_scene_matches_pine(pg, cuda, tmp_path):
     print(f"mixed relMSE {e:.3e}")
 
 
+@pytest.mark.parametrize("name", ["spheres", "mixed"])
+def test_4096_spp_images_match_pine(pg, cuda, tmp_path, name):
+    """The bar of the path: with pine's per-pixel sample sequence, images at
+    4096 spp match pine's own CPU render within relMSE < 1e-3."""
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    import make_golden as mg
+    from pine_amd import workloads as W
+    fx = golden("render_4096")
+    if name == "mixed":
+        v, i = mg.mixed_mesh()
+        W.write_obj(str(tmp_path / "blob.obj"), v, i)
+    film, info = render_text(pg, fx[name + "_scene"].item().decode(), str(tmp_path))
+    e = check_film(film, fx[name + "_film"], 1e-3)
+    print(f"{name} at 4096 spp: relMSE {e:.3e} vs pine")
+
+
 def test_occluded_box_matches_pine(pg, cuda, tmp_path):
     """The path-guiding scene (512,000 clutter triangles, light behind a
     partition) rendered plain: pine's film at 2048 spp, depth 8.  pine's

@@ -54,6 +54,7 @@ def lib():
         L.po_bvh_trace.restype = ctypes.c_double
         L.po_halton_values.argtypes = [ctypes.c_int] * 7 + [vp]
         L.po_sobol_values.argtypes = [ctypes.c_int] * 8 + [vp]
+        L.po_noise.argtypes = [vp, vp, vp, i64, vp]
         _lib = L
     return _lib
 
@@ -101,6 +102,17 @@ def sobol_values(w, h, spp, ndims, px0, py0, npix, strategy):
     out = np.zeros(npix * spp * ndims, np.float32)
     lib().po_sobol_values(w, h, spp, ndims, px0, py0, npix, strategy, _p(out))
     return out.reshape(npix, spp, ndims)
+
+
+def noise(points, freq, octaves):
+    """pine's PerlinNoise (seeds 0..2), Turbulence and Turbulence3D at points
+    [n, 3]: returns [n, 7] (see pinecpu.cpp)."""
+    pts = np.ascontiguousarray(points, np.float32)
+    fr = np.ascontiguousarray(freq, np.float32)
+    oc = np.ascontiguousarray(octaves, np.float32)
+    out = np.zeros((len(pts), 8), np.float32)
+    lib().po_noise(_p(pts), _p(fr), _p(oc), len(pts), _p(out))
+    return out[:, :7]
 
 
 class CpuBVH:

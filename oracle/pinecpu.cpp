@@ -11,6 +11,9 @@
 //   - po_bvh_*             : same answers as po_trace_brute (any valid tree gives
 //                            the same closest t)
 //   - po_halton_values     : golden vectors of pine's HaltonSampler (harness)
+//   - po_noise             : pine's PerlinNoise / Turbulence / Turbulence3D at 2,000
+//                            points (harness golden); closed form: gradient noise
+//                            is 1/2 at every lattice point
 //   - po_sobol_values      : golden vectors of pine's SobolSampler (harness), all
 //                            four scramblers; the per-pixel index is found by
 //                            search (closed form: its first two Sobol dimensions
@@ -833,7 +836,104 @@ struct Sobol {
     }
 };
 
+// ---- Perlin noise (noise.cpp:7-52) -----------------------------------------------
+// Corner gradients come from pine's RNG (rng.h:62-105) seeded with Hash(vec3i
+// corner, seed) (rng.h:17-29): the corner's three ints as two little-endian
+// 64-bit words, then the seed's own hash.
+struct PineRng {
+    uint64_t a, b;
+    explicit PineRng(uint64_t seed) {
+        a = SplitMix(seed);
+        b = SplitMix(seed);
+    }
+    static uint64_t SplitMix(uint64_t& s) {  // rng.h:36-41
+        uint64_t r = s += 0x9E3779B97f4A7C15ULL;
+        r = (r ^ (r >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        r = (r ^ (r >> 27)) * 0x94D049BB133111EBULL;
+        return r ^ (r >> 31);
+    }
+    uint64_t Next() {
+        const uint64_t s0 = a;
+        uint64_t s1 = b;
+        const uint64_t r = s0 + s1;
+        s1 ^= s0;
+        a = ((s0 << 24) | (s0 >> 40)) ^ s1 ^ (s1 << 16);
+        b = (s1 << 37) | (s1 >> 27);
+        return r;
+    }
+    float Uniformf() {
+        const uint64_t u = Next();
+        const uint32_t w = (uint32_t)u + (uint32_t)(u >> 32);
+        return std::min(w * 0x1p-32f, 0x1.fffffep-1f);
+    }
+};
+
+float PerlinNoise(V3 q, float frequency, int seed) {
+    q = q * frequency;
+    const float fx = Floor(q.x), fy = Floor(q.y), fz = Floor(q.z);
+    const float w[3] = {q.x - fx, q.y - fy, q.z - fz};
+    float s3[3];
+    for (int k = 0; k < 3; k++)
+        s3[k] = w[k] * w[k] * (3.0f - 2 * w[k]);
+    float ret = 0.0f;
+    for (int x = 0; x < 2; x++)
+        for (int y = 0; y < 2; y++)
+            for (int z = 0; z < 2; z++) {
+                const int32_t cx = (int32_t)(fx + x), cy = (int32_t)(fy + y), cz = (int32_t)(fz + z);
+                uint64_t h = Hash64u(0 ^ ((uint64_t)(uint32_t)cx | (uint64_t)(uint32_t)cy << 32));
+                h = Hash64u(h ^ (uint64_t)(uint32_t)cz);
+                h = Hash64u(h ^ Hash64u((uint64_t)(uint32_t)seed));
+                PineRng rng(h);
+                const float u0 = rng.Uniformf(), u1 = rng.Uniformf();
+                const float phi = u0 * kPi * 2, theta = u1 * kPi;
+                const V3 g = v3(Sin(theta) * Cos(phi), Sin(theta) * Sin(phi), Cos(theta));
+                const V3 d = v3(s3[0] - x, s3[1] - y, s3[2] - z);
+                // PerlinInterp (vecmath.h:1063-1075): its 1.0 literals make the
+                // weights double
+                const double wt = (x * s3[0] + (1 - x) * (1.0 - s3[0])) * (y * s3[1] + (1 - y) * (1.0 - s3[1])) *
+                                  (z * s3[2] + (1 - z) * (1.0 - s3[2]));
+                ret = (float)(ret + wt * Dot(g, d));
+            }
+    return 0.5f * (1.0f + ret);
+}
+
+float Turbulence(V3 q, float frequency, int octaves) {
+    float acc = 0.0f, wgt = 1.0f;
+    for (int i = 0; i < octaves; i++) {
+        acc += wgt * PerlinNoise(q, frequency, 0);
+        wgt *= 0.5f;
+        q = q * 2.0f;
+    }
+    const float r = acc / (2.0f - wgt * 2);
+    return r * r;
+}
+
+V3 Turbulence3D(V3 q, float frequency, int octaves) {
+    V3 acc = v3(0, 0, 0);
+    float wgt = 1.0f;
+    for (int i = 0; i < octaves; i++) {
+        acc = acc + v3(PerlinNoise(q, frequency, 0), PerlinNoise(q, frequency, 1), PerlinNoise(q, frequency, 2)) * wgt;
+        wgt *= 0.5f;
+        q = q * 2.0f;
+    }
+    const V3 r = acc / (2.0f - wgt * 2);
+    return v3(r.x * r.x, r.y * r.y, r.z * r.z);
+}
+
 extern "C" {
+
+// Noise at points: out[8 i + 0..7] = perlin seeds 0..2, turbulence, turbulence3d xyz.
+void po_noise(const float* pts, const float* freq, const float* octaves, int64_t n, float* out) {
+    for (int64_t i = 0; i < n; i++) {
+        const V3 q = v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+        const V3 t3 = Turbulence3D(q, freq[i], (int)octaves[i]);
+        const float v[8] = {PerlinNoise(q, freq[i], 0), PerlinNoise(q, freq[i], 1), PerlinNoise(q, freq[i], 2),
+                            Turbulence(q, freq[i], (int)octaves[i]), t3.x, t3.y, t3.z};
+        for (int k = 0; k < 7; k++)
+            out[8 * i + k] = v[k];
+        out[8 * i + 7] = 0.0f;
+    }
+}
 
 // Sobol values in the harness's layout (as po_halton_values).
 void po_sobol_values(int w, int h, int spp, int ndims, int px0, int py0, int npix, int strategy,

@@ -24,6 +24,10 @@
 //   atmosphere <out.bin> <sx> <sy> <sz> <r> <g> <b> <sizeX> <sizeY>
 //       The Atmosphere environment light's colour table and sun colour as its
 //       constructor computes them (light.cpp:81-92): sizeX*sizeY*3 floats, then 3.
+//   noise  <out.bin> <n>
+//       PerlinNoise (seed 0..2), Turbulence and Turbulence3D (noise.cpp:7-52)
+//       at n seeded points in [-8, 8)^3 with frequencies 1..7 and 1..5 octaves:
+//       per point x, y, z, frequency, octaves, perlin0..2, turbulence, t3d xyz.
 //   render <scene.txt> <out.bin> [threads] [sample0] [atmosphere.bin]
 //       With atmosphere.bin (sizeX*sizeY*3 floats, then 3), every copy pine
 //       holds of the scene's Atmosphere light (scene->envLight, scene->lights,
@@ -50,6 +54,7 @@
 #include <core/filter.h>
 #include <util/fileio.h>
 #include <core/sampler.h>
+#include <core/noise.h>
 #include <core/lowdiscrepancy.h>
 #include <impl/accel/bvh.h>
 #include <util/parameters.h>
@@ -479,6 +484,29 @@ int ModeSobol(int argc, char** argv) {
     return 0;
 }
 
+int ModeNoise(int argc, char** argv) {
+    if (argc < 4)
+        return 2;
+    const int n = atoi(argv[3]);
+    RNG rng(12345);
+    std::vector<float> out;
+    for (int i = 0; i < n; i++) {
+        vec3 p = (rng.Uniform3f() * 2.0f - vec3(1.0f)) * 8.0f;
+        const float freq = (float)(1 + i % 7);
+        const int oct = 1 + (i / 7) % 5;
+        vec3 t3 = Turbulence3D(p, freq, oct);
+        const float vals[12] = {p.x, p.y, p.z, freq, (float)oct, PerlinNoise(p, freq, 0),
+                                PerlinNoise(p, freq, 1), PerlinNoise(p, freq, 2), Turbulence(p, freq, oct),
+                                t3.x, t3.y, t3.z};
+        out.insert(out.end(), vals, vals + 12);
+    }
+    FILE* f = fopen(argv[2], "wb");
+    fwrite(out.data(), sizeof(float), out.size(), f);
+    fclose(f);
+    printf("{\"floats\": %zu}\n", out.size());
+    return 0;
+}
+
 int ModeAtmosphere(int argc, char** argv) {
     if (argc < 11)
         return 2;
@@ -638,6 +666,8 @@ int main(int argc, char** argv) {
         return ModeSobol(argc, argv);
     if (mode == "atmosphere")
         return ModeAtmosphere(argc, argv);
+    if (mode == "noise")
+        return ModeNoise(argc, argv);
     if (mode == "render")
         return ModeRender(argc, argv);
     fprintf(stderr, "unknown mode\n");

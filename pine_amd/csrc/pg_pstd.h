@@ -36,6 +36,13 @@ PG_HD float ceil(float v) {
     return (float)i;
 }
 
+// math.h:128-133 (cast-based fract; v - trunc(v) moved into [0, 1))
+PG_HD float fract(float v) {
+    v = v - (float)(int32_t)v;
+    v = v + 1.0f;
+    return v - (float)(int32_t)v;
+}
+
 // math.h:141-151: four Newton steps from 2^(e/2).
 PG_HD float sqrt(float y) {
     float x = exp2i(ieeeexp(y) / 2);

@@ -316,6 +316,9 @@ struct Layer {
     f3 albedo;
     float roughness;
     float eta;
+    // Node-graph programs for the inputs (offsets into the scene's program
+    // table, -1: the constant above).
+    int32_t prog_albedo, prog_roughness, prog_eta;
 };
 
 enum MaterialKind : int32_t { kMatNone = -1, kMatLayered = 0, kMatEmissive = 1 };
@@ -326,6 +329,8 @@ struct MaterialDev {
     int32_t nlayers;
     Layer layers[kMaxLayers];  // evaluation order: descending layer index
     f3 emission;
+    int32_t has_nodes;  // some input is a node-graph program
+    int32_t prog_bump;  // bumpMap program, -1 = none
 };
 
 struct BSDFSample {

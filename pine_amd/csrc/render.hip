@@ -19,6 +19,7 @@
 #include "pg_guide.h"
 #include "pg_isect.h"
 #include "pg_medium.h"
+#include "pg_nodes.h"
 #include "pg_spectral.h"
 #include "pg_shade.h"
 #include "pg_sobol_dirnums.h"
@@ -110,6 +111,7 @@ struct SceneDev {
     int32_t g_depth;  // vertices of depth < g_depth sample from the guide
     spec::Tables spec;  // spectral mode: RGB <-> spectrum tables
     int32_t viz_type;   // Viz integrator: VizType
+    const nodes::Inst* node_prog;  // material node-graph programs (MaterialDev::has_nodes)
     const float* g_params;  // FP32 parameters (biases used in FP32)
     const __bf16* g_wf;     // BF16 forward weights
     f3 g_bmin, g_binv;      // scene bounds: network position input = (p - bmin) * binv
@@ -361,6 +363,68 @@ __device__ Hit surface(const SceneDev& s, int32_t prim, float t, float u, float 
         h.n = sh.n;
     }
     return h;
+}
+
+// Surface parameterisation of a hit, as the shapes' Intersect members fill
+// Interaction::uv, dpdu, dpdv: triangles (geometry.h:292, 311-314 via
+// bvh.cpp:695-697) barycentric (u, v) and the edges v1 - v0, v2 - v0;
+// spheres (geometry.cpp:140-145) (phi, theta) of the normal and the spherical
+// tangents; rects (geometry.cpp:191-193) (u, v) + 1/2 and the unit edges;
+// planes (geometry.cpp:104-107) the hit point's coordinates along u, v.
+__device__ void surface_param(const SceneDev& s, int32_t prim, const Hit& h, float t, float u, float v,
+                              f3 o, f3 d, f3& uv, f3& dpdu, f3& dpdv) {
+    const ShapeDev& sh = s.shapes[h.shape];
+    if (sh.type == kShapeMesh) {
+        const float* tv = s.tri_verts + 9 * (int64_t)(sh.tri_offset + (prim - sh.first_prim));
+        const f3 v0 = mk(tv[0], tv[1], tv[2]), v1 = mk(tv[3], tv[4], tv[5]), v2 = mk(tv[6], tv[7], tv[8]);
+        uv = mk(u, v, 0.0f);
+        dpdu = v1 - v0;
+        dpdv = v2 - v0;
+    } else if (sh.type == kShapeSphere) {
+        // CartesianToSpherical (vecmath.h:1243-1250)
+        float phi = pstd::atan2(h.n.y, h.n.x);
+        phi = phi < 0.0f ? kPi * 2.0f + phi : phi;
+        const float theta = pstd::acos(h.n.z);
+        const float st = pstd::sin(theta), ct = pstd::cos(theta);
+        const float sp = pstd::sin(phi), cp = pstd::cos(phi);
+        dpdu = mk(st * -sp, st * cp, 0.0f);
+        dpdv = mk(ct * cp, ct * sp, -st);
+        uv = mk(phi, theta, 0.0f);
+    } else if (sh.type == kShapeRect) {
+        uv = mk(u, v, 0.0f);
+        dpdu = sh.e0;
+        dpdv = sh.e1;
+    } else {
+        const f3 p = o + t * d;
+        uv = mk(dot(p - sh.p0, sh.e0), dot(p - sh.p0, sh.e1), 0.0f);
+        dpdu = sh.e0;
+        dpdv = sh.e1;
+    }
+}
+
+// A material with node-graph inputs, made concrete at a surface point: the
+// bump map first replaces the shading normal (Material::BumpNormal,
+// material.cpp:49-63, as PathIntegrator::Li does before anything else,
+// path.cpp:48), then each input program is evaluated with NodeEvalCtx (p, the
+// bumped n, uv) (material.h:14-33) into the layer's constant.
+__device__ void apply_nodes(const SceneDev& s, MaterialDev& m, Hit& h, int32_t prim, float t, float u, float v,
+                            f3 o, f3 d) {
+    f3 uv, dpdu, dpdv;
+    surface_param(s, prim, h, t, u, v, o, d, uv, dpdu, dpdv);
+    nodes::Ctx c{h.p, h.n, uv.x, uv.y};
+    if (m.prog_bump >= 0) {
+        h.n = nodes::bump_normal(s.node_prog + m.prog_bump, c, dpdu, dpdv);
+        c.n = h.n;
+    }
+    for (int i = 0; i < m.nlayers; i++) {
+        Layer& l = m.layers[i];
+        if (l.prog_albedo >= 0)
+            l.albedo = nodes::eval(s.node_prog + l.prog_albedo, c);
+        if (l.prog_roughness >= 0)
+            l.roughness = nodes::eval(s.node_prog + l.prog_roughness, c).x;
+        if (l.prog_eta >= 0)
+            l.eta = nodes::eval(s.node_prog + l.prog_eta, c).x;
+    }
 }
 
 // EmissiveMaterial::Le through MaterialEvalCtx (material.h:14-33, 60-65): the
@@ -619,7 +683,9 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev s, PathsDev P, int32_t n
             nextO = offset_ray_origin(h.p, face_same_hemisphere(h.n, rd));
             alive = true;
         } else {
-            const MaterialDev& m = s.materials[sh.material];
+            MaterialDev m = s.materials[sh.material];
+            if (m.has_nodes)
+                apply_nodes(s, m, h, hit.prim, hit.t, uv.u, uv.v, ro, rd);
             const f3 wi = -rd;
             if (m.kind == kMatEmissive) {
                 f3 le = emitted(m, h.n, wi);
@@ -795,43 +861,6 @@ __global__ __launch_bounds__(256) void k_resolve_ao(PathsDev P, int32_t n_shadow
 
 // VizIntegrator::Li (viz.cpp:27-41), Position and Normal: the camera ray's
 // hit point or shading normal as the colour, 0 for a miss.
-// Surface parameterisation of a hit, as the shapes' Intersect members fill
-// Interaction::uv, dpdu, dpdv: triangles (geometry.h:292, 311-314 via
-// bvh.cpp:695-697) barycentric (u, v) and the edges v1 - v0, v2 - v0;
-// spheres (geometry.cpp:140-145) (phi, theta) of the normal and the spherical
-// tangents; rects (geometry.cpp:191-193) (u, v) + 1/2 and the unit edges;
-// planes (geometry.cpp:104-107) the hit point's coordinates along u, v.
-__device__ void surface_param(const SceneDev& s, int32_t prim, const Hit& h, float t, float u, float v,
-                              f3 o, f3 d, f3& uv, f3& dpdu, f3& dpdv) {
-    const ShapeDev& sh = s.shapes[h.shape];
-    if (sh.type == kShapeMesh) {
-        const float* tv = s.tri_verts + 9 * (int64_t)(sh.tri_offset + (prim - sh.first_prim));
-        const f3 v0 = mk(tv[0], tv[1], tv[2]), v1 = mk(tv[3], tv[4], tv[5]), v2 = mk(tv[6], tv[7], tv[8]);
-        uv = mk(u, v, 0.0f);
-        dpdu = v1 - v0;
-        dpdv = v2 - v0;
-    } else if (sh.type == kShapeSphere) {
-        // CartesianToSpherical (vecmath.h:1243-1250)
-        float phi = pstd::atan2(h.n.y, h.n.x);
-        phi = phi < 0.0f ? kPi * 2.0f + phi : phi;
-        const float theta = pstd::acos(h.n.z);
-        const float st = pstd::sin(theta), ct = pstd::cos(theta);
-        const float sp = pstd::sin(phi), cp = pstd::cos(phi);
-        dpdu = mk(st * -sp, st * cp, 0.0f);
-        dpdv = mk(ct * cp, ct * sp, -st);
-        uv = mk(phi, theta, 0.0f);
-    } else if (sh.type == kShapeRect) {
-        uv = mk(u, v, 0.0f);
-        dpdu = sh.e0;
-        dpdv = sh.e1;
-    } else {
-        const f3 p = o + t * d;
-        uv = mk(dot(p - sh.p0, sh.e0), dot(p - sh.p0, sh.e1), 0.0f);
-        dpdu = sh.e0;
-        dpdv = sh.e1;
-    }
-}
-
 // ColorMap (color.cpp:25-35): black -> dark blue -> green -> red.
 __device__ f3 color_map(float v) {
     const f3 k0 = mk(0.0f, 0.0f, 0.0f), k1 = mk(0.0f, 0.0f, 0.5f), k2 = mk(0.0f, 1.0f, 0.0f), k3 = mk(1.0f, 0.0f, 0.0f);
@@ -908,14 +937,16 @@ __global__ __launch_bounds__(256) void k_shade_rw(SceneDev s, PathsDev P, int32_
             L = L + beta * env_color(s, rd);
     } else {
         const pg_hit_uv uv = P.uvs[k];
-        const Hit h = surface(s, hit.prim, hit.t, uv.u, uv.v, ro, rd);
+        Hit h = surface(s, hit.prim, hit.t, uv.u, uv.v, ro, rd);
         const ShapeDev& sh = s.shapes[h.shape];
         if (sh.material < 0) {
             nextD = rd;
             nextO = offset_ray_origin(h.p, face_same_hemisphere(h.n, rd));
             alive = true;
         } else {
-            const MaterialDev& m = s.materials[sh.material];
+            MaterialDev m = s.materials[sh.material];
+            if (m.has_nodes)
+                apply_nodes(s, m, h, hit.prim, hit.t, uv.u, uv.v, ro, rd);
             const f3 wi = -rd;
             if (m.kind == kMatEmissive) {
                 L = L + beta * emitted(m, h.n, wi);
@@ -1010,7 +1041,9 @@ __global__ __launch_bounds__(256) void k_shade_spec(SceneDev s, PathsDev P, int3
         const pg_hit_uv uv = P.uvs[k];
         Hit h = surface(s, hit.prim, hit.t, uv.u, uv.v, ro, rd);
         const ShapeDev& sh = s.shapes[h.shape];
-        const MaterialDev& m = s.materials[sh.material];
+        MaterialDev m = s.materials[sh.material];
+        if (m.has_nodes)
+            apply_nodes(s, m, h, hit.prim, hit.t, uv.u, uv.v, ro, rd);
         const f3 wi = -rd;
         if (m.kind == kMatEmissive) {
             const float w = depth == 0 ? 1.0f
@@ -1358,7 +1391,7 @@ __global__ __launch_bounds__(256) void k_shade_rw_gen(SceneDev s, PathsDev P, in
                 L = L + beta * env_color(s, rd);
         } else {
             const pg_hit_uv uv = P.uvs[k];
-            const Hit h = surface(s, hit.prim, hit.t, uv.u, uv.v, ro, rd);
+            Hit h = surface(s, hit.prim, hit.t, uv.u, uv.v, ro, rd);
             const ShapeDev& sh = s.shapes[h.shape];
             const bool transition = sh.mi_inside != sh.mi_outside;
             const int min_ = transition ? sh.mi_inside : med;
@@ -1369,7 +1402,9 @@ __global__ __launch_bounds__(256) void k_shade_rw_gen(SceneDev s, PathsDev P, in
                 next_med = medium_for(rd, h.n, min_, mout);
                 alive = true;
             } else {
-                const MaterialDev& m = s.materials[sh.material];
+                MaterialDev m = s.materials[sh.material];
+                if (m.has_nodes)
+                    apply_nodes(s, m, h, hit.prim, hit.t, uv.u, uv.v, ro, rd);
                 const f3 wi = -rd;
                 if (m.kind == kMatEmissive) {
                     L = L + beta * emitted(m, h.n, wi);
@@ -1893,6 +1928,7 @@ struct pg_scene {
     int spp = 1, max_depth = 4;
     int atmo_w = 0, atmo_h = 0;  // Atmosphere colour table size (0: no Atmosphere)
     bool atmo_accurate = false;  // table with correctly rounded exp/sqrt/pow (extension)
+    std::vector<nodes::Inst> node_progs;  // material node-graph programs
     int sampler_kind = kSamplerHalton;
     bool halton_permute = true;            // HaltonSampler randomizeStrategy permutateDigits
     int sobol_randomize = kSobolFastOwen;  // SobolSampler randomizeStrategy
@@ -1998,52 +2034,179 @@ ConstNode const_node(const Params& p, const std::string& what) {
     throw unsupported("node type \"" + type + "\" for " + what);
 }
 
-Layer make_layer(const Params& p) {
+// CreateNode (node.cpp:32-97) compiled for one evaluation mode (EvalFloat or
+// EvalVec3, which node.h defines separately per node) into a postfix program
+// of nodes::Inst.  Constant trees fold to one kConst.
+struct NodeCompiler {
+    std::vector<nodes::Inst>* out;
+    int depth = 0, max_depth = 0;
+    void emit(nodes::Op op, int arg = 0, float x = 0, float y = 0, float z = 0, int push = 0) {
+        out->push_back({op, arg, x, y, z});
+        depth += push;
+        max_depth = std::max(max_depth, depth);
+    }
+    void constant(float f, f3 v, bool vec) {
+        if (vec)
+            emit(nodes::kConst, 0, v.x, v.y, v.z, 1);
+        else
+            emit(nodes::kConst, 0, f, f, f, 1);
+    }
+    void node(const Params& p, bool vec, const std::string& what) {
+        using namespace nodes;
+        const std::string type = p.get_string("type");
+        auto in = [&](const char* k, bool v) { node(p.sub(k), v, what); };
+        if (type == "Constant") {
+            float v[3] = {0, 0, 0};
+            p.get_vec3("vec3", v);
+            constant(p.get_float("float"), mk(v[0], v[1], v[2]), vec);
+        } else if (type == "Position" || type == "Normal" || type == "TexCoord") {
+            if (!vec)  // Node::EvalFloat: 0
+                constant(0.0f, mk(0, 0, 0), false);
+            else
+                emit(type == "Position" ? kPosition : type == "Normal" ? kNormal : kTexCoord, 0, 0, 0, 0, 1);
+        } else if (type == "Decompose") {
+            in("input", true);
+            emit(kComponent, std::min(std::max(p.get_int("dimension"), 0), 2));
+        } else if (type == "Composite") {
+            if (!vec) {
+                constant(0.0f, mk(0, 0, 0), false);
+            } else {
+                in("inputX", false), in("inputY", false), in("inputZ", false);
+                emit(kCompose, 0, 0, 0, 0, -2);
+            }
+        } else if (type == "Add" || type == "Substract" || type == "Multiply" || type == "Divide") {
+            in("input", vec), in("factor", vec);
+            emit(type == "Add" ? kAdd : type == "Substract" ? kSub : type == "Multiply" ? kMul : kDiv, 0, 0, 0, 0, -1);
+        } else if (type == "MultiplyAdd") {
+            in("input", vec), in("mulFactor", vec), in("addFactor", vec);
+            emit(kMulAdd, 0, 0, 0, 0, -2);
+        } else if (type == "Length") {
+            in("input", true);
+            emit(kLength);
+        } else if (type == "Sqr" || type == "Sqrt" || type == "Sin" || type == "Cos" || type == "Tan" ||
+                   type == "Fract" || type == "Invert") {
+            in("input", vec);
+            emit(type == "Sqr" ? kSqr : type == "Sqrt" ? kSqrt : type == "Sin" ? kSin : type == "Cos" ? kCos
+                 : type == "Tan" ? kTan : type == "Fract" ? kFract : kInvert);
+        } else if (type == "Pow") {
+            in("input", vec), in("exp", false);
+            emit(kPow, 0, 0, 0, 0, -1);
+        } else if (type == "Checkerboard") {
+            in("input", true), in("frequency", false);
+            emit(kChecker, 0, 0, 0, 0, -1);
+        } else if (type == "Noise" || type == "Noise3D") {
+            in("input", true), in("frequency", false), in("octaves", false);
+            emit(type == "Noise" ? kNoise : kNoise3D, vec ? 1 : 0, 0, 0, 0, -2);
+        } else if (type == "Texture") {
+            throw unsupported("Texture node (image files) for " + what);
+        } else if (type.empty() || !p.get_string("@").empty()) {
+            // node.cpp:89-91: a plain value (or an unknown typed block) is a
+            // Constant of its "float" / "vec3" keys, defaulting to the value.
+            const std::string at = p.get_string("@");
+            float v[3] = {0, 0, 0};
+            pine_stofs(at, v, 3);
+            if (p.has_value("vec3"))
+                p.get_vec3("vec3", v);
+            if (type.empty() && at.empty() && !p.has_value("float") && !p.has_value("vec3"))
+                throw std::runtime_error(what + ": missing value");
+            constant(p.has_value("float") ? p.get_float("float") : pine_stof(at), mk(v[0], v[1], v[2]), vec);
+        } else {
+            fprintf(stderr, "[NodeInput][Create]Unknown type \"%s\"\n", type.c_str());
+            constant(p.get_float("float", 0.5f), mk(0.5f, 0.5f, 0.5f), vec);
+        }
+    }
+};
+
+// Compiles a material input; returns its program offset in `progs`, or -1 with
+// the value in f / v when the input is a constant.
+int compile_input(const Params& p, bool vec, const std::string& what, std::vector<nodes::Inst>& progs, float& f,
+                  f3& v, bool always_program = false) {
+    std::vector<nodes::Inst> body;
+    NodeCompiler c{&body};
+    c.node(p, vec, what);
+    if (!always_program && body.size() == 1 && body[0].op == nodes::kConst) {
+        f = body[0].x;
+        v = mk(body[0].x, body[0].y, body[0].z);
+        return -1;
+    }
+    if (c.max_depth > nodes::kStack)
+        throw unsupported("node graph deeper than 16 values for " + what);
+    const int at = (int)progs.size();
+    progs.push_back({nodes::kProgram, (int)body.size(), 0, 0, 0});
+    progs.insert(progs.end(), body.begin(), body.end());
+    return at;
+}
+
+Layer make_layer(const Params& p, std::vector<nodes::Inst>& progs) {
     std::string type = p.get_string("type");
     Layer l{};
+    l.prog_albedo = l.prog_roughness = l.prog_eta = -1;
+    float f;
+    f3 v;
+    auto vec3_in = [&](const char* k, f3& dst, int& prog) {
+        prog = compile_input(p.sub(k), true, k, progs, f, v);
+        dst = v;
+    };
+    auto float_in = [&](const char* k, float& dst, int& prog) {
+        prog = compile_input(p.sub(k), false, k, progs, f, v);
+        dst = f;
+    };
     if (type == "Dielectric") {
         l.type = kDielectric;
-        l.albedo = p.has_subset("albedo") ? const_node(p.sub("albedo"), "albedo").v : mk(1, 1, 1);
-        l.roughness = const_node(p.sub("roughness"), "roughness").f;
-        l.eta = const_node(p.sub("eta"), "eta").f;
+        if (p.has_subset("albedo"))
+            vec3_in("albedo", l.albedo, l.prog_albedo);
+        else
+            l.albedo = mk(1, 1, 1);  // bxdf.cpp:176-180
+        float_in("roughness", l.roughness, l.prog_roughness);
+        float_in("eta", l.eta, l.prog_eta);
     } else if (type == "Conductor") {
         l.type = kConductor;
-        l.albedo = const_node(p.sub("albedo"), "albedo").v;
-        l.roughness = const_node(p.sub("roughness"), "roughness").f;
+        vec3_in("albedo", l.albedo, l.prog_albedo);
+        float_in("roughness", l.roughness, l.prog_roughness);
     } else {
         if (type != "Diffuse")
             fprintf(stderr, "[BSDF][Create]Unknown type \"%s\"\n", type.c_str());
         l.type = kDiffuse;
-        l.albedo = const_node(p.sub("albedo"), "albedo").v;
+        vec3_in("albedo", l.albedo, l.prog_albedo);
     }
     return l;
 }
 
-MaterialDev make_material(const Params& p) {
+MaterialDev make_material(const Params& p, std::vector<nodes::Inst>& progs) {
     MaterialDev m{};
-    if (p.has_subset("bumpMap"))
-        throw unsupported("bump maps");
+    m.prog_bump = -1;
     std::string type = p.get_string("type");
     if (type == "Emissive") {
         m.kind = kMatEmissive;
         m.emission = const_node(p.sub("color"), "color").v;
-        return m;
+    } else {
+        if (type != "Layered")
+            fprintf(stderr, "[Material][Create]Unknown type \"%s\"\n", type.c_str());
+        m.kind = kMatLayered;
+        // LayeredMaterial ctor (material.cpp:37-47): layers by descending index.
+        std::vector<std::pair<int, const Params*>> layers;
+        for (auto& kv : p.subsets)
+            if (kv.first.compare(0, 5, "layer") == 0 && !kv.second.empty())
+                layers.push_back({pine_stoi(kv.first.substr(5)), &kv.second.back()});
+        std::stable_sort(layers.begin(), layers.end(),
+                         [](const auto& a, const auto& b) { return a.first > b.first; });
+        if ((int)layers.size() > kMaxLayers)
+            throw unsupported("more than 4 material layers");
+        m.nlayers = (int)layers.size();
+        for (int i = 0; i < m.nlayers; i++)
+            m.layers[i] = make_layer(*layers[i].second, progs);
     }
-    if (type != "Layered")
-        fprintf(stderr, "[Material][Create]Unknown type \"%s\"\n", type.c_str());
-    m.kind = kMatLayered;
-    // LayeredMaterial ctor (material.cpp:37-47): layers by descending index.
-    std::vector<std::pair<int, const Params*>> layers;
-    for (auto& kv : p.subsets)
-        if (kv.first.compare(0, 5, "layer") == 0 && !kv.second.empty())
-            layers.push_back({pine_stoi(kv.first.substr(5)), &kv.second.back()});
-    std::stable_sort(layers.begin(), layers.end(),
-                     [](const auto& a, const auto& b) { return a.first > b.first; });
-    if ((int)layers.size() > kMaxLayers)
-        throw unsupported("more than 4 material layers");
-    m.nlayers = (int)layers.size();
+    // CreateMaterial (material.cpp:78-79): any material may carry a bump map.
+    if (p.has_subset("bumpMap")) {
+        float f;
+        f3 v;
+        // Always a program: even a constant map renormalises Cross(dpdu, dpdv).
+        m.prog_bump = compile_input(p.sub("bumpMap"), false, "bumpMap", progs, f, v, true);
+    }
+    m.has_nodes = m.prog_bump >= 0;
     for (int i = 0; i < m.nlayers; i++)
-        m.layers[i] = make_layer(*layers[i].second);
+        if (m.layers[i].prog_albedo >= 0 || m.layers[i].prog_roughness >= 0 || m.layers[i].prog_eta >= 0)
+            m.has_nodes = 1;
     return m;
 }
 
@@ -2293,7 +2456,7 @@ void build_scene(pg_scene& S) {
     std::map<std::string, int> mat_index;
     for (const Params& p : root.all("Material")) {
         mat_index[p.get_string("name")] = (int)S.materials.size();
-        S.materials.push_back(make_material(p));
+        S.materials.push_back(make_material(p, S.node_progs));
     }
     // Media by name (fileio.cpp:262-263).
     std::map<std::string, int> med_index;
@@ -2559,6 +2722,14 @@ void build_scene(pg_scene& S) {
     }
     if (uses_media)
         S.general = true;
+    // The split bounce saves a vertex as (p, n, wi) and samples its BSDF later,
+    // without the surface's uv: node-graph materials run in the fused kernels
+    // and in the RandomWalk kernels only.
+    if (S.general && S.integrator == kIntPath)
+        for (const MaterialDev& m : S.materials)
+            if (m.has_nodes)
+                throw unsupported("node-graph materials in a Path scene with participating media or "
+                                  "material-less shapes");
     const Params& fp = cp.sub("film");
     int size[2];
     const int defsize[2] = {720, 480};
@@ -3020,7 +3191,8 @@ int finish_scene(pg_scene& S, const pg_build_options* opts) {
         (rc = upload(S, sums, &sd.halton.primeSums)) || (rc = upload(S, S.prim_shape, &S.dev.prim_shape)) ||
         (rc = upload(S, S.tri_verts, &S.dev.tri_verts)) || (rc = upload(S, S.shapes, &S.dev.shapes)) ||
         (rc = upload(S, S.materials, &S.dev.materials)) || (rc = upload(S, S.lights, &S.dev.lights)) ||
-        (rc = upload(S, S.light_func, &S.dev.light_func)) || (rc = upload(S, S.light_cdf, &S.dev.light_cdf)))
+        (rc = upload(S, S.light_func, &S.dev.light_func)) || (rc = upload(S, S.light_cdf, &S.dev.light_cdf)) ||
+        (rc = upload(S, S.node_progs, &S.dev.node_prog)))
         return rc;
     // Media: density grids and brick majorants, then the MediumDev table.
     S.lds_medium = -1;

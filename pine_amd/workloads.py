@@ -527,3 +527,62 @@ def occluded_box_scene_text(obj_file, width=1024, height=1024, spp=64, max_depth
     lines += rect((0, 1.95, -0.72), (0.5, 0, 0), (0, 0, 0.4), "lamp")
     lines += ["Shape: TriangleMesh{", f"    file: {obj_file}", "    material: clutter", "}"]
     return "\n".join(lines) + "\n"
+
+
+def nodes_scene_text(width=160, height=90, spp=64, max_depth=6, output="nodes.bmp"):
+    """A scene whose materials are pine node graphs (node.h): a plane with a
+    checkerboard albedo over its texture coordinates, a sphere with a noise bump
+    map, a conductor with a Noise3D albedo and a checkerboard roughness, and
+    spheres whose inputs combine the arithmetic nodes (Composite, Decompose,
+    Sin, Cos, Tan, Pow, Sqr, Sqrt, Length, Fract, Invert, Add, Substract,
+    Divide, MultiplyAdd, Position, Normal, TexCoord).  One emissive rect."""
+    lines = [
+        "Integrator: Path{", f"    maxDepth: {max_depth}", "    sampler: Halton{",
+        f"        samplesPerPixel: {spp}", "    }", "}",
+        "Camera: ThinLen{", "    film{", f"        outputFileName: {output}",
+        f"        size: {width} {height}", "        filter: LanczosSinc{", "            radius: 0.5",
+        "        }", "    }", "    from: 0 3.5 13", "    to: 0 0.4 0", "    fov: 0.42", "}",
+        "Shape: Rect{", "    position: 6 12 5", "    ex: 3 0 0", "    ey: 0 0 3", "    material: lamp", "}",
+        "Shape: Plane{", "    position: 0 0 0", "    normal: 0 1 0", "    material: floor", "}",
+    ]
+    for k, (x, mat) in enumerate([(-3.0, "bumpy"), (-1.5, "rusty"), (0.0, "arith"), (1.5, "glassy"),
+                                  (3.0, "trig")]):
+        lines += ["Shape: Sphere{", f"    position: {x} 0.6 {0.4 * (k % 2)}", "    radius: 0.6",
+                  f"    material: {mat}", "}"]
+    lines += [
+        "Material lamp: Emissive{", "    color: 60", "}",
+        "Material floor: Layered{", "    layer0: Diffuse{", "        albedo: MultiplyAdd{",
+        "            input: Checkerboard{", "                input: TexCoord", "                frequency: 3.0",
+        "            }", "            mulFactor: 0.8", "            addFactor: 0.1", "        }", "    }", "}",
+        "Material bumpy: Layered{", "    layer0: Diffuse{", "        albedo: 0.8 0.75 0.7", "    }",
+        "    bumpMap: Multiply{", "        input: Noise{", "            input: Position",
+        "            frequency: 12", "            octaves: 3.0", "        }", "        factor: 0.04", "    }", "}",
+        "Material rusty: Layered{", "    layer0: Conductor{", "        albedo: Noise3D{",
+        "            input: Position", "            frequency: 6", "            octaves: 3", "        }",
+        "        roughness: MultiplyAdd{", "            input: Checkerboard{", "                input: TexCoord",
+        "                frequency: 2.0", "            }", "            mulFactor: 0.3",
+        "            addFactor: 0.15", "        }", "    }", "}",
+        "Material arith: Layered{", "    layer0: Diffuse{", "        albedo: Composite{",
+        "            inputX: Fract{", "                input: Decompose{", "                    input: Position",
+        "                    dimension: 0", "                }", "            }",
+        "            inputY: Sqr{", "                input: Decompose{", "                    input: Normal",
+        "                    dimension: 1", "                }", "            }",
+        "            inputZ: Invert{", "                input: Divide{", "                    input: Length{",
+        "                        input: Position", "                    }", "                    factor: 4.0",
+        "                }", "            }", "        }", "    }", "}",
+        "Material glassy: Layered{", "    layer0: Diffuse{", "        albedo: 0.2 0.5 0.3", "    }",
+        "    layer1: Dielectric{", "        roughness: Sqrt{", "            input: Substract{",
+        "                input: 0.2", "                factor: Multiply{",
+        "                    input: 0.15", "                    factor: Checkerboard{",
+        "                        input: Position", "                        frequency: 4",
+        "                    }", "                }", "            }", "        }", "        eta: Add{",
+        "            input: 1.3", "            factor: 0.2", "        }", "    }", "}",
+        "Material trig: Layered{", "    layer0: Diffuse{", "        albedo: Pow{", "            input: Composite{",
+        "                inputX: Sin{", "                    input: Decompose{", "                        input: TexCoord",
+        "                        dimension: 0", "                    }", "                }",
+        "                inputY: Cos{", "                    input: Decompose{",
+        "                        input: TexCoord", "                        dimension: 1",
+        "                    }", "                }", "                inputZ: Tan{", "                    input: 0.5",
+        "                }", "            }", "            exp: 2.0", "        }", "    }", "}",
+    ]
+    return "\n".join(lines) + "\n"

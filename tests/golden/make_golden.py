@@ -17,9 +17,11 @@ Cases
   halton      pine's Halton sampler values for a spread of pixels
   sobol       pine's Sobol sampler values (four scramblers, several film sizes,
               one run past dimension 1024)
+  noise       pine's PerlinNoise (seeds 0..2), Turbulence and Turbulence3D at 2,000 points
   atmosphere  pine's Atmosphere colour table and sun colour for two suns
   render_spheres  the seeded spheres scene (64 spheres, area light) at
               96x96, 256 spp, maxDepth 8, rendered by pine's PathIntegrator
+  render_nodes    node-graph materials: checkerboard, noise bump, Noise3D, arithmetic nodes
   render_4096     pine at 4096 spp: the spheres scene (64x64) and the mixed scene (96x72)
   render_sobol    the spheres scene under the Sobol sampler (fastOwen 96x96 256 spp,
               Owen 40x24 128 spp)
@@ -219,6 +221,16 @@ def case_sobol():
     print("sobol:", ", ".join(out), flush=True)
 
 
+def case_noise():
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "n.bin")
+        subprocess.run([HARNESS, "noise", p, "2000"], check=True, capture_output=True)
+        a = np.fromfile(p, dtype=np.float32).reshape(-1, 12)
+    np.savez_compressed(os.path.join(OUT, "noise.npz"), points=a[:, :3], freq=a[:, 3], octaves=a[:, 4],
+                        perlin=a[:, 5:8], turbulence=a[:, 8], turbulence3d=a[:, 9:12])
+    print("noise:", a.shape, flush=True)
+
+
 def case_atmosphere():
     out = {}
     for (sun, col, size) in [((2, 6, 3), (1, 1, 1), (64, 32)), ((-1, 0.3, 0.5), (1.5, 1.2, 0.9), (48, 40))]:
@@ -347,6 +359,17 @@ def case_render_mixed():
     print("render_mixed:", line, flush=True)
 
 
+def case_render_nodes():
+    """pine's node-graph materials (node.h) under the Path integrator: the
+    nodes scene of pine_amd.workloads at 160x90, 256 spp."""
+    text = W.nodes_scene_text(width=160, height=90, spp=256)
+    with tempfile.TemporaryDirectory() as d:
+        film, line = run_render(text, d)
+    np.savez_compressed(os.path.join(OUT, "render_nodes.npz"), scene=np.bytes_(text), film=film,
+                        harness=np.bytes_(line))
+    print("render_nodes:", line, flush=True)
+
+
 def case_render_4096():
     """pine's PathIntegrator at 4096 spp: the spheres scene at 64x64 and the
     mixed scene (point, directional and area lights, layered materials, a rough
@@ -469,9 +492,11 @@ CASES = {
     "halton": case_halton,
     "sobol": case_sobol,
     "atmosphere": case_atmosphere,
+    "noise": case_noise,
     "render_spheres": case_render_spheres,
     "render_sobol": case_render_sobol,
     "render_4096": case_render_4096,
+    "render_nodes": case_render_nodes,
     "render_atmosphere": case_render_atmosphere,
     "render_mixed": case_render_mixed,
     "render_cloud": case_render_cloud,

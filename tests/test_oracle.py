@@ -233,6 +233,27 @@ def test_sobol_pixels_tile_the_sequence(oracle, strategy):
             assert ((v >= j) & (v <= j + 1)).all(), (sample, d)
 
 
+def test_noise_matches_pine(oracle):
+    """pine's PerlinNoise (seeds 0..2), Turbulence and Turbulence3D (noise.cpp)
+    at 2,000 seeded points, frequencies 1..7, 1..5 octaves: bit for bit."""
+    g = golden("noise")
+    got = oracle.noise(g["points"], g["freq"], g["octaves"])
+    ref = np.concatenate([g["perlin"], g["turbulence"][:, None], g["turbulence3d"]], 1)
+    assert np.array_equal(got.view(np.int32), ref.view(np.int32))
+
+
+def test_gradient_noise_is_one_half_on_the_lattice(oracle):
+    """Closed form: at a lattice point the smoothstep weights select the corner
+    itself, whose offset is zero, so every gradient term vanishes and
+    PerlinNoise = 0.5 (1 + 0) exactly, for any seed; Turbulence of one octave
+    is then (0.5 / (2 - 1))^2 = 0.25."""
+    rng = np.random.default_rng(5)
+    pts = rng.integers(-50, 50, size=(200, 3)).astype(np.float32)
+    out = oracle.noise(pts, np.ones(200, np.float32), np.ones(200, np.float32))
+    assert (out[:, :3] == 0.5).all()
+    assert (out[:, 3] == 0.25).all() and (out[:, 4:] == 0.25).all()
+
+
 def test_guide_oracle_encoding_is_the_frequency_basis():
     """Every (input, octave, sin/cos) pair and every raw input appears exactly
     once, in the first 225 columns; each column equals sin / cos(pi 2^(k+oct) x_c)

@@ -65,6 +65,24 @@ def test_mixed_scene_matches_pine(pg, cuda, tmp_path):
     print(f"mixed relMSE {e:.3e}")
 
 
+def test_node_graph_materials_match_pine(pg, cuda, tmp_path):
+    """pine's material node graphs (node.h, node.cpp:32-97): a checkerboard over
+    a plane's texture coordinates, a Noise bump map (Material::BumpNormal), a
+    Noise3D conductor albedo with a checkerboard roughness, and inputs built
+    from the arithmetic nodes, against pine's film of the same scene text."""
+    fx = golden("render_nodes")
+    film, info = render_text(pg, fx["scene"].item().decode(), str(tmp_path))
+    e = check_film(film, fx["film"], 1e-3)
+    print(f"node-graph scene relMSE {e:.3e} vs pine")
+
+
+def test_node_graph_materials_fail_loudly_in_the_split_bounce(pg, cuda, tmp_path):
+    from pine_amd import workloads as W
+    text = W.nodes_scene_text(width=8, height=8, spp=1) + "Shape: Rect{\n    position: 0 50 0\n    ex: 1 0 0\n    ey: 0 0 1\n}\n"
+    with pytest.raises(pg.PineError, match="node-graph"):
+        pg.Scene(text=text)
+
+
 @pytest.mark.parametrize("name", ["spheres", "mixed"])
 def test_4096_spp_images_match_pine(pg, cuda, tmp_path, name):
     """The bar of the path: with pine's per-pixel sample sequence, images at

@@ -1894,6 +1894,49 @@ __global__ __launch_bounds__(256) void k_film(SceneDev s, PathsDev P, int32_t n_
     film[4 * own + 3] = acc[3];
 }
 
+// Film::AddSample (film.h:26-45) for filters wider than a pixel, as a gather:
+// each film pixel visits the samples of the pixels within reach (a sample
+// lies inside its own pixel, so it can touch pixels up to ceil(radius + 1/2)
+// away), tests whether it is inside that sample's [Ceil(pf - r), Floor(pf + r)]
+// footprint exactly as AddSample computes it, and adds the filter-table
+// weight.  Fixed order (rows, columns, samples), no atomics.
+__global__ __launch_bounds__(256) void k_film_gather(SceneDev s, PathsDev P, int32_t n_pixels, int32_t spb,
+                                                     float* film) {
+    const int32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pixel >= n_pixels)
+        return;
+    const int x = pixel % s.width, y = pixel / s.width;
+    const float rx = s.filter_radius[0], ry = s.filter_radius[1];
+    const int Rx = (int)ceilf(rx + 0.5f), Ry = (int)ceilf(ry + 0.5f);
+    const size_t own = (size_t)(s.height - 1 - y) * s.width + x;
+    float acc[4] = {film[4 * own + 0], film[4 * own + 1], film[4 * own + 2], film[4 * own + 3]};
+    const int qy0 = y - Ry > 0 ? y - Ry : 0, qy1 = y + Ry < s.height - 1 ? y + Ry : s.height - 1;
+    const int qx0 = x - Rx > 0 ? x - Rx : 0, qx1 = x + Rx < s.width - 1 ? x + Rx : s.width - 1;
+    for (int qy = qy0; qy <= qy1; qy++)
+        for (int qx = qx0; qx <= qx1; qx++)
+            for (int sl = 0; sl < spb; sl++) {
+                const int32_t i = (qy * s.width + qx) * spb + sl;
+                const float fx = P.pfilm[2 * i] * (float)s.width - 0.5f;
+                const float fy = P.pfilm[2 * i + 1] * (float)s.height - 0.5f;
+                const int x0 = (int)pstd::ceil(fx - rx), y0 = (int)pstd::ceil(fy - ry);
+                const int x1 = (int)pstd::floor(fx + rx), y1 = (int)pstd::floor(fy + ry);
+                if (x < x0 || x > x1 || y < y0 || y > y1)
+                    continue;
+                const float dx = fabsp((float)x - fx) / rx, dy = fabsp((float)y - fy) / ry;
+                const int tx = (int)(16.0f * (dx < kOneMinusEps ? dx : kOneMinusEps));
+                const int ty = (int)(16.0f * (dy < kOneMinusEps ? dy : kOneMinusEps));
+                const float w = s.filter_table[ty * 16 + tx];
+                acc[0] += P.L[3 * i] * w;
+                acc[1] += P.L[3 * i + 1] * w;
+                acc[2] += P.L[3 * i + 2] * w;
+                acc[3] += w;
+            }
+    film[4 * own + 0] = acc[0];
+    film[4 * own + 1] = acc[1];
+    film[4 * own + 2] = acc[2];
+    film[4 * own + 3] = acc[3];
+}
+
 __global__ __launch_bounds__(256) void k_film_spill(PathsDev P, float* film) {
     const int32_t n = min(*P.spill_count, P.spill_capacity);
     for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
@@ -2741,6 +2784,47 @@ void build_scene(pg_scene& S) {
     float radius[2];
     const float defr[2] = {0.5f, 0.5f};
     flp.get_vec2("radius", radius, defr);
+    // Film's filter table (film.cpp:22-28): the filter at the 16 x 16 cell
+    // centres of [0, radius)^2; CreateFilter (filter.cpp:5-22) with filter.h's
+    // Evaluate members.  An unknown type is a Box with a warning.
+    if (ftype != "Box" && ftype != "Triangle" && ftype != "Gaussian" && ftype != "Mitchell" &&
+        ftype != "LanczosSinc") {
+        fprintf(stderr, "[Filter][Create]Unknown type \"%s\"\n", ftype.c_str());
+        ftype = "Box";
+    }
+    const float tau = flp.get_float("tau", 3.0f);
+    const float galpha = flp.get_float("alpha", 1.0f);
+    const float mB = flp.get_float("B", 1.0f / 3.0f), mC = flp.get_float("C", 1.0f / 3.0f);
+    // GaussianFilter's constructor; MitchellFilter's invRadius = 1.0 / radius (double).
+    const float expX = pstd::exp(-galpha * (radius[0] * radius[0]));
+    const float expY = pstd::exp(-galpha * (radius[1] * radius[1]));
+    const float invRx = (float)(1.0 / (double)radius[0]), invRy = (float)(1.0 / (double)radius[1]);
+    auto sinc = [](float x) {
+        x = fabsp(x);
+        if (x < 1e-5f)
+            return 1.0f;
+        return pstd::sin(kPi * x) / (kPi * x);
+    };
+    auto wsinc = [&](float x, float r) {  // LanczosSincFilter (filter.h:72-96)
+        x = fabsp(x);
+        if (x > r)
+            return 0.0f;
+        return sinc(x) * sinc(x / tau);
+    };
+    auto gauss = [&](float d, float expv) {  // GaussianFilter::Gaussian (filter.h:35-37)
+        const float g = pstd::exp(-galpha * d * d) - expv;
+        return g > 0.0f ? g : 0.0f;
+    };
+    auto mitchell = [&](float x) {  // MitchellFilter::Mitchell1D (filter.h:47-57)
+        x = fabsp(2.0f * x);
+        if (x > 1.0f)
+            return ((-mB - 6.0f * mC) * x * x * x + (6.0f * mB + 30.0f * mC) * x * x + (-12.0f * mB - 48.0f * mC) * x +
+                    (8.0f * mB + 24.0f * mC)) *
+                   (1.0f / 6.0f);
+        return ((12.0f - 9.0f * mB - 6.0f * mC) * x * x * x + (-18.0f + 12.0f * mB + 6.0f * mC) * x * x +
+                (6.0f - 2.0f * mB)) *
+               (1.0f / 6.0f);
+    };
     float table[256];
     for (int y = 0, o = 0; y < 16; y++)
         for (int x = 0; x < 16; x++) {
@@ -2749,24 +2833,16 @@ void build_scene(pg_scene& S) {
             float w;
             if (ftype == "Box") {
                 w = 1.0f;
-            } else if (ftype == "LanczosSinc") {
-                // LanczosSincFilter (filter.h:72-96)
-                float tau = flp.get_float("tau", 3.0f);
-                auto sinc = [](float x) {
-                    x = fabsp(x);
-                    if (x < 1e-5f)
-                        return 1.0f;
-                    return pstd::sin(kPi * x) / (kPi * x);
-                };
-                auto wsinc = [&](float x, float r) {
-                    x = fabsp(x);
-                    if (x > r)
-                        return 0.0f;
-                    return sinc(x) * sinc(x / tau);
-                };
-                w = wsinc(px, radius[0]) * wsinc(py, radius[1]);
+            } else if (ftype == "Triangle") {
+                const float tx = radius[0] - fabsp(px), ty = radius[1] - fabsp(py);
+                w = (tx > 0.0f ? tx : 0.0f) * (ty > 0.0f ? ty : 0.0f);
+            } else if (ftype == "Gaussian") {
+                w = gauss(px, expX) * gauss(py, expY);
+            } else if (ftype == "Mitchell") {
+                // pine divides y by the inverse radius (filter.h:45)
+                w = mitchell(px * invRx) * mitchell(py / invRy);
             } else {
-                throw unsupported("film filter \"" + ftype + "\"");
+                w = wsinc(px, radius[0]) * wsinc(py, radius[1]);
             }
             table[o++] = w;
         }
@@ -3758,10 +3834,15 @@ static int render_enqueue(pg_scene* S, int spp, int spb_req, hipStream_t st, Gui
         }
         if (S->spectral)
             hipLaunchKernelGGL(k_spec_to_rgb, dim3((np + B - 1) / B), dim3(B), 0, st, S->dev, P, np);
-        PG_HIP(hipMemsetAsync(P.spill_count, 0, sizeof(int32_t), st));
-        hipLaunchKernelGGL(k_film, dim3((npix + B - 1) / B), dim3(B), 0, st, S->dev, P, (int32_t)npix,
-                           nb, S->d_film);
-        hipLaunchKernelGGL(k_film_spill, dim3(64), dim3(B), 0, st, P, S->d_film);
+        if (S->dev.filter_radius[0] > 0.5f || S->dev.filter_radius[1] > 0.5f) {
+            hipLaunchKernelGGL(k_film_gather, dim3((npix + B - 1) / B), dim3(B), 0, st, S->dev, P, (int32_t)npix,
+                               nb, S->d_film);
+        } else {
+            PG_HIP(hipMemsetAsync(P.spill_count, 0, sizeof(int32_t), st));
+            hipLaunchKernelGGL(k_film, dim3((npix + B - 1) / B), dim3(B), 0, st, S->dev, P, (int32_t)npix,
+                               nb, S->d_film);
+            hipLaunchKernelGGL(k_film_spill, dim3(64), dim3(B), 0, st, P, S->d_film);
+        }
     }
     PG_HIP(hipGetLastError());
     PG_HIP(hipEventRecord(t->e1, st));

@@ -21,6 +21,7 @@ Cases
   atmosphere  pine's Atmosphere colour table and sun colour for two suns
   render_spheres  the seeded spheres scene (64 spheres, area light) at
               96x96, 256 spp, maxDepth 8, rendered by pine's PathIntegrator
+  render_filters  Triangle, Gaussian, Mitchell and an unknown (Box) film filter, 64x48
   render_nodes    node-graph materials: checkerboard, noise bump, Noise3D, arithmetic nodes
   render_4096     pine at 4096 spp: the spheres scene (64x64) and the mixed scene (96x72)
   render_sobol    the spheres scene under the Sobol sampler (fastOwen 96x96 256 spp,
@@ -359,6 +360,36 @@ def case_render_mixed():
     print("render_mixed:", line, flush=True)
 
 
+FILTER_CASES = {
+    "triangle": "filter: Triangle{\n            radius: 1.0\n        }",
+    "gaussian": "filter: Gaussian{\n            radius: 1.5\n            alpha: 2.0\n        }",
+    "mitchell": "filter: Mitchell{\n            radius: 2.0 1.5\n        }",
+    "unknown": "filter: Gauss{\n            radius: 0.75\n        }",
+}
+
+
+def filter_scene_text(name):
+    text = W.spheres_scene_text(seed=0, width=64, height=48, spp=64, max_depth=6)
+    old = "filter: LanczosSinc{\n            radius: 0.5\n        }"
+    assert old in text
+    return text.replace(old, FILTER_CASES[name])
+
+
+def case_render_filters():
+    """The film filters of filter.h (Triangle, Gaussian, Mitchell with pine's
+    y-axis quirk, and an unknown type, which pine renders with a Box) on the
+    spheres scene, 64x48, 64 spp: wider footprints than one pixel."""
+    out = {}
+    for name in FILTER_CASES:
+        text = filter_scene_text(name)
+        with tempfile.TemporaryDirectory() as d:
+            film, line = run_render(text, d)
+        out[name + "_scene"] = np.bytes_(text)
+        out[name + "_film"] = film
+        print(f"render_filters {name}:", line, flush=True)
+    np.savez_compressed(os.path.join(OUT, "render_filters.npz"), **out)
+
+
 def case_render_nodes():
     """pine's node-graph materials (node.h) under the Path integrator: the
     nodes scene of pine_amd.workloads at 160x90, 256 spp."""
@@ -497,6 +528,7 @@ CASES = {
     "render_sobol": case_render_sobol,
     "render_4096": case_render_4096,
     "render_nodes": case_render_nodes,
+    "render_filters": case_render_filters,
     "render_atmosphere": case_render_atmosphere,
     "render_mixed": case_render_mixed,
     "render_cloud": case_render_cloud,

@@ -65,6 +65,16 @@ def test_mixed_scene_matches_pine(pg, cuda, tmp_path):
     print(f"mixed relMSE {e:.3e}")
 
 
+@pytest.mark.parametrize("name", ["triangle", "gaussian", "mitchell", "unknown"])
+def test_film_filters_match_pine(pg, cuda, tmp_path, name):
+    """Film filter tables of filter.h (wider than a pixel: each sample lands in
+    several pixels) against pine's films of the same scene."""
+    fx = golden("render_filters")
+    film, info = render_text(pg, fx[name + "_scene"].item().decode(), str(tmp_path))
+    e = check_film(film, fx[name + "_film"], 1e-3)
+    print(f"filter {name}: relMSE {e:.3e}")
+
+
 def test_node_graph_materials_match_pine(pg, cuda, tmp_path):
     """pine's material node graphs (node.h, node.cpp:32-97): a checkerboard over
     a plane's texture coordinates, a Noise bump map (Material::BumpNormal), a

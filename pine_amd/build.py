@@ -6,6 +6,7 @@ pine's x86-64 build does.  Packed FP32 instructions are disabled for device
 code (-packed-fp32-ops).
 """
 import os
+import re
 import subprocess
 import sys
 from concurrent.futures import ThreadPoolExecutor
@@ -36,6 +37,25 @@ def _run(cmd):
     return p.stdout
 
 
+def kernel_resources(src):
+    """{kernel symbol: {"VGPRs", "ScratchSize", "Occupancy", ...}} from the
+    report the last build of `src` (a HIP_SRC name) wrote, or None."""
+    path = os.path.join(OBJ, src + ".resources.txt")
+    if not os.path.exists(path):
+        return None
+    out, name = {}, None
+    for line in open(path):
+        m = re.search(r"Function Name: (\S+)", line)
+        if m:
+            name = m.group(1)
+            out[name] = {}
+            continue
+        m = re.search(r"remark:\s+(\w+)(?: \[[^]]*\])?: (\d+)", line)
+        if m and name:
+            out[name][m.group(1)] = int(m.group(2))
+    return out
+
+
 def build(verbose=False):
     """Compiles every source and links libpinehip.so; returns its path."""
     os.makedirs(OBJ, exist_ok=True)
@@ -45,11 +65,17 @@ def build(verbose=False):
         jobs.append(["g++", *CXXFLAGS, "-c", os.path.join(CSRC, s), "-o", o])
     for s in HIP_SRC:
         o = os.path.join(OBJ, s + ".o")
-        jobs.append([HIPCC, *HIPFLAGS, "-c", os.path.join(CSRC, s), "-o", o])
+        jobs.append([HIPCC, *HIPFLAGS, "-Rpass-analysis=kernel-resource-usage", "-c",
+                     os.path.join(CSRC, s), "-o", o])
     with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
-        for out in ex.map(_run, jobs):
-            if verbose and out.strip():
-                print(out)
+        outs = list(ex.map(_run, jobs))
+    for job, out in zip(jobs, outs):
+        if job[0] == HIPCC:
+            # The compiler's per-kernel register, scratch and occupancy report.
+            with open(job[-1][:-2] + ".resources.txt", "w") as f:
+                f.write("\n".join(l for l in out.splitlines() if "kernel-resource-usage" in l) + "\n")
+        elif verbose and out.strip():
+            print(out)
     objs = [os.path.join(OBJ, s + ".o") for s in HOST_SRC + HIP_SRC]
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp, *objs, "-lpthread"])

@@ -142,10 +142,12 @@ PG_HD f3 turbulence3d(f3 np, float frequency, int octaves) {
 
 PG_HD f3 splat(float v) { return mk(v, v, v); }
 
-// Runs one program.  A program the host compiled never needs more than kStack
-// values (the compiler checks), so the stack accesses stay in bounds.
-PG_HD f3 eval(const Inst* prog, const Ctx& c) {
-    f3 st[kStack];
+// Runs one program on the value stack `st` (kStack entries of the caller's
+// storage: the shading kernels pass per-slot device memory, so that the
+// interpreter adds no scratch memory to them).  A program the host compiled
+// never needs more than kStack values (the compiler checks), so the stack
+// accesses stay in bounds.
+PG_HD f3 eval(const Inst* prog, const Ctx& c, f3* st) {
     int sp = 0;
     const int n = prog[0].arg;
     for (int k = 1; k <= n; k++) {
@@ -234,16 +236,16 @@ PG_HD f3 eval(const Inst* prog, const Ctx& c) {
 // Material::BumpNormal (material.cpp:49-63): finite differences of the bump
 // map along u and v (step 0.01, applied to uv and to p along dpdu / dpdv), the
 // perturbed tangents' cross product turned to the side of n.
-PG_HD f3 bump_normal(const Inst* prog, const Ctx& c, f3 dpdu, f3 dpdv) {
+PG_HD f3 bump_normal(const Inst* prog, const Ctx& c, f3 dpdu, f3 dpdv, f3* st) {
     const float delta = 0.01f;
     Ctx c0 = c, c1 = c;
     c0.u = c.u + delta;
     c1.v = c.v + delta;
     c0.p = c.p + dpdu * delta;
     c1.p = c.p + dpdv * delta;
-    const float b = eval(prog, c).x;
-    const float dddu = (eval(prog, c0).x - b) * (1.0f / delta);
-    const float dddv = (eval(prog, c1).x - b) * (1.0f / delta);
+    const float b = eval(prog, c, st).x;
+    const float dddu = (eval(prog, c0, st).x - b) * (1.0f / delta);
+    const float dddv = (eval(prog, c1, st).x - b) * (1.0f / delta);
     const f3 du = dpdu + dddu * c.n, dv = dpdv + dddv * c.n;
     const f3 n = normalize(cross(du, dv));
     return dot(n, c.n) < 0 ? -n : n;

@@ -190,6 +190,11 @@ struct PathsDev {
     int32_t* rec_boff;
     int32_t* rec_n;
     int32_t rec_depth;
+    // Scenes with material node graphs: each shading slot's copy of its
+    // material with the node-graph inputs evaluated, and the interpreter's
+    // value stack (nodes::kStack entries per slot).
+    MaterialDev* node_mat;
+    f3* node_stack;
 };
 
 }  // namespace pg
@@ -200,7 +205,7 @@ struct PathsDev {
 namespace {
 
 // ThinLenCamera::GenRay (camera.cpp:31-41) for pFilm in [0,1]^2.
-__device__ void gen_ray(const CameraDev& c, float fx, float fy, float u0, float u1, f3& o, f3& d) {
+__device__ __forceinline__ void gen_ray(const CameraDev& c, float fx, float fy, float u0, float u1, f3& o, f3& d) {
     float sx = (fx - 0.5f) * 2.0f * c.fov2d[0];
     float sy = (fy - 0.5f) * 2.0f * c.fov2d[1];
     f3 dir = normalize(mk(sx, sy, 1.0f));
@@ -274,7 +279,7 @@ __global__ __launch_bounds__(256) void k_raygen(SceneDev s, PathsDev P, int32_t 
 // Sky::Color / SkyColor (light.cpp:74-76, color.cpp:91-99): a gradient in the
 // angle from +y (SkyColor swaps y and z, then takes acos of z), 8 x sunColor
 // exactly along the sun direction.
-__device__ f3 sky_color(const SceneDev& s, f3 d) {
+__device__ __forceinline__ f3 sky_color(const SceneDev& s, f3 d) {
     if (d.x == s.sky_dir.x && d.y == s.sky_dir.y && d.z == s.sky_dir.z)
         return 8.0f * s.sky_color;
     const float theta = pstd::acos(d.y) * kInvPi;
@@ -293,14 +298,14 @@ __device__ f3 sky_color(const SceneDev& s, f3 d) {
 // nearest-texel mode crashes on such scenes).  The flat index is formed the
 // same way here and wrapped into the table, so every read pine makes inside
 // its table is the same texel, and none leaves the table.
-__device__ f3 atmo_texel(const SceneDev& s, int x, int y) {
+__device__ __forceinline__ f3 atmo_texel(const SceneDev& s, int x, int y) {
     const int64_t n = (int64_t)s.atmo_w * s.atmo_h;
     int64_t flat = ((int64_t)y * s.atmo_w + x) % n;
     flat = flat < 0 ? flat + n : flat;
     const float* c = s.atmo_colors + 3 * flat;
     return mk(c[0], c[1], c[2]);
 }
-__device__ f3 atmosphere_color(const SceneDev& s, f3 d) {
+__device__ __forceinline__ f3 atmosphere_color(const SceneDev& s, f3 d) {
     // InverseUniformSphereMampling (sampling.h:44-47) via CartesianToSpherical
     // (vecmath.h:1243-1250).
     float phi = pstd::atan2(d.y, d.x);
@@ -325,10 +330,10 @@ __device__ f3 atmosphere_color(const SceneDev& s, f3 d) {
 // EnvironmentLight::Color and ::Pdf (light.h:150-160): Sky's pdf is 0,
 // Atmosphere's 1 / 4pi (its sampling pdf is half that; pine uses this value in
 // the BSDF-side MIS weight).
-__device__ f3 env_color(const SceneDev& s, f3 d) {
+__device__ __forceinline__ f3 env_color(const SceneDev& s, f3 d) {
     return s.env_kind == kEnvAtmosphere ? atmosphere_color(s, d) : sky_color(s, d);
 }
-__device__ float env_pdf(const SceneDev& s) { return s.env_kind == kEnvAtmosphere ? 1.0f / (kPi * 4.0f) : 0.0f; }
+__device__ __forceinline__ float env_pdf(const SceneDev& s) { return s.env_kind == kEnvAtmosphere ? 1.0f / (kPi * 4.0f) : 0.0f; }
 
 struct Hit {
     f3 p, n;
@@ -338,7 +343,7 @@ struct Hit {
 // Surface data after a closest hit, as BVH::Intersect's post-hit closure and the
 // shapes' Intersect members fill the Interaction (bvh.cpp:690-708,
 // geometry.cpp:95-195).
-__device__ Hit surface(const SceneDev& s, int32_t prim, float t, float u, float v, f3 o, f3 d) {
+__device__ __forceinline__ Hit surface(const SceneDev& s, int32_t prim, float t, float u, float v, f3 o, f3 d) {
     Hit h;
     h.shape = s.prim_shape[prim];
     const ShapeDev& sh = s.shapes[h.shape];
@@ -371,7 +376,7 @@ __device__ Hit surface(const SceneDev& s, int32_t prim, float t, float u, float 
 // spheres (geometry.cpp:140-145) (phi, theta) of the normal and the spherical
 // tangents; rects (geometry.cpp:191-193) (u, v) + 1/2 and the unit edges;
 // planes (geometry.cpp:104-107) the hit point's coordinates along u, v.
-__device__ void surface_param(const SceneDev& s, int32_t prim, const Hit& h, float t, float u, float v,
+__device__ __forceinline__ void surface_param(const SceneDev& s, int32_t prim, const Hit& h, float t, float u, float v,
                               f3 o, f3 d, f3& uv, f3& dpdu, f3& dpdv) {
     const ShapeDev& sh = s.shapes[h.shape];
     if (sh.type == kShapeMesh) {
@@ -407,29 +412,39 @@ __device__ void surface_param(const SceneDev& s, int32_t prim, const Hit& h, flo
 // material.cpp:49-63, as PathIntegrator::Li does before anything else,
 // path.cpp:48), then each input program is evaluated with NodeEvalCtx (p, the
 // bumped n, uv) (material.h:14-33) into the layer's constant.
-__device__ void apply_nodes(const SceneDev& s, MaterialDev& m, Hit& h, int32_t prim, float t, float u, float v,
-                            f3 o, f3 d) {
+// The material `src` with its node-graph inputs evaluated at this shading
+// point, written to the slot's own copy in device memory (P.node_mat; the
+// interpreter's stack is P.node_stack).  The copy and the stack stay out of
+// registers: a local copy, indexed by layer at run time, would put the
+// material and the stack in scratch memory in every scene.
+__device__ __forceinline__ const MaterialDev* apply_nodes(const SceneDev& s, const PathsDev& P, int64_t slot,
+                                          const MaterialDev* src, Hit& h, int32_t prim, float t, float u,
+                                          float v, f3 o, f3 d) {
+    MaterialDev* m = P.node_mat + slot;
+    f3* st = P.node_stack + nodes::kStack * slot;
+    *m = *src;
     f3 uv, dpdu, dpdv;
     surface_param(s, prim, h, t, u, v, o, d, uv, dpdu, dpdv);
     nodes::Ctx c{h.p, h.n, uv.x, uv.y};
-    if (m.prog_bump >= 0) {
-        h.n = nodes::bump_normal(s.node_prog + m.prog_bump, c, dpdu, dpdv);
+    if (m->prog_bump >= 0) {
+        h.n = nodes::bump_normal(s.node_prog + m->prog_bump, c, dpdu, dpdv, st);
         c.n = h.n;
     }
-    for (int i = 0; i < m.nlayers; i++) {
-        Layer& l = m.layers[i];
+    for (int i = 0; i < m->nlayers; i++) {
+        Layer& l = m->layers[i];
         if (l.prog_albedo >= 0)
-            l.albedo = nodes::eval(s.node_prog + l.prog_albedo, c);
+            l.albedo = nodes::eval(s.node_prog + l.prog_albedo, c, st);
         if (l.prog_roughness >= 0)
-            l.roughness = nodes::eval(s.node_prog + l.prog_roughness, c).x;
+            l.roughness = nodes::eval(s.node_prog + l.prog_roughness, c, st).x;
         if (l.prog_eta >= 0)
-            l.eta = nodes::eval(s.node_prog + l.prog_eta, c).x;
+            l.eta = nodes::eval(s.node_prog + l.prog_eta, c, st).x;
     }
+    return m;
 }
 
 // EmissiveMaterial::Le through MaterialEvalCtx (material.h:14-33, 60-65): the
 // emission when wi, expressed in the frame of n, points above the surface.
-__device__ f3 emitted(const MaterialDev& m, f3 n, f3 wi) {
+__device__ __forceinline__ f3 emitted(const MaterialDev& m, f3 n, f3 wi) {
     M3 w2n = inverse(frame_from(n));
     f3 wl = mul(w2n, wi);
     return cos_theta(wl) > 0.0f ? m.emission : mk(0, 0, 0);
@@ -438,7 +453,7 @@ __device__ f3 emitted(const MaterialDev& m, f3 n, f3 wi) {
 // PowerLightSampler::SampleLight / UniformLightSampler::SampleLight
 // (lightsampler.cpp:6-34) with Distribution1D::SampleDiscrete
 // (distribution.h:27-52): pdf is func[i]/n, not normalised, as in pine.
-__device__ int sample_light(const SceneDev& s, float u, float& pdf) {
+__device__ __forceinline__ int sample_light(const SceneDev& s, float u, float& pdf) {
     const int n = s.n_lights;
     if (s.light_uniform) {
         int idx = (int)(u * (float)n);
@@ -483,7 +498,7 @@ struct LightSampleD {
 };
 
 // Light::Sample (light.cpp:9-63) and Shape::Sample (geometry.h:435-442).
-__device__ LightSampleD sample_light_point(const SceneDev& s, const LightDev& L, f3 p, float u0,
+__device__ __forceinline__ LightSampleD sample_light_point(const SceneDev& s, const LightDev& L, f3 p, float u0,
                                            float u1) {
     LightSampleD ls;
     ls.delta = false;
@@ -589,7 +604,7 @@ __device__ __forceinline__ bool guidable(const SceneDev& s, int4 info) {
 // strategy drew wo, so it stays unbiased for any guide.  With training on,
 // the vertex is recorded (network input, direction, pdfs, path radiance so far
 // and throughput after the direction).  False when the sample carries nothing.
-__device__ bool guided_direction(const SceneDev& s, PathsDev& P, SamplerState& ss, int32_t i, int depth,
+__device__ __forceinline__ bool guided_direction(const SceneDev& s, PathsDev& P, SamplerState& ss, int32_t i, int depth,
                                  const float* gp, const MaterialDev& m, f3 p, f3 n, f3 wi, f3 L_before,
                                  f3& beta, f3& wo_out) {
     const M3 n2w = frame_from(n);
@@ -683,9 +698,10 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev s, PathsDev P, int32_t n
             nextO = offset_ray_origin(h.p, face_same_hemisphere(h.n, rd));
             alive = true;
         } else {
-            MaterialDev m = s.materials[sh.material];
-            if (m.has_nodes)
-                apply_nodes(s, m, h, hit.prim, hit.t, uv.u, uv.v, ro, rd);
+            const MaterialDev* mp = s.materials + sh.material;
+            if (mp->has_nodes)
+                mp = apply_nodes(s, P, k, mp, h, hit.prim, hit.t, uv.u, uv.v, ro, rd);
+            const MaterialDev& m = *mp;
             const f3 wi = -rd;
             if (m.kind == kMatEmissive) {
                 f3 le = emitted(m, h.n, wi);
@@ -862,7 +878,7 @@ __global__ __launch_bounds__(256) void k_resolve_ao(PathsDev P, int32_t n_shadow
 // VizIntegrator::Li (viz.cpp:27-41), Position and Normal: the camera ray's
 // hit point or shading normal as the colour, 0 for a miss.
 // ColorMap (color.cpp:25-35): black -> dark blue -> green -> red.
-__device__ f3 color_map(float v) {
+__device__ __forceinline__ f3 color_map(float v) {
     const f3 k0 = mk(0.0f, 0.0f, 0.0f), k1 = mk(0.0f, 0.0f, 0.5f), k2 = mk(0.0f, 1.0f, 0.0f), k3 = mk(1.0f, 0.0f, 0.0f);
     auto lerp = [](float t, f3 a, f3 b) { return (1.0f - t) * a + t * b; };  // pstd::lerp
     if (v < 1.0f / 3.0f)
@@ -944,9 +960,10 @@ __global__ __launch_bounds__(256) void k_shade_rw(SceneDev s, PathsDev P, int32_
             nextO = offset_ray_origin(h.p, face_same_hemisphere(h.n, rd));
             alive = true;
         } else {
-            MaterialDev m = s.materials[sh.material];
-            if (m.has_nodes)
-                apply_nodes(s, m, h, hit.prim, hit.t, uv.u, uv.v, ro, rd);
+            const MaterialDev* mp = s.materials + sh.material;
+            if (mp->has_nodes)
+                mp = apply_nodes(s, P, k, mp, h, hit.prim, hit.t, uv.u, uv.v, ro, rd);
+            const MaterialDev& m = *mp;
             const f3 wi = -rd;
             if (m.kind == kMatEmissive) {
                 L = L + beta * emitted(m, h.n, wi);
@@ -1041,9 +1058,10 @@ __global__ __launch_bounds__(256) void k_shade_spec(SceneDev s, PathsDev P, int3
         const pg_hit_uv uv = P.uvs[k];
         Hit h = surface(s, hit.prim, hit.t, uv.u, uv.v, ro, rd);
         const ShapeDev& sh = s.shapes[h.shape];
-        MaterialDev m = s.materials[sh.material];
-        if (m.has_nodes)
-            apply_nodes(s, m, h, hit.prim, hit.t, uv.u, uv.v, ro, rd);
+        const MaterialDev* mp = s.materials + sh.material;
+        if (mp->has_nodes)
+            mp = apply_nodes(s, P, k, mp, h, hit.prim, hit.t, uv.u, uv.v, ro, rd);
+        const MaterialDev& m = *mp;
         const f3 wi = -rd;
         if (m.kind == kMatEmissive) {
             const float w = depth == 0 ? 1.0f
@@ -1160,7 +1178,7 @@ __global__ __launch_bounds__(256) void k_spec_to_rgb(SceneDev s, PathsDev P, int
 
 // Adds a thread's density-lookup count to the global counter: one atomic per
 // wave after a cross-lane sum.  Every lane of the block reaches this call.
-__device__ void add_lookups(unsigned long long* counter, int n) {
+__device__ __forceinline__ void add_lookups(unsigned long long* counter, int n) {
     for (int off = 32; off > 0; off >>= 1)
         n += __shfl_down(n, off, 64);
     if ((threadIdx.x & 63) == 0 && n > 0)
@@ -1175,7 +1193,7 @@ __device__ __forceinline__ const float* majorants(const SceneDev& s, int m, int 
 
 // Loads the brick majorants of `lds_medium` into LDS (once per workgroup of a
 // grid-stride kernel).
-__device__ const float* stage_majorants(const SceneDev& s, int lds_medium, float* lds) {
+__device__ __forceinline__ const float* stage_majorants(const SceneDev& s, int lds_medium, float* lds) {
     if (lds_medium < 0)
         return nullptr;
     const MediumDev& m = s.media[lds_medium];
@@ -1194,7 +1212,7 @@ __device__ __forceinline__ int medium_for(f3 w, f3 n, int inside, int outside) {
 // for a surface (mat >= 0) or medium (mat < 0, phase g) vertex.  The record is
 // written even when the contribution is black: IntersectTr still runs, and in
 // media it consumes sampler dimensions.
-__device__ void light_record(const SceneDev& s, PathsDev& P, SamplerState& ss, int32_t i, f3 p,
+__device__ __forceinline__ void light_record(const SceneDev& s, PathsDev& P, SamplerState& ss, int32_t i, f3 p,
                              f3 n, f3 wi, int mat, float g, int med_in, int med_out) {
     const float ul = get_1d(s.sampler, ss);  // SampleLight's Get1D runs even with no lights
     if (s.n_lights <= 0)
@@ -1236,7 +1254,7 @@ __device__ void light_record(const SceneDev& s, PathsDev& P, SamplerState& ss, i
     P.seg_med[0][r] = medium_for(ls.wo, n, med_in, med_out);
 }
 
-__device__ void save_vertex(PathsDev& P, int32_t i, int kind, f3 p, f3 n, f3 wi, int mat,
+__device__ __forceinline__ void save_vertex(PathsDev& P, int32_t i, int kind, f3 p, f3 n, f3 wi, int mat,
                             int med_in, int med_out) {
     float* v = P.vtx + 9 * (int64_t)i;
     v[0] = p.x, v[1] = p.y, v[2] = p.z;
@@ -1402,9 +1420,10 @@ __global__ __launch_bounds__(256) void k_shade_rw_gen(SceneDev s, PathsDev P, in
                 next_med = medium_for(rd, h.n, min_, mout);
                 alive = true;
             } else {
-                MaterialDev m = s.materials[sh.material];
-                if (m.has_nodes)
-                    apply_nodes(s, m, h, hit.prim, hit.t, uv.u, uv.v, ro, rd);
+                const MaterialDev* mp = s.materials + sh.material;
+                if (mp->has_nodes)
+                    mp = apply_nodes(s, P, k, mp, h, hit.prim, hit.t, uv.u, uv.v, ro, rd);
+                const MaterialDev& m = *mp;
                 const f3 wi = -rd;
                 if (m.kind == kMatEmissive) {
                     L = L + beta * emitted(m, h.n, wi);
@@ -3322,6 +3341,10 @@ int ensure_paths(pg_scene& S, int64_t n) {
             (rc = dalloc(S, n, &P.seg_med[0])) || (rc = dalloc(S, n, &P.seg_med[1])) ||
             (rc = dalloc(S, n, &P.seg_hits)) || (rc = dalloc(S, n, &P.seg_uvs)) ||
             (rc = dalloc(S, 1, &P.lookups)))
+            return rc;
+    }
+    if (!S.node_progs.empty()) {
+        if ((rc = dalloc(S, n, &P.node_mat)) || (rc = dalloc(S, nodes::kStack * n, &P.node_stack)))
             return rc;
     }
     S.path_capacity = n;

@@ -216,3 +216,43 @@ def test_viz_bvh_heat_map_is_on_the_colour_map(pg, cuda, tmp_path):
     nodes = heat[below] * 100
     assert np.abs(nodes - np.round(nodes)).max() < 2e-3
     print(f"Viz Bvh: mean nodes {np.mean(np.minimum(heat, 1)) * 100:.1f}, saturated {1 - below.mean():.3f}")
+
+
+def as_node_graphs(text):
+    """Every constant albedo, roughness and eta of a scene text rewritten as an
+    Add node of the constant and 0 (node.h Add): the same values, computed by
+    the node-graph interpreter at every shading point."""
+    import re
+    out = []
+    for line in text.splitlines():
+        m = re.match(r"(\s*)(albedo|roughness|eta): ([-0-9.e ]+)$", line)
+        if m:
+            ind, key, val = m.groups()
+            out += [f"{ind}{key}: Add{{", f"{ind}    input: {val}", f"{ind}    factor: 0", f"{ind}}}"]
+        else:
+            out.append(line)
+    return "\n".join(out) + "\n"
+
+
+@pytest.mark.parametrize("mode", ["path", "spectral", "randomwalk"])
+def test_constant_node_graphs_render_the_same_bits(pg, cuda, tmp_path, mode):
+    """A node graph that evaluates to its constant renders the same image, bit
+    for bit, as the constant itself, in each kernel that runs the interpreter
+    (the fused Path bounce, the spectral bounce, RandomWalk): each shading
+    slot's material copy and value stack in device memory hold that shading
+    point's own material."""
+    from pine_amd import workloads as W
+    if mode == "randomwalk":
+        text = golden("render_randomwalk")["scene"].item().decode()
+    else:
+        text = W.spheres_scene_text(seed=0, width=64, height=48, spp=8, n_spheres=8)
+    nodes = as_node_graphs(text)
+    assert nodes.count("Add{") >= 3
+    films = []
+    for t in (text, nodes):
+        s = pg.Scene(text=t, directory=str(tmp_path))
+        s.render(spectral=mode == "spectral")
+        films.append(s.film())
+        s.release()
+    assert np.isfinite(films[0]).all() and films[0][..., :3].max() > 0
+    assert np.array_equal(films[0].view(np.int32), films[1].view(np.int32))

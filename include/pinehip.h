@@ -199,6 +199,7 @@ typedef struct pg_scene_info {
     int64_t paths_in_flight; /* paths per batch when samples_per_batch is automatic */
     int32_t trace_block, trace_flags, bvh_width;  /* traversal launch and node width */
     int64_t last_extension_rays;  /* path vertices traced by the last render (closest-hit rays) */
+    int32_t apply_tone_mapping;   /* the film's applyToneMapping (film.cpp:11), default 1 */
 } pg_scene_info;
 int pg_scene_get_info(const pg_scene* s, pg_scene_info* info);
 

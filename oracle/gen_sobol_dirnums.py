@@ -12,7 +12,7 @@ every column pine's sampler can reach (columns 0..31: pine's Sobol index is a
 32-bit value, lowdiscrepancy.h:139-158).  Dimension 0 is the van der Corput
 sequence and needs no numbers.
 
-Run once, where /root/reference exists:  python pine_amd/csrc/gen_sobol_dirnums.py
+Run once, where /root/reference exists:  python oracle/gen_sobol_dirnums.py
 """
 import os
 import re
@@ -21,7 +21,8 @@ import sys
 import numpy as np
 
 REF = "/root/reference/src/util/sobolmetrices.cpp"
-OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "pg_sobol_dirnums.h")
+OUT = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "pine_amd", "csrc",
+                   "pg_sobol_dirnums.h")
 NDIM, NCOL = 1024, 52
 
 
@@ -106,7 +107,7 @@ def main():
     for s, a, m in rows:
         stream += [s, a] + m
     with open(OUT, "w") as f:
-        f.write("// Generated by gen_sobol_dirnums.py: Joe & Kuo (2008) Sobol direction numbers\n")
+        f.write("// Generated by oracle/gen_sobol_dirnums.py: Joe & Kuo (2008) Sobol direction numbers\n")
         f.write("// (new-joe-kuo-6.21201, as pine's src/util/sobolmetrices.cpp holds them expanded)\n")
         f.write("// for dimensions 1..1023, each as {s, a, m_1..m_s}: degree, the polynomial's\n")
         f.write("// middle coefficients a_1..a_{s-1} (a_1 the top bit) and the initial odd m_k < 2^k.\n")

@@ -40,6 +40,12 @@
 //       linear float image; pine itself only writes tone-mapped 8-bit BMPs.
 //       out.bin: i32 w, i32 h, then h*w*4 floats (sum w*r, sum w*g, sum w*b,
 //       sum w) in pine's pixel order (row 0 = top, film.h:60-62).
+//   finalize <film.bin> <out.bmp> <toneMapping 0|1>
+//       pine's own image output for a film of render's layout: a Film with
+//       applyToneMapping as given whose pixels (Film::GetPixel, film.h:59-61)
+//       hold the film's sums, then Film::Finalize(1) (film.cpp:45-100: divide
+//       by the weight, Uncharted2 filmic curve, gamma 1/2.2 with pstd::pow,
+//       SaveImage -> WriteImageBMP, fileio.cpp:88-184).
 //
 // File formats (little endian):
 //   scene.bin: u32 'PGS1', u32 nshapes, then per shape u32 type and payload:
@@ -52,6 +58,7 @@
 #include <core/scene.h>
 #include <core/integrator.h>
 #include <core/filter.h>
+#include <core/film.h>
 #include <util/fileio.h>
 #include <core/sampler.h>
 #include <core/noise.h>
@@ -646,6 +653,36 @@ int ModeRender(int argc, char** argv) {
     return 0;
 }
 
+int ModeFinalize(int argc, char** argv) {
+    if (argc < 5)
+        return 2;
+    FILE* f = fopen(argv[2], "rb");
+    if (!f)
+        return 1;
+    int32_t hdr[2];
+    if (fread(hdr, 4, 2, f) != 2)
+        return 1;
+    const vec2i size(hdr[0], hdr[1]);
+    std::vector<float> acc((size_t)size.x * size.y * 4);
+    if (fread(acc.data(), sizeof(float), acc.size(), f) != acc.size())
+        return 1;
+    fclose(f);
+    Film film(size, CreateFilter(Parameters()), argv[3], atoi(argv[4]) != 0, false);
+    film.Clear();
+    for (int row = 0; row < size.y; row++)
+        for (int x = 0; x < size.x; x++) {
+            // Row 0 of the film file is the top: GetPixel's y = size.y - 1 - row.
+            const float* a = &acc[((size_t)row * size.x + x) * 4];
+            Pixel& px = film.GetPixel(vec2i(x, size.y - 1 - row));
+            px.rgb[0] = a[0];
+            px.rgb[1] = a[1];
+            px.rgb[2] = a[2];
+            px.weight = a[3];
+        }
+    film.Finalize(1.0f);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -670,6 +707,8 @@ int main(int argc, char** argv) {
         return ModeNoise(argc, argv);
     if (mode == "render")
         return ModeRender(argc, argv);
+    if (mode == "finalize")
+        return ModeFinalize(argc, argv);
     fprintf(stderr, "unknown mode\n");
     return 2;
 }

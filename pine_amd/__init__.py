@@ -71,7 +71,8 @@ class _SceneInfo(ctypes.Structure):
                 ("guide_steps", ctypes.c_int64), ("guide_records", ctypes.c_int64),
                 ("graph_captures", ctypes.c_int64), ("paths_in_flight", ctypes.c_int64),
                 ("trace_block", ctypes.c_int32), ("trace_flags", ctypes.c_int32),
-                ("bvh_width", ctypes.c_int32), ("last_extension_rays", ctypes.c_int64)]
+                ("bvh_width", ctypes.c_int32), ("last_extension_rays", ctypes.c_int64),
+                ("apply_tone_mapping", ctypes.c_int32)]
 
 
 class _GuidedOptions(ctypes.Structure):

@@ -3460,6 +3460,7 @@ extern "C" int pg_scene_get_info(const pg_scene* S, pg_scene_info* info) {
     info->graph_captures = S->graph_captures;
     info->last_extension_rays = S->last_ext_rays;
     info->paths_in_flight = S->batch_paths;
+    info->apply_tone_mapping = S->tone_map ? 1 : 0;
     pg_accel_stats as{};
     if (pg_accel_get_stats(S->accel, &as) == PG_OK) {
         info->bvh_width = as.width;

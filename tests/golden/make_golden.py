@@ -38,6 +38,9 @@ Cases
   render_randomwalk_cloud  pine's RandomWalk integrator in the cloud scene (48x48, 256 spp)
   slab_through, slab_shadow   constant-density grid scenes whose expectation
               has a closed form (transmittance exp(-sigma d L)), 48x48, 1024 spp
+  bmp         pine's tone-mapped BMP output (Film::Finalize) of the spheres,
+              mixed and nodes golden films and of a synthetic film, tone
+              mapping on and off
 
 Trace fixtures store ray *indices* into the deterministic ray generator plus a
 checksum of the selected rays, and pine's answers: hit shape, prim within the
@@ -513,6 +516,50 @@ def case_slab(view):
     print(f"slab_{view}:", line, flush=True)
 
 
+def synthetic_film(w=64, h=40, seed=17):
+    """A film of accumulators spanning what Finalize meets: radiance
+    log-uniform over 1e-5..1e4 per channel, black pixels, weights of several
+    magnitudes and a row of zero-weight pixels."""
+    rng = np.random.default_rng(seed)
+    wt = rng.choice(np.float32([0.25, 1.0, 7.5, 64.0, 1000.0]), (h, w))
+    rgb = np.exp(rng.uniform(np.log(1e-5), np.log(1e4), (h, w, 3)))
+    rgb[rng.random((h, w)) < 0.05] = 0.0
+    film = np.concatenate([rgb * wt[..., None], wt[..., None]], -1).astype(np.float32)
+    film[h // 2] = 0.0
+    return film
+
+
+def harness_finalize(film, tone_map):
+    """pine's Film::Finalize of a film (BMP bytes), through the harness."""
+    h, w, _ = film.shape
+    with tempfile.TemporaryDirectory() as d:
+        fp, bp = os.path.join(d, "film.bin"), os.path.join(d, "out.bmp")
+        with open(fp, "wb") as f:
+            f.write(np.array([w, h], np.int32).tobytes())
+            f.write(np.ascontiguousarray(film, np.float32).tobytes())
+        subprocess.run([HARNESS, "finalize", fp, bp, str(int(tone_map))], check=True,
+                       capture_output=True)
+        return np.fromfile(bp, np.uint8)
+
+
+def golden_film(name):
+    return np.load(os.path.join(OUT, name + ".npz"))["film"]
+
+
+def case_bmp():
+    """pine's BMP output (Film::Finalize + SaveImage, film.cpp:45-100,
+    fileio.cpp:88-184) of golden films and a synthetic one, with and without
+    the filmic tone mapping."""
+    out = {"synthetic_film": synthetic_film()}
+    films = {"spheres": golden_film("render_spheres"), "mixed": golden_film("render_mixed"),
+             "nodes": golden_film("render_nodes"), "synthetic": out["synthetic_film"]}
+    for name, film in films.items():
+        for tm in (1, 0):
+            out[f"{name}_tm{tm}"] = harness_finalize(film, tm)
+    np.savez_compressed(os.path.join(OUT, "bmp.npz"), **out)
+    print("bmp:", ", ".join(k for k in out), flush=True)
+
+
 CASES = {
     "trace_5k": lambda: case_trace("trace_5k", W.ICOSPHERE_5K, W.RAYS_SMALL),
     "trace_82k": lambda: case_trace("trace_82k", W.ICOSPHERE_82K, W.RAYS_SMALL),
@@ -537,6 +584,7 @@ CASES = {
     "other_integrators": case_other_integrators,
     "slab_through": lambda: case_slab("through"),
     "slab_shadow": lambda: case_slab("shadow"),
+    "bmp": case_bmp,
 }
 
 if __name__ == "__main__":

@@ -22,7 +22,8 @@ pytestmark = pytest.mark.gpu
 def test_atmosphere_matches_pine_with_the_accurate_table(pg, cuda, tmp_path, name):
     fx = golden("render_atmosphere")
     film, info = render_text(pg, fx[name + "_scene"].item().decode(), str(tmp_path))
-    e = check_film(film, fx[name + "_film"], 1e-3)
+    # measured 2.5e-4 (accurate_interp: pine's atan2 near the poles) and 6.5e-6
+    e = check_film(film, fx[name + "_film"], {"accurate_interp": 1e-3}.get(name, 6e-5))
     print(f"Atmosphere {name}: relMSE {e:.3e}")
 
 

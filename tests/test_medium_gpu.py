@@ -5,7 +5,8 @@ slab_*) and against closed-form transmittance.
 
 Bars:
   - pine's tracking (global majorant) replays pine's sample sequence: relMSE
-    vs pine < 1e-3 and the same non-finite pixels (pine's own NaNs);
+    vs pine about 10x above what the kernels reach (1e-7 for the cloud, 1e-12
+    for the slabs) and the same non-finite pixels (pine's own NaNs);
   - brick majorants are unbiased but consume a different number of samples:
     checked against pine's image statistically and against the closed forms;
   - constant-density grids: pixel means of the closed form exp(-sigma tau)
@@ -68,7 +69,7 @@ def test_cloud_matches_pine(pg, cuda, tmp_path):
           f"{int((~fin_o).sum())} pine {int((~fin_r).sum())}")
     assert (~fin_o).sum() <= (~fin_r).sum() + 2
     assert both.mean() > 0.99
-    assert e < 1e-3
+    assert e < 1e-7  # measured 1.0e-8
 
 
 def test_randomwalk_through_media_matches_pine(pg, cuda, tmp_path):
@@ -86,7 +87,7 @@ def test_randomwalk_through_media_matches_pine(pg, cuda, tmp_path):
     exact = float((film[..., :3] == ref[..., :3]).all(-1).mean())
     print(f"RandomWalk cloud relMSE {e:.3e}, bit-exact pixels {exact:.4f}")
     assert both.mean() > 0.99
-    assert e < 1e-3
+    assert e < 1e-10  # measured 6.7e-12
 
 
 def test_cloud_bricks_agree_with_pine(pg, cuda, tmp_path):
@@ -129,7 +130,7 @@ def test_slab_closed_form(pg, cuda, tmp_path, view, bricks):
         ref = linear(fx["film"]).mean(-1)
         e = rel_mse(img, ref)
         print(f"  vs pine relMSE {e:.3e}")
-        assert e < 1e-3
+        assert e < 1e-12  # measured 0 (through) and 5.8e-16 (shadow)
 
 
 def test_general_mode_reproduces_fused_mode(pg, cuda, tmp_path):

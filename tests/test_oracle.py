@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 from conftest import golden
+from ties import certify_ties, mesh_prim_geom, single_prim
 
 
 def ulps(a, b):
@@ -100,9 +101,10 @@ def test_brute_force_matches_pine_bvh(pg, oracle, case):
     hit = fx["prim"] >= 0
     assert np.array_equal(prim >= 0, hit)
     same = prim == fx["prim"]
-    # Certified ties only: pine's own t for both ids must be bit-equal.
-    assert same.mean() > 0.999
     assert np.array_equal(t[hit].view(np.int32), fx["t"][hit].view(np.int32))
+    # Certified ties only: both ids hit at pine's t, the smaller one kept.
+    ties = certify_ties(oracle, rays, prim, fx["prim"], fx["t"], mesh_prim_geom(oracle, v, i))
+    print(f"{case}: {ties} certified ties")
     assert np.array_equal(u[hit & same].view(np.int32), fx["u"][hit & same].view(np.int32))
     assert np.array_equal(vv[hit & same].view(np.int32), fx["v"][hit & same].view(np.int32))
     assert np.array_equal(occ, fx["occluded"].astype(np.int32))
@@ -138,8 +140,14 @@ def test_brute_force_shapes_match_pine(oracle):
                    np.where(fx["shape"] >= 0, len(tri) + fx["shape"], -1))
     hit = exp >= 0
     assert np.array_equal(prim >= 0, hit)
-    assert (prim == exp).mean() > 0.999
     assert np.array_equal(t[hit].view(np.int32), fx["t"][hit].view(np.int32))
+
+    def geom(pid):
+        if pid < len(tri):
+            return single_prim(tri=tri[pid])
+        return single_prim(ana=ana[pid - len(tri)])
+    ties = certify_ties(oracle, fx["rays"], prim, exp, fx["t"], geom)
+    print(f"shapes: {ties} certified ties")
     assert np.array_equal(occ, fx["occluded"].astype(np.int32))
 
 
@@ -164,8 +172,8 @@ def test_cpu_bvh_port_agrees_with_brute_force(pg, oracle):
     b = oracle.CpuBVH(tri)
     p2, t2, _ = b.trace(rays)
     assert np.array_equal(t1.view(np.int32), t2.view(np.int32))
-    tie_free = p1 == p2
-    assert tie_free.mean() > 0.999
+    ties = certify_ties(oracle, rays, p1, p2, t1, mesh_prim_geom(oracle, v, i))
+    print(f"port vs brute force: {ties} certified ties")
 
 
 def test_halton_matches_pine(oracle):
@@ -324,3 +332,20 @@ def test_guide_oracle_vmf_closed_forms():
                          rng.uniform(size=n))
         mean_cos = (s @ mu[0, 0]).mean()
         assert abs(mean_cos - (1 / np.tanh(kappa) - 1 / kappa)) < 4e-3
+
+
+def test_tie_certifier_accepts_ties_and_rejects_other_ids(oracle):
+    """certify_ties on two exact duplicate triangles (ids 0 and 3) and a third
+    triangle behind them (id 1): id 3 in place of 0 is a certified tie, id 1
+    (hit, but farther) is not, nor is the larger id kept on a tie."""
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [0, 1, 1]], np.float32)
+    i = np.array([[0, 1, 2], [3, 4, 5], [2, 1, 0], [0, 1, 2]], np.uint32)
+    rays = np.array([[0.2, 0.2, -1, 0, 0, 0, 1, 3.4e38]], np.float32)
+    prim, t, *_ = oracle.trace_brute(oracle.tri9(v, i), rays)
+    assert prim.tolist() == [0] and t[0] == np.float32(1.0)
+    geom = mesh_prim_geom(oracle, v, i)
+    assert certify_ties(oracle, rays, prim, np.array([3], np.int32), t, geom) == 1
+    with pytest.raises(AssertionError, match="uncertified"):
+        certify_ties(oracle, rays, prim, np.array([1], np.int32), t, geom)
+    with pytest.raises(AssertionError, match="larger id"):
+        certify_ties(oracle, rays, np.array([3], np.int32), prim, t, geom)

@@ -3,8 +3,13 @@ files (tests/golden/render_*.npz, written by pine's PathIntegrator through the
 oracle harness with the film's filter weights).
 
 Same scene text, same Halton sequence per pixel: the filter weights must agree
-to float rounding and the linear images to a relative MSE far below the noise
-of an independent render."""
+to float rounding, the linear images to a relative MSE far below the noise of
+an independent render, and a floor of pixels must be bit-identical to pine's.
+Each bar sits about 10x above what the kernels reach on MI355X (noted beside
+it), so a regression in the shading, MIS or filter arithmetic fails the test
+long before it would be visible: paths whose rounding differs from pine's in
+the last bit and then branch (Russian roulette, a dielectric's reflect/refract
+choice) are what the remaining difference consists of."""
 import os
 
 import numpy as np
@@ -33,7 +38,9 @@ def render_text(pg, text, directory, spp=0, spb=0):
     return film, info
 
 
-def check_film(film, ref, tol):
+def check_film(film, ref, tol, min_exact=0.0):
+    """relMSE of the linear images below `tol`, filter weights equal to float
+    rounding, and at least `min_exact` of the pixels bit-identical to pine's."""
     assert film.shape == ref.shape
     assert np.isfinite(film).all()
     e = rel_mse(linear(film), linear(ref))
@@ -42,13 +49,14 @@ def check_film(film, ref, tol):
     print(f"relMSE {e:.3e}, weight mismatches {wdiff}, bit-exact pixels {exact:.4f}")
     np.testing.assert_allclose(film[..., 3], ref[..., 3], rtol=1e-5, atol=1e-6)
     assert e < tol, f"relMSE {e:.3e} >= {tol}"
+    assert exact >= min_exact, f"bit-exact pixels {exact:.4f} < {min_exact}"
     return e
 
 
 def test_spheres_scene_matches_pine(pg, cuda, tmp_path):
     fx = golden("render_spheres")
     film, info = render_text(pg, fx["scene"].item().decode(), str(tmp_path))
-    e = check_film(film, fx["film"], 1e-3)
+    e = check_film(film, fx["film"], 2.5e-4, 0.35)  # measured 2.7e-5, 0.44 exact
     print(f"spheres relMSE {e:.3e}, {info['last_rays']} rays in {info['last_render_ms']:.1f} ms")
 
 
@@ -61,7 +69,7 @@ def test_mixed_scene_matches_pine(pg, cuda, tmp_path):
     v, i = mg.mixed_mesh()
     W.write_obj(str(tmp_path / "blob.obj"), v, i)
     film, info = render_text(pg, fx["scene"].item().decode(), str(tmp_path))
-    e = check_film(film, fx["film"], 1e-3)
+    e = check_film(film, fx["film"], 1.5e-5, 0.2)  # measured 1.2e-6, 0.27 exact
     print(f"mixed relMSE {e:.3e}")
 
 
@@ -71,7 +79,8 @@ def test_film_filters_match_pine(pg, cuda, tmp_path, name):
     several pixels) against pine's films of the same scene."""
     fx = golden("render_filters")
     film, info = render_text(pg, fx[name + "_scene"].item().decode(), str(tmp_path))
-    e = check_film(film, fx[name + "_film"], 1e-3)
+    # measured 3.9e-7 (gaussian) .. 1.4e-6 (mitchell), 0.31..0.43 exact
+    e = check_film(film, fx[name + "_film"], 1.5e-5, 0.25)
     print(f"filter {name}: relMSE {e:.3e}")
 
 
@@ -82,7 +91,7 @@ def test_node_graph_materials_match_pine(pg, cuda, tmp_path):
     from the arithmetic nodes, against pine's film of the same scene text."""
     fx = golden("render_nodes")
     film, info = render_text(pg, fx["scene"].item().decode(), str(tmp_path))
-    e = check_film(film, fx["film"], 1e-3)
+    e = check_film(film, fx["film"], 3e-6, 0.1)  # measured 2.7e-7, 0.14 exact
     print(f"node-graph scene relMSE {e:.3e} vs pine")
 
 
@@ -106,7 +115,9 @@ def test_4096_spp_images_match_pine(pg, cuda, tmp_path, name):
         v, i = mg.mixed_mesh()
         W.write_obj(str(tmp_path / "blob.obj"), v, i)
     film, info = render_text(pg, fx[name + "_scene"].item().decode(), str(tmp_path))
-    e = check_film(film, fx[name + "_film"], 1e-3)
+    # measured 8.0e-7 / 2.0e-6, 0.39 / 0.22 exact; the north-star bar is 1e-3
+    tol, floor = {"spheres": (1e-5, 0.3), "mixed": (2e-5, 0.18)}[name]
+    e = check_film(film, fx[name + "_film"], tol, floor)
     print(f"{name} at 4096 spp: relMSE {e:.3e} vs pine")
 
 
@@ -131,7 +142,7 @@ def test_occluded_box_matches_pine(pg, cuda, tmp_path):
     close = float((np.abs(a - b) <= 1e-4 * np.abs(b) + 1e-7).all(-1).mean())
     print(f"box relMSE {e:.3e} over finite pixels, {close:.4f} within 1e-4, "
           f"{info['last_render_ms']:.1f} ms")
-    assert e < 1e-3 and close > 0.99
+    assert e < 1e-8 and close > 0.99  # measured 8.2e-10, 0.9985 within 1e-4
 
 
 def test_batching_does_not_change_the_image(pg, cuda, tmp_path):
@@ -165,6 +176,14 @@ Camera: ThinLen{
         pg.Scene(text=text)
 
 
+# (relMSE bar, bit-exact pixel floor): AO is bit-exact; the Viz images measured
+# normal 9.2e-8 / 0.92, position 2.8e-7 / 0.35, texcoord 7.1e-7 / 0.35,
+# dpdu 1.0e-7 / 0.92, dpdv 5.9e-8 / 0.92.
+AO_VIZ_BARS = {"render_ao": (1e-30, 1.0), "render_viz_normal": (1e-6, 0.85),
+               "render_viz_position": (3e-6, 0.3), "render_viz_texcoord": (7e-6, 0.3),
+               "render_viz_dpdu": (1e-6, 0.85), "render_viz_dpdv": (1e-6, 0.85)}
+
+
 @pytest.mark.parametrize("name", ["render_ao", "render_viz_normal", "render_viz_position",
                                   "render_viz_texcoord", "render_viz_dpdu", "render_viz_dpdv"])
 def test_ao_and_viz_match_pine(pg, cuda, tmp_path, name):
@@ -178,14 +197,14 @@ def test_ao_and_viz_match_pine(pg, cuda, tmp_path, name):
     v, i = mg.mixed_mesh()
     W.write_obj(str(tmp_path / "blob.obj"), v, i)
     film, info = render_text(pg, fx["scene"].item().decode(), str(tmp_path))
-    e = check_film(film, fx["film"], 1e-3)
+    e = check_film(film, fx["film"], *AO_VIZ_BARS[name])
     print(f"{name} relMSE {e:.3e}")
 
 
 def test_randomwalk_matches_pine(pg, cuda, tmp_path):
     fx = golden("render_randomwalk")
     film, info = render_text(pg, fx["scene"].item().decode(), str(tmp_path))
-    e = check_film(film, fx["film"], 1e-3)
+    e = check_film(film, fx["film"], 6e-4, 0.55)  # measured 6.4e-5, 0.64 exact
     print(f"randomwalk relMSE {e:.3e}")
 
 

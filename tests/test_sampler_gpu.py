@@ -85,5 +85,6 @@ def test_halton_without_permutation_is_the_radical_inverse(pg, cuda):
 def test_sobol_render_matches_pine(pg, cuda, tmp_path, name):
     fx = golden("render_sobol")
     film, info = render_text(pg, fx[name + "_scene"].item().decode(), str(tmp_path))
-    e = check_film(film, fx[name + "_film"], 1e-3)
+    # measured 8.4e-6 (fastOwen) and 2.4e-5 (Owen), 0.43 exact
+    e = check_film(film, fx[name + "_film"], 2.5e-4, 0.35)
     print(f"Sobol {name} relMSE {e:.3e}")

@@ -216,6 +216,7 @@ def batch_leg(args, pg, W, work):
     each scene first autotuned (pg_scene_autotune)."""
     t0 = time.time()
     items = W.batch_workload(args.batch_scenes)
+    W.write_box_obj(os.path.join(work, W.BOX_OBJ))
     scenes = []
     for it in items:
         v, i = W.make_mesh(it["mesh"])
@@ -244,7 +245,8 @@ def batch_leg(args, pg, W, work):
         s.release()
     out = {
         "workload": f"{len(items)} scenes: displaced icosphere of log-uniform 10k..2M triangles "
-                    f"in a seeded 16-sphere scene, 256^2..1024^2, 16..128 spp, maxDepth 8",
+                    f"and 4..24 instanced boxes in a seeded 16-sphere scene, 256^2..1024^2, "
+                    f"16..128 spp, maxDepth 8",
         "triangles": int(sum(it["triangles"] for it in items)), "camera_samples": int(samples),
         "samples_per_s": round(samples / (res["ms"] * 1e-3), 1),
         "mrays_per_s": round(res["rays"] / (res["ms"] * 1e-3) * 1e-6, 2),

@@ -14,6 +14,8 @@
 //       before it trusts the replay's triangle index.
 //   time   <scene.bin> <rays.bin> [threads]
 //       Times the real BVH::Intersect over the ray file (CPU Mrays/s).
+//   (time and render wait for a line on stdin before their timed region when
+//   PG_HARNESS_GATE is set; see Gate().)
 //   math   <out.bin>
 //       pstd libm outputs (pstd/math.h:141-351) on a fixed input grid.
 //   halton <out.bin> <filmW> <filmH> <spp> <ndims> <px0> <py0> <npix>
@@ -271,6 +273,20 @@ void ParallelRange(int64_t n, int threads, F&& f) {
         th.join();
 }
 
+// With PG_HARNESS_GATE set in the environment, the timed modes print "ready"
+// once their scene and BVH are built and wait for a line on stdin before the
+// timed region, so a driver can build several in parallel and time them one
+// at a time.
+void Gate() {
+    if (!getenv("PG_HARNESS_GATE"))
+        return;
+    printf("ready\n");
+    fflush(stdout);
+    char buf[16];
+    if (!fgets(buf, sizeof(buf), stdin))
+        exit(5);
+}
+
 double Now() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
         .count();
@@ -349,6 +365,7 @@ int ModeTime(int argc, char** argv) {
     bvh.Initialize(&scene);
     double buildMs = (Now() - t0) * 1e3;
     std::atomic<int64_t> nhit{0};
+    Gate();
     double t1 = Now();
     ParallelRange((int64_t)rays.size(), threads, [&](int64_t i) {
         Ray ray = ToRay(rays[i]);
@@ -593,6 +610,7 @@ int ModeRender(int argc, char** argv) {
                 {(x + 0.5f) / TW * filter.Radius().x, (y + 0.5f) / TW * filter.Radius().y});
     std::vector<float> acc((size_t)size.x * size.y * 4, 0.0f);
     std::vector<std::mutex> locks(1024);
+    Gate();
     double t0 = Now();
     std::atomic<int64_t> next{0};
     std::vector<std::thread> pool;

@@ -173,9 +173,12 @@ def batch_workload(n=BATCH_SCENES, seed=7, min_triangles=10_000, max_triangles=2
     """The batched-rendering workload: n scenes of uneven size.  Scene k is the
     seeded spheres scene (materials and light as the spheres workload, 16
     spheres) with a displaced icosphere at its centre whose triangle count is
-    log-uniform in [min_triangles, max_triangles]; square resolution uniform in
-    {256, 384, ..., 1024}, spp uniform in {16, 24, ..., 128}, maxDepth 8.
-    Returns one dict per scene; batch_scene_text writes its scene file."""
+    log-uniform in [min_triangles, max_triangles], plus 4..24 instances of one
+    box mesh (BOX_OBJ, 12 triangles) standing on the ground, each scaled
+    U(0.3, 2.5) per axis and given one of the scene's sphere materials;
+    square resolution uniform in {256, 384, ..., 1024}, spp uniform in
+    {16, 24, ..., 128}, maxDepth 8.  Returns one dict per scene;
+    batch_scene_text writes its scene file."""
     rng = np.random.default_rng(seed)
     out = []
     for k in range(n):
@@ -183,20 +186,58 @@ def batch_workload(n=BATCH_SCENES, seed=7, min_triangles=10_000, max_triangles=2
         res = int(rng.choice(np.arange(256, 1025, 128)))
         spp = int(rng.choice(np.arange(16, 129, 8)))
         freq = int(np.ceil(np.sqrt(tris / 20.0)))
+        brng = np.random.default_rng(seed * 1000 + k + 500_000)
+        boxes = []
+        for _ in range(int(brng.integers(4, 25))):
+            sc = brng.uniform(0.3, 2.5, 3)
+            pos = (float(brng.uniform(-12, 12)), float(sc[1] / 2), float(brng.uniform(-12, 12)))
+            boxes.append(dict(scale=tuple(float(x) for x in sc), position=pos,
+                              material=int(brng.integers(0, 16))))
         out.append(dict(index=k, triangles=tris, mesh=MeshSpec(freq, 0.1, seed * 1000 + k, tris),
-                        width=res, height=res, spp=spp, scene_seed=seed * 1000 + k))
+                        width=res, height=res, spp=spp, scene_seed=seed * 1000 + k, boxes=boxes))
     return out
 
 
-def batch_scene_text(item, obj_file):
-    """Scene file of one batch_workload item (its mesh in obj_file)."""
+def reduced_batch_item(item, res=64, spp=32, max_triangles=40_000):
+    """A batch_workload item at a smaller film, spp and mesh (the icosphere
+    truncated to max_triangles), same scene otherwise: for parity renders
+    that pine itself finishes in seconds."""
+    it = dict(item)
+    tris = min(item["triangles"], max_triangles)
+    m = item["mesh"]
+    it.update(width=res, height=res, spp=spp, triangles=tris,
+              mesh=MeshSpec(m.freq, m.amp, m.seed, tris))
+    return it
+
+
+BOX_OBJ = "box.obj"
+
+
+def write_box_obj(path):
+    """The unit cube [-0.5, 0.5]^3 as 8 vertices and 12 outward triangles."""
+    v = [(x, y, z) for x in (-0.5, 0.5) for y in (-0.5, 0.5) for z in (-0.5, 0.5)]
+    f = [(1, 2, 4), (1, 4, 3), (5, 7, 8), (5, 8, 6), (1, 5, 6), (1, 6, 2),
+         (3, 4, 8), (3, 8, 7), (1, 3, 7), (1, 7, 5), (2, 6, 8), (2, 8, 4)]
+    with open(path, "w") as fh:
+        for p in v:
+            fh.write("v " + " ".join(_fmt(x) for x in p) + "\n")
+        for t in f:
+            fh.write("f %d %d %d\n" % t)
+
+
+def batch_scene_text(item, obj_file, box_file=BOX_OBJ):
+    """Scene file of one batch_workload item (its icosphere in obj_file, the
+    instanced box in box_file)."""
     text = spheres_scene_text(seed=item["scene_seed"], width=item["width"], height=item["height"],
                               spp=item["spp"], max_depth=8, n_spheres=16,
                               output=f"batch{item['index']}.bmp")
-    return text + "\n".join([
-        "Material batch_mesh: Layered{", "    layer0: Diffuse{", "        albedo: 0.6 0.55 0.5",
-        "    }", "}", "Shape: TriangleMesh{", f"    file: {obj_file}", "    scale: 3 3 3",
-        "    position: 0 3 0", "    material: batch_mesh", "}"]) + "\n"
+    out = ["Material batch_mesh: Layered{", "    layer0: Diffuse{", "        albedo: 0.6 0.55 0.5",
+           "    }", "}", "Shape: TriangleMesh{", f"    file: {obj_file}", "    scale: 3 3 3",
+           "    position: 0 3 0", "    material: batch_mesh", "}"]
+    for b in item.get("boxes", ()):
+        out += ["Shape: TriangleMesh{", f"    file: {box_file}", f"    scale: {_v3(b['scale'])}",
+                f"    position: {_v3(b['position'])}", f"    material: m{b['material']}", "}"]
+    return text + "\n".join(out) + "\n"
 
 
 def write_obj(path, vertices, indices):

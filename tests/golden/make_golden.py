@@ -38,6 +38,11 @@ Cases
   render_randomwalk_cloud  pine's RandomWalk integrator in the cloud scene (48x48, 256 spp)
   slab_through, slab_shadow   constant-density grid scenes whose expectation
               has a closed form (transmittance exp(-sigma d L)), 48x48, 1024 spp
+  render_media  GridMedium rayMarching, nearest-voxel (interpolate: false) and
+              both with a 3-voxel step (48x48, 256 spp, depth 16);
+              HomogeneousMedium in a material-less sphere and around the camera
+  render_batch  five scenes of the 64-scene batched workload at reduced size
+              (40..80 pixels square, 16..64 spp, icosphere <= 40,000 triangles)
   bmp         pine's tone-mapped BMP output (Film::Finalize) of the spheres,
               mixed and nodes golden films and of a synthetic film, tone
               mapping on and off
@@ -516,6 +521,119 @@ def case_slab(view):
     print(f"slab_{view}:", line, flush=True)
 
 
+def with_medium_keys(text, keys):
+    """The first Medium block of a scene text with extra `key: value` lines."""
+    lines = "".join(f"    {k}: {v}\n" for k, v in keys.items())
+    return text.replace("    scale: ", lines + "    scale: ", 1)
+
+
+def fog_sphere_scene_text(width=48, height=48, spp=256, max_depth=12):
+    """A material-less sphere filled with a HomogeneousMedium (medium.cpp:33-53:
+    Tr = exp(-sigma_t tmax), Sample picks a channel, then exp free flight) over
+    a diffuse ground under pine's Sky, seen from outside: rays and shadow rays
+    enter and leave the medium through the sphere (mediumInterface.inside),
+    with per-channel sigma_t so the chromatic branch is taken."""
+    return "\n".join([
+        "Integrator: Path{", f"    maxDepth: {max_depth}", "    sampler: Halton{",
+        f"        samplesPerPixel: {spp}", "    }", "}",
+        "Camera: ThinLen{", "    film{", "        outputFileName: fog.bmp",
+        f"        size: {width} {height}", "        filter: LanczosSinc{", "            radius: 0.5",
+        "        }", "    }", "    from: 0.5 1.2 6", "    to: 0 0.4 0", "    fov: 0.8", "}",
+        "Material ground: Layered{", "    layer0: Diffuse{", "        albedo: 0.5 0.45 0.4", "    }", "}",
+        "Material red: Layered{", "    layer0: Diffuse{", "        albedo: 0.8 0.2 0.1", "    }", "}",
+        "Medium fog: Homogeneous{", "    sigma_a: 0.05 0.1 0.2", "    sigma_s: 0.9 0.6 0.3",
+        "    g: 0.4", "}",
+        "Light: Environment{", "    environment: Sky{", "        sunDirection: -1 4 2",
+        "        sunColor: 1 1 1", "    }", "}",
+        "Shape: Rect{", "    position: 0 -1 0", "    ex: 0 0 40", "    ey: 40 0 0",
+        "    material: ground", "}",
+        "Shape: Sphere{", "    position: 0 0.4 0", "    radius: 1.3", "    medium: fog", "}",
+        "Shape: Sphere{", "    position: 1.6 -0.4 0.8", "    radius: 0.6", "    material: red", "}",
+    ]) + "\n"
+
+
+def fog_camera_scene_text():
+    """The spheres scene (12 spheres, area light) with the camera in a thin
+    chromatic HomogeneousMedium that every ray carries on through the
+    medium-less surfaces (bvh.cpp:702-706)."""
+    text = W.spheres_scene_text(seed=2, width=48, height=40, spp=256, max_depth=8, n_spheres=12)
+    text = text.replace("    fov: ", "    medium: fog\n    fov: ", 1)
+    return text + "\n".join(["Medium fog: Homogeneous{", "    sigma_a: 0.004 0.008 0.012",
+                             "    sigma_s: 0.03 0.02 0.01", "    g: -0.3", "}"]) + "\n"
+
+
+MEDIA_CASES = {
+    # name: (scene text builder, needs the 32^3 cloud volume)
+    "raymarch": (lambda: with_medium_keys(
+        W.cloud_scene_text("cloud32.vol", width=48, height=48, spp=256, max_depth=16,
+                           passthrough=True), {"samplingMethod": "rayMarching"}), True),
+    "raymarch_nearest_step3": (lambda: with_medium_keys(
+        W.cloud_scene_text("cloud32.vol", width=48, height=48, spp=256, max_depth=16),
+        {"samplingMethod": "rayMarching", "rayMarchingStepSize": "3", "interpolate": "false"}), True),
+    "nearest": (lambda: with_medium_keys(
+        W.cloud_scene_text("cloud32.vol", width=48, height=48, spp=256, max_depth=16,
+                           passthrough=True), {"interpolate": "false"}), True),
+    "fog_sphere": (fog_sphere_scene_text, False),
+    "fog_camera": (fog_camera_scene_text, False),
+}
+
+
+def case_render_media():
+    """GridMedium with samplingMethod: rayMarching (medium.cpp:82-93, 118-135),
+    with interpolate: false (nearest voxel, medium.cpp:141-160), both together
+    with a 3-voxel step, and HomogeneousMedium (medium.cpp:33-53) inside a
+    material-less sphere and around the camera."""
+    out = {}
+    for name, (make, vol) in MEDIA_CASES.items():
+        text = make()
+        with tempfile.TemporaryDirectory() as d:
+            if vol:
+                cloud_volume(d)
+            film, line = run_render(text, d)
+        out[name + "_scene"] = np.bytes_(text)
+        out[name + "_film"] = film
+        out[name + "_harness"] = np.bytes_(line)
+        print(f"render_media {name}:", line, flush=True)
+    np.savez_compressed(os.path.join(OUT, "render_media.npz"), **out)
+
+
+# Batch workload items rendered by pine at reduced size: (index, film side, spp).
+BATCH_GOLDEN = [(0, 64, 32), (9, 48, 64), (23, 80, 16), (41, 56, 48), (62, 40, 40)]
+
+
+def batch_golden_items():
+    items = W.batch_workload()
+    return [W.reduced_batch_item(items[k], res=r, spp=spp) for k, r, spp in BATCH_GOLDEN]
+
+
+def write_batch_files(d, item):
+    """The item's icosphere OBJ (batch<k>.obj) and the instanced box in d."""
+    v, i = W.make_mesh(item["mesh"])
+    name = f"batch{item['index']}.obj"
+    W.write_obj(os.path.join(d, name), v, i)
+    W.write_box_obj(os.path.join(d, W.BOX_OBJ))
+    return name
+
+
+def case_render_batch():
+    """Five scenes of the batched workload (displaced icosphere, 16 spheres,
+    instanced boxes; seeded as the 64-scene batch) at reduced film, spp and
+    mesh size, each rendered by pine's PathIntegrator."""
+    out = {}
+    for it in batch_golden_items():
+        with tempfile.TemporaryDirectory() as d:
+            name = write_batch_files(d, it)
+            text = W.batch_scene_text(it, name)
+            film, line = run_render(text, d)
+        key = f"b{it['index']}"
+        out[key + "_scene"] = np.bytes_(text)
+        out[key + "_film"] = film
+        out[key + "_harness"] = np.bytes_(line)
+        print(f"render_batch {key} ({it['triangles']} triangles, {len(it['boxes'])} boxes):", line,
+              flush=True)
+    np.savez_compressed(os.path.join(OUT, "render_batch.npz"), **out)
+
+
 def synthetic_film(w=64, h=40, seed=17):
     """A film of accumulators spanning what Finalize meets: radiance
     log-uniform over 1e-5..1e4 per channel, black pixels, weights of several
@@ -585,6 +703,8 @@ CASES = {
     "slab_through": lambda: case_slab("through"),
     "slab_shadow": lambda: case_slab("shadow"),
     "bmp": case_bmp,
+    "render_media": case_render_media,
+    "render_batch": case_render_batch,
 }
 
 if __name__ == "__main__":

@@ -90,6 +90,36 @@ def test_randomwalk_through_media_matches_pine(pg, cuda, tmp_path):
     assert e < 1e-10  # measured 6.7e-12
 
 
+# relMSE bars vs pine's films of the same scene text, about 10x what the
+# kernels reach (measured 6.4e-12, 5.7e-6, 4.8e-10, 3.7e-9, 4.4e-6).
+MEDIA_BARS = {"raymarch": 1e-10, "raymarch_nearest_step3": 6e-5, "nearest": 5e-9,
+              "fog_sphere": 4e-8, "fog_camera": 5e-5}
+
+
+@pytest.mark.parametrize("name", list(MEDIA_BARS))
+def test_media_variants_match_pine(pg, cuda, tmp_path, name):
+    """GridMedium's rayMarching method (medium.cpp:82-93, 118-135: fixed steps
+    with one jitter, exp of the summed optical depth), nearest-voxel lookups
+    (interpolate: false, medium.cpp:141-160), both with a 3-voxel step, and
+    HomogeneousMedium (medium.cpp:33-53) inside a material-less sphere and
+    around the camera, against pine's films: same sample sequence, same
+    non-finite pixels as pine (up to two more or fewer)."""
+    fx = golden("render_media")
+    film, info = render(pg, fx[name + "_scene"].item().decode(), cloud_dir(tmp_path))
+    assert info["general_mode"] == 1 and info["n_media"] == 1
+    ref = fx[name + "_film"]
+    np.testing.assert_allclose(film[..., 3], ref[..., 3], rtol=1e-5, atol=1e-6)
+    ours, theirs = linear(film), linear(ref)
+    fin_o, fin_r = np.isfinite(ours).all(-1), np.isfinite(theirs).all(-1)
+    both = fin_o & fin_r
+    e = rel_mse(ours[both], theirs[both])
+    exact = float((film[..., :3] == ref[..., :3]).all(-1).mean())
+    print(f"{name}: relMSE {e:.3e}, bit-exact pixels {exact:.4f}, non-finite ours "
+          f"{int((~fin_o).sum())} pine {int((~fin_r).sum())}")
+    assert abs(int((~fin_o).sum()) - int((~fin_r).sum())) <= 2
+    assert both.mean() > 0.99
+    assert e < MEDIA_BARS[name]
+
 def test_cloud_bricks_agree_with_pine(pg, cuda, tmp_path):
     """8-voxel majorant bricks on the 32^3 grid: a different sample sequence,
     the same expectation."""

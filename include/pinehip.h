@@ -355,6 +355,21 @@ int pg_guide_gradient(pg_guide* g, const float* d_inputs, const float* d_dirs,
  * h_loss receives the last step's mean loss.  Synchronous. */
 int pg_guide_train(pg_guide* g, const float* d_inputs, const float* d_dirs, const float* d_weight,
                    const float* d_pbsdf, int64_t n, int32_t steps, int32_t batch, float* h_loss);
+/* pg_guide_train without allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in pine or elsewhere in this ABI.  Same
+ * kernels and arithmetic as pg_guide_train, so the same parameters bit for
+ * bit, but: stream-ordered (no synchronisation; nothing is read back), the
+ * last step's mean loss goes to device memory d_loss (1 float, may be NULL),
+ * and each step's partial-gradient rows live in a static device array of
+ * PG_GUIDE_STATIC_MAX_BATCH / 64 rows (about 3.6 MB per GPU, held until the
+ * process ends), so batch must not exceed PG_GUIDE_STATIC_MAX_BATCH
+ * (PG_E_ARG otherwise).  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (other streams or threads) race. */
+#define PG_GUIDE_STATIC_MAX_BATCH 2048
+int pg_guide_train_static(pg_guide* g, const float* d_inputs, const float* d_dirs,
+                          const float* d_weight, const float* d_pbsdf, int64_t n, int32_t steps,
+                          int32_t batch, float* d_loss, void* stream);
 int pg_guide_release(pg_guide* g);
 
 /* Guided rendering (no counterpart in pine): `train_passes` passes of

@@ -32,11 +32,12 @@ EXPORTED = [
     "pg_guide_forward", "pg_guide_pdf", "pg_guide_sample", "pg_guide_gradient", "pg_guide_train",
     "pg_guide_release", "pg_render_guided", "pg_trace_closest_dev", "pg_trace_occluded_dev",
     "pg_accel_rebuild", "pg_render_batch", "pg_scene_autotune", "pg_write_obj",
-    "pg_spectral_tables", "pg_scene_sampler_values",
+    "pg_spectral_tables", "pg_scene_sampler_values", "pg_guide_train_static",
 ]
 RENDER_NO_GRAPH = 1
 RENDER_SPECTRAL = 2
 GUIDE_N_PARAMS = 27888
+GUIDE_STATIC_MAX_BATCH = 2048
 
 
 class PineError(RuntimeError):
@@ -159,6 +160,7 @@ def lib():
     L.pg_guide_sample.argtypes = [vp, vp, vp, i64, vp, vp]
     L.pg_guide_gradient.argtypes = [vp, vp, vp, vp, vp, i64, vp, ctypes.POINTER(ctypes.c_float)]
     L.pg_guide_train.argtypes = [vp, vp, vp, vp, vp, i64, i32, i32, ctypes.POINTER(ctypes.c_float)]
+    L.pg_guide_train_static.argtypes = [vp, vp, vp, vp, vp, i64, i32, i32, vp, vp]
     L.pg_guide_release.argtypes = [vp]
     L.pg_render_guided.argtypes = [vp, vp, ctypes.POINTER(_GuidedOptions), vp]
     L.pg_trace_closest_dev.argtypes = [vp, vp, vp, i64, vp, vp, vp]
@@ -494,6 +496,14 @@ class Guide:
         check(lib().pg_guide_train(self.h, d_x.ptr, d_dirs.ptr, d_weight.ptr, d_pbsdf.ptr,
                                    d_x.shape[0], steps, batch, ctypes.byref(loss)))
         return loss.value
+
+    def train_static(self, d_x, d_dirs, d_weight, d_pbsdf, steps, batch, d_loss=None, stream=None):
+        """pg_guide_train_static: the same steps with no device allocation,
+        stream-ordered; the last step's mean loss goes to device memory
+        d_loss (one float) when given.  batch <= GUIDE_STATIC_MAX_BATCH."""
+        check(lib().pg_guide_train_static(self.h, d_x.ptr, d_dirs.ptr, d_weight.ptr, d_pbsdf.ptr,
+                                          d_x.shape[0], steps, batch,
+                                          d_loss.ptr if d_loss is not None else None, stream))
 
     def release(self):
         if self.h:

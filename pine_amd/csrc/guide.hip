@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -491,27 +492,85 @@ const float* guide_params_dev(const pg_guide* G) { return G->params; }
 int guide_freq_offset(const pg_guide* G) { return G->oct; }
 const __bf16* guide_wf_dev(const pg_guide* G) { return G->wf; }
 
-// One Adam step on samples [first, first + count) of the arrays.
-int guide_train_step(pg_guide* G, const float* x, const float* dir, const float* weight,
-                     const float* pb, int64_t n, int64_t first, int64_t count, uint64_t perm_a,
-                     hipStream_t st) {
+// One Adam step on samples [first, first + count) of the arrays, with the
+// partial-gradient rows and per-workgroup losses in `partials` / `loss_part`
+// (at least (count + 63) / 64 rows).  Stream-ordered; allocates nothing.
+static int enqueue_step(pg_guide* G, const float* x, const float* dir, const float* weight,
+                        const float* pb, int64_t n, int64_t first, int64_t count, uint64_t perm_a,
+                        float* partials, float* loss_part, hipStream_t st) {
     const int blocks = (int)((count + kQ - 1) / kQ);
-    if (int rc = ensure_partials(G, blocks))
-        return rc;
     Batch B{x, dir, weight, pb, n, first, first + count, 1.0f / (float)count, perm_a, G->oct};
     hipLaunchKernelGGL(k_guide_grad, dim3(blocks), dim3(256), 0, st, G->params, G->wf, G->wt, B,
-                       G->partials, G->loss_part);
+                       partials, loss_part);
     G->last_blocks = blocks;
     G->step++;
     const float bc1 = 1.0f - std::pow(0.9f, (float)G->step);
     const float bc2 = 1.0f - std::pow(0.999f, (float)G->step);
     hipLaunchKernelGGL(k_guide_adam, dim3((kParams + 15) / 16), dim3(256), 0, st, G->params,
-                       G->adam_m, G->adam_v, G->grad, G->partials, blocks, G->lr, bc1, bc2, 1,
+                       G->adam_m, G->adam_v, G->grad, partials, blocks, G->lr, bc1, bc2, 1,
                        G->wf, G->wt);
     PG_HIP(hipGetLastError());
     return PG_OK;
 }
+
+// One Adam step on samples [first, first + count) of the arrays.
+int guide_train_step(pg_guide* G, const float* x, const float* dir, const float* weight,
+                     const float* pb, int64_t n, int64_t first, int64_t count, uint64_t perm_a,
+                     hipStream_t st) {
+    if (int rc = ensure_partials(G, (int)((count + kQ - 1) / kQ)))
+        return rc;
+    return enqueue_step(G, x, dir, weight, pb, n, first, count, perm_a, G->partials, G->loss_part, st);
+}
 }  // namespace pg
+
+// ---------------------------------------------------------------------------
+// Training without allocation (pg_guide_train_static): the partial-gradient
+// rows and per-workgroup losses of one step live in a file-scope device array,
+// one copy per GPU, held until the process ends.  Its device address is taken
+// once per device with hipGetSymbolAddress; host code uses only that address.
+
+namespace {
+
+constexpr int kStaticBlocks = PG_GUIDE_STATIC_MAX_BATCH / kQ;
+static_assert(PG_GUIDE_STATIC_MAX_BATCH % kQ == 0, "static batch limit: whole workgroups");
+constexpr size_t kScratchFloats = (size_t)kStaticBlocks * kParams + kStaticBlocks;
+
+__device__ __align__(256) float scratch[kScratchFloats];
+
+constexpr int kMaxDevices = 64;
+float* g_scratch[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// Device address of this GPU's scratch array (the current device's).
+int scratch_of_current_device(float** out) {
+    int dev = 0;
+    PG_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        return pg_fail(PG_E_ARG, "pg_guide_train_static: device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        PG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p)
+            return pg_fail(PG_E_HIP, "pg_guide_train_static: no device address for the scratch array");
+        g_scratch[dev] = static_cast<float*>(p);
+    }
+    *out = g_scratch[dev];
+    return PG_OK;
+}
+
+// The mean loss of a step: its per-workgroup losses summed in workgroup order
+// in double precision, as pg_guide_train sums them on the host.
+__global__ void k_loss_sum(const float* loss_part, int blocks, float* out) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        double s = 0.0;
+        for (int b = 0; b < blocks; b++)
+            s += loss_part[b];
+        *out = (float)s;
+    }
+}
+
+}  // namespace
 
 // ---------------------------------------------------------------------------
 // C ABI
@@ -661,6 +720,41 @@ extern "C" int pg_guide_train(pg_guide* G, const float* d_inputs, const float* d
         for (float v : lp)
             s += v;
         *h_loss = (float)s;
+    }
+    return PG_OK;
+}
+
+extern "C" int pg_guide_train_static(pg_guide* G, const float* d_inputs, const float* d_dirs,
+                                     const float* d_weight, const float* d_pbsdf, int64_t n,
+                                     int32_t steps, int32_t batch, float* d_loss, void* stream) {
+    const bool ok = d_inputs && d_dirs && d_weight && d_pbsdf;
+    if (int rc = check_n(G, n, ok ? d_inputs : nullptr, "pg_guide_train_static"))
+        return rc;
+    if (n == 0 || steps < 0 || batch <= 0)
+        return pg_fail(PG_E_ARG, "pg_guide_train_static: empty batch or bad step count");
+    if (batch > PG_GUIDE_STATIC_MAX_BATCH)
+        return pg_fail(PG_E_ARG, "pg_guide_train_static: batch " + std::to_string(batch) +
+                                     " exceeds PG_GUIDE_STATIC_MAX_BATCH (" +
+                                     std::to_string(PG_GUIDE_STATIC_MAX_BATCH) + ")");
+    float* sc = nullptr;
+    if (int rc = scratch_of_current_device(&sc))
+        return rc;
+    float* partials = sc;
+    float* loss_part = sc + (size_t)kStaticBlocks * kParams;
+    const hipStream_t st = (hipStream_t)stream;
+    int64_t first = 0;
+    for (int s = 0; s < steps; s++) {
+        const int64_t count = std::min<int64_t>(batch, n - first);
+        if (int rc = enqueue_step(G, d_inputs, d_dirs, d_weight, d_pbsdf, n, first, count, 1, partials,
+                                  loss_part, st))
+            return rc;
+        first += count;
+        if (first >= n)
+            first = 0;
+    }
+    if (d_loss && steps > 0) {
+        hipLaunchKernelGGL(k_loss_sum, dim3(1), dim3(64), 0, st, loss_part, G->last_blocks, d_loss);
+        PG_HIP(hipGetLastError());
     }
     return PG_OK;
 }

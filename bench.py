@@ -13,27 +13,39 @@ Prints one JSON line.  The roofline object accounts the dominant kernel
 read + 8 B hit written) + node bytes x nodes fetched + record bytes x prims
 tested, the counts taken from the instrumented variant of the same kernel over
 the same rays; duration = average of HIP events around each launch on the
-launch stream.  cpu_baseline times the oracle's CPU port of pine's traversal
-(pine-order stack traversal of a binned-SAH binary BVH) on the host cores.
+launch stream.  `traffic` is measured in the same run: rocprofv3 --pmc
+FETCH_SIZE and WRITE_SIZE over the same launches in two child runs, scaled by
+a known-size copy.  cpu_baseline is pine's own BVH::Intersect (pine built from
+its sources, oracle/_ref/ref_harness) over all rays on the host cores; the
+oracle's CPU port of the traversal is reported beside it.
 
-Secondary workloads, each next to pine itself (oracle/_ref/ref_harness, pine's
-own PathIntegrator built from its sources) on the host cores, at matched
-relMSE (the GPU reproduces pine's per-pixel sample sequence):
+Secondary workloads, each next to pine itself on the host cores:
   path_tracing       the seeded 64-spheres scene, 512x512, 64 spp, maxDepth 8 (also
                      in spectral mode, four wavelengths per path; GPU only)
-  mesh_path_tracing  the 2^20-triangle icosphere in a lit scene, 512x512
+  mesh_path_tracing  the 2^20-triangle icosphere in a lit scene, 512x512, 64 spp
   volume             a 256^3 fBm density grid, 1024x1024, 256 spp, maxDepth 64,
                      16^3-voxel majorant bricks, HG g = 0.85, albedo 0.99
 and, GPU only:
-  batch              64 scenes of uneven size (10k..2M triangles, 256^2..1024^2,
-                     16..128 spp) autotuned, then rendered one by one and as one
-                     batch over concurrent streams with graph-replayed bounce loops
+  batch              64 scenes of uneven size (10k..2M triangles plus instanced
+                     boxes, 256^2..1024^2, 16..128 spp) autotuned, then rendered one
+                     by one and as one batch over concurrent streams with
+                     graph-replayed bounce loops
   guided             the occluded-light box: plain vs learned-guided path tracing,
                      time to reach the plain tracer's 512-spp relMSE
+
+Every child process (pine's runs, the traffic runs) is started and finished
+before this process makes its first HIP call, and pine's timed regions run
+one at a time with nothing else on the host.  Under a profiler that preloads
+its library (rocprofv3) no child is started at all.
 """
 import argparse
+import csv
+import glob
 import json
+import math
 import os
+import shutil
+import signal
 import subprocess
 import sys
 import tempfile
@@ -45,33 +57,106 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+
+# Queue-record bytes of the fused path-tracing pipeline (the algorithmic
+# traffic of its wavefront stages; scene data such as BVH nodes, triangles and
+# materials is not counted):
+#   per camera sample  108 B: raygen writes the ray (32) and the path state
+#                      (L, beta, pdf, sampler index and dimension, depth,
+#                      film position, queue slot: 56); the film pass reads L
+#                      and the film position (20);
+#   per extension ray  220 B: trace reads queue slot and ray (36) and writes
+#                      hit and uv (16); shade reads slot, hit, uv, ray and
+#                      path state (96) and writes the next ray, its queue slot
+#                      and the updated state (72);
+#   per shadow ray     128 B: shade writes ray, path id and contribution (48);
+#                      the occlusion trace reads the ray and writes a flag
+#                      (36); resolve reads flag, path id and contribution and
+#                      updates L (44).
+QUEUE_BYTES_SAMPLE, QUEUE_BYTES_EXT, QUEUE_BYTES_SHADOW = 108, 220, 128
+# Guide network evaluation: 2 x 25,449 multiply-adds of the 225-64-64-64-41 MLP.
+GUIDE_FLOP_PER_QUERY = 2 * 25449
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_threads():
-    return min(os.cpu_count() or 1, 16)
+def profiled():
+    """True when this process runs under a profiler that preloads its library:
+    then no child process may be started (the profiler has already set up the
+    GPU in this process)."""
+    return any(k.startswith("ROCPROF") for k in os.environ) or \
+        "rocprof" in os.environ.get("LD_PRELOAD", "")
+
+
+def cpu_share():
+    """The CPUs this process may run on: its affinity mask, capped by a cgroup
+    (v2) CPU quota when one is set.  Pine's ParallelFor uses
+    hardware_concurrency(); threads beyond this share would only time-slice."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = os.cpu_count() or 1
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()
+        if q != "max":
+            quota = int(q) / int(period)
+    except (OSError, ValueError):
+        pass
+    n = aff if quota is None else max(1, min(aff, int(math.floor(quota))))
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return n, {"threads": n, "host_cpus": os.cpu_count(), "affinity_cpus": aff,
+               "cgroup_cpu_quota": quota, "cpu_model": model}
 
 
 class PineRun:
-    """pine's PathIntegrator on a scene file, in a child process (the harness's
-    render mode); its samples/s excludes scene loading and BVH build, as pine's
-    own [Rendering]Average line does."""
+    """pine itself (oracle/_ref/ref_harness: pine's library built from its
+    sources) in a child process: `render` (pine's PathIntegrator on a scene
+    file; samples/s excludes scene loading and BVH build, as pine's own
+    [Rendering]Average line does) or `time` (pine's BVH::Intersect over a ray
+    file).  With gate=True the child builds its scene and BVH, prints
+    "ready" and waits for go() before its timed region, so several children
+    can build at once and still be timed one at a time."""
 
-    def __init__(self, scene_text, files=(), threads=None):
+    def __init__(self, args, scene_text=None, files=(), gate=False):
         self.dir = tempfile.TemporaryDirectory()
         for name, writer in files:
             writer(os.path.join(self.dir.name, name))
-        with open(os.path.join(self.dir.name, "scene.txt"), "w") as f:
-            f.write(scene_text)
-        self.threads = threads or cpu_threads()
+        if scene_text is not None:
+            with open(os.path.join(self.dir.name, "scene.txt"), "w") as f:
+                f.write(scene_text)
+        env = dict(os.environ)
+        if gate:
+            env["PG_HARNESS_GATE"] = "1"
         self.t0 = time.time()
-        self.p = subprocess.Popen([HARNESS, "render", "./scene.txt", "film.bin", str(self.threads)],
-                                  cwd=self.dir.name, stdout=subprocess.PIPE,
-                                  stderr=subprocess.PIPE, text=True)
+        self.p = subprocess.Popen([HARNESS, *args], cwd=self.dir.name, stdin=subprocess.PIPE,
+                                  stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, env=env)
+
+    def wait_ready(self):
+        """Waits for the child's "ready" line (pine's own log lines come first)."""
+        while True:
+            line = self.p.stdout.readline()
+            if not line:
+                raise RuntimeError(f"pine run ended before getting ready: "
+                                   f"{self.p.stderr.read()[-500:]}")
+            if line.strip() == "ready":
+                break
+        self.build_s = time.time() - self.t0
+
+    def go(self):
+        self.p.stdin.write("go\n")
+        self.p.stdin.flush()
 
     def result(self, timeout=900):
         out, err = self.p.communicate(timeout=timeout)
@@ -92,25 +177,159 @@ def place(src):
         try:
             os.link(src, dst)
         except OSError:
-            import shutil
             shutil.copyfile(src, dst)
     return w
 
 
-def cpu_baseline_from(r, what):
+def write_mesh_bin(path, v, i):
+    """The harness's scene.bin with one triangle mesh."""
+    with open(path, "wb") as f:
+        f.write(np.array([0x31534750, 1, 0, len(v), len(i)], np.uint32).tobytes())
+        f.write(np.ascontiguousarray(v, np.float32).tobytes())
+        f.write(np.ascontiguousarray(i, np.uint32).tobytes())
+
+
+def write_rays_bin(path, rays):
+    with open(path, "wb") as f:
+        f.write(np.array([len(rays)], np.uint64).tobytes())
+        f.write(np.ascontiguousarray(rays, np.float32).tobytes())
+
+
+def cpu_baseline_from(r, what, share):
     if r is None:
         return None
     n = r["w"] * r["h"] * r["spp"]
     return {"value": round(r["msamples_per_s"] * 1e6, 1), "unit": "samples/s",
-            "cores": r["threads"], "kind": "reference",
+            "cores": r["threads"], "kind": "reference", "host": share,
             "sample": f"{what}: pine renders {r['w']}x{r['h']} at {r['spp']} spp ({n} samples, "
-                      f"{r['render_s']:.1f}s render, {r['wall_s']:.0f}s with scene load and BVH build)"}
+                      f"{r['render_s']:.1f}s render, {r['wall_s']:.0f}s with scene load and BVH "
+                      f"build), alone on the host before any GPU work"}
 
 
-def traversal(args, pg, hip, W, v, i):
+def pine_baselines(args, W, work, v, i, rays, threads, share):
+    """pine on the host cores, every timed region alone and all of it before
+    this process makes its first HIP call (no child process is started once
+    the GPU is in use).  pine's two single-threaded BVH builds of the
+    2^20-triangle mesh (about 1.5 minutes each) run side by side; their timed
+    parts then run one at a time:
+      mesh      the mesh scene at the GPU leg's size (512x512, 64 spp);
+      trace     BVH::Intersect (bvh.cpp:669-709) over all 16,777,216 rays;
+      spheres   the spheres scene at the GPU leg's size (512x512, 64 spp);
+      volume    the 256^3 cloud at the GPU leg's film (1024x1024) with 8 spp
+                (pine has no majorant bricks: its global majorant)."""
     t0 = time.time()
+    T = str(threads)
+    out = {}
+    obj = os.path.join(work, "ico1m.obj")
+    mesh_bin, rays_bin = os.path.join(work, "ico1m.bin"), os.path.join(work, "rays16m.bin")
+    write_mesh_bin(mesh_bin, v, i)
+    write_rays_bin(rays_bin, rays)
+    runs = {}
+    if not args.no_pt:
+        runs["mesh"] = PineRun(["render", "./scene.txt", "film.bin", T],
+                               W.mesh_scene_text("ico1m.obj", width=512, height=512, spp=64),
+                               files=[("ico1m.obj", place(obj))], gate=True)
+    runs["trace"] = PineRun(["time", "scene.bin", "rays.bin", T],
+                            files=[("scene.bin", place(mesh_bin)), ("rays.bin", place(rays_bin))],
+                            gate=True)
+    for r in runs.values():
+        r.wait_ready()
+    log(f"[bench] pine: BVH builds done in {time.time() - t0:.0f} s")
+    for k, r in runs.items():
+        r.go()
+        out[k] = r.result()
+        if out[k]:
+            out[k]["build_s"] = round(r.build_s, 1)
+    os.remove(rays_bin)
+    if not args.no_pt:
+        text = W.spheres_scene_text(seed=0, width=512, height=512, spp=args.pt_spp, max_depth=8)
+        out["spheres"] = PineRun(["render", "./scene.txt", "film.bin", T], text).result()
+        vtext = W.cloud_scene_text("cloud256.vol", width=1024, height=1024, spp=8, max_depth=64)
+        out["volume"] = PineRun(["render", "./scene.txt", "film.bin", T], vtext,
+                                files=[("cloud256.vol", place(os.path.join(work, "cloud256.vol")))]
+                                ).result()
+    log(f"[bench] pine baselines done in {time.time() - t0:.0f} s on {threads} threads")
+    return out
+
+
+def traffic_probe(args):
+    """HBM bytes per closest-hit launch, measured now: two child runs of this
+    script's --traffic-probe mode under rocprofv3 --pmc (FETCH_SIZE, then
+    WRITE_SIZE), made before this process touches the GPU.  Each counter is
+    scaled by what it reads for a device-to-device copy of the 512 MiB ray
+    buffer in the same run (the copy reads and writes exactly its size).
+    Returns a dict, or None with the reason logged."""
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        log("[bench] traffic: rocprofv3 not found")
+        return None
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp()
+        cmd = [exe, "--pmc", counter, "-d", d, "-o", "run", "--output-format", "csv", "--",
+               sys.executable, os.path.abspath(__file__), "--traffic-probe",
+               "--width", str(args.width), "--block", str(args.block), "--flags", str(args.flags),
+               "--leaf", str(args.leaf)]
+        p = subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                             start_new_session=True)
+        try:
+            out, _ = p.communicate(timeout=180)
+        except subprocess.TimeoutExpired:
+            os.killpg(p.pid, signal.SIGKILL)
+            p.communicate()
+            log(f"[bench] traffic: {counter} run timed out")
+            shutil.rmtree(d, ignore_errors=True)
+            return None
+        rows = []
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(f) as fh:
+                rows += list(csv.DictReader(fh))
+        shutil.rmtree(d, ignore_errors=True)
+        rows = [r for r in rows if r.get("Counter_Name") == counter]
+        rows.sort(key=lambda r: int(r.get("Dispatch_Id") or 0))
+        trace = [float(r["Counter_Value"]) for r in rows if "k_trace" in r.get("Kernel_Name", "")]
+        copy = [float(r["Counter_Value"]) for r in rows if "copyBuffer" in r.get("Kernel_Name", "")]
+        if p.returncode != 0 or len(trace) < 2 or not copy:
+            log(f"[bench] traffic: {counter} run gave rc {p.returncode}, {len(trace)} trace and "
+                f"{len(copy)} copy rows: {out[-400:]}")
+            return None
+        # The copy kernel moves 512 MiB each way: scale = true KiB / counted KiB.
+        scale = (512 * 1024) / copy[-1]
+        # Dispatch 0 is the instrumented pass; the plain launches follow.
+        kib = float(np.mean(trace[1:]))
+        res[counter] = {"kib_per_dispatch": round(kib, 1), "copy_scale": round(scale, 4),
+                        "bytes": kib * scale * 1024}
+    return {"bytes": int(res["FETCH_SIZE"]["bytes"] + res["WRITE_SIZE"]["bytes"]),
+            "fetch": res["FETCH_SIZE"], "write": res["WRITE_SIZE"],
+            "method": "rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, on this script's "
+                      "--traffic-probe (one instrumented + 3 plain launches and a 512 MiB copy), "
+                      "each counter scaled by its reading of the copy"}
+
+
+def traffic_probe_main(args):
+    """--traffic-probe: the traversal workload's launches and the calibration
+    copy, nothing else (run under rocprofv3 by traffic_probe)."""
+    import pine_amd as pg
+    from pine_amd import hip
+    from pine_amd import workloads as W
+    v, i = W.make_mesh(W.ICOSPHERE_1M)
     rays = W.make_rays(W.RAYS_16M, v, i)
-    log(f"[bench] rays generated in {time.time() - t0:.2f}s: {len(i)} triangles, {len(rays)} rays")
+    acc = pg.Accel()
+    acc.add_mesh(v, i)
+    acc.build(width=args.width, leaf_max=args.leaf)
+    acc.set_launch(args.block, args.flags)
+    d_rays = hip.DeviceArray.from_numpy(rays)
+    d_hits = hip.DeviceArray((d_rays.shape[0], 2), np.int32)
+    acc.trace_counts(d_rays)
+    for _ in range(3):
+        acc.trace_closest(d_rays, d_hits)
+    dst = hip.DeviceArray(d_rays.shape, np.float32)
+    hip.check(hip.rt().hipMemcpy(dst.ptr, d_rays.ptr, d_rays.nbytes, hip.hipMemcpyDeviceToDevice))
+    hip.synchronize()
+    print("traffic probe done", flush=True)
+
+
+def traversal(args, pg, hip, W, v, i, rays, traffic):
 
     acc = pg.Accel()
     acc.add_mesh(v, i)
@@ -170,11 +389,20 @@ def traversal(args, pg, hip, W, v, i):
     roofline = {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": measured_traffic(st["width"], args.leaf, args.block, launch_flags),
+        "traffic": traffic["bytes"] if traffic else None,
         "kernel": f"k_trace<W={st['width']}>",
-        "bytes_per_launch": {"compulsory": compulsory, "traversal": traversal_bytes},
+        "bytes_per_launch": {"compulsory": compulsory, "traversal": traversal_bytes,
+                             "node_fetch": nodes_fetched * node_bytes,
+                             "counters": traffic["bytes"] if traffic else None},
+        # The algorithmic figure counts every node visit at its full size,
+        # L2 and MALL hits included; the counters see only L2 misses.
+        "frac_counters": round(traffic["bytes"] / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+        if traffic else None,
         "compulsory_frac": round(compulsory / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+        "limiter": "VALU issue and L2-miss latency, not HBM bandwidth: 72% VALU busy, L2 hit "
+                   "rate 49% (profiles/k_trace2_icosphere1m_rays16m_counters.md)",
         "per_ray": {"nodes": round(nodes_fetched / n, 3), "prims": round(prims_tested / n, 3)},
+        "traffic_method": traffic["method"] if traffic else "not measured in this run",
     }
     config = {"workload": "bvh-traversal: icosphere 1,048,576 tris, 16,777,216 rays",
               "triangles": int(len(i)), "rays": int(n), "node_width": st["width"],
@@ -184,21 +412,38 @@ def traversal(args, pg, hip, W, v, i):
             "rays": rays, "hits": hits}
 
 
-def traversal_cpu(trav, oracle, v, i):
+def traversal_cpu(trav, oracle, v, i, threads, share):
+    """The oracle's CPU port of pine's traversal (pine-order stack traversal
+    of a binned-SAH binary BVH) in this process, on the same rays: a second
+    CPU figure beside pine's own BVH::Intersect."""
     rays, hits = trav["rays"], trav["hits"]
-    threads = cpu_threads()
     tri = oracle.tri9(v, i)
     tb = time.time()
     cb = oracle.CpuBVH(tri)
     build_s = time.time() - tb
     prim, t, secs = cb.trace(rays, threads=threads)
     cpu = {"value": round(len(rays) / secs * 1e-6, 3), "unit": "Mrays/s", "cores": threads,
-           "kind": "port",
+           "kind": "port", "host": share,
            "sample": f"all {len(rays)} rays of the stream, same mesh; pine-order BVH2 "
                      f"traversal, tree built in {build_s:.1f}s (not timed)",
            "agrees_with_gpu_t": round(float((t.view(np.int32) == hits[:, 1]).mean()), 6)}
-    log(f"[bench] cpu baseline {cpu['value']} Mrays/s on {threads} threads ({secs:.2f}s)")
+    log(f"[bench] cpu port {cpu['value']} Mrays/s on {threads} threads ({secs:.2f}s)")
     return cpu
+
+
+def queue_roofline(inf, samples, ms):
+    """Whole-pipeline roofline of a fused path-tracing render from its
+    queue-record bytes (QUEUE_BYTES_*) over the render's time on its stream."""
+    ext = int(inf["last_extension_rays"])
+    shadow = int(inf["last_rays"]) - ext
+    b = samples * QUEUE_BYTES_SAMPLE + ext * QUEUE_BYTES_EXT + shadow * QUEUE_BYTES_SHADOW
+    achieved = b / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes": int(b), "extension_rays": ext, "shadow_rays": shadow,
+            "basis": f"queue records: {QUEUE_BYTES_SAMPLE} B per camera sample, "
+                     f"{QUEUE_BYTES_EXT} B per extension ray, {QUEUE_BYTES_SHADOW} B per shadow "
+                     f"ray, over the whole render time (all kernels)"}
 
 
 def render_gpu(pg, text, directory, warm_spp=2):
@@ -240,6 +485,8 @@ def batch_leg(args, pg, W, work):
     seq_s = time.perf_counter() - t0
     seq_rays = sum(s.info()["last_rays"] for s in scenes)
     res = pg.render_batch(scenes, streams=args.batch_streams)
+    ext = sum(int(s.info()["last_extension_rays"]) for s in scenes)
+    roof = queue_roofline({"last_extension_rays": ext, "last_rays": res["rays"]}, samples, res["ms"])
     res_ng = pg.render_batch(scenes, streams=args.batch_streams, graph=False)
     for s in scenes:
         s.release()
@@ -257,6 +504,7 @@ def batch_leg(args, pg, W, work):
         "speedup_vs_sequential": round(seq_s * 1e3 / res["ms"], 3),
         "graph_speedup": round(res_ng["ms"] / res["ms"], 3),
         "rays_match_sequential": bool(res["rays"] == seq_rays),
+        "roofline": roof,
         "setup_s": round(setup_s, 1), "autotune_s": round(tune_s, 1),
         "tuned": {"trace_block": sorted({t["trace_block"] for t in tuned}),
                   "postpone_leaves": sum(1 for t in tuned if t["trace_flags"] & 1),
@@ -298,6 +546,20 @@ def guided_leg(args, pg, W, work):
         train_ms.append(inf["last_guided_ms"] - inf["last_render_ms"])
         ge.append(M.relmse(M.linear(s.film()), ref))
         g.release()
+    # Guide inference on the matrix cores: one more guided render of the same
+    # size drawn launch by launch, its inference kernels timed by HIP events.
+    g = pg.Guide(seed=3, learning_rate=3e-3)
+    inf = s.render_guided(g, train_passes=8, train_spp=4, final_spp=64, sample_offset=1 << 21,
+                          keep_passes=8, graph=False)
+    g.release()
+    iq, ims = int(inf["guide_infer_queries"]), float(inf["guide_infer_ms"])
+    tflops = iq * GUIDE_FLOP_PER_QUERY / (ims * 1e-3) / 1e12 if ims > 0 else None
+    mfma = {"bound": "mfma", "achieved": round(tflops, 1) if tflops else None,
+            "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 4) if tflops else None, "traffic": None,
+            "kernel": "k_guide_infer", "queries": iq, "kernel_ms": round(ims, 2),
+            "basis": f"{GUIDE_FLOP_PER_QUERY} FLOP per network evaluation (2 x 25,449 MACs) over "
+                     f"the inference kernels' time in a 64-spp guided render"}
     s.release()
     target = pe[3]
     tp, tg = M.time_to_error(pt, pe, target), M.time_to_error(gt, ge, target)
@@ -317,26 +579,10 @@ def guided_leg(args, pg, W, work):
         "equal_time_speedup": round(tp / tg, 3) if tp and tg else None,
         # 64 spp in both images: plain 64, guided 32 kept training + 32 final
         "equal_spp_error_ratio_64": round(pe[0] / ge[0], 3),
+        "roofline": mfma,
     }
     log(f"[bench] guided: plain {tp} ms vs guided {tg} ms to relMSE {target:.3g}")
     return out
-
-
-def measured_traffic(width, leaf, block, flags):
-    """HBM bytes per launch of the trace kernel for this launch configuration,
-    as measured by rocprofv3 FETCH_SIZE + WRITE_SIZE (scaled by a known-size
-    copy) and recorded in the kernel's profile summary; None when that
-    configuration has no recorded measurement."""
-    import re
-    path = os.path.join(ROOT, "profiles", "k_trace2_icosphere1m_rays16m.md")
-    if not os.path.exists(path):
-        return None
-    key = f"width={width} leaf={leaf} block={block} flags={flags}"
-    for line in open(path):
-        m = re.match(r"\s*traffic_bytes_per_launch k_trace\S* (.*): (\d+)\s*$", line)
-        if m and m.group(1).strip() == key:
-            return int(m.group(2))
-    return None
 
 
 def main():
@@ -350,6 +596,8 @@ def main():
                     help="pg_accel_set_launch flags (1 postpone leaves, 2 XCD remap); -1 = default")
     ap.add_argument("--leaf", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 traffic measurement of the traversal kernel")
     ap.add_argument("--no-pt", action="store_true", help="skip the path-tracing workloads")
     ap.add_argument("--pt-spp", type=int, default=64)
     ap.add_argument("--volume-spp", type=int, default=256)
@@ -365,36 +613,48 @@ def main():
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one device-to-device copy of the ray buffer (512 MiB), the "
                          "known-size kernel that scales rocprof's FETCH_SIZE/WRITE_SIZE")
+    ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the traversal "
               "shards by rays with the hierarchy replicated, see DESIGN.md.", file=sys.stderr)
         sys.exit(2)
+    if args.traffic_probe:
+        traffic_probe_main(args)
+        return
 
+    # Host-side preparation, then every child process (pine's CPU baselines,
+    # the rocprofv3 traffic runs), all before this process makes a HIP call.
     import pine_amd as pg
-    from pine_amd import hip
     from pine_amd import workloads as W
+    threads, share = cpu_share()
+    no_children = profiled()
+    if no_children:
+        log("[bench] under a profiler: no child processes (CPU baselines and traffic skipped)")
+    cpu_legs = not args.no_cpu and os.path.exists(HARNESS) and not no_children
+    work = tempfile.TemporaryDirectory()
+    t0 = time.time()
+    v, i = W.make_mesh(W.ICOSPHERE_1M)
+    rays = W.make_rays(W.RAYS_16M, v, i)
+    log(f"[bench] mesh and rays in {time.time() - t0:.2f}s: {len(i)} triangles, {len(rays)} rays")
+    if not args.no_pt:
+        W.write_obj(os.path.join(work.name, "ico1m.obj"), v, i)
+        pg.write_volume(os.path.join(work.name, "cloud256.vol"), pg.gen_density_grid(W.CLOUD_GRID))
+    pine = pine_baselines(args, W, work.name, v, i, rays, threads, share) if cpu_legs else {}
+    traffic = None
+    if not args.no_traffic and not no_children:
+        t0 = time.time()
+        traffic = traffic_probe(args)
+        log(f"[bench] traffic measured in {time.time() - t0:.0f} s: "
+            f"{traffic['bytes'] / 1e9:.2f} GB per launch" if traffic else "[bench] traffic: none")
 
+    from pine_amd import hip
     if hip.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
 
-    cpu_legs = not args.no_cpu and os.path.exists(HARNESS)
-    work = tempfile.TemporaryDirectory()
-    v, i = W.make_mesh(W.ICOSPHERE_1M)
-    # pine's mesh scene spends about two minutes in its single-threaded BVH
-    # build; it runs beside the GPU-only measurements, and the CPU-heavy legs
-    # wait for it.
-    obj = os.path.join(work.name, "ico1m.obj")
-    if not args.no_pt:
-        W.write_obj(obj, v, i)
-    pine_mesh = None
-    if cpu_legs and not args.no_pt:
-        pine_mesh = PineRun(W.mesh_scene_text("ico1m.obj", width=128, height=128, spp=64),
-                            files=[("ico1m.obj", place(obj))])
-
-    trav = traversal(args, pg, hip, W, v, i)
+    trav = traversal(args, pg, hip, W, v, i, rays, traffic)
 
     pt = mesh_pt = vol = None
     if not args.no_pt:
@@ -402,10 +662,12 @@ def main():
         text = W.spheres_scene_text(seed=0, width=512, height=512, spp=args.pt_spp, max_depth=8)
         inf = render_gpu(pg, text, work.name)
         ms = inf["last_render_ms"]
+        samples = 512 * 512 * args.pt_spp
         pt = {"workload": f"spheres scene 512x512, {args.pt_spp} spp, maxDepth 8",
-              "samples_per_s": round(512 * 512 * args.pt_spp / (ms * 1e-3), 1),
+              "samples_per_s": round(samples / (ms * 1e-3), 1),
               "render_ms": round(ms, 2),
-              "mrays_per_s": round(inf["last_rays"] / (ms * 1e-3) * 1e-6, 2)}
+              "mrays_per_s": round(inf["last_rays"] / (ms * 1e-3) * 1e-6, 2),
+              "roofline": queue_roofline(inf, samples, ms)}
         log(f"[bench] spheres: {pt['samples_per_s'] / 1e6:.1f} M samples/s")
         # The same scene with four wavelengths per path (spectral mode).
         sc = pg.Scene(text=text, directory=work.name)
@@ -413,7 +675,7 @@ def main():
         sc.render(spectral=True)
         sms = sc.info()["last_render_ms"]
         sc.release()
-        pt["spectral"] = {"samples_per_s": round(512 * 512 * args.pt_spp / (sms * 1e-3), 1),
+        pt["spectral"] = {"samples_per_s": round(samples / (sms * 1e-3), 1),
                           "render_ms": round(sms, 2), "wavelengths_per_path": 4}
         log(f"[bench] spheres, spectral: {pt['spectral']['samples_per_s'] / 1e6:.1f} M samples/s")
         # 2^20-triangle mesh scene.
@@ -421,14 +683,14 @@ def main():
         text = W.mesh_scene_text("ico1m.obj", width=512, height=512, spp=mesh_spp)
         inf = render_gpu(pg, text, work.name)
         ms = inf["last_render_ms"]
+        samples = 512 * 512 * mesh_spp
         mesh_pt = {"workload": f"2^20-triangle icosphere scene 512x512, {mesh_spp} spp, maxDepth 8",
-                   "samples_per_s": round(512 * 512 * mesh_spp / (ms * 1e-3), 1),
+                   "samples_per_s": round(samples / (ms * 1e-3), 1),
                    "render_ms": round(ms, 2),
-                   "mrays_per_s": round(inf["last_rays"] / (ms * 1e-3) * 1e-6, 2)}
+                   "mrays_per_s": round(inf["last_rays"] / (ms * 1e-3) * 1e-6, 2),
+                   "roofline": queue_roofline(inf, samples, ms)}
         log(f"[bench] mesh: {mesh_pt['samples_per_s'] / 1e6:.1f} M samples/s")
         # Volume: 256^3 grid with majorant bricks.
-        grid = pg.gen_density_grid(W.CLOUD_GRID)
-        pg.write_volume(os.path.join(work.name, "cloud256.vol"), grid)
         text = W.cloud_scene_text("cloud256.vol", width=1024, height=1024, spp=args.volume_spp,
                                   max_depth=64, bricks=W.CLOUD_BRICK)
         inf = render_gpu(pg, text, work.name)
@@ -439,8 +701,10 @@ def main():
                            f"{W.CLOUD_BRICK}^3 majorant bricks",
                "samples_per_s": round(1024 * 1024 * args.volume_spp / (ms * 1e-3), 1),
                "render_ms": round(ms, 2), "density_lookups": int(lookups),
+               "mrays_per_s": round(inf["last_rays"] / (ms * 1e-3) * 1e-6, 2),
                "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                            "traffic": None,
                             "basis": "36 B per density lookup (8 taps + majorant) over the "
                                      "whole render time"}}
         log(f"[bench] volume: {vol['samples_per_s'] / 1e6:.1f} M samples/s, {lookups} lookups")
@@ -452,25 +716,30 @@ def main():
         guided = guided_leg(args, pg, W, work.name)
 
     cpu = None
+    if pine.get("trace"):
+        r = pine["trace"]
+        cpu = {"value": round(r["mrays_per_s"], 3), "unit": "Mrays/s", "cores": r["threads"],
+               "kind": "reference", "host": share,
+               "sample": f"pine's own BVH::Intersect (bvh.cpp:669-709) over all {r['rays']} rays "
+                         f"of the stream on pine's own BVH of the same mesh ({r['trace_s']:.2f} s "
+                         f"timed; its single-threaded build took {r['build_ms'] / 1e3:.0f} s, "
+                         f"not timed), alone on the host before any GPU work"}
+        log(f"[bench] pine BVH::Intersect: {cpu['value']} Mrays/s on {r['threads']} threads")
+    for res, key, what in ((pt, "spheres", "same scene file"),
+                           (mesh_pt, "mesh", "same scene file"),
+                           (vol, "volume", "same grid, medium and film with pine's global majorant "
+                                           "(no bricks), 8 spp")):
+        if res is not None and pine.get(key):
+            res["cpu_baseline"] = cpu_baseline_from(pine[key], what, share)
+            res["speedup_vs_cpu"] = round(res["samples_per_s"] / res["cpu_baseline"]["value"], 1)
+            log(f"[bench] {res['workload'][:32]}: {res['speedup_vs_cpu']}x pine on "
+                f"{res['cpu_baseline']['cores']} cores")
+    port = None
     if not args.no_cpu:
         import oracle
-        if pine_mesh is not None:
-            mesh_pt["cpu_baseline"] = cpu_baseline_from(pine_mesh.result(),
-                                                        "same mesh scene at lower resolution")
-        if pt is not None and cpu_legs:
-            text = W.spheres_scene_text(seed=0, width=512, height=512, spp=args.pt_spp, max_depth=8)
-            pt["cpu_baseline"] = cpu_baseline_from(PineRun(text).result(), "same scene file")
-            vtext = W.cloud_scene_text("cloud256.vol", width=256, height=256, spp=16, max_depth=64)
-            vol_file = os.path.join(work.name, "cloud256.vol")
-            r = PineRun(vtext, files=[("cloud256.vol", place(vol_file))]).result()
-            vol["cpu_baseline"] = cpu_baseline_from(r, "same grid and medium with pine's global "
-                                                       "majorant, lower resolution and spp")
-        for res in (pt, mesh_pt, vol):
-            if res and res.get("cpu_baseline"):
-                res["speedup_vs_cpu"] = round(res["samples_per_s"] / res["cpu_baseline"]["value"], 1)
-                log(f"[bench] {res['workload'][:32]}: {res['speedup_vs_cpu']}x pine on "
-                    f"{res['cpu_baseline']['cores']} cores")
-        cpu = traversal_cpu(trav, oracle, v, i)
+        port = traversal_cpu(trav, oracle, v, i, threads, share)
+        if cpu is None:
+            cpu = port
     work.cleanup()
 
     out = {
@@ -481,6 +750,7 @@ def main():
         "config": trav["config"],
         "roofline": trav["roofline"],
         "cpu_baseline": cpu,
+        "cpu_port": port if port is not cpu else None,
         "path_tracing": pt,
         "mesh_path_tracing": mesh_pt,
         "volume": vol,

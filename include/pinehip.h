@@ -200,6 +200,9 @@ typedef struct pg_scene_info {
     int32_t trace_block, trace_flags, bvh_width;  /* traversal launch and node width */
     int64_t last_extension_rays;  /* path vertices traced by the last render (closest-hit rays) */
     int32_t apply_tone_mapping;   /* the film's applyToneMapping (film.cpp:11), default 1 */
+    double guide_infer_ms;        /* last guided render drawn launch by launch (PG_RENDER_NO_GRAPH):
+                                     time of its guide-inference kernels on the GPU */
+    int64_t guide_infer_queries;  /* ... and the network evaluations they made */
 } pg_scene_info;
 int pg_scene_get_info(const pg_scene* s, pg_scene_info* info);
 
@@ -219,6 +222,11 @@ typedef struct pg_render_options {
  * 1931 matching functions.  Gray scenes render as in RGB in expectation.  The
  * fused pipeline only (scenes without media or material-less shapes). */
 #define PG_RENDER_SPECTRAL 2
+/* Track brick-majorant media as persistent lanes (each lane takes the next ray
+ * of its workgroup's share of the queue when its own is done) instead of one
+ * ray per thread.  Same image bit for bit; measured 8% slower on the 256^3
+ * cloud, so it is not the default. */
+#define PG_RENDER_LANE_TRACKING 4
 
 /* The spectral mode's tables: Q (RGB -> spectral basis coefficients), M (XYZ ->
  * RGB), both row-major 3x3, and the film weight scale 470 / (4 * integral of

@@ -36,6 +36,7 @@ EXPORTED = [
 ]
 RENDER_NO_GRAPH = 1
 RENDER_SPECTRAL = 2
+RENDER_LANE_TRACKING = 4
 GUIDE_N_PARAMS = 27888
 GUIDE_STATIC_MAX_BATCH = 2048
 
@@ -73,7 +74,8 @@ class _SceneInfo(ctypes.Structure):
                 ("graph_captures", ctypes.c_int64), ("paths_in_flight", ctypes.c_int64),
                 ("trace_block", ctypes.c_int32), ("trace_flags", ctypes.c_int32),
                 ("bvh_width", ctypes.c_int32), ("last_extension_rays", ctypes.c_int64),
-                ("apply_tone_mapping", ctypes.c_int32)]
+                ("apply_tone_mapping", ctypes.c_int32), ("guide_infer_ms", ctypes.c_double),
+                ("guide_infer_queries", ctypes.c_int64)]
 
 
 class _GuidedOptions(ctypes.Structure):
@@ -376,8 +378,10 @@ class Scene:
         check(lib().pg_scene_get_info(self.h, ctypes.byref(s)))
         return {k: getattr(s, k) for k, _ in _SceneInfo._fields_}
 
-    def render(self, spp=0, samples_per_batch=0, sample_offset=0, graph=True, spectral=False):
-        flags = (0 if graph else RENDER_NO_GRAPH) | (RENDER_SPECTRAL if spectral else 0)
+    def render(self, spp=0, samples_per_batch=0, sample_offset=0, graph=True, spectral=False,
+               lane_tracking=False):
+        flags = (0 if graph else RENDER_NO_GRAPH) | (RENDER_SPECTRAL if spectral else 0) | \
+            (RENDER_LANE_TRACKING if lane_tracking else 0)
         o = _RenderOptions(spp, samples_per_batch, sample_offset, flags)
         check(lib().pg_render(self.h, ctypes.byref(o), _stream_handle()))
         return self.info()

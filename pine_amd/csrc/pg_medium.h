@@ -13,10 +13,14 @@
 //    (the exponential step restarts at each brick face).  Unbiased and far
 //    fewer null collisions in sparse grids; consumes a different number of
 //    sampler dimensions, so images agree with pine's statistically, not
-//    sample for sample.  The brick majorants of a launch are staged in LDS.
+//    sample for sample, and it uses exact logarithms and float densities
+//    rather than pine's arithmetic.  The brick majorants of a launch are
+//    staged in LDS.
 #pragma once
 
 #include "pg_shade.h"
+
+#include <cmath>
 
 namespace pg {
 
@@ -163,6 +167,47 @@ PG_HD float grid_density(const MediumDev& m, f3 p) {
     return ret;
 }
 
+// --- brick-majorant tracking: this library's own estimator -------------------
+// Tracking against brick majorants does not replay pine's sample sequence, so
+// it need not reproduce pine's arithmetic: free flights use the hardware log2
+// (pine's bisection log2 costs 32 IEEE divisions and is off by up to 2^-9,
+// which skews the exponential distribution) and densities a float trilinear
+// interpolation.
+
+// log2 of x > 0: v_log_f32 on the device.
+PG_HD float fast_log2(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_logf(x);
+#else
+    return std::log2(x);
+#endif
+}
+
+// A free flight from t at extinction `rate`: t - ln(1 - u) / rate.
+PG_HD float brick_flight(float t, float u, float rate) {
+    return t - (fast_log2(1.0f - u) * 0.693147182f) / rate;
+}
+
+// GridMedium::Density (medium.cpp:139-155) in float: trilinear weights as
+// lerps over x, then y, then z; nearest voxel when interpolate is off.
+PG_HD float grid_density_fast(const MediumDev& m, f3 p) {
+    const float px = p.x * (float)m.size[0], py = p.y * (float)m.size[1],
+                pz = p.z * (float)m.size[2];
+    const int ix = (int)px, iy = (int)py, iz = (int)pz;
+    if (!m.interpolate)
+        return grid_voxel(m, ix, iy, iz);
+    const float fx = px - (float)ix, fy = py - (float)iy, fz = pz - (float)iz;
+    const float c000 = grid_voxel(m, ix, iy, iz), c001 = grid_voxel(m, ix, iy, iz + 1);
+    const float c010 = grid_voxel(m, ix, iy + 1, iz), c011 = grid_voxel(m, ix, iy + 1, iz + 1);
+    const float c100 = grid_voxel(m, ix + 1, iy, iz), c101 = grid_voxel(m, ix + 1, iy, iz + 1);
+    const float c110 = grid_voxel(m, ix + 1, iy + 1, iz),
+                c111 = grid_voxel(m, ix + 1, iy + 1, iz + 1);
+    const float c00 = fmaf(fx, c100 - c000, c000), c01 = fmaf(fx, c101 - c001, c001);
+    const float c10 = fmaf(fx, c110 - c010, c010), c11 = fmaf(fx, c111 - c011, c011);
+    const float c0 = fmaf(fy, c10 - c00, c00), c1 = fmaf(fy, c11 - c01, c01);
+    return fmaf(fz, c1 - c0, c0);
+}
+
 // One free-flight step of pine's trackers: t -= log(1 - u) * invMaxDensity /
 // sigma_t, with pstd::log's Ln2 folded into the majorant as pine's build does.
 PG_HD float track_step(const MediumDev& m, float t, float u) {
@@ -238,19 +283,28 @@ struct MediumEvent {
     bool scattered;
 };
 
+// Media a tracking kernel is compiled for: every kind (kM 0), only grids
+// tracked with brick majorants by delta / ratio tracking (kM 1), whose kernels
+// then carry no code or registers for the other methods, or a scene whose only
+// medium is such a grid (kM 2: the medium's fields are uniform across the wave
+// and stay in scalar registers).
+enum MediumVariant : int { kMediaAny = 0, kMediaBricks = 1, kMediaOneBrickGrid = 2 };
+
 // Medium::Sample (medium.h:70-73) for a ray of the path; `maj` holds the brick
-// majorants of grid media in brick mode (LDS copy or global).
+// majorants of grid media in brick mode (LDS copy or global).  kS: the
+// sampler variant (get_1d_t), kM: the media variant.
+template <int kS = kSamplerAny, int kM = kMediaAny>
 PG_HD MediumEvent medium_sample(const MediumDev& m, const float* maj, const SamplerDev& hs,
                                 SamplerState& ss, f3 o, f3 d, float tmax, int& lookups) {
     MediumEvent ev;
     ev.scattered = false;
     ev.weight = mk(1.0f, 1.0f, 1.0f);
-    if (m.type == kMediumHomogeneous) {
+    if (kM == kMediaAny && m.type == kMediumHomogeneous) {
         // HomogeneousMedium::Sample (medium.cpp:37-53)
-        int ch = (int)(get_1d(hs, ss) * 3.0f);
+        int ch = (int)(get_1d_t<kS>(hs, ss) * 3.0f);
         ch = ch < 0 ? 0 : (ch > 2 ? 2 : ch);
         const float st = ch == 0 ? m.sigma_t3.x : (ch == 1 ? m.sigma_t3.y : m.sigma_t3.z);
-        const float dist = -pstd::log(1.0f - get_1d(hs, ss)) / st;
+        const float dist = -pstd::log(1.0f - get_1d_t<kS>(hs, ss)) / st;
         const bool sampled = dist < tmax;
         const float t = dist < tmax ? dist : tmax;
         const f3 tr = mk(pstd::exp(-m.sigma_t3.x * t), pstd::exp(-m.sigma_t3.y * t),
@@ -273,10 +327,10 @@ PG_HD MediumEvent medium_sample(const MediumDev& m, const float* maj, const Samp
     if (!unit_box_hit(r, t0, t1))
         return ev;
     const f3 albedo = m.sigma_s / m.sigma_t;
-    if (m.method == kRayMarching) {
-        const float lnXi = -pstd::log(1.0f - get_1d(hs, ss));
+    if (kM == kMediaAny && m.method == kRayMarching) {
+        const float lnXi = -pstd::log(1.0f - get_1d_t<kS>(hs, ss));
         const float stepSize = m.march_step;
-        const float jitter = stepSize * (get_1d(hs, ss) - 0.5f);
+        const float jitter = stepSize * (get_1d_t<kS>(hs, ss) - 0.5f);
         float t = t0, sum = 0.0f;
         while (t < t1) {
             lookups++;
@@ -291,15 +345,15 @@ PG_HD MediumEvent medium_sample(const MediumDev& m, const float* maj, const Samp
         }
         return ev;
     }
-    if (m.brick == 0) {
+    if (kM == kMediaAny && m.brick == 0) {
         float t = t0;
         while (true) {
-            t = track_step(m, t, get_1d(hs, ss));
+            t = track_step(m, t, get_1d_t<kS>(hs, ss));
             if (!(t < t1))  // also ends on NaN, where pine would loop forever
                 return ev;
             const f3 pm = ray_at(r, t);
             lookups++;
-            if (grid_density(m, pm) * m.inv_max_density > get_1d(hs, ss)) {
+            if (grid_density(m, pm) * m.inv_max_density > get_1d_t<kS>(hs, ss)) {
                 ev.p = to_world(m, pm);
                 ev.scattered = true;
                 ev.weight = albedo;
@@ -318,12 +372,12 @@ PG_HD MediumEvent medium_sample(const MediumDev& m, const float* maj, const Samp
         if (mj > 0.0f && rate > 0.0f) {
             const float inv_mj = 1.0f / mj;
             while (true) {
-                t = t - pstd::log(1.0f - get_1d(hs, ss)) / rate;
+                t = brick_flight(t, get_1d_t<kS>(hs, ss), rate);
                 if (!(t < texit))
                     break;
                 const f3 pm = ray_at(r, t);
                 lookups++;
-                if (grid_density(m, pm) * inv_mj > get_1d(hs, ss)) {
+                if (grid_density_fast(m, pm) * inv_mj > get_1d_t<kS>(hs, ss)) {
                     ev.p = to_world(m, pm);
                     ev.scattered = true;
                     ev.weight = albedo;
@@ -340,9 +394,10 @@ PG_HD MediumEvent medium_sample(const MediumDev& m, const float* maj, const Samp
 }
 
 // Medium::Tr (medium.h:66-69): transmittance over [0, tmax] of the ray.
+template <int kS = kSamplerAny, int kM = kMediaAny>
 PG_HD f3 medium_tr(const MediumDev& m, const float* maj, const SamplerDev& hs, SamplerState& ss,
                    f3 o, f3 d, float tmax, int& lookups) {
-    if (m.type == kMediumHomogeneous) {
+    if (kM == kMediaAny && m.type == kMediumHomogeneous) {
         // HomogeneousMedium::Tr (medium.cpp:33-35): Exp(-ray.tmax * sigma_t)
         return mk(pstd::exp(-tmax * m.sigma_t3.x), pstd::exp(-tmax * m.sigma_t3.y),
                   pstd::exp(-tmax * m.sigma_t3.z));
@@ -352,10 +407,10 @@ PG_HD f3 medium_tr(const MediumDev& m, const float* maj, const SamplerDev& hs, S
     float t0, t1;
     if (!unit_box_hit(r, t0, t1))
         return mk(1.0f, 1.0f, 1.0f);
-    if (m.method == kRayMarching) {
+    if (kM == kMediaAny && m.method == kRayMarching) {
         float tr = 0.0f;
         const float stepSize = m.march_step;
-        const float jitter = stepSize * (get_1d(hs, ss) - 0.5f);
+        const float jitter = stepSize * (get_1d_t<kS>(hs, ss) - 0.5f);
         float t = t0;
         while (t < t1) {
             lookups++;
@@ -367,10 +422,10 @@ PG_HD f3 medium_tr(const MediumDev& m, const float* maj, const SamplerDev& hs, S
     }
     float tr = 1.0f;
     const float rrThreshold = 0.1f;
-    if (m.brick == 0) {
+    if (kM == kMediaAny && m.brick == 0) {
         float t = t0;
         while (true) {
-            t = track_step(m, t, get_1d(hs, ss));
+            t = track_step(m, t, get_1d_t<kS>(hs, ss));
             if (!(t < t1))
                 break;
             lookups++;
@@ -378,7 +433,7 @@ PG_HD f3 medium_tr(const MediumDev& m, const float* maj, const SamplerDev& hs, S
             tr *= 1.0f - max0(density * m.inv_max_density, 0.0f);
             if (tr < rrThreshold) {
                 const float q = max0(1.0f - tr, 0.05f);
-                if (get_1d(hs, ss) < q)
+                if (get_1d_t<kS>(hs, ss) < q)
                     return mk(0.0f, 0.0f, 0.0f);
                 tr /= 1.0f - q;
             }
@@ -395,15 +450,15 @@ PG_HD f3 medium_tr(const MediumDev& m, const float* maj, const SamplerDev& hs, S
         if (mj > 0.0f && rate > 0.0f) {
             const float inv_mj = 1.0f / mj;
             while (true) {
-                t = t - pstd::log(1.0f - get_1d(hs, ss)) / rate;
+                t = brick_flight(t, get_1d_t<kS>(hs, ss), rate);
                 if (!(t < texit))
                     break;
                 lookups++;
-                const float density = grid_density(m, ray_at(r, t));
+                const float density = grid_density_fast(m, ray_at(r, t));
                 tr *= 1.0f - max0(density * inv_mj, 0.0f);
                 if (tr < rrThreshold) {
                     const float q = max0(1.0f - tr, 0.05f);
-                    if (get_1d(hs, ss) < q)
+                    if (get_1d_t<kS>(hs, ss) < q)
                         return mk(0.0f, 0.0f, 0.0f);
                     tr /= 1.0f - q;
                 }

@@ -68,6 +68,11 @@ struct HaltonDev {
     const uint16_t* perms;  // ComputeRadicalInversePermutations
     const int* primes;      // first 1000 primes
     const int* primeSums;   // prefix sums of primes
+    // Per dimension, one 16-byte record: prime base, offset of its permutation
+    // in perms, 1/base and the series term perm[0] / (base + 1), as floats'
+    // bits (the two quotients computed once on the host, IEEE as the device
+    // would).
+    const int4* dims;
 };
 
 PG_HD uint64_t inverse_radical_inverse(uint64_t inverse, int base, int nDigits) {
@@ -94,32 +99,58 @@ PG_HD uint64_t halton_index(const HaltonDev& h, int px, int py, int sampleIndex)
     return (uint64_t)(hi + (int64_t)sampleIndex * h.stride);
 }
 
+// ScrambledRadicalInverse for an index below 2^32: the digit loop in 32 bits,
+// four digits a round so that their permutation loads are in flight together
+// (the digits are combined in the same order as one at a time).
+PG_HD float scrambled_radical_inverse32(const HaltonDev& h, int dim, uint32_t a) {
+    const int4 t = h.dims[dim];
+    const uint32_t b = (uint32_t)t.x;
+    const uint16_t* perm = h.perms + t.y;
+    const float invBase = __builtin_bit_cast(float, t.z);
+    float invBaseN = 1.0f;
+    uint64_t reversed = 0;
+    while (a) {
+        uint32_t dg[4];
+        int nd = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t next = a / b;
+            dg[j] = a - next * b;
+            nd += a != 0u ? 1 : 0;
+            a = next;
+        }
+        uint32_t pv[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            pv[j] = j < nd ? (uint32_t)perm[dg[j]] : 0u;
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if (j < nd) {
+                reversed = reversed * (uint64_t)b + pv[j];
+                invBaseN *= invBase;
+            }
+    }
+    const float series = __builtin_bit_cast(float, t.w);
+    const float v = ((float)reversed + series) * invBaseN;
+    return v < kOneMinusEps ? v : kOneMinusEps;
+}
+
 // ScrambledRadicalInverse.  Indices stay below 2^32 for every film up to
 // 2^32/stride samples per pixel, so the digit loop runs in 32 bits.
 PG_HD float scrambled_radical_inverse(const HaltonDev& h, int dim, uint64_t a64) {
+    if (!(a64 >> 32))
+        return scrambled_radical_inverse32(h, dim, (uint32_t)a64);
     const int base = h.primes[dim];
     const uint16_t* perm = h.perms + h.primeSums[dim];
     const float invBase = 1.0f / (float)base;
     float invBaseN = 1.0f;
     uint64_t reversed = 0;
-    if (a64 >> 32) {
-        while (a64) {
-            uint64_t next = a64 / (uint64_t)base;
-            uint64_t digit = a64 - next * (uint64_t)base;
-            reversed = reversed * (uint64_t)base + perm[digit];
-            invBaseN *= invBase;
-            a64 = next;
-        }
-    } else {
-        uint32_t a = (uint32_t)a64;
-        const uint32_t b = (uint32_t)base;
-        while (a) {
-            uint32_t next = a / b;
-            uint32_t digit = a - next * b;
-            reversed = reversed * (uint64_t)b + perm[digit];
-            invBaseN *= invBase;
-            a = next;
-        }
+    while (a64) {
+        uint64_t next = a64 / (uint64_t)base;
+        uint64_t digit = a64 - next * (uint64_t)base;
+        reversed = reversed * (uint64_t)base + perm[digit];
+        invBaseN *= invBase;
+        a64 = next;
     }
     const float series = (float)perm[0] / ((float)base + 1.0f);
     const float v = ((float)reversed + series) * invBaseN;
@@ -159,6 +190,25 @@ PG_HD float get_1d(const SamplerDev& h, SamplerState& s) {
         s.dim = 2;
     return scrambled_radical_inverse(h.halton, s.dim++, s.index);
 }
+// get_1d compiled for one sampler (kernels instantiated per sampler keep only
+// that sampler's code and registers): kS 0 = the scene's kind at run time,
+// 1 = Halton with sample indices below 2^32, 2 = Sobol.  Same values as get_1d.
+enum SamplerVariant : int { kSamplerAny = 0, kSamplerHalton32 = 1, kSamplerSobolOnly = 2 };
+template <int kS>
+PG_HD float get_1d_t(const SamplerDev& h, SamplerState& s) {
+    if constexpr (kS == kSamplerSobolOnly) {
+        if (s.dim >= kSobolDimensions)
+            s.dim = 2;
+        return sobol_sample(h.sobol, s.dim++, (uint32_t)s.index);
+    } else if constexpr (kS == kSamplerHalton32) {
+        if (s.dim >= 1000)
+            s.dim = 2;
+        return scrambled_radical_inverse32(h.halton, s.dim++, (uint32_t)s.index);
+    } else {
+        return get_1d(h, s);
+    }
+}
+
 PG_HD void get_2d(const SamplerDev& h, SamplerState& s, float& u0, float& u1) {
     if (h.kind == kSamplerSobol) {
         // sampler.cpp:152-158 wraps only when the first dimension is past the

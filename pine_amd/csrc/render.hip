@@ -35,6 +35,7 @@
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 using namespace pg;
@@ -134,7 +135,8 @@ struct PathsDev {
     pg_ray* rays_out;
     pg_hit* hits;
     pg_hit_uv* uvs;
-    int32_t* counters;  // [0] next-queue length, [1] shadow-queue length
+    int32_t* counters;  // [0] next-queue length, [1] shadow-queue length, [2] direction queue,
+                        // [3] next shadow segments
     pg_ray* shadow_rays;
     int32_t* shadow_path;
     float* shadow_contrib;  // 3 per shadow ray
@@ -165,6 +167,8 @@ struct PathsDev {
     pg_hit* seg_hits;
     pg_hit_uv* seg_uvs;
     unsigned long long* lookups;  // density lookups of the tracking kernels (8 taps each)
+    float4* mev;                  // 2 per queue slot: Medium::Sample's event (weight, scattered), (p, -)
+    float* seg_trf;               // per shadow segment: its medium's transmittance (lane tracking)
     // Guiding: mixture of each path's current vertex, and training records
     // (slot = path * rec_depth + depth; rec_pdf 0 = empty slot).
     float* gparams;   // guide::kParamStride per vertex: per path (split bounce) or per queue slot (fused)
@@ -1265,10 +1269,291 @@ __device__ __forceinline__ void save_vertex(PathsDev& P, int32_t i, int kind, f3
     P.queue_b[atomicAdd(&P.counters[2], 1)] = i;
 }
 
-__global__ __launch_bounds__(256) void k_shade_gen(SceneDev s, PathsDev P, int32_t n_active, int32_t lds_medium) {
+// Medium::Sample (medium.h:70-73) for every queued ray whose path is in a
+// medium, ahead of k_shade_gen: the tracking loop (free flights and their
+// dependent density lookups) runs in this lean kernel at several times the
+// shading kernel's occupancy.  The event goes to the ray's queue slot and the
+// sampler dimensions it consumed to the path, so k_shade_gen continues the
+// path's sample sequence exactly where Sample left it.
+template <int kS, int kM>
+__global__ __launch_bounds__(256) void k_medium_track(SceneDev s, PathsDev P, int32_t n_active, int32_t lds_medium) {
     extern __shared__ float lds_maj[];
     const float* lds = stage_majorants(s, lds_medium, lds_maj);
     int lookups = 0;
+    for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n_active;
+         k += gridDim.x * blockDim.x) {
+        const int32_t i = P.queue_in[k];
+        const int med = P.medium[i];
+        if (med < 0)
+            continue;
+        const float4* r4 = reinterpret_cast<const float4*>(&P.rays_in[k]);
+        const float4 ra = r4[0], rb = r4[1];
+        const pg_hit hit = P.hits[k];
+        // Intersect leaves ray.tmax at the hit distance (bvh.cpp:652-709).
+        const float tmax = hit.prim >= 0 ? hit.t : kFloatMax;
+        SamplerState ss;
+        ss.index = P.sidx[i];
+        ss.dim = P.sdim[i];
+        const int mi = kM == kMediaOneBrickGrid ? 0 : med;
+        const MediumEvent ev = medium_sample<kS, kM>(s.media[mi], majorants(s, mi, lds_medium, lds), s.sampler,
+                                                     ss, mk(ra.x, ra.y, ra.z), mk(rb.x, rb.y, rb.z), tmax, lookups);
+        P.sdim[i] = ss.dim;
+        P.mev[2 * (int64_t)k] = make_float4(ev.weight.x, ev.weight.y, ev.weight.z, ev.scattered ? 1.0f : 0.0f);
+        P.mev[2 * (int64_t)k + 1] = make_float4(ev.p.x, ev.p.y, ev.p.z, 0.0f);
+    }
+    add_lookups(P.lookups, lookups);
+}
+
+// --- lane tracking ------------------------------------------------------------
+// Delta and ratio tracking against brick majorants as persistent lanes
+// (PG_RENDER_LANE_TRACKING): each lane takes the next ray of its workgroup's
+// share whenever its own is finished, so a wave is not held by its longest
+// path.  Measured on the 256^3 cloud it is 8% slower than one ray per thread
+// (the branch for every phase runs predicated each step), so it is opt-in.  Each call of the step functions below does
+// one piece of the brick walk of medium_sample / medium_tr (enter a brick, one
+// free flight, leave a brick) with the same arithmetic and sampler draws, so
+// the results are those of the per-path loops bit for bit.
+
+// Gives each idle lane of the wave the next index of a work counter (one
+// atomic per wave).  Every lane of the wave calls it; busy lanes get -1.  The
+// counter is the workgroup's own, in LDS, over its contiguous share of the
+// queue: one global counter for every wave of the chip serialises its atomics
+// (measured: slower than no refilling at all).
+__device__ __forceinline__ int32_t lanes_acquire(bool idle, int32_t* ctr) {
+    const uint64_t need = __ballot(idle);
+    if (need == 0)
+        return -1;
+    const int lane = __lane_id();
+    const int leader = __ffsll((unsigned long long)need) - 1;
+    int32_t base = 0;
+    if (lane == leader)
+        base = atomicAdd(ctr, __popcll(need));
+    base = __shfl(base, leader);
+    return idle ? base + __popcll(need & ((1ull << lane) - 1ull)) : -1;
+}
+
+enum TrackPhase : int { kEnter = 0, kFlight = 1, kLeave = 2 };
+
+struct LaneTrack {
+    MRay r;
+    BrickWalk w;
+    float t, t1, texit, rate, inv_mj;
+    int phase;
+};
+
+// One step of the brick walk of a grid medium: returns 1 when the walk ends
+// (left the grid or the ray's range), 2 when a free flight lands at r(t)
+// (the caller tests that collision), 0 otherwise.
+template <int kS>
+__device__ __forceinline__ int track_step_bricks(const MediumDev& m, const float* maj, const SamplerDev& hs,
+                                                 SamplerState& ss, LaneTrack& L) {
+    if (L.phase == kEnter) {
+        L.texit = fminf(brick_exit(L.w), L.t1);
+        const float mj = maj[brick_index(m, L.w)];
+        L.rate = m.sigma_t * mj;
+        if (mj > 0.0f && L.rate > 0.0f) {
+            L.inv_mj = 1.0f / mj;
+            L.phase = kFlight;
+        } else {
+            L.phase = kLeave;
+        }
+    }
+    if (L.phase == kFlight) {
+        L.t = brick_flight(L.t, get_1d_t<kS>(hs, ss), L.rate);
+        if (L.t < L.texit)
+            return 2;
+        L.phase = kLeave;
+    }
+    // kLeave
+    if (L.texit >= L.t1)
+        return 1;
+    L.t = L.texit;
+    if (!brick_advance(m, L.w))
+        return 1;
+    L.phase = kEnter;
+    return 0;
+}
+
+// Medium::Sample of brick-tracked grid media as persistent lanes (see
+// k_medium_track for what is read and written).
+template <int kS, int kM>
+__global__ __launch_bounds__(256) void k_medium_track_lanes(SceneDev s, PathsDev P, int32_t n_active,
+                                                            int32_t lds_medium) {
+    extern __shared__ float lds_maj[];
+    const float* lds = stage_majorants(s, lds_medium, lds_maj);
+    int lookups = 0;
+    int32_t k = -1, i = 0, med = 0;
+    SamplerState ss;
+    LaneTrack L;
+    // This workgroup's share of the queue, handed out through an LDS counter.
+    __shared__ int32_t next;
+    const int32_t share = (n_active + gridDim.x - 1) / gridDim.x;
+    const int32_t end = min(n_active, (int32_t)blockIdx.x * share + share);
+    if (threadIdx.x == 0)
+        next = (int32_t)blockIdx.x * share;
+    __syncthreads();
+    bool exhausted = false;
+    while (true) {
+        if (!exhausted) {
+            const bool idle = k < 0;
+            const int32_t got = lanes_acquire(idle, &next);
+            if (idle && got < end) {
+                k = got;
+                i = P.queue_in[k];
+                med = P.medium[i];
+                if (med < 0) {
+                    k = -1;
+                } else {
+                    const int mi = kM == kMediaOneBrickGrid ? 0 : med;
+                    const MediumDev& m = s.media[mi];
+                    const float4* r4 = reinterpret_cast<const float4*>(&P.rays_in[k]);
+                    const float4 ra = r4[0], rb = r4[1];
+                    const pg_hit hit = P.hits[k];
+                    // Intersect leaves ray.tmax at the hit distance (bvh.cpp:652-709).
+                    const float tmax = hit.prim >= 0 ? hit.t : kFloatMax;
+                    ss.index = P.sidx[i];
+                    ss.dim = P.sdim[i];
+                    L.r = to_model(m, mk(ra.x, ra.y, ra.z), mk(rb.x, rb.y, rb.z), 0.0f, tmax);
+                    float t0, t1;
+                    if (!unit_box_hit(L.r, t0, t1)) {
+                        P.mev[2 * (int64_t)k] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+                        k = -1;
+                    } else {
+                        brick_walk_init(m, L.r, t0, L.w);
+                        L.t = t0;
+                        L.t1 = t1;
+                        L.phase = kEnter;
+                    }
+                }
+            }
+            exhausted = __any(idle && got >= end);
+        }
+        if (!__any(k >= 0)) {
+            if (exhausted)
+                break;
+            continue;
+        }
+        if (k < 0)
+            continue;
+        const int mi = kM == kMediaOneBrickGrid ? 0 : med;
+        const MediumDev& m = s.media[mi];
+        const int st = track_step_bricks<kS>(m, majorants(s, mi, lds_medium, lds), s.sampler, ss, L);
+        bool scattered = false;
+        f3 pm;
+        if (st == 2) {
+            pm = ray_at(L.r, L.t);
+            lookups++;
+            scattered = grid_density_fast(m, pm) * L.inv_mj > get_1d_t<kS>(s.sampler, ss);
+        }
+        if (scattered) {
+            const f3 albedo = m.sigma_s / m.sigma_t;
+            const f3 p = to_world(m, pm);
+            P.mev[2 * (int64_t)k] = make_float4(albedo.x, albedo.y, albedo.z, 1.0f);
+            P.mev[2 * (int64_t)k + 1] = make_float4(p.x, p.y, p.z, 0.0f);
+        } else if (st == 1) {
+            P.mev[2 * (int64_t)k] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+        }
+        if (scattered || st == 1) {
+            P.sdim[i] = ss.dim;
+            k = -1;
+        }
+    }
+    add_lookups(P.lookups, lookups);
+}
+
+// Medium::Tr of the shadow segments in brick-tracked grid media (ratio
+// tracking with pine's Russian roulette, medium.cpp:55-94) as persistent
+// lanes: the transmittance of segment k goes to seg_trf[k] (1 outside media),
+// the sampler dimensions it used to its path.
+template <int kS, int kM>
+__global__ __launch_bounds__(256) void k_shadow_track_lanes(SceneDev s, PathsDev P, int32_t n_seg, int32_t src,
+                                                            int32_t lds_medium) {
+    extern __shared__ float lds_maj[];
+    const float* lds = stage_majorants(s, lds_medium, lds_maj);
+    int lookups = 0;
+    int32_t k = -1, i = 0, med = 0;
+    float tr = 1.0f;
+    SamplerState ss;
+    LaneTrack L;
+    // This workgroup's share of the queue, handed out through an LDS counter.
+    __shared__ int32_t next;
+    const int32_t share = (n_seg + gridDim.x - 1) / gridDim.x;
+    const int32_t end = min(n_seg, (int32_t)blockIdx.x * share + share);
+    if (threadIdx.x == 0)
+        next = (int32_t)blockIdx.x * share;
+    __syncthreads();
+    bool exhausted = false;
+    while (true) {
+        if (!exhausted) {
+            const bool idle = k < 0;
+            const int32_t got = lanes_acquire(idle, &next);
+            if (idle && got < end) {
+                k = got;
+                med = P.seg_med[src][k];
+                if (med < 0) {
+                    P.seg_trf[k] = 1.0f;
+                    k = -1;
+                } else {
+                    const int mi = kM == kMediaOneBrickGrid ? 0 : med;
+                    const MediumDev& m = s.media[mi];
+                    i = P.shadow_path[P.seg_rec[src][k]];
+                    const float4* r4 = reinterpret_cast<const float4*>(&P.seg_rays[src][k]);
+                    const float4 ra = r4[0], rb = r4[1];
+                    const pg_hit hit = P.seg_hits[k];
+                    const float tmax = hit.prim >= 0 ? hit.t : rb.w;
+                    ss.index = P.sidx[i];
+                    ss.dim = P.sdim[i];
+                    L.r = to_model(m, mk(ra.x, ra.y, ra.z), mk(rb.x, rb.y, rb.z), 0.0f, tmax);
+                    float t0, t1;
+                    if (!unit_box_hit(L.r, t0, t1)) {
+                        P.seg_trf[k] = 1.0f;
+                        k = -1;
+                    } else {
+                        brick_walk_init(m, L.r, t0, L.w);
+                        L.t = t0;
+                        L.t1 = t1;
+                        L.phase = kEnter;
+                        tr = 1.0f;
+                    }
+                }
+            }
+            exhausted = __any(idle && got >= end);
+        }
+        if (!__any(k >= 0)) {
+            if (exhausted)
+                break;
+            continue;
+        }
+        if (k < 0)
+            continue;
+        const int mi = kM == kMediaOneBrickGrid ? 0 : med;
+        const MediumDev& m = s.media[mi];
+        const int st = track_step_bricks<kS>(m, majorants(s, mi, lds_medium, lds), s.sampler, ss, L);
+        bool done = st == 1;
+        if (st == 2) {
+            lookups++;
+            const float density = grid_density_fast(m, ray_at(L.r, L.t));
+            tr *= 1.0f - max0(density * L.inv_mj, 0.0f);
+            if (tr < 0.1f) {
+                const float q = max0(1.0f - tr, 0.05f);
+                if (get_1d_t<kS>(s.sampler, ss) < q) {
+                    tr = 0.0f;
+                    done = true;
+                } else {
+                    tr /= 1.0f - q;
+                }
+            }
+        }
+        if (done) {
+            P.seg_trf[k] = tr;
+            P.sdim[i] = ss.dim;
+            k = -1;
+        }
+    }
+    add_lookups(P.lookups, lookups);
+}
+
+__global__ __launch_bounds__(256) void k_shade_gen(SceneDev s, PathsDev P, int32_t n_active) {
     for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n_active;
          k += gridDim.x * blockDim.x) {
         const int32_t i = P.queue_in[k];
@@ -1287,13 +1572,14 @@ __global__ __launch_bounds__(256) void k_shade_gen(SceneDev s, PathsDev P, int32
         bool alive = false;
         f3 nextO, nextD;
         int next_med = med;
-        // Intersect leaves ray.tmax at the hit distance (bvh.cpp:652-709).
-        const float tmax = hit.prim >= 0 ? hit.t : kFloatMax;
         MediumEvent ev;
         ev.scattered = false;
         if (med >= 0) {
-            ev = medium_sample(s.media[med], majorants(s, med, lds_medium, lds), s.sampler, ss, ro,
-                               rd, tmax, lookups);
+            // k_medium_track's event for this slot (sdim above already follows it).
+            const float4 e0 = P.mev[2 * (int64_t)k], e1 = P.mev[2 * (int64_t)k + 1];
+            ev.weight = mk(e0.x, e0.y, e0.z);
+            ev.scattered = e0.w != 0.0f;
+            ev.p = mk(e1.x, e1.y, e1.z);
             beta = beta * ev.weight;
         }
         if (ev.scattered) {
@@ -1351,7 +1637,6 @@ __global__ __launch_bounds__(256) void k_shade_gen(SceneDev s, PathsDev P, int32
             store_ray(&P.rays_out[slot], nextO, nextD, 0.0f, kFloatMax);
         }
     }
-    add_lookups(P.lookups, lookups);
 }
 
 // RandomWalkIntegrator::Li (randomwalk.cpp:6-62) with participating media:
@@ -1467,9 +1752,12 @@ __global__ __launch_bounds__(256) void k_shade_rw_gen(SceneDev s, PathsDev P, in
 // One segment of IntersectTr (integrator.cpp:68-87): transmittance of the
 // segment's medium up to the closest hit, then stop at a surface with a
 // material, or continue towards p2 through a material-less one.
+// kPre: the segments' transmittance was tracked by k_shadow_track_lanes
+// (seg_trf); otherwise it is tracked here, one segment per thread.
+template <int kS, int kM, bool kPre>
 __global__ __launch_bounds__(256) void k_shadow_tr(SceneDev s, PathsDev P, int32_t n_seg, int32_t src, int32_t lds_medium) {
     extern __shared__ float lds_maj[];
-    const float* lds = stage_majorants(s, lds_medium, lds_maj);
+    const float* lds = kPre ? nullptr : stage_majorants(s, lds_medium, lds_maj);
     const int dst = 1 - src;
     int lookups = 0;
     for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n_seg; k += gridDim.x * blockDim.x) {
@@ -1481,13 +1769,17 @@ __global__ __launch_bounds__(256) void k_shadow_tr(SceneDev s, PathsDev P, int32
         const pg_hit hit = P.seg_hits[k];
         const int med = P.seg_med[src][k];
         f3 tr = mk(P.sh_tr[3 * r], P.sh_tr[3 * r + 1], P.sh_tr[3 * r + 2]);
-        if (med >= 0) {
+        if (kPre && med >= 0) {
+            const float f = P.seg_trf[k];
+            tr = tr * mk(f, f, f);
+        } else if (med >= 0) {
             SamplerState ss;
             ss.index = P.sidx[i];
             ss.dim = P.sdim[i];
             const float tmax = hit.prim >= 0 ? hit.t : rb.w;
-            tr = tr * medium_tr(s.media[med], majorants(s, med, lds_medium, lds), s.sampler, ss, ro,
-                                rd, tmax, lookups);
+            const int mi = kM == kMediaOneBrickGrid ? 0 : med;
+            tr = tr * medium_tr<kS, kM>(s.media[mi], majorants(s, mi, lds_medium, lds), s.sampler, ss, ro,
+                                        rd, tmax, lookups);
             P.sdim[i] = ss.dim;
         }
         if (hit.prim >= 0) {
@@ -2044,8 +2336,20 @@ struct pg_scene {
     int64_t graph_captures = 0;
     int64_t batch_paths = 4 << 20;       // paths per batch when not given (tunable)
     pg_build_options build_opts{};       // the accel's build options (rebuilds keep them)
+    // Guide inference kernel time and queries of the last launch-by-launch
+    // render (events around each k_guide_infer launch; graph replays skip it).
+    // Tracking-kernel variants of the current render: sampler (kSampler*) and
+    // media (kMedia*), chosen per render from the scene and its sample range.
+    int track_sampler = 0, track_media = 0;
+    bool lane_tracking = false;  // brick-media tracking as persistent lanes (PG_RENDER_LANE_TRACKING)
+    hipEvent_t infer_ev[2] = {nullptr, nullptr};
+    double infer_ms = 0.0;
+    int64_t infer_queries = 0;
 
     ~pg_scene() {
+        for (hipEvent_t e : infer_ev)
+            if (e)
+                (void)hipEventDestroy(e);
         if (graph_exec)
             (void)hipGraphExecDestroy(graph_exec);
         if (capture_stream)
@@ -3260,6 +3564,17 @@ int finish_scene(pg_scene& S, const pg_build_options* opts) {
             for (int j = 0; j < primes[d]; j++)
                 perms[o + j] = (uint16_t)j;
     }
+    // Per-dimension records of the radical inverse (base, permutation offset,
+    // 1/base, perm[0] / (base + 1)).
+    std::vector<int4> dimtab(primes.size());
+    for (size_t d = 0; d < primes.size(); d++) {
+        const float inv_base = 1.0f / (float)primes[d];
+        const float series = (float)perms[sums[d]] / ((float)primes[d] + 1.0f);
+        int ib, sb;
+        std::memcpy(&ib, &inv_base, 4);
+        std::memcpy(&sb, &series, 4);
+        dimtab[d] = make_int4(primes[d], sums[d], ib, sb);
+    }
     SamplerDev& sd = S.dev.sampler;
     sd.kind = S.sampler_kind;
     halton_setup(sd.halton, S.dev.width, S.dev.height);
@@ -3283,6 +3598,7 @@ int finish_scene(pg_scene& S, const pg_build_options* opts) {
             return rc;
     }
     if ((rc = upload(S, perms, &sd.halton.perms)) || (rc = upload(S, primes, &sd.halton.primes)) ||
+        (rc = upload(S, dimtab, &sd.halton.dims)) ||
         (rc = upload(S, sums, &sd.halton.primeSums)) || (rc = upload(S, S.prim_shape, &S.dev.prim_shape)) ||
         (rc = upload(S, S.tri_verts, &S.dev.tri_verts)) || (rc = upload(S, S.shapes, &S.dev.shapes)) ||
         (rc = upload(S, S.materials, &S.dev.materials)) || (rc = upload(S, S.lights, &S.dev.lights)) ||
@@ -3323,7 +3639,7 @@ int ensure_paths(pg_scene& S, int64_t n) {
         (rc = dalloc(S, n, &P.depth)) || (rc = dalloc(S, 2 * n, &P.pfilm)) ||
         (rc = dalloc(S, n, &P.queue_in)) || (rc = dalloc(S, n, &P.queue_out)) ||
         (rc = dalloc(S, n, &P.rays_in)) || (rc = dalloc(S, n, &P.rays_out)) ||
-        (rc = dalloc(S, n, &P.hits)) || (rc = dalloc(S, n, &P.uvs)) || (rc = dalloc(S, 4, &P.counters)) ||
+        (rc = dalloc(S, n, &P.hits)) || (rc = dalloc(S, n, &P.uvs)) || (rc = dalloc(S, 8, &P.counters)) ||
         (rc = dalloc(S, n, &P.shadow_rays)) || (rc = dalloc(S, n, &P.shadow_path)) ||
         (rc = dalloc(S, 3 * n, &P.shadow_contrib)) || (rc = dalloc(S, n, &P.occluded)) ||
         (rc = dalloc(S, 1, &P.spill_count)))
@@ -3340,7 +3656,8 @@ int ensure_paths(pg_scene& S, int64_t n) {
             (rc = dalloc(S, n, &P.seg_rec[0])) || (rc = dalloc(S, n, &P.seg_rec[1])) ||
             (rc = dalloc(S, n, &P.seg_med[0])) || (rc = dalloc(S, n, &P.seg_med[1])) ||
             (rc = dalloc(S, n, &P.seg_hits)) || (rc = dalloc(S, n, &P.seg_uvs)) ||
-            (rc = dalloc(S, 1, &P.lookups)))
+            (rc = dalloc(S, 1, &P.lookups)) || (rc = dalloc(S, 2 * n, &P.mev)) ||
+            (rc = dalloc(S, n, &P.seg_trf)))
             return rc;
     }
     if (!S.node_progs.empty()) {
@@ -3461,6 +3778,8 @@ extern "C" int pg_scene_get_info(const pg_scene* S, pg_scene_info* info) {
     info->last_extension_rays = S->last_ext_rays;
     info->paths_in_flight = S->batch_paths;
     info->apply_tone_mapping = S->tone_map ? 1 : 0;
+    info->guide_infer_ms = S->infer_ms;
+    info->guide_infer_queries = S->infer_queries;
     pg_accel_stats as{};
     if (pg_accel_get_stats(S->accel, &as) == PG_OK) {
         info->bvh_width = as.width;
@@ -3484,12 +3803,68 @@ static int tracking_grid(int64_t n, int block) {
     return (int)std::max<int64_t>(1, std::min<int64_t>((n + block - 1) / block, 4096));
 }
 
+// Grid of a persistent-lane tracking kernel: the chip's resident workgroups
+// (four or five 256-lane workgroups per CU at these kernels' registers), fewer
+// for short queues.
+static int lane_grid(int64_t n, int block) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>((n + block - 1) / block, 1280));
+}
+
+// Calls f(sampler variant, media variant) as integral constants, so the
+// tracking kernels are launched in the instantiation that fits the render.
+template <class F>
+static void launch_tracking(int ks, int km, F&& f) {
+    using S0 = std::integral_constant<int, kSamplerAny>;
+    using S1 = std::integral_constant<int, kSamplerHalton32>;
+    using S2 = std::integral_constant<int, kSamplerSobolOnly>;
+    using M0 = std::integral_constant<int, kMediaAny>;
+    using M1 = std::integral_constant<int, kMediaBricks>;
+    using M2 = std::integral_constant<int, kMediaOneBrickGrid>;
+    if (km == kMediaOneBrickGrid) {
+        if (ks == kSamplerHalton32)
+            f(S1{}, M2{});
+        else if (ks == kSamplerSobolOnly)
+            f(S2{}, M2{});
+        else
+            f(S0{}, M2{});
+    } else if (km == kMediaBricks) {
+        if (ks == kSamplerHalton32)
+            f(S1{}, M1{});
+        else if (ks == kSamplerSobolOnly)
+            f(S2{}, M1{});
+        else
+            f(S0{}, M1{});
+    } else {
+        if (ks == kSamplerHalton32)
+            f(S1{}, M0{});
+        else if (ks == kSamplerSobolOnly)
+            f(S2{}, M0{});
+        else
+            f(S0{}, M0{});
+    }
+}
+
 // One bounce in general mode; `active` becomes the next queue's length.
 static int bounce_general(pg_scene* S, PathsDev& Q, int32_t& active, hipStream_t st, int64_t& rays) {
     const int B = 256;
     const size_t lds = S->lds_medium >= 0 ? S->med_majorant[S->lds_medium].size() * sizeof(float) : 0;
-    hipLaunchKernelGGL(k_shade_gen, dim3(tracking_grid(active, B)), dim3(B), lds, st, S->dev, Q, active,
-                       S->lds_medium);
+    // PG_RENDER_LANE_TRACKING: brick-tracked media run their tracking as
+    // persistent lanes.
+    const bool lanes = S->lane_tracking && S->track_media != kMediaAny;
+    if (!S->media.empty())
+        launch_tracking(S->track_sampler, S->track_media, [&](auto ks, auto km) {
+            if constexpr (decltype(km)::value != kMediaAny) {
+                if (lanes) {
+                    hipLaunchKernelGGL((k_medium_track_lanes<decltype(ks)::value, decltype(km)::value>),
+                                       dim3(lane_grid(active, B)), dim3(B), lds, st, S->dev, Q, active,
+                                       S->lds_medium);
+                    return;
+                }
+            }
+            hipLaunchKernelGGL((k_medium_track<decltype(ks)::value, decltype(km)::value>),
+                               dim3(tracking_grid(active, B)), dim3(B), lds, st, S->dev, Q, active, S->lds_medium);
+        });
+    hipLaunchKernelGGL(k_shade_gen, dim3((active + B - 1) / B), dim3(B), 0, st, S->dev, Q, active);
     int32_t counts[4];
     PG_HIP(hipMemcpyAsync(counts, Q.counters, sizeof(counts), hipMemcpyDeviceToHost, st));
     PG_HIP(hipStreamSynchronize(st));
@@ -3506,8 +3881,21 @@ static int bounce_general(pg_scene* S, PathsDev& Q, int32_t& active, hipStream_t
         PG_HIP(hipMemsetAsync(Q.counters + 3, 0, sizeof(int32_t), st));
         if (int rc = pg_trace_closest(S->accel, Q.seg_rays[src], n_seg, Q.seg_hits, Q.seg_uvs, st))
             return rc;
-        hipLaunchKernelGGL(k_shadow_tr, dim3(tracking_grid(n_seg, B)), dim3(B), lds, st, S->dev, Q, n_seg,
-                           src, S->lds_medium);
+        launch_tracking(S->track_sampler, S->track_media, [&](auto ks, auto km) {
+            if constexpr (decltype(km)::value != kMediaAny) {
+                if (lanes) {
+                    hipLaunchKernelGGL((k_shadow_track_lanes<decltype(ks)::value, decltype(km)::value>),
+                                       dim3(lane_grid(n_seg, B)), dim3(B), lds, st, S->dev, Q, n_seg, src,
+                                       S->lds_medium);
+                    hipLaunchKernelGGL((k_shadow_tr<decltype(ks)::value, decltype(km)::value, true>),
+                                       dim3((n_seg + B - 1) / B), dim3(B), 0, st, S->dev, Q, n_seg, src, -1);
+                    return;
+                }
+            }
+            hipLaunchKernelGGL((k_shadow_tr<decltype(ks)::value, decltype(km)::value, false>),
+                               dim3(tracking_grid(n_seg, B)), dim3(B), lds, st, S->dev, Q, n_seg, src,
+                               S->lds_medium);
+        });
         int32_t next = 0;
         PG_HIP(hipMemcpyAsync(&next, Q.counters + 3, sizeof(int32_t), hipMemcpyDeviceToHost, st));
         PG_HIP(hipStreamSynchronize(st));
@@ -3712,6 +4100,23 @@ static int render_enqueue(pg_scene* S, int spp, int spb_req, hipStream_t st, Gui
         PG_HIP(hipMemsetAsync(S->g_rays, 0, 2 * sizeof(unsigned long long), st));
     }
     int64_t rays = 0, ext_rays = 0;
+    S->infer_ms = 0.0;
+    S->infer_queries = 0;
+    // Tracking kernels compiled for this render: Halton indices stay below
+    // 2^32 when stride * (last sample index + 1) does (a pixel's offset is
+    // below the stride); brick-only kernels when every medium is a grid
+    // tracked with brick majorants.
+    if (S->dev.sampler.kind == kSamplerSobol)
+        S->track_sampler = kSamplerSobolOnly;
+    else
+        S->track_sampler = (uint64_t)S->dev.sampler.halton.stride * ((uint64_t)sample_offset + (uint64_t)spp) <=
+                                   (1ull << 32) ? kSamplerHalton32 : kSamplerAny;
+    S->track_media = kMediaBricks;
+    for (const MediumDev& m : S->media)
+        if (!(m.type == kMediumGrid && m.method == kDeltaTracking && m.brick > 0))
+            S->track_media = kMediaAny;
+    if (S->track_media == kMediaBricks && S->media.size() == 1)
+        S->track_media = kMediaOneBrickGrid;
     const int B = 256;
     for (int s0 = 0; s0 < spp; s0 += spb) {
         const int nb = std::min(spb, spp - s0);
@@ -3730,7 +4135,7 @@ static int render_enqueue(pg_scene* S, int spp, int spb_req, hipStream_t st, Gui
         // which pine does not count (path.cpp:42-46); the bound guards those.
         for (int bounce = 0; active > 0 && bounce < kMaxBounces; bounce++) {
             ext_rays += active;
-            PG_HIP(hipMemsetAsync(Q.counters, 0, 4 * sizeof(int32_t), st));
+            PG_HIP(hipMemsetAsync(Q.counters, 0, 8 * sizeof(int32_t), st));
             const bool viz_bvh = S->integrator == kIntViz && S->dev.viz_type == kVizBvh;
             if (viz_bvh) {
                 if (S->viz_counts_capacity < active) {
@@ -3787,11 +4192,18 @@ static int render_enqueue(pg_scene* S, int spp, int spb_req, hipStream_t st, Gui
                 }
                 active = counts[0];
             } else if (!S->general) {
-                if (S->dev.guided && bounce < S->dev.g_depth) {
+                const bool infer = S->dev.guided && bounce < S->dev.g_depth;
+                if (infer) {
+                    if (!S->infer_ev[0]) {
+                        PG_HIP(hipEventCreate(&S->infer_ev[0]));
+                        PG_HIP(hipEventCreate(&S->infer_ev[1]));
+                    }
                     hipLaunchKernelGGL(k_guide_gather, dim3((active + B - 1) / B), dim3(B), 0, st, S->dev, Q,
                                        active, nullptr);
+                    PG_HIP(hipEventRecord(S->infer_ev[0], st));
                     hipLaunchKernelGGL(k_guide_infer<1>, dim3(infer_blocks(active)), dim3(kInferThreads), 0, st,
                                        S->dev, Q, active, nullptr);
+                    PG_HIP(hipEventRecord(S->infer_ev[1], st));
                     hipLaunchKernelGGL(k_shade<true>, dim3((active + B - 1) / B), dim3(B), 0, st, S->dev, Q,
                                        active, nullptr);
                 } else {
@@ -3801,6 +4213,12 @@ static int render_enqueue(pg_scene* S, int spp, int spb_req, hipStream_t st, Gui
                 int32_t counts[2];
                 PG_HIP(hipMemcpyAsync(counts, Q.counters, sizeof(counts), hipMemcpyDeviceToHost, st));
                 PG_HIP(hipStreamSynchronize(st));
+                if (infer) {
+                    float ims = 0.0f;
+                    PG_HIP(hipEventElapsedTime(&ims, S->infer_ev[0], S->infer_ev[1]));
+                    S->infer_ms += ims;
+                    S->infer_queries += active;
+                }
                 rays += active + counts[1];
                 if (counts[1] > 0) {
                     if (int rc = pg_trace_occluded(S->accel, Q.shadow_rays, counts[1], Q.occluded, st))
@@ -3921,11 +4339,13 @@ extern "C" int pg_render(pg_scene* S, const pg_render_options* opts, void* strea
     S->dev.guided = 0;
     S->use_graph = !(opts && (opts->flags & PG_RENDER_NO_GRAPH));
     S->spectral = opts && (opts->flags & PG_RENDER_SPECTRAL);
+    S->lane_tracking = opts && (opts->flags & PG_RENDER_LANE_TRACKING);
     double ms = 0.0;
     int64_t rays = 0;
     const int rc = render_impl(S, spp, spb, (hipStream_t)stream, nullptr, &ms, &rays, offset);
     S->use_graph = true;
     S->spectral = false;
+    S->lane_tracking = false;
     if (rc)
         return rc;
     S->last_render_ms = ms;
@@ -4006,8 +4426,10 @@ extern "C" int pg_render_batch(pg_scene* const* scenes, int32_t n, const int32_t
         const int sp = spp && spp[i] > 0 ? spp[i] : S->spp;
         S->dev.guided = 0;
         S->use_graph = !(flags & PG_RENDER_NO_GRAPH);
+        S->lane_tracking = (flags & PG_RENDER_LANE_TRACKING) != 0;
         rc = render_enqueue(S, sp, 0, ws[k], nullptr, 0, &tickets[i]);
         S->use_graph = true;
+        S->lane_tracking = false;
     }
     for (int k = 0; k < K && rc == PG_OK; k++)
         hip(hipEventRecord(done[k], ws[k]), "record") && hip(hipStreamWaitEvent(st, done[k], 0), "wait");

@@ -180,3 +180,26 @@ def test_compressed_volumes_fail_loudly(pg, cuda, tmp_path):
     text = W.cloud_scene_text("cloud.compressed", width=8, height=8, spp=1)
     with pytest.raises(pg.PineError, match="compressed"):
         pg.Scene(text=text, directory=str(tmp_path))
+
+
+@pytest.mark.parametrize("bricks", [8, 4])
+def test_lane_tracking_equals_per_path_tracking(pg, cuda, tmp_path, bricks):
+    """Brick-majorant tracking as persistent lanes (PG_RENDER_LANE_TRACKING:
+    each lane takes the next ray when its own is done) gives the image of
+    one-ray-per-thread tracking bit for bit, with the same density lookups: the
+    same free flights, in the same order, from the same sampler dimensions.
+    The cloud scene with its material-less rect, so shadow rays cross several
+    segments."""
+    fx = golden("render_cloud")
+    text = with_bricks(fx["scene"].item().decode(), bricks)
+    d = cloud_dir(tmp_path)
+    out = []
+    for lanes in (True, False):
+        s = pg.Scene(text=text, directory=d)
+        s.render(spp=128, lane_tracking=lanes)
+        out.append((s.film(), s.info()["medium_steps"], s.info()["last_rays"]))
+        s.release()
+    (a, la, ra), (b, lb, rb) = out
+    print(f"bricks {bricks}: {la} lookups, {ra} rays")
+    assert la == lb and ra == rb and la > 0
+    assert np.array_equal(a.view(np.int32), b.view(np.int32))

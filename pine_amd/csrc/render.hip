@@ -61,8 +61,11 @@ struct ShapeDev {
     int32_t mi_inside, mi_outside;  // MediumInterface (geometry.cpp:392-409), -1 = none
 };
 
-enum LightType : int32_t { kLightArea = 0, kLightPoint = 1, kLightDirectional = 2, kLightSky = 3, kLightAtmosphere = 4 };
-enum EnvKind : int32_t { kEnvSky = 0, kEnvAtmosphere = 1 };
+enum LightType : int32_t {
+    kLightArea = 0, kLightPoint = 1, kLightDirectional = 2, kLightSky = 3, kLightAtmosphere = 4,
+    kLightConstantEnv = 5  // this library's extension: radiance `color` from every direction
+};
+enum EnvKind : int32_t { kEnvSky = 0, kEnvAtmosphere = 1, kEnvConstant = 2 };
 // VizIntegrator::Type (viz.h:14)
 enum VizType : int32_t { kVizPosition = 0, kVizNormal = 1, kVizTexcoord = 2, kVizDpDu = 3, kVizDpDv = 4, kVizBvh = 5 };
 struct LightDev {
@@ -334,10 +337,16 @@ __device__ __forceinline__ f3 atmosphere_color(const SceneDev& s, f3 d) {
 // EnvironmentLight::Color and ::Pdf (light.h:150-160): Sky's pdf is 0,
 // Atmosphere's 1 / 4pi (its sampling pdf is half that; pine uses this value in
 // the BSDF-side MIS weight).
+// The Constant environment (an extension; pine has no such light) is sampled
+// uniformly over the sphere, so its pdf is 1 / 4pi as well.
 __device__ __forceinline__ f3 env_color(const SceneDev& s, f3 d) {
+    if (s.env_kind == kEnvConstant)
+        return s.sky_color;
     return s.env_kind == kEnvAtmosphere ? atmosphere_color(s, d) : sky_color(s, d);
 }
-__device__ __forceinline__ float env_pdf(const SceneDev& s) { return s.env_kind == kEnvAtmosphere ? 1.0f / (kPi * 4.0f) : 0.0f; }
+__device__ __forceinline__ float env_pdf(const SceneDev& s) {
+    return s.env_kind == kEnvSky ? 0.0f : 1.0f / (kPi * 4.0f);
+}
 
 struct Hit {
     f3 p, n;
@@ -542,6 +551,17 @@ __device__ __forceinline__ LightSampleD sample_light_point(const SceneDev& s, co
             ls.delta = true;
         }
         ls.pdf = ls.pdf * 0.5f;
+        return ls;
+    }
+    if (L.type == kLightConstantEnv) {
+        // A direction uniform on the sphere (UniformSphereSampling,
+        // sampling.h:41-43) and the constant radiance.
+        ls.distance = 1e10f;
+        const float phi = u0 * kPi * 2.0f, theta = pstd::acos(1.0f - 2.0f * u1);
+        const float st = pstd::sin(theta);
+        ls.wo = mk(st * pstd::cos(phi), st * pstd::sin(phi), pstd::cos(theta));
+        ls.pdf = 1.0f / (kPi * 4.0f);
+        ls.Le = L.color;
         return ls;
     }
     if (L.type == kLightDirectional) {
@@ -2861,6 +2881,15 @@ void build_scene(pg_scene& S) {
                 ep.get_vec3("sunColor", c);
                 l.type = kLightSky;
                 S.dev.env_kind = kEnvSky;
+                S.atmo_w = S.atmo_h = 0;
+            } else if (et == "Constant") {
+                // This library's extension (pine would fall back to its
+                // Atmosphere): the same radiance from every direction.
+                const float up[3] = {0.0f, 1.0f, 0.0f};
+                ep.get_vec3("color", c);
+                a[0] = up[0], a[1] = up[1], a[2] = up[2];
+                l.type = kLightConstantEnv;
+                S.dev.env_kind = kEnvConstant;
                 S.atmo_w = S.atmo_h = 0;
             } else {
                 if (et != "Atmosphere")

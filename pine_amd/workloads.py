@@ -351,13 +351,16 @@ CLOUD_BRICK = 16
 
 def cloud_scene_text(volume_file, width=1024, height=1024, spp=256, max_depth=64, bricks=0,
                      sigma_a=0.01, sigma_s=0.99, g=0.85, scale=2.0, passthrough=False,
-                     output="cloud.bmp"):
+                     output="cloud.bmp", lighting="sky"):
     """A density grid seen from outside, the camera inside the medium as in
     pine's scenes/cloud.txt (the grid bounds the density), a diffuse ground and
     pine's Sky environment light (its sun is sampled as a delta light).
     `bricks` > 0 adds the brick-majorant extension parameter.  `passthrough`
     adds a material-less rect in front of the grid (surfaces that rays and
-    shadow rays cross, path.cpp:42-46 and integrator.cpp:80-84)."""
+    shadow rays cross, path.cpp:42-46 and integrator.cpp:80-84).
+    lighting="sun+constant" replaces the Sky by a Directional sun (colour 3,
+    from the Sky's sun direction) and a Constant environment of radiance 0.3
+    (this library's extension: pine has no constant environment light)."""
     out = [
         "Integrator: Path{",
         f"    maxDepth: {max_depth}",
@@ -383,12 +386,24 @@ def cloud_scene_text(volume_file, width=1024, height=1024, spp=256, max_depth=64
         "        albedo: 0.35 0.3 0.25",
         "    }",
         "}",
+    ] + ([
         "Light: Environment{",
         "    environment: Sky{",
         "        sunDirection: -1 6 1",
         "        sunColor: 1 1 1",
         "    }",
         "}",
+    ] if lighting == "sky" else [
+        "Light: Directional{",
+        "    direction: -1 6 1",
+        "    color: 3 3 3",
+        "}",
+        "Light: Environment{",
+        "    environment: Constant{",
+        "        color: 0.3 0.3 0.3",
+        "    }",
+        "}",
+    ]) + [
         "Shape: Rect{",
         "    position: 0 -1.2 0",
         "    ex: 0 0 100",

@@ -219,8 +219,9 @@ typedef struct pg_render_options {
 /* Spectral transport (an extension; pine renders RGB): four wavelengths per
  * path by hero-wavelength sampling, the scene's RGB reflectances, emission and
  * sky upsampled to smooth spectra, radiance projected to RGB through the CIE
- * 1931 matching functions.  Gray scenes render as in RGB in expectation.  The
- * fused pipeline only (scenes without media or material-less shapes). */
+ * 1931 matching functions.  Gray scenes render as in RGB in expectation.  Both
+ * pipelines (the split bounce of media and material-less shapes too); not
+ * combined with the learned guide. */
 #define PG_RENDER_SPECTRAL 2
 /* Track brick-majorant media as persistent lanes (each lane takes the next ray
  * of its workgroup's share of the queue when its own is done) instead of one

@@ -1233,11 +1233,12 @@ __device__ __forceinline__ int medium_for(f3 w, f3 n, int inside, int outside) {
 }
 
 // EstimateDirect's light sample (integrator.cpp:88-119) up to the shadow ray,
-// for a surface (mat >= 0) or medium (mat < 0, phase g) vertex.  The record is
+// for a surface (its material, node-graph inputs evaluated) or medium (mat
+// null, phase g) vertex.  The record is
 // written even when the contribution is black: IntersectTr still runs, and in
 // media it consumes sampler dimensions.
 __device__ __forceinline__ void light_record(const SceneDev& s, PathsDev& P, SamplerState& ss, int32_t i, f3 p,
-                             f3 n, f3 wi, int mat, float g, int med_in, int med_out) {
+                             f3 n, f3 wi, const MaterialDev* mat, float g, int med_in, int med_out) {
     const float ul = get_1d(s.sampler, ss);  // SampleLight's Get1D runs even with no lights
     if (s.n_lights <= 0)
         return;
@@ -1248,8 +1249,8 @@ __device__ __forceinline__ void light_record(const SceneDev& s, PathsDev& P, Sam
     ls.pdf *= lpdf;
     f3 f;
     float scatPdf = 0.0f, pb = 0.0f;
-    if (mat >= 0) {
-        const MaterialDev& m = s.materials[mat];
+    if (mat) {
+        const MaterialDev& m = *mat;
         const M3 w2n = inverse(frame_from(n));
         f = material_f(m, mul(w2n, wi), mul(w2n, ls.wo)) * abs_dot(ls.wo, n);
         pb = material_pdf(m, mul(w2n, wi), mul(w2n, ls.wo));
@@ -1573,6 +1574,29 @@ __global__ __launch_bounds__(256) void k_shadow_track_lanes(SceneDev s, PathsDev
     add_lookups(P.lookups, lookups);
 }
 
+// The four wavelengths' upsampling rows of path i (spectral mode): an RGB
+// quantity of the scene becomes the path's 4-wavelength spectrum.
+struct Lift4 {
+    f3 up[4];
+    __device__ __forceinline__ Lift4(const SceneDev& s, const PathsDev& P, int32_t i) {
+        const float u = P.lam[i];
+        for (int j = 0; j < 4; j++)
+            up[j] = spec::upsample_row(s.spec, spec::wavelength(u, j));
+    }
+    __device__ __forceinline__ Spec4 operator()(f3 c, float scale) const {
+        Spec4 r;
+        for (int j = 0; j < 4; j++)
+            r.v[j] = fmaxf(0.0f, dot(up[j], c)) * scale;
+        return r;
+    }
+};
+
+// kSpec: spectral mode (PG_RENDER_SPECTRAL), every colour of the path a
+// 4-wavelength spectrum (L4, beta4), the scene's RGB factors upsampled; the
+// geometry, sampling and sampler dimensions are those of the RGB bounce.
+// kNodes: the scene has node-graph materials (the interpreter's code and
+// registers only in that instantiation).
+template <bool kSpec, bool kNodes>
 __global__ __launch_bounds__(256) void k_shade_gen(SceneDev s, PathsDev P, int32_t n_active) {
     for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n_active;
          k += gridDim.x * blockDim.x) {
@@ -1582,8 +1606,25 @@ __global__ __launch_bounds__(256) void k_shade_gen(SceneDev s, PathsDev P, int32
         const f3 ro = mk(ra.x, ra.y, ra.z), rd = mk(rb.x, rb.y, rb.z);
         const pg_hit hit = P.hits[k];
         const int depth = P.depth[i];
-        f3 L = mk(P.L[3 * i], P.L[3 * i + 1], P.L[3 * i + 2]);
-        f3 beta = mk(P.beta[3 * i], P.beta[3 * i + 1], P.beta[3 * i + 2]);
+        f3 L, beta;
+        Spec4 L4, beta4;
+        if constexpr (kSpec) {
+            L4 = spec_load(P.L4, i);
+            beta4 = spec_load(P.beta4, i);
+        } else {
+            L = mk(P.L[3 * i], P.L[3 * i + 1], P.L[3 * i + 2]);
+            beta = mk(P.beta[3 * i], P.beta[3 * i + 1], P.beta[3 * i + 2]);
+        }
+        // L += beta * c * w, in RGB as pine computes it, or per wavelength.
+        auto add_light = [&](f3 c, float w, bool weighted) {
+            if constexpr (kSpec) {
+                const Spec4 cs = Lift4(s, P, i)(c, weighted ? w : 1.0f);
+                for (int j = 0; j < 4; j++)
+                    L4.v[j] += beta4.v[j] * cs.v[j];
+            } else {
+                L = weighted ? L + beta * c * w : L + beta * c;
+            }
+        };
         const float bsdfPdf = P.bsdf_pdf[i];
         SamplerState ss;
         ss.index = P.sidx[i];
@@ -1600,22 +1641,31 @@ __global__ __launch_bounds__(256) void k_shade_gen(SceneDev s, PathsDev P, int32
             ev.weight = mk(e0.x, e0.y, e0.z);
             ev.scattered = e0.w != 0.0f;
             ev.p = mk(e1.x, e1.y, e1.z);
-            beta = beta * ev.weight;
+            if constexpr (kSpec) {
+                const Spec4 ws = Lift4(s, P, i)(ev.weight, 1.0f);
+                for (int j = 0; j < 4; j++)
+                    beta4.v[j] *= ws.v[j];
+            } else {
+                beta = beta * ev.weight;
+            }
         }
         if (ev.scattered) {
             if (depth + 1 != s.max_depth) {
                 const f3 zero = mk(0.0f, 0.0f, 0.0f);
-                light_record(s, P, ss, i, ev.p, zero, -rd, -1, s.media[med].g, med, med);
+                light_record(s, P, ss, i, ev.p, zero, -rd, nullptr, s.media[med].g, med, med);
                 save_vertex(P, i, 1, ev.p, zero, -rd, -1, med, med);
             }
         } else if (hit.prim < 0) {
             if (s.has_env) {
                 const f3 le = env_color(s, rd);
-                L = depth == 0 ? L + beta * le : L + beta * le * power_heuristic(bsdfPdf, env_pdf(s));
+                if (depth == 0)
+                    add_light(le, 1.0f, false);
+                else
+                    add_light(le, power_heuristic(bsdfPdf, env_pdf(s)), true);
             }
         } else {
             const pg_hit_uv uv = P.uvs[k];
-            const Hit h = surface(s, hit.prim, hit.t, uv.u, uv.v, ro, rd);
+            Hit h = surface(s, hit.prim, hit.t, uv.u, uv.v, ro, rd);
             const ShapeDev& sh = s.shapes[h.shape];
             // bvh.cpp:702-707: a shape's own interface if it is a transition,
             // else the ray's medium on both sides.
@@ -1628,27 +1678,38 @@ __global__ __launch_bounds__(256) void k_shade_gen(SceneDev s, PathsDev P, int32
                 next_med = medium_for(rd, h.n, min_, mout);
                 alive = true;
             } else {
-                const MaterialDev& m = s.materials[sh.material];
+                // Node-graph inputs evaluated at this point (and the bump map
+                // applied to h.n, path.cpp:48) into the path's own material
+                // copy, which k_shade_gen_b samples after the shadow ray.
+                const MaterialDev* mp = s.materials + sh.material;
+                if (kNodes && mp->has_nodes)
+                    mp = apply_nodes(s, P, i, mp, h, hit.prim, hit.t, uv.u, uv.v, ro, rd);
+                const MaterialDev& m = *mp;
                 if (m.kind == kMatEmissive) {
                     const f3 le = emitted(m, h.n, -rd);
                     if (depth == 0) {
-                        L = L + beta * le;
+                        add_light(le, 1.0f, false);
                     } else {
                         const float lightPdf = hit.t * hit.t / (abs_dot(-rd, h.n) * sh.area);
-                        L = L + beta * le * power_heuristic(bsdfPdf, lightPdf);
+                        add_light(le, power_heuristic(bsdfPdf, lightPdf), true);
                     }
                 } else if (depth + 1 != s.max_depth) {
-                    light_record(s, P, ss, i, h.p, h.n, -rd, sh.material, 0.0f, min_, mout);
+                    light_record(s, P, ss, i, h.p, h.n, -rd, &m, 0.0f, min_, mout);
                     save_vertex(P, i, 0, h.p, h.n, -rd, sh.material, min_, mout);
                 }
             }
         }
-        P.L[3 * i + 0] = L.x;
-        P.L[3 * i + 1] = L.y;
-        P.L[3 * i + 2] = L.z;
-        P.beta[3 * i + 0] = beta.x;
-        P.beta[3 * i + 1] = beta.y;
-        P.beta[3 * i + 2] = beta.z;
+        if constexpr (kSpec) {
+            spec_store(P.L4, i, L4);
+            spec_store(P.beta4, i, beta4);
+        } else {
+            P.L[3 * i + 0] = L.x;
+            P.L[3 * i + 1] = L.y;
+            P.L[3 * i + 2] = L.z;
+            P.beta[3 * i + 0] = beta.x;
+            P.beta[3 * i + 1] = beta.y;
+            P.beta[3 * i + 2] = beta.z;
+        }
         P.sdim[i] = ss.dim;
         if (alive) {
             const int slot = atomicAdd(&P.counters[0], 1);
@@ -1835,6 +1896,7 @@ __global__ __launch_bounds__(256) void k_shadow_tr(SceneDev s, PathsDev P, int32
 // EstimateDirect's tail (integrator.cpp:101-118) and `L += beta * ...`.
 // At a guided vertex the directional strategy is the guide/BSDF mixture, so
 // the light sample's MIS weight uses the mixture pdf of the light direction.
+template <bool kSpec>
 __global__ __launch_bounds__(256) void k_resolve_gen(SceneDev s, PathsDev P, int32_t n_rec) {
     const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n_rec)
@@ -1855,17 +1917,30 @@ __global__ __launch_bounds__(256) void k_resolve_gen(SceneDev s, PathsDev P, int
         const float pmix = alpha * guide::mix_pdf(gp, mk(d.x, d.y, d.z)) + (1.0f - alpha) * d.w;
         w = power_heuristic(pdf, pmix);
     }
-    const f3 ld = c.w != 0.0f ? tr * f * le / pdf : tr * f * w * le / pdf;
-    const f3 beta = mk(P.beta[3 * i], P.beta[3 * i + 1], P.beta[3 * i + 2]);
-    const f3 add = beta * ld;
-    P.L[3 * i + 0] += add.x;
-    P.L[3 * i + 1] += add.y;
-    P.L[3 * i + 2] += add.z;
+    if constexpr (kSpec) {
+        // Transmittance, BSDF or phase value and emission each upsampled at the
+        // path's wavelengths, multiplied per wavelength.
+        const Lift4 lift(s, P, i);
+        const Spec4 ts = lift(tr, 1.0f), fs = lift(f, 1.0f), ls = lift(le, (c.w != 0.0f ? 1.0f : w) / pdf);
+        const Spec4 beta4 = spec_load(P.beta4, i);
+        Spec4 L4 = spec_load(P.L4, i);
+        for (int j = 0; j < 4; j++)
+            L4.v[j] += beta4.v[j] * ts.v[j] * fs.v[j] * ls.v[j];
+        spec_store(P.L4, i, L4);
+    } else {
+        const f3 ld = c.w != 0.0f ? tr * f * le / pdf : tr * f * w * le / pdf;
+        const f3 beta = mk(P.beta[3 * i], P.beta[3 * i + 1], P.beta[3 * i + 2]);
+        const f3 add = beta * ld;
+        P.L[3 * i + 0] += add.x;
+        P.L[3 * i + 1] += add.y;
+        P.L[3 * i + 2] += add.z;
+    }
 }
 
 // Direction sampling after the shadow ray: PhaseFunction::Sample at a medium
 // vertex (path.cpp:19-25), the material's Sample + Russian roulette at a
 // surface (path.cpp:66-86).
+template <bool kSpec>
 __global__ __launch_bounds__(256) void k_shade_gen_b(SceneDev s, PathsDev P, int32_t n_b) {
     const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n_b)
@@ -1874,7 +1949,12 @@ __global__ __launch_bounds__(256) void k_shade_gen_b(SceneDev s, PathsDev P, int
     const float* v = P.vtx + 9 * (int64_t)i;
     const f3 p = mk(v[0], v[1], v[2]), n = mk(v[3], v[4], v[5]), wi = mk(v[6], v[7], v[8]);
     const int4 info = *reinterpret_cast<const int4*>(P.vinfo + 4 * (int64_t)i);
-    f3 beta = mk(P.beta[3 * i], P.beta[3 * i + 1], P.beta[3 * i + 2]);
+    f3 beta;
+    Spec4 beta4;
+    if constexpr (kSpec)
+        beta4 = spec_load(P.beta4, i);
+    else
+        beta = mk(P.beta[3 * i], P.beta[3 * i + 1], P.beta[3 * i + 2]);
     SamplerState ss;
     ss.index = P.sidx[i];
     ss.dim = P.sdim[i];
@@ -1891,10 +1971,10 @@ __global__ __launch_bounds__(256) void k_shade_gen_b(SceneDev s, PathsDev P, int
         nextO = offset_ray_origin(p, face_same_hemisphere(n, wo));
         next_med = info.z;
         alive = true;
-    } else if (s.guided && P.gflag[i]) {
+    } else if (!kSpec && s.guided && P.gflag[i]) {
         f3 wo;
         if (guided_direction(s, P, ss, i, depth, P.gparams + guide::kParamStride * (int64_t)i,
-                             s.materials[info.y], p, n, wi,
+                             s.materials[info.y].has_nodes ? P.node_mat[i] : s.materials[info.y], p, n, wi,
                              mk(P.L[3 * i], P.L[3 * i + 1], P.L[3 * i + 2]), beta, wo)) {
             nextD = wo;
             nextO = offset_ray_origin(p, face_same_hemisphere(n, nextD));
@@ -1909,7 +1989,9 @@ __global__ __launch_bounds__(256) void k_shade_gen_b(SceneDev s, PathsDev P, int
             }
         }
     } else {
-        const MaterialDev& m = s.materials[info.y];
+        // The material as k_shade_gen evaluated it at this vertex (its
+        // node-graph inputs in the path's own copy).
+        const MaterialDev& m = s.materials[info.y].has_nodes ? P.node_mat[i] : s.materials[info.y];
         const M3 n2w = frame_from(n);
         const M3 w2n = inverse(n2w);
         const f3 wiL = mul(w2n, wi);
@@ -1920,26 +2002,48 @@ __global__ __launch_bounds__(256) void k_shade_gen_b(SceneDev s, PathsDev P, int
         if (material_sample(m, wiL, u1, u2a, u2b, bs)) {
             bs.wo = mul(n2w, bs.wo);
             if (!is_black(bs.f) && bs.pdf != 0.0f) {
-                beta = beta * (abs_dot(bs.wo, n) * bs.f / bs.pdf);
+                if constexpr (kSpec) {
+                    const Spec4 fs = Lift4(s, P, i)(bs.f, abs_dot(bs.wo, n) / bs.pdf);
+                    for (int j = 0; j < 4; j++)
+                        beta4.v[j] *= fs.v[j];
+                } else {
+                    beta = beta * (abs_dot(bs.wo, n) * bs.f / bs.pdf);
+                }
                 P.bsdf_pdf[i] = bs.pdf;
                 nextD = bs.wo;
                 nextO = offset_ray_origin(p, face_same_hemisphere(n, nextD));
                 next_med = medium_for(nextD, n, info.z, info.w);
                 alive = true;
                 if (depth > 2) {
-                    const float q = pstd::clamp(1.0f - luminance(beta), 0.05f, 1.0f);
-                    if (get_1d(s.sampler, ss) < q)
-                        alive = false;
-                    else
-                        beta = beta / (1.0f - q);
+                    if constexpr (kSpec) {
+                        // Russian roulette on the mean throughput (as k_shade_spec).
+                        const float mean = 0.25f * (beta4.v[0] + beta4.v[1] + beta4.v[2] + beta4.v[3]);
+                        const float q = pstd::clamp(1.0f - mean, 0.05f, 1.0f);
+                        if (get_1d(s.sampler, ss) < q) {
+                            alive = false;
+                        } else {
+                            for (int j = 0; j < 4; j++)
+                                beta4.v[j] /= (1.0f - q);
+                        }
+                    } else {
+                        const float q = pstd::clamp(1.0f - luminance(beta), 0.05f, 1.0f);
+                        if (get_1d(s.sampler, ss) < q)
+                            alive = false;
+                        else
+                            beta = beta / (1.0f - q);
+                    }
                 }
             }
         }
     }
     depth++;
-    P.beta[3 * i + 0] = beta.x;
-    P.beta[3 * i + 1] = beta.y;
-    P.beta[3 * i + 2] = beta.z;
+    if constexpr (kSpec) {
+        spec_store(P.beta4, i, beta4);
+    } else {
+        P.beta[3 * i + 0] = beta.x;
+        P.beta[3 * i + 1] = beta.y;
+        P.beta[3 * i + 2] = beta.z;
+    }
     P.sdim[i] = ss.dim;
     P.depth[i] = depth;
     if (alive) {
@@ -3117,14 +3221,6 @@ void build_scene(pg_scene& S) {
     }
     if (uses_media)
         S.general = true;
-    // The split bounce saves a vertex as (p, n, wi) and samples its BSDF later,
-    // without the surface's uv: node-graph materials run in the fused kernels
-    // and in the RandomWalk kernels only.
-    if (S.general && S.integrator == kIntPath)
-        for (const MaterialDev& m : S.materials)
-            if (m.has_nodes)
-                throw unsupported("node-graph materials in a Path scene with participating media or "
-                                  "material-less shapes");
     const Params& fp = cp.sub("film");
     int size[2];
     const int defsize[2] = {720, 480};
@@ -3893,7 +3989,12 @@ static int bounce_general(pg_scene* S, PathsDev& Q, int32_t& active, hipStream_t
             hipLaunchKernelGGL((k_medium_track<decltype(ks)::value, decltype(km)::value>),
                                dim3(tracking_grid(active, B)), dim3(B), lds, st, S->dev, Q, active, S->lds_medium);
         });
-    hipLaunchKernelGGL(k_shade_gen, dim3((active + B - 1) / B), dim3(B), 0, st, S->dev, Q, active);
+    {
+        const bool nodes = !S->node_progs.empty();
+        auto kern = S->spectral ? (nodes ? k_shade_gen<true, true> : k_shade_gen<true, false>)
+                                : (nodes ? k_shade_gen<false, true> : k_shade_gen<false, false>);
+        hipLaunchKernelGGL(kern, dim3((active + B - 1) / B), dim3(B), 0, st, S->dev, Q, active);
+    }
     int32_t counts[4];
     PG_HIP(hipMemcpyAsync(counts, Q.counters, sizeof(counts), hipMemcpyDeviceToHost, st));
     PG_HIP(hipStreamSynchronize(st));
@@ -3935,9 +4036,11 @@ static int bounce_general(pg_scene* S, PathsDev& Q, int32_t& active, hipStream_t
         hipLaunchKernelGGL(k_guide_infer<0>, dim3(infer_blocks(n_b)), dim3(kInferThreads), 0, st, S->dev, Q, n_b,
                            nullptr);
     if (n_rec > 0)
-        hipLaunchKernelGGL(k_resolve_gen, dim3((n_rec + B - 1) / B), dim3(B), 0, st, S->dev, Q, n_rec);
+        hipLaunchKernelGGL(S->spectral ? k_resolve_gen<true> : k_resolve_gen<false>, dim3((n_rec + B - 1) / B),
+                           dim3(B), 0, st, S->dev, Q, n_rec);
     if (n_b > 0)
-        hipLaunchKernelGGL(k_shade_gen_b, dim3((n_b + B - 1) / B), dim3(B), 0, st, S->dev, Q, n_b);
+        hipLaunchKernelGGL(S->spectral ? k_shade_gen_b<true> : k_shade_gen_b<false>, dim3((n_b + B - 1) / B),
+                           dim3(B), 0, st, S->dev, Q, n_b);
     PG_HIP(hipMemcpyAsync(counts, Q.counters, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     PG_HIP(hipStreamSynchronize(st));
     active = counts[0];
@@ -4101,9 +4204,8 @@ static int render_enqueue(pg_scene* S, int spp, int spb_req, hipStream_t st, Gui
             return rc;
     }
     if (S->spectral) {
-        if (S->general || S->dev.guided)
-            return pg_fail(PG_E_ARG, "spectral rendering needs the fused pipeline: no media, no material-less "
-                                     "shapes, no guide");
+        if (S->dev.guided)
+            return pg_fail(PG_E_ARG, "spectral rendering is not combined with the learned guide");
         if (int rc = ensure_spectral(*S, n_paths))
             return rc;
     }
@@ -4207,7 +4309,7 @@ static int render_enqueue(pg_scene* S, int spp, int spb_req, hipStream_t st, Gui
                 PG_HIP(hipStreamSynchronize(st));
                 rays += active;
                 active = next;
-            } else if (S->spectral) {
+            } else if (S->spectral && !S->general) {
                 hipLaunchKernelGGL(k_shade_spec, dim3((active + B - 1) / B), dim3(B), 0, st, S->dev, Q, active);
                 int32_t counts[2];
                 PG_HIP(hipMemcpyAsync(counts, Q.counters, sizeof(counts), hipMemcpyDeviceToHost, st));

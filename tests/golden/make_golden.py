@@ -41,6 +41,8 @@ Cases
   render_media  GridMedium rayMarching, nearest-voxel (interpolate: false) and
               both with a 3-voxel step (48x48, 256 spp, depth 16);
               HomogeneousMedium in a material-less sphere and around the camera
+  render_nodes_media  the node-graph scene with a material-less rect in view and
+              with the camera in a homogeneous fog (96x54, 128 spp)
   render_batch  five scenes of the 64-scene batched workload at reduced size
               (40..80 pixels square, 16..64 spp, icosphere <= 40,000 triangles)
   bmp         pine's tone-mapped BMP output (Film::Finalize) of the spheres,
@@ -634,6 +636,38 @@ def case_render_batch():
     np.savez_compressed(os.path.join(OUT, "render_batch.npz"), **out)
 
 
+NODES_MEDIA_CASES = {
+    # The nodes scene in the split bounce: a material-less rect that camera and
+    # shadow rays cross, and a thin gray homogeneous fog around the camera.
+    "rect": "Shape: Rect{\n    position: 0 1.2 6\n    ex: 3 0 0\n    ey: 0 1.5 0\n}\n",
+    "fog": "Medium fog: Homogeneous{\n    sigma_a: 0.01 0.01 0.01\n    sigma_s: 0.04 0.04 0.04\n"
+           "    g: 0.3\n}\n",
+}
+
+
+def nodes_media_scene_text(name):
+    text = W.nodes_scene_text(width=96, height=54, spp=128) + NODES_MEDIA_CASES[name]
+    if name == "fog":
+        text = text.replace("    fov: 0.42", "    medium: fog\n    fov: 0.42", 1)
+    return text
+
+
+def case_render_nodes_media():
+    """pine's node-graph materials in Path scenes that need the split bounce
+    (path.cpp:42-62 evaluates the material at the vertex, then EstimateDirect,
+    then Sample): a material-less rect and a homogeneous fog."""
+    out = {}
+    for name in NODES_MEDIA_CASES:
+        text = nodes_media_scene_text(name)
+        with tempfile.TemporaryDirectory() as d:
+            film, line = run_render(text, d)
+        out[name + "_scene"] = np.bytes_(text)
+        out[name + "_film"] = film
+        out[name + "_harness"] = np.bytes_(line)
+        print(f"render_nodes_media {name}:", line, flush=True)
+    np.savez_compressed(os.path.join(OUT, "render_nodes_media.npz"), **out)
+
+
 def synthetic_film(w=64, h=40, seed=17):
     """A film of accumulators spanning what Finalize meets: radiance
     log-uniform over 1e-5..1e4 per channel, black pixels, weights of several
@@ -705,6 +739,7 @@ CASES = {
     "bmp": case_bmp,
     "render_media": case_render_media,
     "render_batch": case_render_batch,
+    "render_nodes_media": case_render_nodes_media,
 }
 
 if __name__ == "__main__":

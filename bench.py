@@ -376,6 +376,25 @@ def traversal(args, pg, hip, W, v, i, rays, traffic):
     mrays = n / (ms_per_step * 1e-3) * 1e-6
     hits = d_hits.numpy()
     hit_frac = float((hits[:, 0] >= 0).mean())
+    # The same rays by the stackless traversal (parent links, per-ray trail,
+    # wave-wide node fetches; binary trees): same hits bit for bit, its rate.
+    variants = {}
+    if st["width"] == 2 and not (launch_flags & pg.LAUNCH_STACKLESS):
+        acc.set_launch(args.block, launch_flags | pg.LAUNCH_STACKLESS)
+        acc.trace_closest(d_rays, d_hits)
+        ev = [hip.Event() for _ in range(10)]
+        for k in range(5):
+            ev[2 * k].record()
+            acc.trace_closest(d_rays, d_hits)
+            ev[2 * k + 1].record()
+        hip.synchronize()
+        sl_ms = float(np.mean([ev[2 * k].elapsed_ms(ev[2 * k + 1]) for k in range(5)]))
+        same = bool(np.array_equal(d_hits.numpy(), hits))
+        variants["stackless"] = {"kernel_ms": round(sl_ms, 3),
+                                 "mrays_per_s": round(n / (sl_ms * 1e-3) * 1e-6, 1),
+                                 "hits_identical": same}
+        log(f"[bench] stackless traversal: kernel {sl_ms:.3f} ms, same hits: {same}")
+        acc.set_launch(args.block, launch_flags)
     if args.copy_calib:
         dst = hip.DeviceArray(d_rays.shape, np.float32)
         hip.check(hip.rt().hipMemcpy(dst.ptr, d_rays.ptr, d_rays.nbytes, hip.hipMemcpyDeviceToDevice))
@@ -409,7 +428,8 @@ def traversal(args, pg, hip, W, v, i, rays, traffic):
     config = {"workload": "bvh-traversal: icosphere 1,048,576 tris, 16,777,216 rays",
               "triangles": int(len(i)), "rays": int(n), "node_width": st["width"],
               "leaf_max": args.leaf, "block": args.block, "launch_flags": launch_flags,
-              "hit_fraction": round(hit_frac, 4), "build_ms": round(st["build_ms"], 1)}
+              "hit_fraction": round(hit_frac, 4), "build_ms": round(st["build_ms"], 1),
+              "variants": variants}
     return {"mrays": mrays, "ms_per_step": ms_per_step, "roofline": roofline, "config": config,
             "rays": rays, "hits": hits}
 

@@ -158,6 +158,9 @@ int pg_accel_prim_shape(const pg_accel* a, int32_t prim, int32_t* shape, int32_t
  * Every setting returns identical hits; only the speed differs. */
 #define PG_LAUNCH_POSTPONE_LEAVES 1 /* wave tests its leaves together (while-while loop) */
 #define PG_LAUNCH_XCD_REMAP 2       /* each XCD traces one contiguous range of rays */
+#define PG_LAUNCH_STACKLESS 4       /* binary trees: no stack, parent links and a per-ray trail of
+                                       pending far children; nodes shared by the whole wave are
+                                       fetched once by scalar loads (ignores POSTPONE_LEAVES) */
 #define PG_LAUNCH_DEFAULT 0
 int pg_accel_set_launch(pg_accel* a, int32_t block, int32_t flags);
 

@@ -20,6 +20,7 @@ PG_OK = 0
 # pg_accel_set_launch flags (include/pinehip.h)
 LAUNCH_POSTPONE_LEAVES = 1
 LAUNCH_XCD_REMAP = 2
+LAUNCH_STACKLESS = 4
 LAUNCH_DEFAULT = 0
 EXPORTED = [
     "pg_last_error", "pg_version", "pg_accel_create", "pg_accel_add_mesh",

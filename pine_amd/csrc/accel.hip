@@ -374,6 +374,148 @@ __global__ __launch_bounds__(kBlock) void k_trace(const float4* __restrict__ nod
     }
 }
 
+// --- stackless traversal with wave-wide node fetches (binary nodes) ---------
+// No stack: a 64-bit trail holds one bit per level of the current path, set
+// when the level's far child is still to be visited, and each node stores its
+// parent (float slot 14).  Children are ordered by the ray direction's sign on
+// the node's split axis (slot 15), so the far child is known again when the
+// walk climbs back to a node.  When every active lane of the wave is at the
+// same node (the top levels of the tree, and longer for coherent rays), the
+// node is fetched once for the wave by scalar loads.
+__device__ __forceinline__ void fetch2(const float4* __restrict__ nodes, int32_t P, float4& a, float4& b,
+                                       float4& c, float4& d) {
+    const int32_t p0 = __builtin_amdgcn_readfirstlane(P);
+    if (__all(P == p0)) {
+        const float4* nd = nodes + 4 * (int64_t)p0;
+        a = nd[0], b = nd[1], c = nd[2], d = nd[3];
+    } else {
+        const float4* nd = nodes + 4 * (int64_t)P;
+        a = nd[0], b = nd[1], c = nd[2], d = nd[3];
+    }
+}
+
+// Near and far child of a binary node for this ray (slot 15: axis | 4 when
+// child 1 lies lower on it; the lower child is near for a positive direction).
+__device__ __forceinline__ bool near_is_c1(const RayIn& r, int32_t info) {
+    const int axis = info & 3;
+    const float dir = axis == 0 ? r.d.x : (axis == 1 ? r.d.y : r.d.z);
+    return (dir >= 0.0f) == ((info & 4) != 0);
+}
+
+template <int kBlock, int kMode, bool kStats>
+__global__ __launch_bounds__(kBlock) void k_trace_stackless(const float4* __restrict__ nodes,
+                                                            const float4* __restrict__ tris,
+                                                            const float4* __restrict__ ana,
+                                                            const pg_ray* __restrict__ rays, int64_t n,
+                                                            pg_hit* __restrict__ hits,
+                                                            pg_hit_uv* __restrict__ uvs,
+                                                            int32_t* __restrict__ occluded,
+                                                            int2* __restrict__ counts, int xcd_remap,
+                                                            const int32_t* __restrict__ d_n) {
+    const int64_t i = logical_block(xcd_remap != 0) * kBlock + threadIdx.x;
+    if (i >= (d_n ? (int64_t)*d_n : n))
+        return;
+    const RayIn r = load_ray(rays, i);
+    float best_t = r.tmax, bu = 0.0f, bv = 0.0f;
+    int32_t best_prim = -1;
+    int visited = 0, tested = 0;
+    bool found_any = false;
+    int32_t P = 0;       // the inner node whose children are being visited
+    uint64_t trail = 0;  // bit 0: the current level's far child is pending
+    bool done = false;
+    while (!done) {
+        // Visit P: test both child boxes, go to the near hit child.
+        float4 a, b, c, d;
+        fetch2(nodes, P, a, b, c, d);
+        visited++;
+        const int32_t c0 = __float_as_int(d.x), c1 = __float_as_int(d.y), info = __float_as_int(d.w);
+        const float lim = kMode == kAnyHit ? r.tmax : best_t;
+        float t0, t1;
+        const bool h0 = slab(r, a.x, a.y, a.z, a.w, b.x, b.y, lim, t0) && c0 != kEmptyChild;
+        const bool h1 = slab(r, b.z, b.w, c.x, c.y, c.z, c.w, lim, t1) && c1 != kEmptyChild;
+        const bool nc1 = near_is_c1(r, info);
+        const bool hn = nc1 ? h1 : h0, hf = nc1 ? h0 : h1;
+        trail <<= 1;
+        int32_t next = 0;
+        bool have = true;
+        if (hn) {
+            next = nc1 ? c1 : c0;
+            if (hf)
+                trail |= 1;
+        } else if (hf) {
+            next = nc1 ? c0 : c1;
+        } else {
+            have = false;
+        }
+        if (have && next >= 0) {
+            P = next;
+            continue;
+        }
+        // A leaf, or nothing: test it, then climb to the next pending far child.
+        while (true) {
+            if (have) {
+                if (leaf_test<kMode>(next, tris, ana, r, best_t, best_prim, bu, bv, tested)) {
+                    found_any = true;
+                    done = true;
+                    break;
+                }
+                have = false;
+            }
+            if (trail & 1) {
+                trail ^= 1;
+                const float4 dd = nodes[4 * (int64_t)P + 3];
+                const int32_t f = near_is_c1(r, __float_as_int(dd.w)) ? __float_as_int(dd.x)
+                                                                       : __float_as_int(dd.y);
+                if (f >= 0) {
+                    P = f;
+                    break;
+                }
+                next = f;
+                have = true;
+                continue;
+            }
+            if (P == 0) {
+                done = true;
+                break;
+            }
+            trail >>= 1;
+            P = __float_as_int(nodes[4 * (int64_t)P + 3].z);
+        }
+    }
+    if (kStats)
+        counts[i] = make_int2(visited, tested);
+    if (kMode != kClosest) {
+        occluded[i] = found_any ? 1 : 0;
+    } else {
+        pg_hit h;
+        h.prim = best_prim;
+        h.t = best_prim >= 0 ? best_t : r.tmax;
+        hits[i] = h;
+        if (uvs) {
+            pg_hit_uv uv;
+            uv.u = bu;
+            uv.v = bv;
+            uvs[i] = uv;
+        }
+    }
+}
+
+template <int kMode, bool kStats>
+void launch_stackless(const pg_accel* a, int64_t grid, const pg_ray* rays, int64_t n, pg_hit* hits,
+                      pg_hit_uv* uvs, int32_t* occ, int2* counts, hipStream_t s, const int32_t* d_n) {
+    const int remap = (a->launch_flags & PG_LAUNCH_XCD_REMAP) ? 1 : 0;
+#define PG_LAUNCH(B)                                                                              \
+    hipLaunchKernelGGL((k_trace_stackless<B, kMode, kStats>), dim3((unsigned)grid), dim3(B), 0, s, \
+                       a->d_nodes, a->d_tris, a->d_ana, rays, n, hits, uvs, occ, counts, remap, d_n)
+    if (a->block == 64)
+        PG_LAUNCH(64);
+    else if (a->block == 256)
+        PG_LAUNCH(256);
+    else
+        PG_LAUNCH(128);
+#undef PG_LAUNCH
+}
+
 template <int W, int kMode, bool kStats, bool kPostpone>
 void launch_w(const pg_accel* a, int64_t grid, size_t lds, const pg_ray* rays, int64_t n,
               pg_hit* hits, pg_hit_uv* uvs, int32_t* occ, int2* counts, hipStream_t s,
@@ -399,7 +541,9 @@ hipError_t launch_trace(const pg_accel* a, const pg_ray* rays, int64_t n, pg_hit
     const int64_t grid = (n + a->block - 1) / a->block;
     const size_t lds = (size_t)a->stack_entries * a->block * sizeof(int32_t);
     const bool postpone = (a->launch_flags & PG_LAUNCH_POSTPONE_LEAVES) != 0;
-    if (a->width == 2) {
+    if (a->width == 2 && (a->launch_flags & PG_LAUNCH_STACKLESS)) {
+        launch_stackless<kMode, kStats>(a, grid, rays, n, hits, uvs, occ, counts, s, d_n);
+    } else if (a->width == 2) {
         if (postpone)
             launch_w<2, kMode, kStats, true>(a, grid, lds, rays, n, hits, uvs, occ, counts, s, d_n);
         else
@@ -753,7 +897,7 @@ extern "C" int pg_accel_set_launch(pg_accel* a, int32_t block, int32_t flags) {
         return pg_fail(PG_E_ARG, "pg_accel_set_launch: null accel");
     if (block != 0 && block != 64 && block != 128 && block != 256)
         return pg_fail(PG_E_ARG, "pg_accel_set_launch: block must be 64, 128 or 256");
-    if (flags < -1 || flags > (PG_LAUNCH_POSTPONE_LEAVES | PG_LAUNCH_XCD_REMAP))
+    if (flags < -1 || flags > (PG_LAUNCH_POSTPONE_LEAVES | PG_LAUNCH_XCD_REMAP | PG_LAUNCH_STACKLESS))
         return pg_fail(PG_E_ARG, "pg_accel_set_launch: unknown flags");
     if (block)
         a->block = block;

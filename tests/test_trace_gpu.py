@@ -149,10 +149,11 @@ def test_uv_output_matches_pine(pg, cuda):
 
 @pytest.mark.parametrize("width", WIDTHS)
 @pytest.mark.parametrize("block", [64, 128, 256])
-@pytest.mark.parametrize("flags", [0, 1, 2, 3])
+@pytest.mark.parametrize("flags", [0, 1, 2, 3, 4, 6])
 def test_launch_shapes_agree(pg, oracle, cuda, width, block, flags):
-    """Every block size, traversal loop (leaves postponed or not) and XCD
-    remap gives pine's hits; the any-hit and occlusion modes too."""
+    """Every block size, traversal loop (leaves postponed or not, stack or
+    stackless with wave-wide node fetches) and XCD remap gives pine's hits;
+    the any-hit and occlusion modes too."""
     fx = golden("trace_82k")
     v, i, rays = _mesh_rays(fx)
     a = _accel(pg, v, i, width, block, flags)
@@ -219,18 +220,20 @@ def test_single_triangle_and_empty_batch(pg, oracle, cuda):
 
 def test_full_benchmark_stream_is_tree_independent(pg, oracle, cuda):
     """All 16,777,216 benchmark rays: the closest hit does not depend on the
-    hierarchy, so the binary and 4-wide trees must agree bit for bit, and a
-    sample must agree with the CPU port of pine's traversal."""
+    hierarchy or the traversal, so the binary and 4-wide trees and the
+    stackless traversal must agree bit for bit, and a sample must agree with
+    the CPU port of pine's traversal."""
     from pine_amd import workloads as W
     v, i = W.make_mesh(W.ICOSPHERE_1M)
     rays = W.make_rays(W.RAYS_16M, v, i)
     d = cuda.DeviceArray.from_numpy(rays)
     outs = []
-    for width in WIDTHS:
-        a = _accel(pg, v, i, width)
+    for width, flags in ((2, -1), (4, -1), (2, pg.LAUNCH_STACKLESS)):
+        a = _accel(pg, v, i, width, 128, flags)
         outs.append(a.trace_closest(d).numpy())
         a.release()
     assert np.array_equal(outs[0], outs[1])
+    assert np.array_equal(outs[0], outs[2])  # stackless traversal, same hits bit for bit
     hitfrac = (outs[0][:, 0] >= 0).mean()
     assert 0.05 < hitfrac < 0.9
     sel = np.arange(0, len(rays), 251)[:65536]

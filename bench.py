@@ -540,12 +540,22 @@ def batch_leg(args, pg, W, work):
     return out
 
 
+GUIDED_RR_DEPTH = 4      # Russian roulette from depth 4 in the guided renders (pine: 2)
+GUIDED_TRAIN = dict(train_passes=8, train_spp=4, keep_passes=4)
+GUIDED_KEPT_SPP = GUIDED_TRAIN["train_spp"] * GUIDED_TRAIN["keep_passes"]
+
+
 def guided_leg(args, pg, W, work):
-    """Learned path guiding on the occluded-light box at equal time: the plain
-    tracer and the guided one (8 training passes of 4 spp, kept in the image,
-    then the final render) at several spp each, relMSE against a reference of the plain
-    tracer (pine_amd.metrics: mean of 1024-spp ranges without singular ones);
-    time for each to reach the plain tracer's error at 512 spp."""
+    """Learned path guiding on the occluded-light box at equal time.  Three
+    tracers, each at five sample counts, relMSE against a reference of the
+    plain tracer (pine_amd.metrics: mean of 1024-spp ranges without singular
+    ones): the plain tracer (pine's Russian roulette from depth 2), the plain
+    tracer with roulette from depth 4, and the guided tracer (8 training
+    passes of 4 spp, the last 4 kept in the image, roulette from depth 4) with
+    the same samples per pixel in the image as the plain one.  Reported: time
+    for each to reach the plain tracer's error at 512 spp, and the split of
+    the gain into the later roulette (plain rr4 vs plain) and guiding itself
+    (guided vs plain rr4)."""
     from pine_amd import metrics as M
     res = args.guide_res
     v, i = W.occluded_box_clutter()
@@ -555,16 +565,23 @@ def guided_leg(args, pg, W, work):
     t0 = time.time()
     ref = M.robust_reference(s, 1024, args.guide_ref_chunks)
     ref_s = time.time() - t0
-    pt, pe, gt, ge = [], [], [], []
-    for spp in (64, 128, 256, 512, 1024):
-        s.render(spp=spp, sample_offset=1 << 20)
-        pt.append(s.info()["last_render_ms"])
-        pe.append(M.relmse(M.linear(s.film()), ref))
-    train_ms = []
-    for spp in (32, 64, 128, 256, 512):
+    spps = (32, 64, 128, 256, 512, 1024)
+
+    def plain(rr):
+        ms, err = [], []
+        for spp in spps:
+            s.render(spp=spp, sample_offset=1 << 20, rr_depth=rr)
+            ms.append(s.info()["last_render_ms"])
+            err.append(M.relmse(M.linear(s.film()), ref))
+        return ms, err
+
+    pt, pe = plain(0)
+    qt, qe = plain(GUIDED_RR_DEPTH)
+    gt, ge, train_ms = [], [], []
+    for spp in spps:
         g = pg.Guide(seed=3, learning_rate=3e-3)
-        inf = s.render_guided(g, train_passes=8, train_spp=4, final_spp=spp, sample_offset=1 << 21,
-                              keep_passes=8)
+        inf = s.render_guided(g, final_spp=spp - GUIDED_KEPT_SPP, sample_offset=1 << 21,
+                              rr_depth=GUIDED_RR_DEPTH, **GUIDED_TRAIN)
         gt.append(inf["last_guided_ms"])
         train_ms.append(inf["last_guided_ms"] - inf["last_render_ms"])
         ge.append(M.relmse(M.linear(s.film()), ref))
@@ -572,8 +589,8 @@ def guided_leg(args, pg, W, work):
     # Guide inference on the matrix cores: one more guided render of the same
     # size drawn launch by launch, its inference kernels timed by HIP events.
     g = pg.Guide(seed=3, learning_rate=3e-3)
-    inf = s.render_guided(g, train_passes=8, train_spp=4, final_spp=64, sample_offset=1 << 21,
-                          keep_passes=8, graph=False)
+    inf = s.render_guided(g, final_spp=64 - GUIDED_KEPT_SPP, sample_offset=1 << 21,
+                          rr_depth=GUIDED_RR_DEPTH, graph=False, **GUIDED_TRAIN)
     g.release()
     iq, ims = int(inf["guide_infer_queries"]), float(inf["guide_infer_ms"])
     tflops = iq * GUIDE_FLOP_PER_QUERY / (ims * 1e-3) / 1e12 if ims > 0 else None
@@ -584,27 +601,42 @@ def guided_leg(args, pg, W, work):
             "basis": f"{GUIDE_FLOP_PER_QUERY} FLOP per network evaluation (2 x 25,449 MACs) over "
                      f"the inference kernels' time in a 64-spp guided render"}
     s.release()
-    target = pe[3]
-    tp, tg = M.time_to_error(pt, pe, target), M.time_to_error(gt, ge, target)
+    target = pe[spps.index(512)]
+    tp, tq, tg = (M.time_to_error(t, e, target) for t, e in ((pt, pe), (qt, qe), (gt, ge)))
+    # A series already below the target at its cheapest point reports that
+    # point's time: its speedup is then a lower bound.
+    bracketed = all(e[0] > target for e in (pe, qe, ge))
+
+    def series(ms, err):
+        return {"ms": [round(x, 2) for x in ms], "relmse": [float(f"{x:.4g}") for x in err]}
+
+    def ratio(a, b):
+        return round(a / b, 3) if a and b else None
+
     out = {
         "workload": f"occluded-light box (512,000 clutter triangles), {res}x{res}, maxDepth 8; "
-                    f"guide trained 8 passes x 4 spp, whose 32 spp stay in the image",
+                    f"guide trained {GUIDED_TRAIN['train_passes']} passes x "
+                    f"{GUIDED_TRAIN['train_spp']} spp, the last {GUIDED_TRAIN['keep_passes']} "
+                    f"kept in the image; Russian roulette from depth {GUIDED_RR_DEPTH} when guided",
         "reference": f"plain tracer, {args.guide_ref_chunks} x 1024 spp ({ref_s:.0f} s)",
-        "plain": {"spp": [64, 128, 256, 512, 1024], "ms": [round(x, 2) for x in pt],
-                  "relmse": [float(f"{x:.4g}") for x in pe]},
-        "guided": {"final_spp": [32, 64, 128, 256, 512], "spp_in_image": [32 + x for x in (32, 64, 128, 256, 512)],
-                   "ms_incl_training": [round(x, 2) for x in gt],
-                   "relmse": [float(f"{x:.4g}") for x in ge],
-                   "training_ms": round(float(np.mean(train_ms)), 2)},
+        "spp_in_image": list(spps),
+        "plain": series(pt, pe),
+        f"plain_rr{GUIDED_RR_DEPTH}": series(qt, qe),
+        "guided": dict(series(gt, ge), final_spp=[x - GUIDED_KEPT_SPP for x in spps],
+                       training_ms=round(float(np.mean(train_ms)), 2)),
         "target_relmse": float(f"{target:.4g}"),
         "plain_ms_to_target": round(tp, 2) if tp else None,
+        f"plain_rr{GUIDED_RR_DEPTH}_ms_to_target": round(tq, 2) if tq else None,
         "guided_ms_to_target": round(tg, 2) if tg else None,
-        "equal_time_speedup": round(tp / tg, 3) if tp and tg else None,
-        # 64 spp in both images: plain 64, guided 32 kept training + 32 final
-        "equal_spp_error_ratio_64": round(pe[0] / ge[0], 3),
+        "equal_time_speedup": ratio(tp, tg),
+        "speedups_are": "interpolated" if bracketed else "lower bounds (a series starts below the target)",
+        "equal_time_speedup_from_roulette": ratio(tp, tq),
+        "equal_time_speedup_from_guiding": ratio(tq, tg),
+        "equal_spp_error_ratio": [round(a / b, 3) for a, b in zip(pe, ge)],
         "roofline": mfma,
     }
-    log(f"[bench] guided: plain {tp} ms vs guided {tg} ms to relMSE {target:.3g}")
+    log(f"[bench] guided: plain {tp:.1f} ms, plain rr{GUIDED_RR_DEPTH} {tq:.1f} ms, guided {tg:.1f} ms "
+        f"to relMSE {target:.3g}" if tp and tq and tg else f"[bench] guided: target {target:.3g} not reached")
     return out
 
 

@@ -214,6 +214,9 @@ typedef struct pg_render_options {
     int32_t samples_per_batch;  /* 0 = automatic (the scene's paths in flight, 4M unless tuned) */
     int32_t sample_offset;      /* first sample index per pixel (progressive rendering) */
     int32_t flags;              /* PG_RENDER_* bits */
+    int32_t rr_depth;           /* Russian roulette (path.cpp:76-82) only after this depth; 0 =
+                                   pine's 2.  Other depths keep the estimator unbiased but are no
+                                   longer pine's sample sequence */
 } pg_render_options;
 /* Run the fused pipeline's bounce loop launch by launch, counts read back per
  * bounce, instead of replaying it as a recorded HIP graph (the default; both
@@ -410,6 +413,9 @@ typedef struct pg_guided_options {
     int32_t keep_passes;    /* the last keep_passes training passes stay in the final film (0 =
                                none): each is an unbiased guided render of its own sample range,
                                so the film holds train_spp * keep_passes + final_spp samples */
+    int32_t rr_depth;       /* Russian roulette (pine's rule, path.cpp:76-82) only after this
+                               depth in guided renders (0 = pine's 2); any depth keeps the
+                               estimator unbiased */
 } pg_guided_options;
 int pg_render_guided(pg_scene* s, pg_guide* g, const pg_guided_options* o, void* stream);
 

@@ -84,12 +84,14 @@ class _GuidedOptions(ctypes.Structure):
                 ("steps_per_pass", ctypes.c_int32), ("batch", ctypes.c_int32),
                 ("record_depth", ctypes.c_int32), ("final_spp", ctypes.c_int32),
                 ("sample_offset", ctypes.c_int32), ("guide_depth", ctypes.c_int32),
-                ("flags", ctypes.c_int32), ("keep_passes", ctypes.c_int32)]
+                ("flags", ctypes.c_int32), ("keep_passes", ctypes.c_int32),
+                ("rr_depth", ctypes.c_int32)]
 
 
 class _RenderOptions(ctypes.Structure):
     _fields_ = [("spp", ctypes.c_int32), ("samples_per_batch", ctypes.c_int32),
-                ("sample_offset", ctypes.c_int32), ("flags", ctypes.c_int32)]
+                ("sample_offset", ctypes.c_int32), ("flags", ctypes.c_int32),
+                ("rr_depth", ctypes.c_int32)]
 
 
 class _BatchOptions(ctypes.Structure):
@@ -380,10 +382,10 @@ class Scene:
         return {k: getattr(s, k) for k, _ in _SceneInfo._fields_}
 
     def render(self, spp=0, samples_per_batch=0, sample_offset=0, graph=True, spectral=False,
-               lane_tracking=False):
+               lane_tracking=False, rr_depth=0):
         flags = (0 if graph else RENDER_NO_GRAPH) | (RENDER_SPECTRAL if spectral else 0) | \
             (RENDER_LANE_TRACKING if lane_tracking else 0)
-        o = _RenderOptions(spp, samples_per_batch, sample_offset, flags)
+        o = _RenderOptions(spp, samples_per_batch, sample_offset, flags, rr_depth)
         check(lib().pg_render(self.h, ctypes.byref(o), _stream_handle()))
         return self.info()
 
@@ -395,11 +397,12 @@ class Scene:
 
     def render_guided(self, guide, train_passes=16, train_spp=8, steps_per_pass=0, batch=1024,
                       record_depth=5, final_spp=0, sample_offset=0, guide_depth=0, graph=True,
-                      keep_passes=0):
+                      keep_passes=0, rr_depth=0):
         """Trains `guide` on this scene, then renders final_spp with it (pg_render_guided);
         the last keep_passes training passes stay in the film."""
         o = _GuidedOptions(train_passes, train_spp, steps_per_pass, batch, record_depth, final_spp,
-                           sample_offset, guide_depth, 0 if graph else RENDER_NO_GRAPH, keep_passes)
+                           sample_offset, guide_depth, 0 if graph else RENDER_NO_GRAPH, keep_passes,
+                           rr_depth)
         check(lib().pg_render_guided(self.h, guide.h, ctypes.byref(o), _stream_handle()))
         return self.info()
 

@@ -183,7 +183,7 @@ int main(int argc, char** argv) {
         fprintf(stderr, "[pine]%s\n", pg_last_error());
         return 1;
     }
-    pg_render_options opts = {spp, 0, 0, 0};
+    pg_render_options opts = {spp, 0, 0, 0, 0};
     if (pg_render(scene, &opts, nullptr) != PG_OK) {
         fprintf(stderr, "[pine]%s\n", pg_last_error());
         pg_scene_release(scene);

@@ -113,6 +113,7 @@ struct SceneDev {
     // Learned path guiding (general mode only).
     int32_t guided;
     int32_t g_depth;  // vertices of depth < g_depth sample from the guide
+    int32_t rr_depth;  // Russian roulette after this depth (pine's: 2, path.cpp:76)
     spec::Tables spec;  // spectral mode: RGB <-> spectrum tables
     int32_t viz_type;   // Viz integrator: VizType
     const nodes::Inst* node_prog;  // material node-graph programs (MaterialDev::has_nodes)
@@ -784,7 +785,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev s, PathsDev P, int32_t n
                         nextD = wo;
                         nextO = offset_ray_origin(h.p, face_same_hemisphere(h.n, nextD));
                         alive = true;
-                        if (depth > 2) {
+                        if (depth > s.rr_depth) {
                             float q = pstd::clamp(1.0f - luminance(beta), 0.05f, 1.0f);
                             if (get_1d(s.sampler, ss) < q)
                                 alive = false;
@@ -806,7 +807,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev s, PathsDev P, int32_t n
                             nextD = bs.wo;
                             nextO = offset_ray_origin(h.p, face_same_hemisphere(h.n, nextD));
                             alive = true;
-                            if (depth > 2) {
+                            if (depth > s.rr_depth) {
                                 float q = pstd::clamp(1.0f - luminance(beta), 0.05f, 1.0f);
                                 if (get_1d(s.sampler, ss) < q)
                                     alive = false;
@@ -1138,7 +1139,7 @@ __global__ __launch_bounds__(256) void k_shade_spec(SceneDev s, PathsDev P, int3
                     nextD = bs.wo;
                     nextO = offset_ray_origin(h.p, face_same_hemisphere(h.n, nextD));
                     alive = true;
-                    if (depth > 2) {
+                    if (depth > s.rr_depth) {
                         const float mean = 0.25f * (beta.v[0] + beta.v[1] + beta.v[2] + beta.v[3]);
                         const float q = pstd::clamp(1.0f - mean, 0.05f, 1.0f);
                         if (get_1d(s.sampler, ss) < q) {
@@ -1980,7 +1981,7 @@ __global__ __launch_bounds__(256) void k_shade_gen_b(SceneDev s, PathsDev P, int
             nextO = offset_ray_origin(p, face_same_hemisphere(n, nextD));
             next_med = medium_for(nextD, n, info.z, info.w);
             alive = true;
-            if (depth > 2) {
+            if (depth > s.rr_depth) {
                 const float q = pstd::clamp(1.0f - luminance(beta), 0.05f, 1.0f);
                 if (get_1d(s.sampler, ss) < q)
                     alive = false;
@@ -2014,7 +2015,7 @@ __global__ __launch_bounds__(256) void k_shade_gen_b(SceneDev s, PathsDev P, int
                 nextO = offset_ray_origin(p, face_same_hemisphere(n, nextD));
                 next_med = medium_for(nextD, n, info.z, info.w);
                 alive = true;
-                if (depth > 2) {
+                if (depth > s.rr_depth) {
                     if constexpr (kSpec) {
                         // Russian roulette on the mean throughput (as k_shade_spec).
                         const float mean = 0.25f * (beta4.v[0] + beta4.v[1] + beta4.v[2] + beta4.v[3]);
@@ -2436,13 +2437,13 @@ struct pg_scene {
     // device memory), replayed for every batch while its key holds.
     struct GraphKey {
         int64_t capacity = -1, accel_gen = -1, paths_gen = -1;
-        int guided = 0, oct = 0, max_depth = 0, g_depth = 0;
+        int guided = 0, oct = 0, max_depth = 0, g_depth = 0, rr_depth = 0;
         const void* g_params = nullptr;
         const void* g_wf = nullptr;
         bool operator==(const GraphKey& o) const {
             return capacity == o.capacity && accel_gen == o.accel_gen && paths_gen == o.paths_gen &&
                    guided == o.guided && oct == o.oct && max_depth == o.max_depth && g_depth == o.g_depth &&
-                   g_params == o.g_params && g_wf == o.g_wf;
+                   rr_depth == o.rr_depth && g_params == o.g_params && g_wf == o.g_wf;
         }
     };
     bool use_graph = true;
@@ -3327,6 +3328,7 @@ void build_scene(pg_scene& S) {
     d.width = size[0];
     d.height = size[1];
     d.max_depth = S.max_depth;
+    d.rr_depth = 2;  // PathIntegrator::Li: Russian roulette after depth 2 (path.cpp:76)
     d.n_lights = nl;
     d.light_uniform = S.light_uniform ? 1 : 0;
     d.light_func_int = S.light_func_int;
@@ -4108,6 +4110,7 @@ static int ensure_graph(pg_scene* S, const PathsDev& P, int64_t cap) {
     key.paths_gen = S->paths_gen;
     key.guided = S->dev.guided;
     key.g_depth = S->dev.g_depth;
+    key.rr_depth = S->dev.rr_depth;
     key.oct = S->dev.g_oct;
     key.max_depth = S->max_depth;
     key.g_params = S->dev.g_params;
@@ -4464,10 +4467,13 @@ static int render_impl(pg_scene* S, int spp, int spb_req, hipStream_t st, GuideT
 extern "C" int pg_render(pg_scene* S, const pg_render_options* opts, void* stream) {
     if (!S)
         return pg_fail(PG_E_ARG, "pg_render: null scene");
+    if (opts && opts->rr_depth < 0)
+        return pg_fail(PG_E_ARG, "pg_render: rr_depth must be >= 0");
     const int spp = opts && opts->spp > 0 ? opts->spp : S->spp;
     const int spb = opts && opts->samples_per_batch > 0 ? opts->samples_per_batch : 0;
     const int offset = opts && opts->sample_offset > 0 ? opts->sample_offset : 0;
     S->dev.guided = 0;
+    S->dev.rr_depth = opts && opts->rr_depth > 0 ? opts->rr_depth : 2;
     S->use_graph = !(opts && (opts->flags & PG_RENDER_NO_GRAPH));
     S->spectral = opts && (opts->flags & PG_RENDER_SPECTRAL);
     S->lane_tracking = opts && (opts->flags & PG_RENDER_LANE_TRACKING);
@@ -4556,6 +4562,7 @@ extern "C" int pg_render_batch(pg_scene* const* scenes, int32_t n, const int32_t
         pg_scene* S = scenes[i];
         const int sp = spp && spp[i] > 0 ? spp[i] : S->spp;
         S->dev.guided = 0;
+        S->dev.rr_depth = 2;
         S->use_graph = !(flags & PG_RENDER_NO_GRAPH);
         S->lane_tracking = (flags & PG_RENDER_LANE_TRACKING) != 0;
         rc = render_enqueue(S, sp, 0, ws[k], nullptr, 0, &tickets[i]);
@@ -4599,6 +4606,7 @@ extern "C" int pg_render_batch(pg_scene* const* scenes, int32_t n, const int32_t
 // One probe render of the scene as configured; its time on the stream.
 static int probe(pg_scene* S, int spp, hipStream_t st, double* ms) {
     S->dev.guided = 0;
+    S->dev.rr_depth = 2;
     int64_t rays = 0;
     return render_impl(S, spp, 0, st, nullptr, ms, &rays, 0);
 }
@@ -4701,6 +4709,8 @@ extern "C" int pg_render_guided(pg_scene* S, pg_guide* G, const pg_guided_option
                                 void* stream) {
     if (!S || !G)
         return pg_fail(PG_E_ARG, "pg_render_guided: null scene or guide");
+    if (o && o->rr_depth < 0)
+        return pg_fail(PG_E_ARG, "pg_render_guided: rr_depth must be >= 0");
     const int passes = o && o->train_passes > 0 ? o->train_passes : 16;
     const int train_spp = o && o->train_spp > 0 ? o->train_spp : 8;
     const int steps = o && o->steps_per_pass > 0 ? o->steps_per_pass : 0;
@@ -4711,6 +4721,7 @@ extern "C" int pg_render_guided(pg_scene* S, pg_guide* G, const pg_guided_option
     const int keep = o && o->keep_passes > 0 ? std::min(o->keep_passes, passes) : 0;
     SceneDev& d = S->dev;
     d.g_depth = o && o->guide_depth > 0 ? o->guide_depth : (1 << 30);
+    d.rr_depth = o && o->rr_depth > 0 ? o->rr_depth : 2;
     d.g_params = guide_params_dev(G);
     d.g_wf = guide_wf_dev(G);
     d.g_oct = guide_freq_offset(G);

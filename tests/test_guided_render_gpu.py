@@ -34,10 +34,28 @@ def scene(pg, box, res, spp=64):
                                                    max_depth=8), directory=d)
 
 
-def test_guided_estimator_is_unbiased(pg, cuda, tmp_path):
+def blocks(a):
+    return a.reshape(8, 12, 8, 12, 3).mean(axis=(1, 3))
+
+
+def check_converges(img, ref, what):
+    """img and ref, two 2048-spp renders of the 96 x 96 spheres scene, agree
+    in their 8 x 8 block means: within 1% over the image, 3% block by block."""
+    ok = np.isfinite(img).all(-1) & np.isfinite(ref).all(-1)
+    assert ok.all()
+    bg, br = blocks(img), blocks(ref)
+    rel = np.abs(bg - br).mean() / br.mean()
+    print(f"{what}: block mean rel. difference {rel:.4f}, image means {bg.mean():.5f} vs {br.mean():.5f}")
+    assert abs(bg.mean() - br.mean()) < 0.01 * br.mean()
+    assert rel < 0.03
+
+
+@pytest.mark.parametrize("rr_depth", [0, 4])
+def test_guided_estimator_is_unbiased(pg, cuda, tmp_path, rr_depth):
     """The spheres scene (diffuse spheres are guided, dielectric and conductor
     ones are not): the guided image converges to the plain tracer's, block by
-    block, whatever the guide learned."""
+    block, whatever the guide learned, with pine's Russian roulette depth and
+    with a later one."""
     from conftest import golden
     from pine_amd import metrics as M
     text = golden("render_spheres")["scene"].item().decode()
@@ -45,19 +63,32 @@ def test_guided_estimator_is_unbiased(pg, cuda, tmp_path):
     s.render(spp=2048)
     ref = M.linear(s.film())
     g = pg.Guide(seed=3, learning_rate=3e-3)
-    inf = s.render_guided(g, train_passes=4, train_spp=4, final_spp=2048)
-    img = M.linear(s.film())
-    ok = np.isfinite(img).all(-1) & np.isfinite(ref).all(-1)
-    assert ok.all()
-    def blocks(a):
-        return a.reshape(8, 12, 8, 12, 3).mean(axis=(1, 3))
-    bg, br = blocks(img), blocks(ref)
-    rel = np.abs(bg - br).mean() / br.mean()
-    print(f"guided {inf['last_render_ms']:.1f} ms; block mean rel. difference {rel:.4f}, "
-          f"image means {bg.mean():.5f} vs {br.mean():.5f}")
-    assert abs(bg.mean() - br.mean()) < 0.01 * br.mean()
-    assert rel < 0.03
+    inf = s.render_guided(g, train_passes=4, train_spp=4, final_spp=2048, rr_depth=rr_depth)
+    print(f"guided {inf['last_render_ms']:.1f} ms")
+    check_converges(M.linear(s.film()), ref, f"guided, rr_depth {rr_depth}")
     g.release()
+    s.release()
+
+
+def test_russian_roulette_depth(pg, cuda, tmp_path):
+    """rr_depth 0 is pine's depth 2, sample for sample; a later depth changes
+    the samples but not what they converge to; a negative one is refused."""
+    from conftest import golden
+    from pine_amd import metrics as M
+    text = golden("render_spheres")["scene"].item().decode()
+    s = pg.Scene(text=text, directory=str(tmp_path))
+    s.render(spp=64, sample_offset=77)
+    f0 = s.film()
+    s.render(spp=64, sample_offset=77, rr_depth=2)
+    assert np.array_equal(s.film(), f0)
+    s.render(spp=64, sample_offset=77, rr_depth=5)
+    assert not np.array_equal(s.film(), f0)
+    s.render(spp=2048)
+    ref = M.linear(s.film())
+    s.render(spp=2048, rr_depth=5)
+    check_converges(M.linear(s.film()), ref, "plain, rr_depth 5")
+    with pytest.raises(pg.PineError):
+        s.render(spp=4, rr_depth=-1)
     s.release()
 
 

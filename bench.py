@@ -563,16 +563,24 @@ def guided_leg(args, pg, W, work):
     s = pg.Scene(text=W.occluded_box_scene_text("clutter.obj", width=res, height=res, spp=64,
                                                  max_depth=8), directory=work)
     t0 = time.time()
-    ref = M.robust_reference(s, 1024, args.guide_ref_chunks)
+    ref = M.robust_reference(s, 1024, args.guide_ref_chunks,
+                             progress=lambda c: log(f"[bench] guided: reference {c} x 1024 spp"))
     ref_s = time.time() - t0
     spps = (32, 64, 128, 256, 512, 1024)
+
+    untrimmed = {}
+
+    def score(key, err):
+        img = M.linear(s.film())
+        err.append(M.relmse(img, ref))
+        untrimmed.setdefault(key, []).append(M.relmse_untrimmed(img, ref))
 
     def plain(rr):
         ms, err = [], []
         for spp in spps:
             s.render(spp=spp, sample_offset=1 << 20, rr_depth=rr)
             ms.append(s.info()["last_render_ms"])
-            err.append(M.relmse(M.linear(s.film()), ref))
+            score(f"plain_rr{rr}", err)
         return ms, err
 
     pt, pe = plain(0)
@@ -584,7 +592,7 @@ def guided_leg(args, pg, W, work):
                               rr_depth=GUIDED_RR_DEPTH, **GUIDED_TRAIN)
         gt.append(inf["last_guided_ms"])
         train_ms.append(inf["last_guided_ms"] - inf["last_render_ms"])
-        ge.append(M.relmse(M.linear(s.film()), ref))
+        score("guided", ge)
         g.release()
     # Guide inference on the matrix cores: one more guided render of the same
     # size drawn launch by launch, its inference kernels timed by HIP events.
@@ -607,8 +615,11 @@ def guided_leg(args, pg, W, work):
     # point's time: its speedup is then a lower bound.
     bracketed = all(e[0] > target for e in (pe, qe, ge))
 
-    def series(ms, err):
-        return {"ms": [round(x, 2) for x in ms], "relmse": [float(f"{x:.4g}") for x in err]}
+    def series(ms, err, key):
+        u = untrimmed[key]
+        return {"ms": [round(x, 2) for x in ms], "relmse": [float(f"{x:.4g}") for x in err],
+                "relmse_untrimmed": [float(f"{x:.4g}") for x, _ in u],
+                "nonfinite_pixels": [n for _, n in u]}
 
     def ratio(a, b):
         return round(a / b, 3) if a and b else None
@@ -618,11 +629,14 @@ def guided_leg(args, pg, W, work):
                     f"guide trained {GUIDED_TRAIN['train_passes']} passes x "
                     f"{GUIDED_TRAIN['train_spp']} spp, the last {GUIDED_TRAIN['keep_passes']} "
                     f"kept in the image; Russian roulette from depth {GUIDED_RR_DEPTH} when guided",
-        "reference": f"plain tracer, {args.guide_ref_chunks} x 1024 spp ({ref_s:.0f} s)",
+        "reference": f"plain tracer, {args.guide_ref_chunks} x 1024 spp ({ref_s:.0f} s), "
+                     f"per-pixel mean of the ranges without singular ones",
+        "metric": "relMSE = mean (img - ref)^2 / (ref^2 + 0.01) over pixels, the 0.1% largest "
+                  "dropped (relmse_untrimmed: none dropped, non-finite pixels counted apart)",
         "spp_in_image": list(spps),
-        "plain": series(pt, pe),
-        f"plain_rr{GUIDED_RR_DEPTH}": series(qt, qe),
-        "guided": dict(series(gt, ge), final_spp=[x - GUIDED_KEPT_SPP for x in spps],
+        "plain": series(pt, pe, "plain_rr0"),
+        f"plain_rr{GUIDED_RR_DEPTH}": series(qt, qe, f"plain_rr{GUIDED_RR_DEPTH}"),
+        "guided": dict(series(gt, ge, "guided"), final_spp=[x - GUIDED_KEPT_SPP for x in spps],
                        training_ms=round(float(np.mean(train_ms)), 2)),
         "target_relmse": float(f"{target:.4g}"),
         "plain_ms_to_target": round(tp, 2) if tp else None,
@@ -633,6 +647,9 @@ def guided_leg(args, pg, W, work):
         "equal_time_speedup_from_roulette": ratio(tp, tq),
         "equal_time_speedup_from_guiding": ratio(tq, tg),
         "equal_spp_error_ratio": [round(a / b, 3) for a, b in zip(pe, ge)],
+        # Untrimmed errors are listed per series but give no time-to-error:
+        # single finite paths of ~1e9 radiance in the plain images put their
+        # untrimmed relMSE at ~1e17 at random sample counts.
         "roofline": mfma,
     }
     log(f"[bench] guided: plain {tp:.1f} ms, plain rr{GUIDED_RR_DEPTH} {tq:.1f} ms, guided {tg:.1f} ms "
@@ -663,7 +680,7 @@ def main():
     ap.add_argument("--no-guide", action="store_true", help="skip the path-guiding leg")
     ap.add_argument("--guide-res", type=int, default=1024,
                     help="occluded-box resolution of the guiding leg")
-    ap.add_argument("--guide-ref-chunks", type=int, default=16,
+    ap.add_argument("--guide-ref-chunks", type=int, default=64,
                     help="1024-spp ranges in the guiding leg's reference")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one device-to-device copy of the ray buffer (512 MiB), the "

@@ -33,13 +33,24 @@ def relmse(img, ref, trim=1e-3):
     return float(np.sort(e.ravel())[:keep].mean())
 
 
-def robust_reference(scene, spp_per_chunk, chunks, outlier=1000.0):
+def relmse_untrimmed(img, ref):
+    """(relMSE over every pixel whose value and reference are finite, with no
+    trimming; the number of the other pixels)."""
+    ok = np.isfinite(img).all(-1) & np.isfinite(ref).all(-1)
+    e = ((img - ref) ** 2 / (ref ** 2 + 1e-2)).mean(-1)
+    return float(e[ok].mean()), int((~ok).sum())
+
+
+def robust_reference(scene, spp_per_chunk, chunks, outlier=1000.0, progress=None):
     """Per-pixel mean over `chunks` renders of disjoint sample ranges, leaving
-    out singular ranges (see the module docstring)."""
+    out singular ranges (see the module docstring).  progress(c) is called
+    after every 8th chunk."""
     imgs = []
     for c in range(chunks):
         scene.render(spp=spp_per_chunk, sample_offset=c * spp_per_chunk)
         imgs.append(linear(scene.film()))
+        if progress and (c + 1) % 8 == 0:
+            progress(c + 1)
     return reference_from_chunks(np.stack(imgs), outlier)
 
 

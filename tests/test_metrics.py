@@ -34,3 +34,14 @@ def test_relmse_trims_the_worst_pixels_and_counts_non_finite_as_worst():
     assert abs(M.time_to_error([10, 20, 40], [4.0, 2.0, 1.0], 2.0) - 20.0) < 1e-9
     assert abs(M.time_to_error([10, 20, 40], [4.0, 2.0, 1.0], np.sqrt(2.0)) - 20 * np.sqrt(2.0)) < 1e-9
     assert M.time_to_error([10, 20], [4.0, 2.0], 1.0) is None
+
+
+def test_untrimmed_relmse_counts_nonfinite_pixels_apart():
+    from pine_amd import metrics as M
+    ref = np.ones((4, 4, 3), np.float32)
+    img = ref.copy()
+    img[0, 0] = 2.0                      # error 1 / 1.01 on one pixel of 15 finite ones
+    img[3, 3, 1] = np.nan
+    e, bad = M.relmse_untrimmed(img, ref)
+    assert bad == 1
+    assert abs(e - (1 / 1.01) / 15) < 1e-7

@@ -4,7 +4,10 @@ Headline workload (`value`, Mrays/s): closest-hit BVH traversal of a
 1,048,576-triangle displaced geodesic icosphere by 16,777,216 rays (4,194,304
 primary rays of a 2048x2048 pinhole camera + 12,582,912 cosine-distributed
 rays leaving seeded surface points).  One step = one pg_trace_closest pass
-over all rays, rays and hierarchy resident in HBM.
+over all rays, rays and hierarchy resident in HBM; by default the launch
+first sorts the rays on the device by direction octant and origin cell
+(PG_LAUNCH_REORDER: a key kernel and a radix sort, inside the step) and
+writes every hit to its ray's slot.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--width 2] [--block 128]
 
@@ -371,11 +374,22 @@ def traversal(args, pg, hip, W, v, i, rays, traffic):
         ends[k].record()
     hip.synchronize()
     wall = time.perf_counter() - w0
-    kern_ms = float(np.mean([s.elapsed_ms(e) for s, e in zip(starts, ends)]))
+    step_ms = float(np.mean([s.elapsed_ms(e) for s, e in zip(starts, ends)]))
     ms_per_step = wall * 1e3 / args.steps
     mrays = n / (ms_per_step * 1e-3) * 1e-6
     hits = d_hits.numpy()
     hit_frac = float((hits[:, 0] >= 0).mean())
+    # The traversal kernel alone (a reordered launch also runs the key kernel
+    # and the sort): HIP events the library records around that kernel on its
+    # stream (PG_LAUNCH_TIMED), one launch at a time.
+    acc.set_launch(args.block, launch_flags | pg.LAUNCH_TIMED)
+    trace_ms = []
+    for _ in range(args.steps):
+        acc.trace_closest(d_rays, d_hits)
+        trace_ms.append(acc.stats()["last_trace_ms"])
+    acc.set_launch(args.block, launch_flags)
+    kern_ms = float(np.mean(trace_ms))
+    same_timed = bool(np.array_equal(d_hits.numpy(), hits))
     # The same rays by the stackless traversal (parent links, per-ray trail,
     # wave-wide node fetches; binary trees): same hits bit for bit, its rate.
     variants = {}
@@ -390,18 +404,35 @@ def traversal(args, pg, hip, W, v, i, rays, traffic):
         hip.synchronize()
         sl_ms = float(np.mean([ev[2 * k].elapsed_ms(ev[2 * k + 1]) for k in range(5)]))
         same = bool(np.array_equal(d_hits.numpy(), hits))
-        variants["stackless"] = {"kernel_ms": round(sl_ms, 3),
+        variants["stackless"] = {"step_ms": round(sl_ms, 3), "launch_flags": launch_flags | pg.LAUNCH_STACKLESS,
                                  "mrays_per_s": round(n / (sl_ms * 1e-3) * 1e-6, 1),
                                  "hits_identical": same}
-        log(f"[bench] stackless traversal: kernel {sl_ms:.3f} ms, same hits: {same}")
+        log(f"[bench] stackless traversal: step {sl_ms:.3f} ms, same hits: {same}")
         acc.set_launch(args.block, launch_flags)
     if args.copy_calib:
         dst = hip.DeviceArray(d_rays.shape, np.float32)
         hip.check(hip.rt().hipMemcpy(dst.ptr, d_rays.ptr, d_rays.nbytes, hip.hipMemcpyDeviceToDevice))
         hip.synchronize()
         dst.free()
-    log(f"[bench] {ms_per_step:.3f} ms/step, kernel {kern_ms:.3f} ms, {mrays:.1f} Mrays/s, "
-        f"hit fraction {hit_frac:.3f}")
+    # The same rays in the callers' order, no sort: the reordering's gain.
+    unordered = None
+    if launch_flags & pg.LAUNCH_REORDER:
+        acc.set_launch(args.block, launch_flags & ~pg.LAUNCH_REORDER)
+        acc.trace_closest(d_rays, d_hits)
+        ev = [hip.Event() for _ in range(10)]
+        for k in range(5):
+            ev[2 * k].record()
+            acc.trace_closest(d_rays, d_hits)
+            ev[2 * k + 1].record()
+        hip.synchronize()
+        un_ms = float(np.mean([ev[2 * k].elapsed_ms(ev[2 * k + 1]) for k in range(5)]))
+        unordered = {"step_ms": round(un_ms, 3), "mrays_per_s": round(n / (un_ms * 1e-3) * 1e-6, 1),
+                     "hits_identical": bool(np.array_equal(d_hits.numpy(), hits))}
+        acc.set_launch(args.block, launch_flags)
+        log(f"[bench] rays in the callers' order: step {un_ms:.3f} ms, same hits: "
+            f"{unordered['hits_identical']}")
+    log(f"[bench] {ms_per_step:.3f} ms/step (events {step_ms:.3f} ms), traversal kernel {kern_ms:.3f} ms, "
+        f"{mrays:.1f} Mrays/s, hit fraction {hit_frac:.3f}")
     d_rays.free()
     d_hits.free()
     acc.release()
@@ -420,8 +451,10 @@ def traversal(args, pg, hip, W, v, i, rays, traffic):
         "frac_counters": round(traffic["bytes"] / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
         if traffic else None,
         "compulsory_frac": round(compulsory / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
-        "limiter": "VALU issue and L2-miss latency, not HBM bandwidth: 72% VALU busy, L2 hit "
-                   "rate 49% (profiles/k_trace2_icosphere1m_rays16m_counters.md)",
+        "limiter": "VALU issue (78% busy with the rays reordered), not HBM bandwidth (profiles/"
+                   "k_trace2_reordered_icosphere1m_rays16m.md); the algorithmic figure counts every "
+                   "node visit at 64 B, cache hits included, so it exceeds the HBM peak",
+        "kernel_ms": round(kern_ms, 4),
         "per_ray": {"nodes": round(nodes_fetched / n, 3), "prims": round(prims_tested / n, 3)},
         "traffic_method": traffic["method"] if traffic else "not measured in this run",
     }
@@ -429,7 +462,9 @@ def traversal(args, pg, hip, W, v, i, rays, traffic):
               "triangles": int(len(i)), "rays": int(n), "node_width": st["width"],
               "leaf_max": args.leaf, "block": args.block, "launch_flags": launch_flags,
               "hit_fraction": round(hit_frac, 4), "build_ms": round(st["build_ms"], 1),
-              "variants": variants}
+              "step_ms_events": round(step_ms, 4), "trace_kernel_ms": round(kern_ms, 4),
+              "reorder_ms": round(step_ms - kern_ms, 4) if launch_flags & pg.LAUNCH_REORDER else 0.0,
+              "timed_hits_identical": same_timed, "unordered": unordered, "variants": variants}
     return {"mrays": mrays, "ms_per_step": ms_per_step, "roofline": roofline, "config": config,
             "rays": rays, "hits": hits}
 
@@ -664,8 +699,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--width", type=int, default=2)
     ap.add_argument("--block", type=int, default=128)
-    ap.add_argument("--flags", type=int, default=-1,
-                    help="pg_accel_set_launch flags (1 postpone leaves, 2 XCD remap); -1 = default")
+    ap.add_argument("--flags", type=int, default=8,
+                    help="pg_accel_set_launch flags (1 postpone leaves, 2 XCD remap, 4 stackless, "
+                         "8 rays reordered on the device); default 8; -1 = the library's default")
     ap.add_argument("--leaf", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-traffic", action="store_true",

@@ -145,6 +145,9 @@ typedef struct pg_accel_stats {
     double build_ms;
     int32_t block;         /* traversal workgroup size (pg_accel_set_launch) */
     int32_t launch_flags;  /* PG_LAUNCH_* bits */
+    double last_trace_ms;  /* the traversal kernel alone (not the reordering sort) of the last
+                              launch made with PG_LAUNCH_TIMED, from HIP events recorded around
+                              that kernel on its stream (get_stats waits for it); -1 if none */
 } pg_accel_stats;
 int pg_accel_get_stats(const pg_accel* a, pg_accel_stats* out);
 
@@ -161,6 +164,19 @@ int pg_accel_prim_shape(const pg_accel* a, int32_t prim, int32_t* shape, int32_t
 #define PG_LAUNCH_STACKLESS 4       /* binary trees: no stack, parent links and a per-ray trail of
                                        pending far children; nodes shared by the whole wave are
                                        fetched once by scalar loads (ignores POSTPONE_LEAVES) */
+#define PG_LAUNCH_REORDER 8         /* rays traced in the order of a key (direction octant, Morton
+                                       code of the origin in the hierarchy's bounds) found by a
+                                       counting sort on the device before each launch of 65,536
+                                       rays or more; hits land in the callers' slots, identical to
+                                       an unordered launch.  The sort's buffers belong to the accel
+                                       (allocated at the first such launch, grown on demand; the
+                                       _dev entry points reorder only when they already hold the
+                                       capacity, and never while their stream is being captured
+                                       into a graph: the sort clears memory with memset nodes), so
+                                       concurrent launches on one accel from several streams must
+                                       not use this flag */
+#define PG_LAUNCH_TIMED 16          /* records HIP events around the traversal kernel of each
+                                       launch (pg_accel_stats.last_trace_ms) */
 #define PG_LAUNCH_DEFAULT 0
 int pg_accel_set_launch(pg_accel* a, int32_t block, int32_t flags);
 

@@ -21,6 +21,8 @@ PG_OK = 0
 LAUNCH_POSTPONE_LEAVES = 1
 LAUNCH_XCD_REMAP = 2
 LAUNCH_STACKLESS = 4
+LAUNCH_REORDER = 8
+LAUNCH_TIMED = 16
 LAUNCH_DEFAULT = 0
 EXPORTED = [
     "pg_last_error", "pg_version", "pg_accel_create", "pg_accel_add_mesh",
@@ -56,7 +58,7 @@ class _Stats(ctypes.Structure):
                 ("node_bytes", ctypes.c_int64), ("prim_bytes", ctypes.c_int64),
                 ("width", ctypes.c_int32), ("max_depth", ctypes.c_int32),
                 ("build_ms", ctypes.c_double), ("block", ctypes.c_int32),
-                ("launch_flags", ctypes.c_int32)]
+                ("launch_flags", ctypes.c_int32), ("last_trace_ms", ctypes.c_double)]
 
 
 class _GuideConfig(ctypes.Structure):

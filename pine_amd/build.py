@@ -20,7 +20,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
 HOST_SRC = ["bvh_build.cpp", "workload.cpp", "scene_parse.cpp"]
-HIP_SRC = ["accel.hip", "render.hip", "guide.hip"]
+HIP_SRC = ["accel.hip", "render.hip", "guide.hip", "sort.hip"]
 
 CXXFLAGS = ["-std=c++17", "-O3", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
 HIPFLAGS = [

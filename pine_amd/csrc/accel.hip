@@ -23,6 +23,11 @@
 namespace pg {
 
 thread_local std::string g_last_error;
+
+// sort.hip
+hipError_t reorder_sort_temp_bytes(int64_t n, int end_bit, size_t& bytes);
+hipError_t reorder_sort(void* temp, size_t bytes, const uint32_t* keys_in, uint32_t* keys_out,
+                        uint32_t* perm, int64_t n, int end_bit, hipStream_t s);
 int fail(int code, const std::string& msg) {
     g_last_error = msg;
     return code;
@@ -69,7 +74,31 @@ struct pg_accel {
     int32_t launch_flags = PG_LAUNCH_DEFAULT;
     int stack_entries = 1;  // LDS stack entries per lane, from the built depth
     int64_t generation = 0;  // bumped by every build and launch change (graph reuse key)
+    // Ray reordering (PG_LAUNCH_REORDER): the hierarchy's bounds for the key,
+    // and the sort's device buffers.
+    float root_lo[3] = {0, 0, 0}, root_hi[3] = {0, 0, 0};
+    uint32_t* d_keys = nullptr;  // 2 x sort_cap: keys, then the sorted keys
+    int32_t* d_perm = nullptr;   // key-order slot -> caller's slot
+    void* d_sort_tmp = nullptr;  // the radix sort's temporary storage
+    size_t sort_tmp_bytes = 0;
+    int64_t sort_cap = 0;
+    // PG_LAUNCH_TIMED: events around the last launch's traversal kernel.
+    hipEvent_t ev_start = nullptr, ev_end = nullptr;
+    bool timed_pending = false;
 
+    void free_sort() {
+        if (d_keys)
+            (void)hipFree(d_keys);
+        if (d_perm)
+            (void)hipFree(d_perm);
+        if (d_sort_tmp)
+            (void)hipFree(d_sort_tmp);
+        d_keys = nullptr;
+        d_perm = nullptr;
+        d_sort_tmp = nullptr;
+        sort_tmp_bytes = 0;
+        sort_cap = 0;
+    }
     void free_device() {
         if (d_nodes)
             (void)hipFree(d_nodes);
@@ -80,6 +109,15 @@ struct pg_accel {
         d_nodes = nullptr;
         d_tris = nullptr;
         d_ana = nullptr;
+        free_sort();
+    }
+    void free_events() {
+        if (ev_start)
+            (void)hipEventDestroy(ev_start);
+        if (ev_end)
+            (void)hipEventDestroy(ev_end);
+        ev_start = ev_end = nullptr;
+        timed_pending = false;
     }
 };
 
@@ -301,7 +339,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(const float4* __restrict__ nod
                                                   pg_hit_uv* __restrict__ uvs,
                                                   int32_t* __restrict__ occluded,
                                                   int2* __restrict__ counts, int xcd_remap,
-                                                  const int32_t* __restrict__ d_n) {
+                                                  const int32_t* __restrict__ d_n,
+                                                  const int32_t* __restrict__ perm) {
     extern __shared__ int32_t lds_stack[];
     const int lane = threadIdx.x;
     const int64_t i = logical_block(xcd_remap != 0) * kBlock + lane;
@@ -309,7 +348,10 @@ __global__ __launch_bounds__(kBlock) void k_trace(const float4* __restrict__ nod
     // capacity, as in a captured bounce loop); the blocks past it exit.
     if (i >= (d_n ? (int64_t)*d_n : n))
         return;
-    const RayIn r = load_ray(rays, i);
+    // With reordering, slot i traces ray perm[i] (key order) and its result
+    // goes to that ray's slot.
+    const int64_t o = perm ? (int64_t)perm[i] : i;
+    const RayIn r = load_ray(rays, o);
     float best_t = r.tmax, bu = 0.0f, bv = 0.0f;
     int32_t best_prim = -1;
     Stack st{lds_stack, lane, kBlock, 0};
@@ -357,19 +399,19 @@ __global__ __launch_bounds__(kBlock) void k_trace(const float4* __restrict__ nod
         }
     }
     if (kStats)
-        counts[i] = make_int2(visited, tested);
+        counts[o] = make_int2(visited, tested);
     if (kMode != kClosest) {
-        occluded[i] = found_any ? 1 : 0;
+        occluded[o] = found_any ? 1 : 0;
     } else {
         pg_hit h;
         h.prim = best_prim;
         h.t = best_prim >= 0 ? best_t : r.tmax;
-        hits[i] = h;
+        hits[o] = h;
         if (uvs) {
             pg_hit_uv uv;
             uv.u = bu;
             uv.v = bv;
-            uvs[i] = uv;
+            uvs[o] = uv;
         }
     }
 }
@@ -411,11 +453,15 @@ __global__ __launch_bounds__(kBlock) void k_trace_stackless(const float4* __rest
                                                             pg_hit_uv* __restrict__ uvs,
                                                             int32_t* __restrict__ occluded,
                                                             int2* __restrict__ counts, int xcd_remap,
-                                                            const int32_t* __restrict__ d_n) {
+                                                            const int32_t* __restrict__ d_n,
+                                                            const int32_t* __restrict__ perm) {
     const int64_t i = logical_block(xcd_remap != 0) * kBlock + threadIdx.x;
     if (i >= (d_n ? (int64_t)*d_n : n))
         return;
-    const RayIn r = load_ray(rays, i);
+    // With reordering, slot i traces ray perm[i] (key order) and its result
+    // goes to that ray's slot.
+    const int64_t o = perm ? (int64_t)perm[i] : i;
+    const RayIn r = load_ray(rays, o);
     float best_t = r.tmax, bu = 0.0f, bv = 0.0f;
     int32_t best_prim = -1;
     int visited = 0, tested = 0;
@@ -483,30 +529,32 @@ __global__ __launch_bounds__(kBlock) void k_trace_stackless(const float4* __rest
         }
     }
     if (kStats)
-        counts[i] = make_int2(visited, tested);
+        counts[o] = make_int2(visited, tested);
     if (kMode != kClosest) {
-        occluded[i] = found_any ? 1 : 0;
+        occluded[o] = found_any ? 1 : 0;
     } else {
         pg_hit h;
         h.prim = best_prim;
         h.t = best_prim >= 0 ? best_t : r.tmax;
-        hits[i] = h;
+        hits[o] = h;
         if (uvs) {
             pg_hit_uv uv;
             uv.u = bu;
             uv.v = bv;
-            uvs[i] = uv;
+            uvs[o] = uv;
         }
     }
 }
 
 template <int kMode, bool kStats>
 void launch_stackless(const pg_accel* a, int64_t grid, const pg_ray* rays, int64_t n, pg_hit* hits,
-                      pg_hit_uv* uvs, int32_t* occ, int2* counts, hipStream_t s, const int32_t* d_n) {
+                      pg_hit_uv* uvs, int32_t* occ, int2* counts, hipStream_t s, const int32_t* d_n,
+                      const int32_t* perm) {
     const int remap = (a->launch_flags & PG_LAUNCH_XCD_REMAP) ? 1 : 0;
 #define PG_LAUNCH(B)                                                                              \
     hipLaunchKernelGGL((k_trace_stackless<B, kMode, kStats>), dim3((unsigned)grid), dim3(B), 0, s, \
-                       a->d_nodes, a->d_tris, a->d_ana, rays, n, hits, uvs, occ, counts, remap, d_n)
+                       a->d_nodes, a->d_tris, a->d_ana, rays, n, hits, uvs, occ, counts, remap, d_n, \
+                       perm)
     if (a->block == 64)
         PG_LAUNCH(64);
     else if (a->block == 256)
@@ -519,12 +567,12 @@ void launch_stackless(const pg_accel* a, int64_t grid, const pg_ray* rays, int64
 template <int W, int kMode, bool kStats, bool kPostpone>
 void launch_w(const pg_accel* a, int64_t grid, size_t lds, const pg_ray* rays, int64_t n,
               pg_hit* hits, pg_hit_uv* uvs, int32_t* occ, int2* counts, hipStream_t s,
-              const int32_t* d_n) {
+              const int32_t* d_n, const int32_t* perm) {
     const int remap = (a->launch_flags & PG_LAUNCH_XCD_REMAP) ? 1 : 0;
 #define PG_LAUNCH(B)                                                                           \
     hipLaunchKernelGGL((k_trace<W, B, kMode, kStats, kPostpone>), dim3((unsigned)grid), dim3(B), \
                        lds, s, a->d_nodes, a->d_tris, a->d_ana, rays, n, hits, uvs, occ,         \
-                       counts, remap, d_n)
+                       counts, remap, d_n, perm)
     if (a->block == 64)
         PG_LAUNCH(64);
     else if (a->block == 256)
@@ -534,27 +582,170 @@ void launch_w(const pg_accel* a, int64_t grid, size_t lds, const pg_ray* rays, i
 #undef PG_LAUNCH
 }
 
+// --- ray reordering (PG_LAUNCH_REORDER) ---------------------------------------
+// Rays are traced in the order of a key: the octant of the direction (the
+// major 3 bits), then the Morton code of the origin quantised to 2^c cells per
+// axis of the hierarchy's bounds.  Rays that start close together and head the
+// same way then share a wave and the nodes it fetches (fewer divergent
+// iterations, more L2 hits); every ray's result is the same in any order.  A
+// key kernel writes one key per ray; a device radix sort (sort.hip) turns the
+// keys into perm, the ray index of each key-order slot; the trace kernel of
+// slot i reads ray perm[i] and writes that ray's result.  Host-count launches
+// use 7 bits per axis (24-bit keys, three 8-bit digits); device-count
+// launches, sized for a capacity, use 6 bits per axis and mark the slots past
+// the count with bit 21 so that they sort last (22-bit keys).
+constexpr int64_t kReorderMinRays = 65536;
+
+struct ReorderFrame {
+    float lo[3], scale[3];  // cell = (o - lo) * scale, clamped to [0, 2^bits - 1]
+    int bits;
+    uint32_t past_count;  // key of slots past the device count
+};
+
+__device__ __forceinline__ uint32_t spread3(uint32_t x) {
+    x &= 0x3ffu;
+    x = (x | (x << 16)) & 0x030000ffu;
+    x = (x | (x << 8)) & 0x0300f00fu;
+    x = (x | (x << 4)) & 0x030c30c3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+__device__ __forceinline__ uint32_t cell_of(float o, float lo, float scale, float top) {
+    // fmaxf/fminf map NaN to the bounds, so the conversion is always defined.
+    return (uint32_t)fminf(fmaxf((o - lo) * scale, 0.0f), top);
+}
+
+__global__ __launch_bounds__(256) void k_reorder_keys(const pg_ray* __restrict__ rays, int64_t n,
+                                                      const int32_t* __restrict__ d_n, ReorderFrame f,
+                                                      uint32_t* __restrict__ keys) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n)
+        return;
+    if (d_n && i >= (int64_t)*d_n) {
+        keys[i] = f.past_count;
+        return;
+    }
+    const float4* r4 = reinterpret_cast<const float4*>(rays + i);
+    const float4 a = r4[0], b = r4[1];
+    const uint32_t oct = ((b.x < 0.0f) ? 4u : 0u) | ((b.y < 0.0f) ? 2u : 0u) | ((b.z < 0.0f) ? 1u : 0u);
+    const float top = (float)((1 << f.bits) - 1);
+    const uint32_t m = (spread3(cell_of(a.x, f.lo[0], f.scale[0], top)) << 2) |
+                       (spread3(cell_of(a.y, f.lo[1], f.scale[1], top)) << 1) |
+                       spread3(cell_of(a.z, f.lo[2], f.scale[2], top));
+    keys[i] = (oct << (3 * f.bits)) | m;
+}
+
+// Sizes the accel's sort buffers for `cap` rays (allocating only when
+// `may_alloc`); `ready` tells whether they can serve the launch.
+int ensure_reorder(pg_accel* a, int64_t cap, bool may_alloc, bool& ready) {
+    ready = false;
+    if (cap > (int64_t)INT32_MAX)
+        return PG_OK;
+    if (a->sort_cap < cap) {
+        if (!may_alloc)
+            return PG_OK;
+        a->free_sort();
+        size_t tmp = 0;
+        PG_HIP(reorder_sort_temp_bytes(cap, 24, tmp));
+        PG_HIP(hipMalloc(&a->d_keys, 2 * cap * sizeof(uint32_t)));
+        PG_HIP(hipMalloc(&a->d_perm, cap * sizeof(int32_t)));
+        PG_HIP(hipMalloc(&a->d_sort_tmp, tmp));
+        a->sort_tmp_bytes = tmp;
+        a->sort_cap = cap;
+    }
+    ready = true;
+    return PG_OK;
+}
+
+// Enqueues the key kernel and the sort of rays[0, n) (or of the first *d_n
+// of them) on stream s; afterwards a->d_perm holds the key order.
+hipError_t enqueue_reorder(const pg_accel* a, const pg_ray* rays, int64_t n, const int32_t* d_n,
+                           hipStream_t s) {
+    ReorderFrame f;
+    f.bits = d_n ? 6 : 7;
+    f.past_count = 1u << (3 + 3 * f.bits);
+    const int end_bit = d_n ? 4 + 3 * f.bits : 3 + 3 * f.bits;
+    for (int k = 0; k < 3; k++) {
+        const float ext = a->root_hi[k] - a->root_lo[k];
+        f.lo[k] = a->root_lo[k];
+        f.scale[k] = ext > 0.0f ? (float)(1 << f.bits) / ext : 0.0f;
+    }
+    hipLaunchKernelGGL(k_reorder_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, rays, n, d_n, f,
+                       a->d_keys);
+    size_t need = 0;
+    const hipError_t q = reorder_sort_temp_bytes(n, end_bit, need);
+    if (q != hipSuccess)
+        return q;
+    if (need > a->sort_tmp_bytes)
+        return hipErrorInvalidValue;  // not reached: storage was sized for 24-bit keys at capacity
+    return reorder_sort(a->d_sort_tmp, a->sort_tmp_bytes, a->d_keys, a->d_keys + a->sort_cap,
+                        reinterpret_cast<uint32_t*>(a->d_perm), n, end_bit, s);
+}
+
+// Launches the traversal over rays[0, n) (or the first *d_n).  With
+// PG_LAUNCH_REORDER and at least kReorderMinRays rays it first sorts the rays
+// by key; host-count launches may allocate the sort's buffers, device-count
+// launches (which a caller may be recording into a graph) use them only when
+// they already hold the capacity.
 template <int kMode, bool kStats>
-hipError_t launch_trace(const pg_accel* a, const pg_ray* rays, int64_t n, pg_hit* hits,
+hipError_t launch_trace(pg_accel* a, const pg_ray* rays, int64_t n, pg_hit* hits,
                         pg_hit_uv* uvs, int32_t* occ, int2* counts, hipStream_t s,
                         const int32_t* d_n = nullptr) {
     const int64_t grid = (n + a->block - 1) / a->block;
     const size_t lds = (size_t)a->stack_entries * a->block * sizeof(int32_t);
     const bool postpone = (a->launch_flags & PG_LAUNCH_POSTPONE_LEAVES) != 0;
+    const int32_t* perm = nullptr;
+    if ((a->launch_flags & PG_LAUNCH_REORDER) && n >= kReorderMinRays) {
+        bool ready = false;
+        // A launch being recorded into a graph keeps the callers' order: the
+        // sort's library code clears memory with memset nodes.
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        const hipError_t ce = hipStreamIsCapturing(s, &cap);
+        if (ce != hipSuccess)
+            return ce;
+        if (cap == hipStreamCaptureStatusNone && ensure_reorder(a, n, d_n == nullptr, ready) != PG_OK)
+            return hipErrorOutOfMemory;
+        if (ready) {
+            const hipError_t e = enqueue_reorder(a, rays, n, d_n, s);
+            if (e != hipSuccess)
+                return e;
+            perm = a->d_perm;
+        }
+    }
+    const bool timed = (a->launch_flags & PG_LAUNCH_TIMED) != 0;
+    if (timed) {
+        if (!a->ev_start) {
+            const hipError_t e0 = hipEventCreate(&a->ev_start);
+            if (e0 != hipSuccess)
+                return e0;
+            const hipError_t e1 = hipEventCreate(&a->ev_end);
+            if (e1 != hipSuccess)
+                return e1;
+        }
+        const hipError_t er = hipEventRecord(a->ev_start, s);
+        if (er != hipSuccess)
+            return er;
+    }
     if (a->width == 2 && (a->launch_flags & PG_LAUNCH_STACKLESS)) {
-        launch_stackless<kMode, kStats>(a, grid, rays, n, hits, uvs, occ, counts, s, d_n);
+        launch_stackless<kMode, kStats>(a, grid, rays, n, hits, uvs, occ, counts, s, d_n, perm);
     } else if (a->width == 2) {
         if (postpone)
-            launch_w<2, kMode, kStats, true>(a, grid, lds, rays, n, hits, uvs, occ, counts, s, d_n);
+            launch_w<2, kMode, kStats, true>(a, grid, lds, rays, n, hits, uvs, occ, counts, s, d_n, perm);
         else
-            launch_w<2, kMode, kStats, false>(a, grid, lds, rays, n, hits, uvs, occ, counts, s, d_n);
+            launch_w<2, kMode, kStats, false>(a, grid, lds, rays, n, hits, uvs, occ, counts, s, d_n, perm);
     } else {
         if (postpone)
-            launch_w<4, kMode, kStats, true>(a, grid, lds, rays, n, hits, uvs, occ, counts, s, d_n);
+            launch_w<4, kMode, kStats, true>(a, grid, lds, rays, n, hits, uvs, occ, counts, s, d_n, perm);
         else
-            launch_w<4, kMode, kStats, false>(a, grid, lds, rays, n, hits, uvs, occ, counts, s, d_n);
+            launch_w<4, kMode, kStats, false>(a, grid, lds, rays, n, hits, uvs, occ, counts, s, d_n, perm);
     }
-    return hipGetLastError();
+    const hipError_t e = hipGetLastError();
+    if (e == hipSuccess && timed) {
+        a->timed_pending = true;
+        return hipEventRecord(a->ev_end, s);
+    }
+    return e;
 }
 
 }  // namespace
@@ -731,6 +922,13 @@ extern "C" int pg_accel_build(pg_accel* a, const pg_build_options* opts) {
         for (int k = 0; k < 3; k++)
             scene_mag = std::max(scene_mag, std::max(std::fabs(p.lo[k]), std::fabs(p.hi[k])));
     }
+    // The hierarchy's bounds frame the reordering key's cells.
+    float blo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, bhi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+    for (const BuildPrim& p : prims)
+        for (int k = 0; k < 3; k++) {
+            blo[k] = std::min(blo[k], p.lo[k]);
+            bhi[k] = std::max(bhi[k], p.hi[k]);
+        }
     // Grow child boxes by 2^-18 of the scene's magnitude: float slab tests then
     // never cull a box that holds the true closest hit.
     const float inflate = scene_mag * (1.0f / 262144.0f) + 1e-30f;
@@ -774,6 +972,10 @@ extern "C" int pg_accel_build(pg_accel* a, const pg_build_options* opts) {
     }
     a->width = flat.width;
     a->max_depth = flat.max_depth;
+    for (int k = 0; k < 3; k++) {
+        a->root_lo[k] = n > 0 ? blo[k] : 0.0f;
+        a->root_hi[k] = n > 0 ? bhi[k] : 0.0f;
+    }
     a->stack_entries = stack_entries(flat.width, flat.max_depth);
     a->n_nodes = flat.n_nodes;
     a->n_tri_slots = (int64_t)flat.tri_order.size();
@@ -878,6 +1080,13 @@ extern "C" int pg_accel_get_stats(const pg_accel* a, pg_accel_stats* out) {
     out->build_ms = a->build_ms;
     out->block = a->block;
     out->launch_flags = a->launch_flags;
+    out->last_trace_ms = -1.0;
+    if (a->timed_pending) {
+        float ms = 0.0f;
+        PG_HIP(hipEventSynchronize(a->ev_end));
+        PG_HIP(hipEventElapsedTime(&ms, a->ev_start, a->ev_end));
+        out->last_trace_ms = ms;
+    }
     return PG_OK;
 }
 
@@ -897,7 +1106,9 @@ extern "C" int pg_accel_set_launch(pg_accel* a, int32_t block, int32_t flags) {
         return pg_fail(PG_E_ARG, "pg_accel_set_launch: null accel");
     if (block != 0 && block != 64 && block != 128 && block != 256)
         return pg_fail(PG_E_ARG, "pg_accel_set_launch: block must be 64, 128 or 256");
-    if (flags < -1 || flags > (PG_LAUNCH_POSTPONE_LEAVES | PG_LAUNCH_XCD_REMAP | PG_LAUNCH_STACKLESS))
+    if (flags < -1 ||
+        flags > (PG_LAUNCH_POSTPONE_LEAVES | PG_LAUNCH_XCD_REMAP | PG_LAUNCH_STACKLESS | PG_LAUNCH_REORDER |
+                 PG_LAUNCH_TIMED))
         return pg_fail(PG_E_ARG, "pg_accel_set_launch: unknown flags");
     if (block)
         a->block = block;
@@ -913,6 +1124,7 @@ extern "C" int pg_accel_release(pg_accel* a) {
     if (!a)
         return PG_OK;
     a->free_device();
+    a->free_events();
     delete a;
     return PG_OK;
 }

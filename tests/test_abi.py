@@ -50,7 +50,8 @@ def test_argument_errors_without_gpu(pg):
     assert L.pg_trace_closest(h, ctypes.c_void_p(16), 1, ctypes.c_void_p(16), None, None) == -2
     assert L.pg_trace_any(h, ctypes.c_void_p(16), 1, ctypes.c_void_p(16), None) == -2
     assert L.pg_accel_set_launch(h, 96, -1) == -1
-    assert L.pg_accel_set_launch(h, 64, 8) == -1 and b"flags" in L.pg_last_error()
+    assert L.pg_accel_set_launch(h, 64, 32) == -1 and b"flags" in L.pg_last_error()
+    assert L.pg_accel_set_launch(h, 64, 8 | 16) == 0  # reorder + timed
     assert L.pg_accel_set_launch(h, 0, 3) == 0
     c = (ctypes.c_float * 3)(0, 0, 0)
     assert L.pg_accel_add_sphere(h, c, ctypes.c_float(-1.0), None) == -1

@@ -169,6 +169,130 @@ def test_launch_shapes_agree(pg, oracle, cuda, width, block, flags):
     assert np.array_equal(occ, bocc)
 
 
+def _tiled_82k(fx, reps=4, tail=777):
+    """pine's 82k-triangle golden rays repeated, plus a ragged tail: above the
+    65,536 rays from which PG_LAUNCH_REORDER sorts them."""
+    v, i, rays = _mesh_rays(fx)
+    def tile(x):
+        return np.concatenate([x] * reps + [x[:tail]])
+    big = tile(rays)
+    assert len(big) >= 65536
+    return v, i, big, tile(fx["prim"]), tile(fx["t"]), tile(fx["occluded"])
+
+
+@pytest.mark.parametrize("width", WIDTHS)
+@pytest.mark.parametrize("flags", [8, 9, 10, 12])
+def test_reordered_launches_match_pine(pg, oracle, cuda, width, flags):
+    """PG_LAUNCH_REORDER (alone, with postponed leaves, with the XCD remap,
+    with the stackless traversal) sorts the rays on the device before tracing:
+    every caller slot holds pine's hit, and uv, any-hit, occlusion and the
+    per-ray work counts equal those of the unordered launch."""
+    fx = golden("trace_82k")
+    v, i, big, gprim, gt, gocc = _tiled_82k(fx)
+    a = _accel(pg, v, i, width, 128, flags)
+    b = _accel(pg, v, i, width, 128, flags & ~pg.LAUNCH_REORDER)
+    prim, t, occ = _trace(a, big, cuda)
+    check_vs_pine(prim, t, occ, gprim, gt, gocc, big, mesh_prim_geom(oracle, v, i), oracle)
+    d = cuda.DeviceArray.from_numpy(big)
+    uv_a = cuda.DeviceArray((len(big), 2), np.float32)
+    uv_b = cuda.DeviceArray((len(big), 2), np.float32)
+    ha = a.trace_closest(d, d_uv=uv_a).numpy()
+    hb = b.trace_closest(d, d_uv=uv_b).numpy()
+    assert np.array_equal(ha, hb)
+    assert np.array_equal(uv_a.numpy().view(np.int32), uv_b.numpy().view(np.int32))
+    assert np.array_equal(a.trace_occluded(d).numpy(), b.trace_occluded(d).numpy())
+    assert np.array_equal(a.trace_any(d).numpy(), b.trace_any(d).numpy())
+    ca, cb = a.trace_counts(d)[1].numpy(), b.trace_counts(d)[1].numpy()
+    if not flags & pg.LAUNCH_POSTPONE_LEAVES:
+        # Without speculative descent a ray's work does not depend on its wave.
+        assert np.array_equal(ca, cb)
+    else:
+        assert ca.sum() > 0
+
+
+def test_reordered_device_count_launch(pg, oracle, cuda):
+    """pg_trace_closest_dev with reordering: only the first *d_n rays are
+    sorted and traced (slots past the count keep their contents), and their
+    hits equal the host-count launch's."""
+    import ctypes
+    fx = golden("trace_82k")
+    v, i, big, _, _, _ = _tiled_82k(fx)
+    a = _accel(pg, v, i, 2, 128, pg.LAUNCH_REORDER)
+    d = cuda.DeviceArray.from_numpy(big)
+    full = a.trace_closest(d).numpy()  # sizes the sort's buffers for len(big)
+    for count in (len(big), 70001, 1000):
+        hits = cuda.DeviceArray.from_numpy(np.full((len(big), 2), -7, np.int32))
+        dn = cuda.DeviceArray.from_numpy(np.array([count], np.int32))
+        pg.check(pg.lib().pg_trace_closest_dev(a.h, d.ptr, dn.ptr, ctypes.c_int64(len(big)),
+                                               hits.ptr, None, None))
+        h = hits.numpy()
+        assert np.array_equal(h[:count], full[:count]), count
+        assert (h[count:] == -7).all(), count
+
+
+def test_timed_launch_reports_the_traversal_kernel(pg, cuda):
+    """PG_LAUNCH_TIMED: last_trace_ms is -1 before any timed launch, then the
+    traversal kernel's own duration (inside the whole reordered launch's),
+    and timing changes no hit."""
+    fx = golden("trace_82k")
+    v, i, big, _, _, _ = _tiled_82k(fx)
+    a = _accel(pg, v, i, 2, 128, pg.LAUNCH_REORDER)
+    d = cuda.DeviceArray.from_numpy(big)
+    plain = a.trace_closest(d).numpy()
+    assert a.stats()["last_trace_ms"] == -1.0
+    a.set_launch(128, pg.LAUNCH_REORDER | pg.LAUNCH_TIMED)
+    s0, s1 = cuda.Event(), cuda.Event()
+    s0.record()
+    timed = a.trace_closest(d)
+    s1.record()
+    ms = a.stats()["last_trace_ms"]
+    assert np.array_equal(timed.numpy(), plain)
+    assert 0.0 < ms <= s0.elapsed_ms(s1) + 1e-3
+
+
+def test_reordered_flag_in_a_recorded_graph(pg, cuda):
+    """A device-count launch recorded into a HIP graph keeps the callers'
+    order even with PG_LAUNCH_REORDER (the sort's library code would record
+    memset nodes): every replay, with eager work and synchronisations between
+    the replays, gives the eager launch's hits."""
+    import ctypes
+    fx = golden("trace_82k")
+    v, i, big, _, _, _ = _tiled_82k(fx)
+    a = _accel(pg, v, i, 2, 128, pg.LAUNCH_REORDER)
+    d = cuda.DeviceArray.from_numpy(big)
+    eager = a.trace_closest(d).numpy()
+    R = cuda.rt()
+    vp = ctypes.c_void_p
+    R.hipStreamCreate.argtypes = [ctypes.POINTER(vp)]
+    R.hipStreamBeginCapture.argtypes = [vp, ctypes.c_int]
+    R.hipStreamEndCapture.argtypes = [vp, ctypes.POINTER(vp)]
+    R.hipGraphInstantiate.argtypes = [ctypes.POINTER(vp), vp, vp, vp, ctypes.c_size_t]
+    R.hipGraphLaunch.argtypes = [vp, vp]
+    R.hipStreamSynchronize.argtypes = [vp]
+    R.hipGraphExecDestroy.argtypes = [vp]
+    R.hipGraphDestroy.argtypes = [vp]
+    R.hipStreamDestroy.argtypes = [vp]
+    st, g, ge = vp(), vp(), vp()
+    cuda.check(R.hipStreamCreate(ctypes.byref(st)))
+    dn = cuda.DeviceArray.from_numpy(np.array([len(big)], np.int32))
+    hits = cuda.DeviceArray((len(big), 2), np.int32)
+    cuda.check(R.hipStreamBeginCapture(st, 0))  # hipStreamCaptureModeGlobal
+    rc = pg.lib().pg_trace_closest_dev(a.h, d.ptr, dn.ptr, ctypes.c_int64(len(big)), hits.ptr, None, st)
+    cuda.check(R.hipStreamEndCapture(st, ctypes.byref(g)))
+    pg.check(rc)
+    cuda.check(R.hipGraphInstantiate(ctypes.byref(ge), g, None, None, 0))
+    for _ in range(3):
+        hits.zero()
+        a.trace_closest(d)  # eager reordered work between replays
+        cuda.synchronize()
+        cuda.check(R.hipGraphLaunch(ge, st))
+        cuda.check(R.hipStreamSynchronize(st))
+        assert np.array_equal(hits.numpy(), eager)
+    R.hipGraphExecDestroy(ge)
+    R.hipGraphDestroy(g)
+    R.hipStreamDestroy(st)
+
+
 def test_edge_cases_vs_oracle(pg, oracle, cuda):
     rng = np.random.default_rng(5)
     # Two exact duplicates (prims 0 and 3) -> exact t ties resolved to id 0;
@@ -228,12 +352,16 @@ def test_full_benchmark_stream_is_tree_independent(pg, oracle, cuda):
     rays = W.make_rays(W.RAYS_16M, v, i)
     d = cuda.DeviceArray.from_numpy(rays)
     outs = []
-    for width, flags in ((2, -1), (4, -1), (2, pg.LAUNCH_STACKLESS)):
+    configs = ((2, -1), (4, -1), (2, pg.LAUNCH_STACKLESS), (2, pg.LAUNCH_REORDER),
+               (4, pg.LAUNCH_REORDER), (2, pg.LAUNCH_REORDER | pg.LAUNCH_STACKLESS))
+    for width, flags in configs:
         a = _accel(pg, v, i, width, 128, flags)
         outs.append(a.trace_closest(d).numpy())
         a.release()
-    assert np.array_equal(outs[0], outs[1])
-    assert np.array_equal(outs[0], outs[2])  # stackless traversal, same hits bit for bit
+    for k in range(1, len(configs)):
+        # other trees, the stackless traversal, rays sorted on the device:
+        # the same hits bit for bit
+        assert np.array_equal(outs[0], outs[k]), configs[k]
     hitfrac = (outs[0][:, 0] >= 0).mean()
     assert 0.05 < hitfrac < 0.9
     sel = np.arange(0, len(rays), 251)[:65536]

@@ -1836,13 +1836,18 @@ __global__ __launch_bounds__(256) void k_shade_rw_gen(SceneDev s, PathsDev P, in
 // material, or continue towards p2 through a material-less one.
 // kPre: the segments' transmittance was tracked by k_shadow_track_lanes
 // (seg_trf); otherwise it is tracked here, one segment per thread.
+// d_nseg: the pass's segment count in device memory (NULL: n_seg); the next
+// pass's segments are counted at next_ctr (NULL: P.counters[3]).
 template <int kS, int kM, bool kPre>
-__global__ __launch_bounds__(256) void k_shadow_tr(SceneDev s, PathsDev P, int32_t n_seg, int32_t src, int32_t lds_medium) {
+__global__ __launch_bounds__(256) void k_shadow_tr(SceneDev s, PathsDev P, int32_t n_seg, int32_t src, int32_t lds_medium,
+                                                   const int32_t* d_nseg, int32_t* next_ctr) {
     extern __shared__ float lds_maj[];
     const float* lds = kPre ? nullptr : stage_majorants(s, lds_medium, lds_maj);
     const int dst = 1 - src;
+    const int32_t nseg = d_nseg ? *d_nseg : n_seg;
+    int32_t* const ctr = next_ctr ? next_ctr : &P.counters[3];
     int lookups = 0;
-    for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n_seg; k += gridDim.x * blockDim.x) {
+    for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nseg; k += gridDim.x * blockDim.x) {
         const int32_t r = P.seg_rec[src][k];
         const int32_t i = P.shadow_path[r];
         const float4* r4 = reinterpret_cast<const float4*>(&P.seg_rays[src][k]);
@@ -1881,7 +1886,7 @@ __global__ __launch_bounds__(256) void k_shadow_tr(SceneDev s, PathsDev P, int32
                 float len;
                 const f3 d2 = normalize(p2 - h.p, len);
                 const f3 o2 = offset_ray_origin(h.p, face_same_hemisphere(h.n, d2));
-                const int slot = atomicAdd(&P.counters[3], 1);
+                const int slot = atomicAdd(ctr, 1);
                 store_ray(&P.seg_rays[dst][slot], o2, d2, 0.0f, len * (1.0f - 1e-3f));
                 P.seg_rec[dst][slot] = r;
                 P.seg_med[dst][slot] = medium_for(d2, h.n, min_, mout);
@@ -1897,10 +1902,14 @@ __global__ __launch_bounds__(256) void k_shadow_tr(SceneDev s, PathsDev P, int32
 // EstimateDirect's tail (integrator.cpp:101-118) and `L += beta * ...`.
 // At a guided vertex the directional strategy is the guide/BSDF mixture, so
 // the light sample's MIS weight uses the mixture pdf of the light direction.
+// d_n: the record count in device memory (NULL: n_rec).  ovf: when it holds
+// a nonzero count (shadow segments left after the device-driven passes), the
+// kernel does nothing; the host finishes the segments and launches it again.
 template <bool kSpec>
-__global__ __launch_bounds__(256) void k_resolve_gen(SceneDev s, PathsDev P, int32_t n_rec) {
+__global__ __launch_bounds__(256) void k_resolve_gen(SceneDev s, PathsDev P, int32_t n_rec, const int32_t* d_n,
+                                                     const int32_t* ovf) {
     const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n_rec)
+    if (r >= (d_n ? *d_n : n_rec) || (ovf && *ovf != 0))
         return;
     const f3 tr = mk(P.sh_tr[3 * r], P.sh_tr[3 * r + 1], P.sh_tr[3 * r + 2]);
     if (is_black(tr))
@@ -1941,10 +1950,12 @@ __global__ __launch_bounds__(256) void k_resolve_gen(SceneDev s, PathsDev P, int
 // Direction sampling after the shadow ray: PhaseFunction::Sample at a medium
 // vertex (path.cpp:19-25), the material's Sample + Russian roulette at a
 // surface (path.cpp:66-86).
+// d_n, ovf: as for k_resolve_gen (the direction queue's count).
 template <bool kSpec>
-__global__ __launch_bounds__(256) void k_shade_gen_b(SceneDev s, PathsDev P, int32_t n_b) {
+__global__ __launch_bounds__(256) void k_shade_gen_b(SceneDev s, PathsDev P, int32_t n_b, const int32_t* d_n,
+                                                     const int32_t* ovf) {
     const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n_b)
+    if (k >= (d_n ? *d_n : n_b) || (ovf && *ovf != 0))
         return;
     const int32_t i = P.queue_b[k];
     const float* v = P.vtx + 9 * (int64_t)i;
@@ -3971,7 +3982,18 @@ static void launch_tracking(int ks, int km, F&& f) {
     }
 }
 
-// One bounce in general mode; `active` becomes the next queue's length.
+// Shadow-segment passes of the split bounce run on device-held counts (no
+// host round trip): pass 0 reads the record count (counters[1]), pass p
+// counts its continuations at counters[4 + p].  A shadow ray crosses each
+// material-less surface once, so most scenes finish within these passes;
+// counters[7] holds whatever is left after the last one, and the host then
+// finishes those segments before the records are resolved.
+constexpr int kDeviceSegmentPasses = 4;
+
+// One bounce in general mode; `active` becomes the next queue's length.  One
+// host synchronisation per bounce: every launch after the shading kernel
+// reads its count from device memory and is sized for `active` (records,
+// segments and direction samples never outnumber the bounce's paths).
 static int bounce_general(pg_scene* S, PathsDev& Q, int32_t& active, hipStream_t st, int64_t& rays) {
     const int B = 256;
     const size_t lds = S->lds_medium >= 0 ? S->med_majorant[S->lds_medium].size() * sizeof(float) : 0;
@@ -3997,20 +4019,54 @@ static int bounce_general(pg_scene* S, PathsDev& Q, int32_t& active, hipStream_t
                                 : (nodes ? k_shade_gen<false, true> : k_shade_gen<false, false>);
         hipLaunchKernelGGL(kern, dim3((active + B - 1) / B), dim3(B), 0, st, S->dev, Q, active);
     }
-    int32_t counts[4];
-    PG_HIP(hipMemcpyAsync(counts, Q.counters, sizeof(counts), hipMemcpyDeviceToHost, st));
-    PG_HIP(hipStreamSynchronize(st));
     rays += active;
-    const int32_t n_rec = counts[1], n_b = counts[2];
+    const unsigned grid_all = (unsigned)((active + B - 1) / B);
+    int32_t* C = Q.counters;
     // IntersectTr: trace each segment, track its medium, continue through
     // material-less surfaces until every record is resolved.
-    int32_t n_seg = n_rec;
     int src = 0;
-    // A shadow ray crosses each material-less surface once; the bound only
-    // guards against a surface that re-hits itself.
+    if (!lanes) {
+        for (int pass = 0; pass < kDeviceSegmentPasses; pass++) {
+            const int32_t* d_nseg = pass == 0 ? C + 1 : C + 4 + pass - 1;
+            if (int rc = pg_trace_closest_dev(S->accel, Q.seg_rays[src], d_nseg, active, Q.seg_hits, Q.seg_uvs, st))
+                return rc;
+            launch_tracking(S->track_sampler, S->track_media, [&](auto ks, auto km) {
+                hipLaunchKernelGGL((k_shadow_tr<decltype(ks)::value, decltype(km)::value, false>),
+                                   dim3(tracking_grid(active, B)), dim3(B), lds, st, S->dev, Q, 0, src,
+                                   S->lds_medium, d_nseg, C + 4 + pass);
+            });
+            src = 1 - src;
+        }
+        if (S->dev.guided)
+            hipLaunchKernelGGL(k_guide_infer<0>, dim3(infer_blocks(active)), dim3(kInferThreads), 0, st, S->dev, Q,
+                               active, C + 2);
+        const int32_t* ovf = C + 4 + kDeviceSegmentPasses - 1;
+        hipLaunchKernelGGL(S->spectral ? k_resolve_gen<true> : k_resolve_gen<false>, dim3(grid_all), dim3(B), 0, st,
+                           S->dev, Q, 0, C + 1, ovf);
+        hipLaunchKernelGGL(S->spectral ? k_shade_gen_b<true> : k_shade_gen_b<false>, dim3(grid_all), dim3(B), 0, st,
+                           S->dev, Q, 0, C + 2, ovf);
+    }
+    int32_t counts[8];
+    PG_HIP(hipMemcpyAsync(counts, C, sizeof(counts), hipMemcpyDeviceToHost, st));
+    PG_HIP(hipStreamSynchronize(st));
+    const int32_t n_rec = counts[1], n_b = counts[2];
+    int32_t n_seg = n_rec;
+    if (!lanes) {
+        rays += n_rec;
+        for (int pass = 0; pass + 1 < kDeviceSegmentPasses; pass++)
+            rays += counts[4 + pass];
+        n_seg = counts[4 + kDeviceSegmentPasses - 1];
+        if (n_seg == 0) {
+            active = counts[0];
+            return PG_OK;
+        }
+    }
+    // Host-driven segment passes: the lane-tracking variant throughout, or the
+    // segments left after the device-driven passes (the resolve and direction
+    // kernels above then did nothing).
     for (int pass = 0; n_seg > 0 && pass < kMaxShadowSegments; pass++) {
         rays += n_seg;
-        PG_HIP(hipMemsetAsync(Q.counters + 3, 0, sizeof(int32_t), st));
+        PG_HIP(hipMemsetAsync(C + 3, 0, sizeof(int32_t), st));
         if (int rc = pg_trace_closest(S->accel, Q.seg_rays[src], n_seg, Q.seg_hits, Q.seg_uvs, st))
             return rc;
         launch_tracking(S->track_sampler, S->track_media, [&](auto ks, auto km) {
@@ -4020,30 +4076,31 @@ static int bounce_general(pg_scene* S, PathsDev& Q, int32_t& active, hipStream_t
                                        dim3(lane_grid(n_seg, B)), dim3(B), lds, st, S->dev, Q, n_seg, src,
                                        S->lds_medium);
                     hipLaunchKernelGGL((k_shadow_tr<decltype(ks)::value, decltype(km)::value, true>),
-                                       dim3((n_seg + B - 1) / B), dim3(B), 0, st, S->dev, Q, n_seg, src, -1);
+                                       dim3((n_seg + B - 1) / B), dim3(B), 0, st, S->dev, Q, n_seg, src, -1,
+                                       nullptr, nullptr);
                     return;
                 }
             }
             hipLaunchKernelGGL((k_shadow_tr<decltype(ks)::value, decltype(km)::value, false>),
                                dim3(tracking_grid(n_seg, B)), dim3(B), lds, st, S->dev, Q, n_seg, src,
-                               S->lds_medium);
+                               S->lds_medium, nullptr, nullptr);
         });
         int32_t next = 0;
-        PG_HIP(hipMemcpyAsync(&next, Q.counters + 3, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        PG_HIP(hipMemcpyAsync(&next, C + 3, sizeof(int32_t), hipMemcpyDeviceToHost, st));
         PG_HIP(hipStreamSynchronize(st));
         n_seg = next;
         src = 1 - src;
     }
-    if (S->dev.guided && n_b > 0)
+    if (lanes && S->dev.guided && n_b > 0)
         hipLaunchKernelGGL(k_guide_infer<0>, dim3(infer_blocks(n_b)), dim3(kInferThreads), 0, st, S->dev, Q, n_b,
                            nullptr);
     if (n_rec > 0)
         hipLaunchKernelGGL(S->spectral ? k_resolve_gen<true> : k_resolve_gen<false>, dim3((n_rec + B - 1) / B),
-                           dim3(B), 0, st, S->dev, Q, n_rec);
+                           dim3(B), 0, st, S->dev, Q, n_rec, nullptr, nullptr);
     if (n_b > 0)
         hipLaunchKernelGGL(S->spectral ? k_shade_gen_b<true> : k_shade_gen_b<false>, dim3((n_b + B - 1) / B),
-                           dim3(B), 0, st, S->dev, Q, n_b);
-    PG_HIP(hipMemcpyAsync(counts, Q.counters, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+                           dim3(B), 0, st, S->dev, Q, n_b, nullptr, nullptr);
+    PG_HIP(hipMemcpyAsync(counts, C, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     PG_HIP(hipStreamSynchronize(st));
     active = counts[0];
     return PG_OK;

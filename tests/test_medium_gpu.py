@@ -182,6 +182,37 @@ def test_compressed_volumes_fail_loudly(pg, cuda, tmp_path):
         pg.Scene(text=text, directory=str(tmp_path))
 
 
+@pytest.mark.parametrize("layers", [2, 6])
+def test_shadow_segments_past_the_device_passes(pg, cuda, tmp_path, layers):
+    """Shadow rays that cross many material-less surfaces: the slab's shadow
+    scene with `layers` stacked material-less rects between the ground and
+    the sun.  With 6 layers a shadow ray has 7 segments, more than the split
+    bounce's device-driven passes, and the host finishes the rest; the image
+    must equal the lane-tracking render (whose segment passes all run from the
+    host) bit for bit, with the same density lookups and rays, and keep the
+    closed form."""
+    fx = golden("slab_shadow")
+    text = with_bricks(fx["scene"].item().decode(), 4)
+    text += "".join(f"Shape: Rect{{\n    position: 0 {0.6 + 0.1 * k:.1f} 0\n    ex: 4 0 0\n"
+                    f"    ey: 0 0 4\n}}\n" for k in range(layers))
+    d = cloud_dir(tmp_path)
+    out = []
+    for lanes in (True, False):
+        s = pg.Scene(text=text, directory=d)
+        s.render(lane_tracking=lanes)
+        out.append((s.film(), s.info()["medium_steps"], s.info()["last_rays"]))
+        s.release()
+    (a, la, ra), (b, lb, rb) = out
+    print(f"{layers} layers: {lb} lookups, {rb} rays")
+    assert la == lb and ra == rb and lb > 0
+    assert np.array_equal(a.view(np.int32), b.view(np.int32))
+    img = linear(b).mean(-1)
+    exp, mask, scale = expected_slab("shadow", img)
+    dif = img[mask] - exp[mask]
+    assert abs(dif.mean()) < 3e-3
+    assert np.abs(dif).max() < 0.08
+
+
 @pytest.mark.parametrize("bricks", [8, 4])
 def test_lane_tracking_equals_per_path_tracking(pg, cuda, tmp_path, bricks):
     """Brick-majorant tracking as persistent lanes (PG_RENDER_LANE_TRACKING:

@@ -46,9 +46,28 @@ struct MediumDev {
     int32_t nbricks[3];
     f3 w2m_s, w2m_t;      // world -> unit cube: s * p + t (Translate * Scale, medium.cpp:180)
     float m2w[12];        // Inverse(w2m) columns c0, c1, c2, c3 (xyz of each)
-    const float* density;   // size.x * size.y * size.z, x fastest
+    int32_t dblocks[2];     // density blocks along x and y (see density_blocked)
+    const float* density;   // pine's densities in blocks of 4 x 4 x 2 voxels (density_blocked)
     const float* majorant;  // nbricks.x * nbricks.y * nbricks.z density bounds
 };
+
+// Density layout in HBM: blocks of 4 (x) x 4 (y) x 2 (z) voxels, 32 floats =
+// one 128-byte line each, blocks x-fastest, then y, then z; inside a block x,
+// then y, then z.  The 8 taps of a trilinear lookup touch 2.3 lines on
+// average ((1 + 1/4)(1 + 1/4)(1 + 1/2)) instead of the 4 of pine's x-fastest
+// layout.  The values are pine's; only their addresses change.
+PG_HD int32_t dens_off_x(int x) { return (x >> 2) * 32 + (x & 3); }
+PG_HD int32_t dens_off_y(const MediumDev& m, int y) { return (y >> 2) * (m.dblocks[0] * 32) + (y & 3) * 4; }
+PG_HD int32_t dens_off_z(const MediumDev& m, int z) {
+    return (z >> 1) * (m.dblocks[0] * m.dblocks[1] * 32) + (z & 1) * 16;
+}
+// Blocks per axis and padded length (floats) of a grid of size[3] voxels.
+inline void density_blocks(const int32_t size[3], int32_t blocks[3], int64_t& floats) {
+    blocks[0] = (size[0] + 3) / 4;
+    blocks[1] = (size[1] + 3) / 4;
+    blocks[2] = (size[2] + 1) / 2;
+    floats = (int64_t)blocks[0] * blocks[1] * blocks[2] * 32;
+}
 
 // PhaseHG (medium.cpp:9-12) as pine's build evaluates it:
 // ((1 - g^2) / (4 pi)) / (sqrt(d) * d), d = (2 cos g + g^2) + 1.
@@ -134,7 +153,30 @@ PG_HD f3 ray_at(const MRay& r, float t) { return mk(t * r.d.x + r.o.x, t * r.d.y
 PG_HD float grid_voxel(const MediumDev& m, int x, int y, int z) {
     if (x < 0 || x >= m.size[0] || y < 0 || y >= m.size[1] || z < 0 || z >= m.size[2])
         return 0.0f;
-    return m.density[x + y * m.size[0] + z * m.size[1] * m.size[0]];
+    return m.density[dens_off_x(x) + dens_off_y(m, y) + dens_off_z(m, z)];
+}
+
+// The 8 voxels of a trilinear lookup at (ix, iy, iz): each axis's two
+// offsets and in-grid flags found once, every tap one add of three offsets.
+struct GridTaps {
+    int32_t ox[2], oy[2], oz[2];
+    bool vx[2], vy[2], vz[2];
+};
+PG_HD GridTaps grid_taps(const MediumDev& m, int ix, int iy, int iz) {
+    GridTaps T;
+    for (int k = 0; k < 2; k++) {
+        const int x = ix + k, y = iy + k, z = iz + k;
+        T.vx[k] = x >= 0 && x < m.size[0];
+        T.vy[k] = y >= 0 && y < m.size[1];
+        T.vz[k] = z >= 0 && z < m.size[2];
+        T.ox[k] = dens_off_x(x);
+        T.oy[k] = dens_off_y(m, y);
+        T.oz[k] = dens_off_z(m, z);
+    }
+    return T;
+}
+PG_HD float grid_tap(const MediumDev& m, const GridTaps& T, int a, int b, int c) {
+    return (T.vx[a] && T.vy[b] && T.vz[c]) ? m.density[T.ox[a] + T.oy[b] + T.oz[c]] : 0.0f;
 }
 
 // GridMedium::Density (medium.cpp:139-155).  AABB(0,1).Offset(p) is p itself;
@@ -150,11 +192,11 @@ PG_HD float grid_density(const MediumDev& m, f3 p) {
     const double X1 = (double)(px - (float)ix), Y1 = (double)(py - (float)iy),
                  Z1 = (double)(pz - (float)iz);
     const double X0 = 1.0 - X1, Y0 = 1.0 - Y1, Z0 = 1.0 - Z1;
-    const float c000 = grid_voxel(m, ix, iy, iz), c001 = grid_voxel(m, ix, iy, iz + 1);
-    const float c010 = grid_voxel(m, ix, iy + 1, iz), c011 = grid_voxel(m, ix, iy + 1, iz + 1);
-    const float c100 = grid_voxel(m, ix + 1, iy, iz), c101 = grid_voxel(m, ix + 1, iy, iz + 1);
-    const float c110 = grid_voxel(m, ix + 1, iy + 1, iz),
-                c111 = grid_voxel(m, ix + 1, iy + 1, iz + 1);
+    const GridTaps T = grid_taps(m, ix, iy, iz);
+    const float c000 = grid_tap(m, T, 0, 0, 0), c001 = grid_tap(m, T, 0, 0, 1);
+    const float c010 = grid_tap(m, T, 0, 1, 0), c011 = grid_tap(m, T, 0, 1, 1);
+    const float c100 = grid_tap(m, T, 1, 0, 0), c101 = grid_tap(m, T, 1, 0, 1);
+    const float c110 = grid_tap(m, T, 1, 1, 0), c111 = grid_tap(m, T, 1, 1, 1);
     const double x0y0 = X0 * Y0, x0y1 = X0 * Y1, x1y0 = X1 * Y0, x1y1 = X1 * Y1;
     float ret = (float)(x0y0 * Z0 * (double)c000);
     ret = (float)((double)ret + x0y0 * Z1 * (double)c001);
@@ -197,11 +239,11 @@ PG_HD float grid_density_fast(const MediumDev& m, f3 p) {
     if (!m.interpolate)
         return grid_voxel(m, ix, iy, iz);
     const float fx = px - (float)ix, fy = py - (float)iy, fz = pz - (float)iz;
-    const float c000 = grid_voxel(m, ix, iy, iz), c001 = grid_voxel(m, ix, iy, iz + 1);
-    const float c010 = grid_voxel(m, ix, iy + 1, iz), c011 = grid_voxel(m, ix, iy + 1, iz + 1);
-    const float c100 = grid_voxel(m, ix + 1, iy, iz), c101 = grid_voxel(m, ix + 1, iy, iz + 1);
-    const float c110 = grid_voxel(m, ix + 1, iy + 1, iz),
-                c111 = grid_voxel(m, ix + 1, iy + 1, iz + 1);
+    const GridTaps T = grid_taps(m, ix, iy, iz);
+    const float c000 = grid_tap(m, T, 0, 0, 0), c001 = grid_tap(m, T, 0, 0, 1);
+    const float c010 = grid_tap(m, T, 0, 1, 0), c011 = grid_tap(m, T, 0, 1, 1);
+    const float c100 = grid_tap(m, T, 1, 0, 0), c101 = grid_tap(m, T, 1, 0, 1);
+    const float c110 = grid_tap(m, T, 1, 1, 0), c111 = grid_tap(m, T, 1, 1, 1);
     const float c00 = fmaf(fx, c100 - c000, c000), c01 = fmaf(fx, c101 - c001, c001);
     const float c10 = fmaf(fx, c110 - c010, c010), c11 = fmaf(fx, c111 - c011, c011);
     const float c0 = fmaf(fy, c10 - c00, c00), c1 = fmaf(fy, c11 - c01, c01);

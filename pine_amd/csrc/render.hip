@@ -3748,7 +3748,25 @@ int finish_scene(pg_scene& S, const pg_build_options* opts) {
     int bricked = 0;
     for (size_t k = 0; k < S.media.size(); k++) {
         MediumDev& m = S.media[k];
-        if ((rc = upload(S, S.med_density[k], &m.density)) || (rc = upload(S, S.med_majorant[k], &m.majorant)))
+        // Pine's x-fastest densities regrouped into 4 x 4 x 2-voxel blocks.
+        std::vector<float> blocked;
+        if (m.type == kMediumGrid) {
+            int32_t nb[3];
+            int64_t floats = 0;
+            density_blocks(m.size, nb, floats);
+            if (floats > (int64_t)INT32_MAX)
+                return pg_fail(PG_E_ARG, "density grid too large (more than 2^31 voxels with padding)");
+            m.dblocks[0] = nb[0];
+            m.dblocks[1] = nb[1];
+            blocked.assign((size_t)floats, 0.0f);
+            const std::vector<float>& lin = S.med_density[k];
+            for (int z = 0; z < m.size[2]; z++)
+                for (int y = 0; y < m.size[1]; y++)
+                    for (int x = 0; x < m.size[0]; x++)
+                        blocked[(size_t)(dens_off_x(x) + dens_off_y(m, y) + dens_off_z(m, z))] =
+                            lin[(size_t)x + (size_t)m.size[0] * (y + (size_t)m.size[1] * z)];
+        }
+        if ((rc = upload(S, blocked, &m.density)) || (rc = upload(S, S.med_majorant[k], &m.majorant)))
             return rc;
         if (m.brick > 0) {
             bricked++;

@@ -390,7 +390,7 @@ def traversal(args, pg, hip, W, v, i, rays, traffic):
     acc.set_launch(args.block, launch_flags)
     kern_ms = float(np.mean(trace_ms))
     same_timed = bool(np.array_equal(d_hits.numpy(), hits))
-    # The same rays by the stackless traversal (parent links, per-ray trail,
+    # The same rays by the stackless traversal (per-ray trail, key table,
     # wave-wide node fetches; binary trees): same hits bit for bit, its rate.
     variants = {}
     if st["width"] == 2 and not (launch_flags & pg.LAUNCH_STACKLESS):

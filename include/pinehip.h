@@ -161,9 +161,11 @@ int pg_accel_prim_shape(const pg_accel* a, int32_t prim, int32_t* shape, int32_t
  * Every setting returns identical hits; only the speed differs. */
 #define PG_LAUNCH_POSTPONE_LEAVES 1 /* wave tests its leaves together (while-while loop) */
 #define PG_LAUNCH_XCD_REMAP 2       /* each XCD traces one contiguous range of rays */
-#define PG_LAUNCH_STACKLESS 4       /* binary trees: no stack, parent links and a per-ray trail of
-                                       pending far children; nodes shared by the whole wave are
-                                       fetched once by scalar loads (ignores POSTPONE_LEAVES) */
+#define PG_LAUNCH_STACKLESS 4       /* binary trees of at most 31 levels: no stack; a per-ray trail of
+                                       pending far children and a table of child codes by path key
+                                       (constant-time backtracking); in the top levels a node shared
+                                       by the whole wave is fetched once by scalar loads (ignores
+                                       POSTPONE_LEAVES) */
 #define PG_LAUNCH_REORDER 8         /* rays traced in the order of a key (direction octant, Morton
                                        code of the origin in the hierarchy's bounds) found by a
                                        counting sort on the device before each launch of 65,536

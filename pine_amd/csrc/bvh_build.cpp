@@ -290,28 +290,8 @@ struct Flattener {
         memcpy(node + off + slot, &code, 4);
     }
 
-    // Binary nodes carry, for parent-link (stackless) traversal, the flat
-    // index of their parent (float slot 14, -1 at the root) and their
-    // children's order along the axis where the child boxes' centres lie
-    // farthest apart (slot 15: axis | 4 when child 1's centre is the lower).
-    void put_links(float* node, int64_t parent, const Box& b0, const Box& b1) {
-        int axis = 0;
-        float best = -1.0f;
-        for (int k = 0; k < 3; k++) {
-            const float d = std::fabs((b1.lo[k] + b1.hi[k]) - (b0.lo[k] + b0.hi[k]));
-            if (d > best) {
-                best = d;
-                axis = k;
-            }
-        }
-        const bool c1_low = (b1.lo[axis] + b1.hi[axis]) < (b0.lo[axis] + b0.hi[axis]);
-        const int32_t p = (int32_t)parent, info = axis | (c1_low ? 4 : 0);
-        memcpy(node + 14, &p, 4);
-        memcpy(node + 15, &info, 4);
-    }
-
     // Emits node `id` (inner) at flat index `at`, recursing in depth-first order.
-    void emit(int32_t id, int64_t at, int depth, int64_t parent = -1) {
+    void emit(int32_t id, int64_t at, int depth) {
         const int stride = width == 2 ? 16 : 32;
         out.max_depth = std::max(out.max_depth, depth);
         int32_t kids[8];
@@ -347,11 +327,9 @@ struct Flattener {
                 put_child(node, i, kEmptyChild);
             }
         }
-        if (width == 2)
-            put_links(node, parent, bn[kids[0]].box, cnt > 1 ? bn[kids[1]].box : bn[kids[0]].box);
         for (int i = 0; i < cnt; i++)
             if (inner_at[i] >= 0)
-                emit(kids[i], inner_at[i], depth + 1, at);
+                emit(kids[i], inner_at[i], depth + 1);
     }
 };
 
@@ -367,11 +345,6 @@ FlatBVH build_bvh(std::vector<BuildPrim>& prims, const BuildParams& params, floa
         for (int i = 0; i < out.width; i++) {
             int32_t code = kEmptyChild;
             memcpy(&out.nodes[(out.width == 2 ? 12 : 24) + i], &code, 4);
-        }
-        if (out.width == 2) {
-            const int32_t p = -1, info = 0;
-            memcpy(&out.nodes[14], &p, 4);
-            memcpy(&out.nodes[15], &info, 4);
         }
         return out;
     }
@@ -406,11 +379,6 @@ FlatBVH build_bvh(std::vector<BuildPrim>& prims, const BuildParams& params, floa
                 node[12 + i] = node[16 + i] = node[20 + i] = -1.0f;
             }
             f.put_child(node, i, kEmptyChild);
-        }
-        if (out.width == 2) {
-            const int32_t p = -1, info = 0;
-            memcpy(&out.nodes[14], &p, 4);
-            memcpy(&out.nodes[15], &info, 4);
         }
         out.max_depth = 1;
         return out;

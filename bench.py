@@ -304,11 +304,56 @@ def traffic_probe(args):
         kib = float(np.mean(trace[1:]))
         res[counter] = {"kib_per_dispatch": round(kib, 1), "copy_scale": round(scale, 4),
                         "bytes": kib * scale * 1024}
-    return {"bytes": int(res["FETCH_SIZE"]["bytes"] + res["WRITE_SIZE"]["bytes"]),
-            "fetch": res["FETCH_SIZE"], "write": res["WRITE_SIZE"],
-            "method": "rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, on this script's "
-                      "--traffic-probe (one instrumented + 3 plain launches and a 512 MiB copy), "
-                      "each counter scaled by its reading of the copy"}
+    out = {"bytes": int(res["FETCH_SIZE"]["bytes"] + res["WRITE_SIZE"]["bytes"]),
+           "fetch": res["FETCH_SIZE"], "write": res["WRITE_SIZE"],
+           "method": "rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, on this script's "
+                     "--traffic-probe (one instrumented + 3 plain launches and a 512 MiB copy), "
+                     "each counter scaled by its reading of the copy"}
+    out["valu"] = valu_probe(args, exe)
+    return out
+
+
+VALU_COUNTERS = ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "GRBM_GUI_ACTIVE")
+
+
+def valu_probe(args, exe):
+    """VALU utilisation of the plain traversal launches, measured now in a third
+    child run under rocprofv3 --pmc: busy = 4 x SQ_ACTIVE_INST_VALU (quad-cycles)
+    over 1,024 SIMDs x GRBM_GUI_ACTIVE / 8 (the GUI-active cycles of the eight
+    XCDs are summed).  Returns a dict, or None with the reason logged."""
+    d = tempfile.mkdtemp()
+    cmd = [exe, "--pmc", *VALU_COUNTERS, "-d", d, "-o", "run", "--output-format", "csv", "--",
+           sys.executable, os.path.abspath(__file__), "--traffic-probe",
+           "--width", str(args.width), "--block", str(args.block), "--flags", str(args.flags),
+           "--leaf", str(args.leaf)]
+    p = subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                         start_new_session=True)
+    try:
+        out, _ = p.communicate(timeout=180)
+    except subprocess.TimeoutExpired:
+        os.killpg(p.pid, signal.SIGKILL)
+        p.communicate()
+        log("[bench] valu: counter run timed out")
+        shutil.rmtree(d, ignore_errors=True)
+        return None
+    per = {}
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                if "k_trace" in r.get("Kernel_Name", "") and r.get("Counter_Name") in VALU_COUNTERS:
+                    key = int(r.get("Dispatch_Id") or 0)
+                    per.setdefault(key, {}).setdefault(r["Counter_Name"], 0.0)
+                    per[key][r["Counter_Name"]] += float(r["Counter_Value"])
+    shutil.rmtree(d, ignore_errors=True)
+    launches = [per[k] for k in sorted(per)][1:]  # dispatch 0: the instrumented pass
+    if p.returncode != 0 or not launches or any(len(c) != len(VALU_COUNTERS) for c in launches):
+        log(f"[bench] valu: counter run gave rc {p.returncode}, {len(launches)} launches: {out[-400:]}")
+        return None
+    busy = [4.0 * c["SQ_ACTIVE_INST_VALU"] / (1024.0 * c["GRBM_GUI_ACTIVE"] / 8.0) for c in launches]
+    per_wave = [c["SQ_INSTS_VALU"] / c["SQ_WAVES"] for c in launches]
+    return {"busy": round(float(np.mean(busy)), 4), "insts_per_wave": round(float(np.mean(per_wave)), 1),
+            "method": "rocprofv3 --pmc " + " ".join(VALU_COUNTERS) + " on the same --traffic-probe "
+                      "launches; busy = 4 x SQ_ACTIVE_INST_VALU / (1024 SIMDs x GRBM_GUI_ACTIVE / 8)"}
 
 
 def traffic_probe_main(args):
@@ -451,9 +496,12 @@ def traversal(args, pg, hip, W, v, i, rays, traffic):
         "frac_counters": round(traffic["bytes"] / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
         if traffic else None,
         "compulsory_frac": round(compulsory / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
-        "limiter": "VALU issue (78% busy with the rays reordered), not HBM bandwidth (profiles/"
-                   "k_trace2_reordered_icosphere1m_rays16m.md); the algorithmic figure counts every "
-                   "node visit at 64 B, cache hits included, so it exceeds the HBM peak",
+        "limiter": (f"VALU issue ({traffic['valu']['busy']:.0%} busy, measured in this run), "
+                    if traffic and traffic.get("valu") else "VALU issue (78% busy in "
+                    "profiles/k_trace2_reordered_icosphere1m_rays16m.md), ") +
+                   "not HBM bandwidth; the algorithmic figure counts every node visit at 64 B, "
+                   "cache hits included, so it exceeds the HBM peak",
+        "valu": traffic.get("valu") if traffic else None,
         "kernel_ms": round(kern_ms, 4),
         "per_ray": {"nodes": round(nodes_fetched / n, 3), "prims": round(prims_tested / n, 3)},
         "traffic_method": traffic["method"] if traffic else "not measured in this run",
@@ -755,7 +803,8 @@ def main():
         t0 = time.time()
         traffic = traffic_probe(args)
         log(f"[bench] traffic measured in {time.time() - t0:.0f} s: "
-            f"{traffic['bytes'] / 1e9:.2f} GB per launch" if traffic else "[bench] traffic: none")
+            f"{traffic['bytes'] / 1e9:.2f} GB per launch, VALU busy "
+            f"{traffic['valu']['busy'] if traffic['valu'] else None}" if traffic else "[bench] traffic: none")
 
     from pine_amd import hip
     if hip.device_count() < 1:

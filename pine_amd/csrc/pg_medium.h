@@ -13,9 +13,9 @@
 //    (the exponential step restarts at each brick face).  Unbiased and far
 //    fewer null collisions in sparse grids; consumes a different number of
 //    sampler dimensions, so images agree with pine's statistically, not
-//    sample for sample, and it uses exact logarithms and float densities
-//    rather than pine's arithmetic.  The brick majorants of a launch are
-//    staged in LDS.
+//    sample for sample, and it uses exact logarithms, float densities and
+//    hashed uniform numbers (track_uniform) rather than pine's arithmetic and
+//    sampler values.  The brick majorants of a launch are staged in LDS.
 #pragma once
 
 #include "pg_shade.h"
@@ -213,8 +213,10 @@ PG_HD float grid_density(const MediumDev& m, f3 p) {
 // Tracking against brick majorants does not replay pine's sample sequence, so
 // it need not reproduce pine's arithmetic: free flights use the hardware log2
 // (pine's bisection log2 costs 32 IEEE divisions and is off by up to 2^-9,
-// which skews the exponential distribution) and densities a float trilinear
-// interpolation.
+// which skews the exponential distribution), densities a float trilinear
+// interpolation, and the uniform numbers of flights, collisions and Russian
+// roulette come from track_uniform (a hash of the sample index and the
+// dimension, the dimensions advancing as the sampler's would).
 
 // log2 of x > 0: v_log_f32 on the device.
 PG_HD float fast_log2(float x) {
@@ -414,12 +416,12 @@ PG_HD MediumEvent medium_sample(const MediumDev& m, const float* maj, const Samp
         if (mj > 0.0f && rate > 0.0f) {
             const float inv_mj = 1.0f / mj;
             while (true) {
-                t = brick_flight(t, get_1d_t<kS>(hs, ss), rate);
+                t = brick_flight(t, track_uniform<kS>(hs, ss), rate);
                 if (!(t < texit))
                     break;
                 const f3 pm = ray_at(r, t);
                 lookups++;
-                if (grid_density_fast(m, pm) * inv_mj > get_1d_t<kS>(hs, ss)) {
+                if (grid_density_fast(m, pm) * inv_mj > track_uniform<kS>(hs, ss)) {
                     ev.p = to_world(m, pm);
                     ev.scattered = true;
                     ev.weight = albedo;
@@ -492,7 +494,7 @@ PG_HD f3 medium_tr(const MediumDev& m, const float* maj, const SamplerDev& hs, S
         if (mj > 0.0f && rate > 0.0f) {
             const float inv_mj = 1.0f / mj;
             while (true) {
-                t = brick_flight(t, get_1d_t<kS>(hs, ss), rate);
+                t = brick_flight(t, track_uniform<kS>(hs, ss), rate);
                 if (!(t < texit))
                     break;
                 lookups++;
@@ -500,7 +502,7 @@ PG_HD f3 medium_tr(const MediumDev& m, const float* maj, const SamplerDev& hs, S
                 tr *= 1.0f - max0(density * inv_mj, 0.0f);
                 if (tr < rrThreshold) {
                     const float q = max0(1.0f - tr, 0.05f);
-                    if (get_1d_t<kS>(hs, ss) < q)
+                    if (track_uniform<kS>(hs, ss) < q)
                         return mk(0.0f, 0.0f, 0.0f);
                     tr /= 1.0f - q;
                 }

@@ -209,6 +209,36 @@ PG_HD float get_1d_t(const SamplerDev& h, SamplerState& s) {
     }
 }
 
+// Uniform numbers of this library's brick-majorant trackers (its own
+// estimator; pine's global-majorant tracking keeps the sampler's values):
+// each number takes the next sampler dimension with get_1d_t's bookkeeping
+// (the same wrap points), so the dimensions a path uses after tracking are
+// the ones it would use with the sampler's values, but the value is a hash of
+// (sample index, dimension) instead of the sampler's radical inverse or Sobol
+// point, which cost a digit loop or a matrix walk and dependent table reads
+// per number inside the trackers' inner loops.  Distinct (index, dimension)
+// pairs of a path give independent numbers, so the trackers stay unbiased.
+// The hash: Wellons' lowbias32 mixer, applied to the dimension and the
+// index's high word, then to that XOR the index's low word (bijective in the
+// dimension for a fixed index); the top 24 bits give a float in [0, 1).
+PG_HD uint32_t lowbias32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+template <int kS>
+PG_HD float track_uniform(const SamplerDev& h, SamplerState& s) {
+    const bool sobol = kS == kSamplerSobolOnly || (kS == kSamplerAny && h.kind == kSamplerSobol);
+    if (s.dim >= (sobol ? kSobolDimensions : 1000))
+        s.dim = 2;
+    const uint32_t d = (uint32_t)s.dim++;
+    const uint32_t inner = lowbias32((uint32_t)(s.index >> 32) * 1024u + d);
+    return (float)(lowbias32((uint32_t)s.index ^ inner) >> 8) * 0x1p-24f;
+}
+
 PG_HD void get_2d(const SamplerDev& h, SamplerState& s, float& u0, float& u1) {
     if (h.kind == kSamplerSobol) {
         // sampler.cpp:152-158 wraps only when the first dimension is past the

@@ -1381,7 +1381,7 @@ __device__ __forceinline__ int track_step_bricks(const MediumDev& m, const float
         }
     }
     if (L.phase == kFlight) {
-        L.t = brick_flight(L.t, get_1d_t<kS>(hs, ss), L.rate);
+        L.t = brick_flight(L.t, track_uniform<kS>(hs, ss), L.rate);
         if (L.t < L.texit)
             return 2;
         L.phase = kLeave;
@@ -1465,7 +1465,7 @@ __global__ __launch_bounds__(256) void k_medium_track_lanes(SceneDev s, PathsDev
         if (st == 2) {
             pm = ray_at(L.r, L.t);
             lookups++;
-            scattered = grid_density_fast(m, pm) * L.inv_mj > get_1d_t<kS>(s.sampler, ss);
+            scattered = grid_density_fast(m, pm) * L.inv_mj > track_uniform<kS>(s.sampler, ss);
         }
         if (scattered) {
             const f3 albedo = m.sigma_s / m.sigma_t;
@@ -1558,7 +1558,7 @@ __global__ __launch_bounds__(256) void k_shadow_track_lanes(SceneDev s, PathsDev
             tr *= 1.0f - max0(density * L.inv_mj, 0.0f);
             if (tr < 0.1f) {
                 const float q = max0(1.0f - tr, 0.05f);
-                if (get_1d_t<kS>(s.sampler, ss) < q) {
+                if (track_uniform<kS>(s.sampler, ss) < q) {
                     tr = 0.0f;
                     done = true;
                 } else {

@@ -68,12 +68,31 @@ struct HaltonDev {
     const uint16_t* perms;  // ComputeRadicalInversePermutations
     const int* primes;      // first 1000 primes
     const int* primeSums;   // prefix sums of primes
-    // Per dimension, one 16-byte record: prime base, offset of its permutation
+    // Per dimension, two 16-byte records: prime base, offset of its permutation
     // in perms, 1/base and the series term perm[0] / (base + 1), as floats'
     // bits (the two quotients computed once on the host, IEEE as the device
-    // would).
+    // would); then the base's division magic m and post-shift l - 1 (see
+    // udiv_magic).
     const int4* dims;
 };
+
+// Unsigned 32-bit division by an invariant b >= 2 (Granlund and Montgomery,
+// "Division by invariant integers using multiplication", 1994): with
+// l = ceil(log2 b) and m = floor(2^32 (2^l - b) / b) + 1,
+// a / b = (t + ((a - t) >> 1)) >> (l - 1), t = the high word of m * a, for
+// every 32-bit a.  A multiply-high, a subtract, an add and two shifts instead
+// of the generic division sequence.
+PG_HD uint32_t umulhi32(uint32_t a, uint32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umulhi(a, b);
+#else
+    return (uint32_t)(((uint64_t)a * b) >> 32);
+#endif
+}
+PG_HD uint32_t udiv_magic(uint32_t a, uint32_t m, uint32_t post) {
+    const uint32_t t = umulhi32(m, a);
+    return (t + ((a - t) >> 1)) >> post;
+}
 
 PG_HD uint64_t inverse_radical_inverse(uint64_t inverse, int base, int nDigits) {
     uint64_t index = 0;
@@ -103,8 +122,8 @@ PG_HD uint64_t halton_index(const HaltonDev& h, int px, int py, int sampleIndex)
 // four digits a round so that their permutation loads are in flight together
 // (the digits are combined in the same order as one at a time).
 PG_HD float scrambled_radical_inverse32(const HaltonDev& h, int dim, uint32_t a) {
-    const int4 t = h.dims[dim];
-    const uint32_t b = (uint32_t)t.x;
+    const int4 t = h.dims[2 * dim], dm = h.dims[2 * dim + 1];
+    const uint32_t b = (uint32_t)t.x, m = (uint32_t)dm.x, post = (uint32_t)dm.y;
     const uint16_t* perm = h.perms + t.y;
     const float invBase = __builtin_bit_cast(float, t.z);
     float invBaseN = 1.0f;
@@ -114,7 +133,7 @@ PG_HD float scrambled_radical_inverse32(const HaltonDev& h, int dim, uint32_t a)
         int nd = 0;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            const uint32_t next = a / b;
+            const uint32_t next = udiv_magic(a, m, post);
             dg[j] = a - next * b;
             nd += a != 0u ? 1 : 0;
             a = next;

@@ -3704,14 +3704,21 @@ int finish_scene(pg_scene& S, const pg_build_options* opts) {
     }
     // Per-dimension records of the radical inverse (base, permutation offset,
     // 1/base, perm[0] / (base + 1)).
-    std::vector<int4> dimtab(primes.size());
+    std::vector<int4> dimtab(2 * primes.size());
     for (size_t d = 0; d < primes.size(); d++) {
         const float inv_base = 1.0f / (float)primes[d];
         const float series = (float)perms[sums[d]] / ((float)primes[d] + 1.0f);
         int ib, sb;
         std::memcpy(&ib, &inv_base, 4);
         std::memcpy(&sb, &series, 4);
-        dimtab[d] = make_int4(primes[d], sums[d], ib, sb);
+        dimtab[2 * d] = make_int4(primes[d], sums[d], ib, sb);
+        // udiv_magic's constants for the base: l = ceil(log2 b), m, post-shift l - 1.
+        const uint64_t b = (uint64_t)primes[d];
+        int l = 0;
+        while ((uint64_t(1) << l) < b)
+            l++;
+        const uint32_t m = (uint32_t)(((uint64_t(1) << 32) * ((uint64_t(1) << l) - b)) / b + 1);
+        dimtab[2 * d + 1] = make_int4((int32_t)m, l - 1, 0, 0);
     }
     SamplerDev& sd = S.dev.sampler;
     sd.kind = S.sampler_kind;

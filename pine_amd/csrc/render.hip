@@ -687,12 +687,93 @@ __device__ __forceinline__ bool guided_direction(const SceneDev& s, PathsDev& P,
 // and the light sample's MIS weight uses the mixture pdf.
 // d_active (optional): the active count in device memory, read in place of
 // n_active (launches sized for the queue's capacity).
+// --- queue slots claimed per workgroup -----------------------------------------
+// Device-scope atomics on one address serialise: on MI355X about 11 ns each,
+// so a 4M-path bounce whose waves each claim a queue slot (the compiler already
+// merges a wave's lanes into one atomic) spends ~0.75 ms per counter on them.
+// These helpers claim a whole workgroup's slots with one atomic: the waves'
+// counts go through LDS, one lane adds the workgroup's total, every lane gets
+// the counter's old value plus its rank among the requesting lanes (waves in
+// order, lanes in order).  Every lane of the workgroup must call them (no early
+// return before); they begin with a barrier so that they can follow each other.
+constexpr int kSlotBlock = 256, kSlotWaves = kSlotBlock / 64;
+
+// Two adjacent counters ctr[0], ctr[1] (8-byte aligned) in one 64-bit atomic:
+// the low counter never carries into the high one (counts stay below 2^31).
+__device__ __forceinline__ void block_slots2(bool want_a, bool want_b, int32_t* ctr, int32_t& slot_a,
+                                             int32_t& slot_b) {
+    __shared__ uint32_t cnt[2][kSlotWaves];
+    __shared__ uint32_t base[2];
+    __syncthreads();
+    const int lane = (int)__lane_id(), wave = (int)(threadIdx.x >> 6);
+    const uint64_t ma = __ballot(want_a), mb = __ballot(want_b);
+    if (lane == 0) {
+        cnt[0][wave] = (uint32_t)__popcll(ma);
+        cnt[1][wave] = (uint32_t)__popcll(mb);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t ta = 0, tb = 0;
+        for (int w = 0; w < kSlotWaves; w++) {
+            ta += cnt[0][w];
+            tb += cnt[1][w];
+        }
+        unsigned long long old = 0;
+        if (ta | tb)
+            old = atomicAdd(reinterpret_cast<unsigned long long*>(ctr),
+                            (unsigned long long)ta | ((unsigned long long)tb << 32));
+        base[0] = (uint32_t)old;
+        base[1] = (uint32_t)(old >> 32);
+    }
+    __syncthreads();
+    uint32_t pa = base[0], pb = base[1];
+    for (int w = 0; w < wave; w++) {
+        pa += cnt[0][w];
+        pb += cnt[1][w];
+    }
+    const uint64_t lt = (1ull << lane) - 1ull;
+    slot_a = (int32_t)(pa + (uint32_t)__popcll(ma & lt));
+    slot_b = (int32_t)(pb + (uint32_t)__popcll(mb & lt));
+}
+
+// One counter.
+__device__ __forceinline__ int32_t block_slot(bool want, int32_t* ctr) {
+    __shared__ uint32_t cnt[kSlotWaves];
+    __shared__ uint32_t base;
+    __syncthreads();
+    const int lane = (int)__lane_id(), wave = (int)(threadIdx.x >> 6);
+    const uint64_t m = __ballot(want);
+    if (lane == 0)
+        cnt[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < kSlotWaves; w++)
+            t += cnt[w];
+        base = t ? (uint32_t)atomicAdd(ctr, (int32_t)t) : 0u;
+    }
+    __syncthreads();
+    uint32_t p = base;
+    for (int w = 0; w < wave; w++)
+        p += cnt[w];
+    return (int32_t)(p + (uint32_t)__popcll(m & ((1ull << lane) - 1ull)));
+}
+
 template <bool kGuided>
-__global__ __launch_bounds__(256) void k_shade(SceneDev s, PathsDev P, int32_t n_active, const int32_t* d_active) {
+__global__ __launch_bounds__(kSlotBlock) void k_shade(SceneDev s, PathsDev P, int32_t n_active, const int32_t* d_active) {
+    const int32_t n = d_active ? *d_active : n_active;
+    if ((int64_t)blockIdx.x * blockDim.x >= n)
+        return;  // the whole workgroup
     const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (d_active ? *d_active : n_active))
-        return;
-    const int32_t i = P.queue_in[k];
+    // The shadow ray and the next ray are written after the workgroup has
+    // claimed its queue slots (block_slots2); lanes past the count take part in
+    // the claim only.
+    bool alive = false, shadow = false;
+    f3 nextO, nextD, sh_o, sh_d, sh_c;
+    float sh_tmax = 0.0f;
+    int32_t sh_rec = -1, i = 0;
+    if (k < n) {
+    i = P.queue_in[k];
     const float4* r4 = reinterpret_cast<const float4*>(&P.rays_in[k]);
     const float4 ra = r4[0], rb = r4[1];
     const f3 ro = mk(ra.x, ra.y, ra.z), rd = mk(rb.x, rb.y, rb.z);
@@ -704,8 +785,6 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev s, PathsDev P, int32_t n
     SamplerState ss;
     ss.index = P.sidx[i];
     ss.dim = P.sdim[i];
-    bool alive = false;
-    f3 nextO, nextD;
 
     if (hit.prim < 0) {
         if (s.has_env) {
@@ -765,18 +844,14 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev s, PathsDev P, int32_t n
                     }
                     float w = power_heuristic(ls.pdf, scatPdf);
                     f3 ld = ls.delta ? f * ls.Le / ls.pdf : f * w * ls.Le / ls.pdf;
-                    f3 contrib = beta * ld;
                     if (!is_black(ld)) {
-                        int slot = atomicAdd(&P.counters[1], 1);
-                        f3 so = offset_ray_origin(h.p, face_same_hemisphere(h.n, ls.wo));
-                        store_ray(&P.shadow_rays[slot], so, ls.wo, 0.0f,
-                                  ls.distance * (1.0f - 1e-3f));
-                        P.shadow_path[slot] = i;
-                        P.shadow_contrib[3 * slot + 0] = contrib.x;
-                        P.shadow_contrib[3 * slot + 1] = contrib.y;
-                        P.shadow_contrib[3 * slot + 2] = contrib.z;
-                        if (kGuided && P.rec_depth > 0)
-                            P.shadow_rec[slot] = guided && depth < P.rec_depth ? i * P.rec_depth + depth : -1;
+                        shadow = true;
+                        sh_c = beta * ld;
+                        sh_o = offset_ray_origin(h.p, face_same_hemisphere(h.n, ls.wo));
+                        sh_d = ls.wo;
+                        sh_tmax = ls.distance * (1.0f - 1e-3f);
+                        if (kGuided)
+                            sh_rec = guided && depth < P.rec_depth ? i * P.rec_depth + depth : -1;
                     }
                 }
                 if (guided) {
@@ -829,10 +904,21 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev s, PathsDev P, int32_t n
     P.beta[3 * i + 2] = beta.z;
     P.sdim[i] = ss.dim;
     P.depth[i] = depth;
+    }
+    int32_t slot = 0, sslot = 0;
+    block_slots2(alive, shadow, P.counters, slot, sslot);
     if (alive) {
-        int slot = atomicAdd(&P.counters[0], 1);
         P.queue_out[slot] = i;
         store_ray(&P.rays_out[slot], nextO, nextD, 0.0f, kFloatMax);
+    }
+    if (shadow) {
+        store_ray(&P.shadow_rays[sslot], sh_o, sh_d, 0.0f, sh_tmax);
+        P.shadow_path[sslot] = i;
+        P.shadow_contrib[3 * sslot + 0] = sh_c.x;
+        P.shadow_contrib[3 * sslot + 1] = sh_c.y;
+        P.shadow_contrib[3 * sslot + 2] = sh_c.z;
+        if (kGuided && P.rec_depth > 0)
+            P.shadow_rec[sslot] = sh_rec;
     }
 }
 
@@ -1201,13 +1287,25 @@ __global__ __launch_bounds__(256) void k_spec_to_rgb(SceneDev s, PathsDev P, int
 //   k_resolve_gen L += beta * tr * f * w * Le / pdf
 //   k_shade_gen_b BSDF or phase sample, Russian roulette -> next ray
 
-// Adds a thread's density-lookup count to the global counter: one atomic per
-// wave after a cross-lane sum.  Every lane of the block reaches this call.
+// Adds a thread's density-lookup count to the global counter: a cross-lane
+// sum per wave, the waves' sums through LDS, one atomic per workgroup (see
+// block_slots2 on same-address atomics).  Every lane of the block reaches
+// this call.
 __device__ __forceinline__ void add_lookups(unsigned long long* counter, int n) {
+    __shared__ int wsum[16];
     for (int off = 32; off > 0; off >>= 1)
         n += __shfl_down(n, off, 64);
-    if ((threadIdx.x & 63) == 0 && n > 0)
-        atomicAdd(counter, (unsigned long long)n);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0)
+        wsum[threadIdx.x >> 6] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < (int)(blockDim.x + 63) / 64; w++)
+            t += (unsigned long long)wsum[w];
+        if (t > 0)
+            atomicAdd(counter, t);
+    }
 }
 
 // Brick majorants of medium m: the LDS copy when this launch staged it.
@@ -1238,8 +1336,21 @@ __device__ __forceinline__ int medium_for(f3 w, f3 n, int inside, int outside) {
 // null, phase g) vertex.  The record is
 // written even when the contribution is black: IntersectTr still runs, and in
 // media it consumes sampler dimensions.
-__device__ __forceinline__ void light_record(const SceneDev& s, PathsDev& P, SamplerState& ss, int32_t i, f3 p,
-                             f3 n, f3 wi, const MaterialDev* mat, float g, int med_in, int med_out) {
+// EstimateDirect's light sample at a split-bounce vertex (integrator.cpp:
+// 88-100): light choice, point, BSDF or phase value, MIS weight and the first
+// shadow segment, held in registers until the workgroup has claimed the
+// record's slot (light_record_write).
+struct LightRecord {
+    float4 r0, r1, r2, r3;  // {f, w}, {Le, pdf}, {target p2, delta}, {wo, pb}
+    f3 so;                  // first segment: origin, direction r3.xyz, range stmax
+    float stmax;
+    int32_t seg_med;
+    bool valid;
+};
+
+__device__ __forceinline__ void light_record(const SceneDev& s, SamplerState& ss, f3 p, f3 n, f3 wi,
+                                             const MaterialDev* mat, float g, int med_in, int med_out,
+                                             LightRecord& out) {
     const float ul = get_1d(s.sampler, ss);  // SampleLight's Get1D runs even with no lights
     if (s.n_lights <= 0)
         return;
@@ -1264,22 +1375,32 @@ __device__ __forceinline__ void light_record(const SceneDev& s, PathsDev& P, Sam
     }
     const float w = power_heuristic(ls.pdf, scatPdf);
     // it.SpawnRay(ls.wo, ls.distance) (interaction.h:36-44)
-    const f3 so = offset_ray_origin(p, face_same_hemisphere(n, ls.wo));
-    const float stmax = ls.distance * (1.0f - 1e-3f);
-    const int r = atomicAdd(&P.counters[1], 1);
-    float4* rec = reinterpret_cast<float4*>(P.sh_rec + 16 * (int64_t)r);
-    const f3 p2 = so + stmax * ls.wo;  // IntersectTr's target ray(ray.tmax)
-    rec[0] = make_float4(f.x, f.y, f.z, w);
-    rec[1] = make_float4(ls.Le.x, ls.Le.y, ls.Le.z, ls.pdf);
-    rec[2] = make_float4(p2.x, p2.y, p2.z, ls.delta ? 1.0f : 0.0f);
-    rec[3] = make_float4(ls.wo.x, ls.wo.y, ls.wo.z, pb);
-    P.sh_tr[3 * r + 0] = P.sh_tr[3 * r + 1] = P.sh_tr[3 * r + 2] = 1.0f;
-    P.shadow_path[r] = i;
-    store_ray(&P.seg_rays[0][r], so, ls.wo, 0.0f, stmax);
-    P.seg_rec[0][r] = r;
-    P.seg_med[0][r] = medium_for(ls.wo, n, med_in, med_out);
+    out.so = offset_ray_origin(p, face_same_hemisphere(n, ls.wo));
+    out.stmax = ls.distance * (1.0f - 1e-3f);
+    const f3 p2 = out.so + out.stmax * ls.wo;  // IntersectTr's target ray(ray.tmax)
+    out.r0 = make_float4(f.x, f.y, f.z, w);
+    out.r1 = make_float4(ls.Le.x, ls.Le.y, ls.Le.z, ls.pdf);
+    out.r2 = make_float4(p2.x, p2.y, p2.z, ls.delta ? 1.0f : 0.0f);
+    out.r3 = make_float4(ls.wo.x, ls.wo.y, ls.wo.z, pb);
+    out.seg_med = medium_for(ls.wo, n, med_in, med_out);
+    out.valid = true;
 }
 
+__device__ __forceinline__ void light_record_write(PathsDev& P, int32_t r, int32_t i, const LightRecord& L) {
+    float4* rec = reinterpret_cast<float4*>(P.sh_rec + 16 * (int64_t)r);
+    rec[0] = L.r0;
+    rec[1] = L.r1;
+    rec[2] = L.r2;
+    rec[3] = L.r3;
+    P.sh_tr[3 * r + 0] = P.sh_tr[3 * r + 1] = P.sh_tr[3 * r + 2] = 1.0f;
+    P.shadow_path[r] = i;
+    store_ray(&P.seg_rays[0][r], L.so, mk(L.r3.x, L.r3.y, L.r3.z), 0.0f, L.stmax);
+    P.seg_rec[0][r] = r;
+    P.seg_med[0][r] = L.seg_med;
+}
+
+// The vertex a direction will be sampled at (k_shade_gen_b), stored per path;
+// the path joins the direction queue once the workgroup has claimed slots.
 __device__ __forceinline__ void save_vertex(PathsDev& P, int32_t i, int kind, f3 p, f3 n, f3 wi, int mat,
                             int med_in, int med_out) {
     float* v = P.vtx + 9 * (int64_t)i;
@@ -1288,7 +1409,6 @@ __device__ __forceinline__ void save_vertex(PathsDev& P, int32_t i, int kind, f3
     v[6] = wi.x, v[7] = wi.y, v[8] = wi.z;
     int4 info = make_int4(kind, mat, med_in, med_out);
     *reinterpret_cast<int4*>(P.vinfo + 4 * (int64_t)i) = info;
-    P.queue_b[atomicAdd(&P.counters[2], 1)] = i;
 }
 
 // Medium::Sample (medium.h:70-73) for every queued ray whose path is in a
@@ -1598,10 +1718,19 @@ struct Lift4 {
 // kNodes: the scene has node-graph materials (the interpreter's code and
 // registers only in that instantiation).
 template <bool kSpec, bool kNodes>
-__global__ __launch_bounds__(256) void k_shade_gen(SceneDev s, PathsDev P, int32_t n_active) {
-    for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n_active;
-         k += gridDim.x * blockDim.x) {
-        const int32_t i = P.queue_in[k];
+__global__ __launch_bounds__(kSlotBlock) void k_shade_gen(SceneDev s, PathsDev P, int32_t n_active) {
+    if ((int64_t)blockIdx.x * blockDim.x >= n_active)
+        return;  // the whole workgroup
+    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    // One path per lane; the queue writes wait for the workgroup's slot claim.
+    bool alive = false, vertex = false;
+    f3 nextO, nextD;
+    int32_t i = 0;
+    int next_med = -1;
+    LightRecord lrec;
+    lrec.valid = false;
+    if (k < n_active) {
+        i = P.queue_in[k];
         const float4* r4 = reinterpret_cast<const float4*>(&P.rays_in[k]);
         const float4 ra = r4[0], rb = r4[1];
         const f3 ro = mk(ra.x, ra.y, ra.z), rd = mk(rb.x, rb.y, rb.z);
@@ -1631,9 +1760,7 @@ __global__ __launch_bounds__(256) void k_shade_gen(SceneDev s, PathsDev P, int32
         ss.index = P.sidx[i];
         ss.dim = P.sdim[i];
         const int med = P.medium[i];
-        bool alive = false;
-        f3 nextO, nextD;
-        int next_med = med;
+        next_med = med;
         MediumEvent ev;
         ev.scattered = false;
         if (med >= 0) {
@@ -1653,8 +1780,9 @@ __global__ __launch_bounds__(256) void k_shade_gen(SceneDev s, PathsDev P, int32
         if (ev.scattered) {
             if (depth + 1 != s.max_depth) {
                 const f3 zero = mk(0.0f, 0.0f, 0.0f);
-                light_record(s, P, ss, i, ev.p, zero, -rd, nullptr, s.media[med].g, med, med);
+                light_record(s, ss, ev.p, zero, -rd, nullptr, s.media[med].g, med, med, lrec);
                 save_vertex(P, i, 1, ev.p, zero, -rd, -1, med, med);
+                vertex = true;
             }
         } else if (hit.prim < 0) {
             if (s.has_env) {
@@ -1695,8 +1823,9 @@ __global__ __launch_bounds__(256) void k_shade_gen(SceneDev s, PathsDev P, int32
                         add_light(le, power_heuristic(bsdfPdf, lightPdf), true);
                     }
                 } else if (depth + 1 != s.max_depth) {
-                    light_record(s, P, ss, i, h.p, h.n, -rd, &m, 0.0f, min_, mout);
+                    light_record(s, ss, h.p, h.n, -rd, &m, 0.0f, min_, mout, lrec);
                     save_vertex(P, i, 0, h.p, h.n, -rd, sh.material, min_, mout);
+                    vertex = true;
                 }
             }
         }
@@ -1712,13 +1841,19 @@ __global__ __launch_bounds__(256) void k_shade_gen(SceneDev s, PathsDev P, int32
             P.beta[3 * i + 2] = beta.z;
         }
         P.sdim[i] = ss.dim;
-        if (alive) {
-            const int slot = atomicAdd(&P.counters[0], 1);
-            P.queue_out[slot] = i;
-            P.medium[i] = next_med;
-            store_ray(&P.rays_out[slot], nextO, nextD, 0.0f, kFloatMax);
-        }
     }
+    int32_t slot = 0, r = 0;
+    block_slots2(alive, lrec.valid, P.counters, slot, r);
+    const int32_t b = block_slot(vertex, &P.counters[2]);
+    if (alive) {
+        P.queue_out[slot] = i;
+        P.medium[i] = next_med;
+        store_ray(&P.rays_out[slot], nextO, nextD, 0.0f, kFloatMax);
+    }
+    if (lrec.valid)
+        light_record_write(P, r, i, lrec);
+    if (vertex)
+        P.queue_b[b] = i;
 }
 
 // RandomWalkIntegrator::Li (randomwalk.cpp:6-62) with participating media:
@@ -1952,12 +2087,17 @@ __global__ __launch_bounds__(256) void k_resolve_gen(SceneDev s, PathsDev P, int
 // surface (path.cpp:66-86).
 // d_n, ovf: as for k_resolve_gen (the direction queue's count).
 template <bool kSpec>
-__global__ __launch_bounds__(256) void k_shade_gen_b(SceneDev s, PathsDev P, int32_t n_b, const int32_t* d_n,
-                                                     const int32_t* ovf) {
+__global__ __launch_bounds__(kSlotBlock) void k_shade_gen_b(SceneDev s, PathsDev P, int32_t n_b, const int32_t* d_n,
+                                                            const int32_t* ovf) {
+    const int32_t n = d_n ? *d_n : n_b;
+    if ((int64_t)blockIdx.x * blockDim.x >= n || (ovf && *ovf != 0))
+        return;  // the whole workgroup
     const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (d_n ? *d_n : n_b) || (ovf && *ovf != 0))
-        return;
-    const int32_t i = P.queue_b[k];
+    bool alive = false;
+    f3 nextO, nextD;
+    int next_med = -1, i = 0;
+    if (k < n) {
+    i = P.queue_b[k];
     const float* v = P.vtx + 9 * (int64_t)i;
     const f3 p = mk(v[0], v[1], v[2]), n = mk(v[3], v[4], v[5]), wi = mk(v[6], v[7], v[8]);
     const int4 info = *reinterpret_cast<const int4*>(P.vinfo + 4 * (int64_t)i);
@@ -1971,9 +2111,6 @@ __global__ __launch_bounds__(256) void k_shade_gen_b(SceneDev s, PathsDev P, int
     ss.index = P.sidx[i];
     ss.dim = P.sdim[i];
     int depth = P.depth[i];
-    bool alive = false;
-    f3 nextO, nextD;
-    int next_med = -1;
     if (info.x == 1) {
         float u0, u1;
         get_2d(s.sampler, ss, u0, u1);
@@ -2058,8 +2195,9 @@ __global__ __launch_bounds__(256) void k_shade_gen_b(SceneDev s, PathsDev P, int
     }
     P.sdim[i] = ss.dim;
     P.depth[i] = depth;
+    }
+    const int32_t slot = block_slot(alive, &P.counters[0]);
     if (alive) {
-        const int slot = atomicAdd(&P.counters[0], 1);
         P.queue_out[slot] = i;
         P.medium[i] = next_med;
         store_ray(&P.rays_out[slot], nextO, nextD, 0.0f, kFloatMax);

@@ -950,15 +950,19 @@ __global__ __launch_bounds__(256) void k_resolve_shadow(PathsDev P, int32_t n_sh
 // uniform over the sphere (UniformHemisphereMampling, sampling.h:49-51, with
 // pine's acos/sin/cos) turned into the hemisphere of the face-forward normal;
 // radiance 1 when BVH::Hit finds nothing along it.  Misses give 0.
-__global__ __launch_bounds__(256) void k_shade_ao(SceneDev s, PathsDev P, int32_t n_active) {
+__global__ __launch_bounds__(kSlotBlock) void k_shade_ao(SceneDev s, PathsDev P, int32_t n_active) {
+    if ((int64_t)blockIdx.x * blockDim.x >= n_active)
+        return;  // the whole workgroup
     const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n_active)
-        return;
-    const int32_t i = P.queue_in[k];
+    // The occlusion ray is written after the workgroup's slot claim.
+    bool shadow = false;
+    f3 sh_o, sh_d;
+    int32_t i = 0;
+    if (k < n_active) {
+    i = P.queue_in[k];
     P.L[3 * i + 0] = P.L[3 * i + 1] = P.L[3 * i + 2] = 0.0f;
     const pg_hit hit = P.hits[k];
-    if (hit.prim < 0)
-        return;
+    if (hit.prim >= 0) {
     const float4* r4 = reinterpret_cast<const float4*>(&P.rays_in[k]);
     const float4 ra = r4[0], rb = r4[1];
     const f3 ro = mk(ra.x, ra.y, ra.z), rd = mk(rb.x, rb.y, rb.z);
@@ -973,9 +977,16 @@ __global__ __launch_bounds__(256) void k_shade_ao(SceneDev s, PathsDev P, int32_
     const float phi = u0 * pstd::Pi * 2.0f, theta = pstd::acos(u1);
     const float st = pstd::sin(theta);
     const f3 w = face_same_hemisphere(mk(st * pstd::cos(phi), st * pstd::sin(phi), pstd::cos(theta)), n);
-    const int32_t slot = atomicAdd(&P.counters[1], 1);
-    store_ray(&P.shadow_rays[slot], offset_ray_origin(h.p, face_same_hemisphere(n, w)), w, 0.0f, kFloatMax);
-    P.shadow_path[slot] = i;
+    shadow = true;
+    sh_o = offset_ray_origin(h.p, face_same_hemisphere(n, w));
+    sh_d = w;
+    }
+    }
+    const int32_t slot = block_slot(shadow, &P.counters[1]);
+    if (shadow) {
+        store_ray(&P.shadow_rays[slot], sh_o, sh_d, 0.0f, kFloatMax);
+        P.shadow_path[slot] = i;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_resolve_ao(PathsDev P, int32_t n_shadow) {
@@ -1042,11 +1053,16 @@ __global__ __launch_bounds__(256) void k_shade_viz(SceneDev s, PathsDev P, int32
 // the environment added with throughput and no light sampling; Material::Sample
 // per vertex (Get1D, Get2D); no Russian roulette; material-less surfaces
 // continue straight without counting depth.
-__global__ __launch_bounds__(256) void k_shade_rw(SceneDev s, PathsDev P, int32_t n_active) {
+__global__ __launch_bounds__(kSlotBlock) void k_shade_rw(SceneDev s, PathsDev P, int32_t n_active) {
+    if ((int64_t)blockIdx.x * blockDim.x >= n_active)
+        return;  // the whole workgroup
     const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n_active)
-        return;
-    const int32_t i = P.queue_in[k];
+    // The next ray is written after the workgroup's slot claim.
+    bool alive = false;
+    f3 nextO, nextD;
+    int32_t i = 0;
+    if (k < n_active) {
+    i = P.queue_in[k];
     const float4* r4 = reinterpret_cast<const float4*>(&P.rays_in[k]);
     const float4 ra = r4[0], rb = r4[1];
     const f3 ro = mk(ra.x, ra.y, ra.z), rd = mk(rb.x, rb.y, rb.z);
@@ -1057,8 +1073,6 @@ __global__ __launch_bounds__(256) void k_shade_rw(SceneDev s, PathsDev P, int32_
     SamplerState ss;
     ss.index = P.sidx[i];
     ss.dim = P.sdim[i];
-    bool alive = false;
-    f3 nextO, nextD;
     if (hit.prim < 0) {
         if (s.has_env)
             L = L + beta * env_color(s, rd);
@@ -1104,8 +1118,9 @@ __global__ __launch_bounds__(256) void k_shade_rw(SceneDev s, PathsDev P, int32_
     P.beta[3 * i + 2] = beta.z;
     P.sdim[i] = ss.dim;
     P.depth[i] = depth;
+    }
+    const int32_t slot = block_slot(alive, &P.counters[0]);
     if (alive) {
-        const int slot = atomicAdd(&P.counters[0], 1);
         P.queue_out[slot] = i;
         store_ray(&P.rays_out[slot], nextO, nextD, 0.0f, kFloatMax);
     }
@@ -1129,11 +1144,18 @@ __device__ __forceinline__ void spec_store(float* p, int64_t i, const Spec4& s) 
 // upsampled at the path's wavelengths (spec::upsample_row), products are taken
 // per wavelength, Russian roulette uses the mean throughput.  Geometry,
 // sampling and the sampler's dimension order are those of k_shade.
-__global__ __launch_bounds__(256) void k_shade_spec(SceneDev s, PathsDev P, int32_t n_active) {
+__global__ __launch_bounds__(kSlotBlock) void k_shade_spec(SceneDev s, PathsDev P, int32_t n_active) {
+    if ((int64_t)blockIdx.x * blockDim.x >= n_active)
+        return;  // the whole workgroup
     const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n_active)
-        return;
-    const int32_t i = P.queue_in[k];
+    // Queue writes wait for the workgroup's slot claim (block_slots2).
+    bool alive = false, shadow = false;
+    f3 nextO, nextD, sh_o, sh_d;
+    float sh_tmax = 0.0f;
+    Spec4 sh_c;
+    int32_t i = 0;
+    if (k < n_active) {
+    i = P.queue_in[k];
     const float4* r4 = reinterpret_cast<const float4*>(&P.rays_in[k]);
     const float4 ra = r4[0], rb = r4[1];
     const f3 ro = mk(ra.x, ra.y, ra.z), rd = mk(rb.x, rb.y, rb.z);
@@ -1156,8 +1178,6 @@ __global__ __launch_bounds__(256) void k_shade_spec(SceneDev s, PathsDev P, int3
     SamplerState ss;
     ss.index = P.sidx[i];
     ss.dim = P.sdim[i];
-    bool alive = false;
-    f3 nextO, nextD;
     if (hit.prim < 0) {
         if (s.has_env) {
             const float w = depth == 0 ? 1.0f : power_heuristic(bsdfPdf, env_pdf(s));
@@ -1204,11 +1224,11 @@ __global__ __launch_bounds__(256) void k_shade_spec(SceneDev s, PathsDev P, int3
                     any = any || contrib.v[j] != 0.0f;
                 }
                 if (any) {
-                    int slot = atomicAdd(&P.counters[1], 1);
-                    f3 so = offset_ray_origin(h.p, face_same_hemisphere(h.n, ls.wo));
-                    store_ray(&P.shadow_rays[slot], so, ls.wo, 0.0f, ls.distance * (1.0f - 1e-3f));
-                    P.shadow_path[slot] = i;
-                    spec_store(P.sh4, slot, contrib);
+                    shadow = true;
+                    sh_o = offset_ray_origin(h.p, face_same_hemisphere(h.n, ls.wo));
+                    sh_d = ls.wo;
+                    sh_tmax = ls.distance * (1.0f - 1e-3f);
+                    sh_c = contrib;
                 }
             }
             const float u1 = get_1d(s.sampler, ss);
@@ -1244,10 +1264,17 @@ __global__ __launch_bounds__(256) void k_shade_spec(SceneDev s, PathsDev P, int3
     spec_store(P.beta4, i, beta);
     P.sdim[i] = ss.dim;
     P.depth[i] = depth;
+    }
+    int32_t slot = 0, sslot = 0;
+    block_slots2(alive, shadow, P.counters, slot, sslot);
     if (alive) {
-        int slot = atomicAdd(&P.counters[0], 1);
         P.queue_out[slot] = i;
         store_ray(&P.rays_out[slot], nextO, nextD, 0.0f, kFloatMax);
+    }
+    if (shadow) {
+        store_ray(&P.shadow_rays[sslot], sh_o, sh_d, 0.0f, sh_tmax);
+        P.shadow_path[sslot] = i;
+        spec_store(P.sh4, sslot, sh_c);
     }
 }
 
@@ -1863,13 +1890,20 @@ __global__ __launch_bounds__(kSlotBlock) void k_shade_gen(SceneDev s, PathsDev P
 // path is at maxDepth; emission and the environment are added with the
 // throughput alone; material-less surfaces pass the ray into the medium on
 // their far side without counting depth.
-__global__ __launch_bounds__(256) void k_shade_rw_gen(SceneDev s, PathsDev P, int32_t n_active, int32_t lds_medium) {
+__global__ __launch_bounds__(kSlotBlock) void k_shade_rw_gen(SceneDev s, PathsDev P, int32_t n_active, int32_t lds_medium) {
     extern __shared__ float lds_maj[];
     const float* lds = stage_majorants(s, lds_medium, lds_maj);
     int lookups = 0;
-    for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n_active;
-         k += gridDim.x * blockDim.x) {
-        const int32_t i = P.queue_in[k];
+    // The workgroup strides through the queue together, so that each round
+    // claims its next-ray slots once (block_slot).
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n_active; base += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t k = (int32_t)(base + threadIdx.x);
+        bool alive = false;
+        f3 nextO, nextD;
+        int32_t i = 0;
+        int next_med = -1;
+        if (k < n_active) {
+        i = P.queue_in[k];
         const float4* r4 = reinterpret_cast<const float4*>(&P.rays_in[k]);
         const float4 ra = r4[0], rb = r4[1];
         const f3 ro = mk(ra.x, ra.y, ra.z), rd = mk(rb.x, rb.y, rb.z);
@@ -1881,9 +1915,7 @@ __global__ __launch_bounds__(256) void k_shade_rw_gen(SceneDev s, PathsDev P, in
         ss.index = P.sidx[i];
         ss.dim = P.sdim[i];
         const int med = P.medium[i];
-        bool alive = false;
-        f3 nextO, nextD;
-        int next_med = med;
+        next_med = med;
         const float tmax = hit.prim >= 0 ? hit.t : kFloatMax;
         MediumEvent ev;
         ev.scattered = false;
@@ -1956,8 +1988,9 @@ __global__ __launch_bounds__(256) void k_shade_rw_gen(SceneDev s, PathsDev P, in
         P.beta[3 * i + 2] = beta.z;
         P.sdim[i] = ss.dim;
         P.depth[i] = depth;
+        }
+        const int32_t slot = block_slot(alive, &P.counters[0]);
         if (alive) {
-            const int slot = atomicAdd(&P.counters[0], 1);
             P.queue_out[slot] = i;
             P.medium[i] = next_med;
             store_ray(&P.rays_out[slot], nextO, nextD, 0.0f, kFloatMax);

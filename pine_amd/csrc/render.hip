@@ -2611,6 +2611,7 @@ struct pg_scene {
     int lds_medium = -1;  // medium whose brick majorants the tracking kernels stage in LDS
     int64_t last_lookups = 0;
     bool general = false;
+    int materialless_shapes = 0;  // shapes without a material (the split bounce's pass-throughs)
     float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};  // bounds of the finite shapes
     int64_t guide_capacity = 0, record_capacity = 0;
     double last_guide_ms = 0.0;
@@ -3246,8 +3247,10 @@ void build_scene(pg_scene& S) {
             sh.mi_outside = find_medium(mn);
         if (sh.mi_inside >= 0 || sh.mi_outside >= 0)
             uses_media = true;
-        if (sh.material < 0)
+        if (sh.material < 0) {
             S.general = true;
+            S.materialless_shapes++;
+        }
         float pos[3], e0[3], e1[3];
         int32_t first = 0;
         const int shape_index = (int)S.shapes.size();
@@ -4180,11 +4183,15 @@ static void launch_tracking(int ks, int km, F&& f) {
 
 // Shadow-segment passes of the split bounce run on device-held counts (no
 // host round trip): pass 0 reads the record count (counters[1]), pass p
-// counts its continuations at counters[4 + p].  A shadow ray crosses each
-// material-less surface once, so most scenes finish within these passes;
-// counters[7] holds whatever is left after the last one, and the host then
-// finishes those segments before the records are resolved.
+// counts its continuations at counters[4 + p].  A shadow ray crosses a convex
+// material-less shape at most twice, so a scene with m such shapes gets
+// 1 + 2m passes, at most kDeviceSegmentPasses; counters[4 + passes - 1] holds
+// whatever is left after the last one (a non-convex shape, or more shapes),
+// and the host then finishes those segments before the records are resolved.
 constexpr int kDeviceSegmentPasses = 4;
+static int device_segment_passes(const pg_scene* S) {
+    return std::min(kDeviceSegmentPasses, 1 + 2 * S->materialless_shapes);
+}
 
 // One bounce in general mode; `active` becomes the next queue's length.  One
 // host synchronisation per bounce: every launch after the shading kernel
@@ -4218,11 +4225,12 @@ static int bounce_general(pg_scene* S, PathsDev& Q, int32_t& active, hipStream_t
     rays += active;
     const unsigned grid_all = (unsigned)((active + B - 1) / B);
     int32_t* C = Q.counters;
+    const int passes = device_segment_passes(S);
     // IntersectTr: trace each segment, track its medium, continue through
     // material-less surfaces until every record is resolved.
     int src = 0;
     if (!lanes) {
-        for (int pass = 0; pass < kDeviceSegmentPasses; pass++) {
+        for (int pass = 0; pass < passes; pass++) {
             const int32_t* d_nseg = pass == 0 ? C + 1 : C + 4 + pass - 1;
             if (int rc = pg_trace_closest_dev(S->accel, Q.seg_rays[src], d_nseg, active, Q.seg_hits, Q.seg_uvs, st))
                 return rc;
@@ -4236,7 +4244,7 @@ static int bounce_general(pg_scene* S, PathsDev& Q, int32_t& active, hipStream_t
         if (S->dev.guided)
             hipLaunchKernelGGL(k_guide_infer<0>, dim3(infer_blocks(active)), dim3(kInferThreads), 0, st, S->dev, Q,
                                active, C + 2);
-        const int32_t* ovf = C + 4 + kDeviceSegmentPasses - 1;
+        const int32_t* ovf = C + 4 + passes - 1;
         hipLaunchKernelGGL(S->spectral ? k_resolve_gen<true> : k_resolve_gen<false>, dim3(grid_all), dim3(B), 0, st,
                            S->dev, Q, 0, C + 1, ovf);
         hipLaunchKernelGGL(S->spectral ? k_shade_gen_b<true> : k_shade_gen_b<false>, dim3(grid_all), dim3(B), 0, st,
@@ -4249,9 +4257,9 @@ static int bounce_general(pg_scene* S, PathsDev& Q, int32_t& active, hipStream_t
     int32_t n_seg = n_rec;
     if (!lanes) {
         rays += n_rec;
-        for (int pass = 0; pass + 1 < kDeviceSegmentPasses; pass++)
+        for (int pass = 0; pass + 1 < passes; pass++)
             rays += counts[4 + pass];
-        n_seg = counts[4 + kDeviceSegmentPasses - 1];
+        n_seg = counts[4 + passes - 1];
         if (n_seg == 0) {
             active = counts[0];
             return PG_OK;

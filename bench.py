@@ -551,9 +551,9 @@ def queue_roofline(inf, samples, ms):
                      f"ray, over the whole render time (all kernels)"}
 
 
-def render_gpu(pg, text, directory, warm_spp=2):
+def render_gpu(pg, text, directory):
     scene = pg.Scene(text=text, directory=directory)
-    scene.render(spp=warm_spp)  # allocations and first launches
+    scene.render()  # warm, untimed: path-state slot sized, bounce loop recorded for the batch size
     inf = scene.render()
     scene.release()
     return inf
@@ -581,14 +581,18 @@ def batch_leg(args, pg, W, work):
     tuned = [s.autotune(probe_spp=1, try_width=True) for s in scenes] if not args.no_tune else []
     tune_s = time.time() - t0
     samples = sum(it["width"] * it["height"] * it["spp"] for it in items)
-    for s in scenes:  # warm: record every scene's bounce loop
-        s.render(spp=1)
+    # Warm passes, untimed: size the path-state slots (slot 0 for renders one
+    # by one, one per stream for the batch) and record every scene's bounce
+    # loop for its batch size on each.
+    for s in scenes:
+        s.render()
     pg.hip.synchronize()
     t0 = time.perf_counter()
     for s in scenes:
         s.render()
     seq_s = time.perf_counter() - t0
     seq_rays = sum(s.info()["last_rays"] for s in scenes)
+    pg.render_batch(scenes, streams=args.batch_streams)
     res = pg.render_batch(scenes, streams=args.batch_streams)
     ext = sum(int(s.info()["last_extension_rays"]) for s in scenes)
     roof = queue_roofline({"last_extension_rays": ext, "last_rays": res["rays"]}, samples, res["ms"])
@@ -828,7 +832,7 @@ def main():
         log(f"[bench] spheres: {pt['samples_per_s'] / 1e6:.1f} M samples/s")
         # The same scene with four wavelengths per path (spectral mode).
         sc = pg.Scene(text=text, directory=work.name)
-        sc.render(spp=2, spectral=True)
+        sc.render(spectral=True)  # warm, untimed
         sc.render(spectral=True)
         sms = sc.info()["last_render_ms"]
         sc.release()

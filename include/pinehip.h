@@ -273,8 +273,10 @@ int pg_render(pg_scene* s, const pg_render_options* opts, void* stream);
  * of PixelIntegrator::Render (integrator.cpp:121-150) over many scenes; pine
  * renders one scene per process.  Synchronous. */
 typedef struct pg_batch_options {
-    int32_t streams;  /* concurrent streams, 1..16 (0 = 4) */
-    int32_t flags;    /* PG_RENDER_* bits for every scene */
+    int32_t streams;          /* concurrent streams, 1..16 (0 = 4) */
+    int32_t flags;            /* PG_RENDER_* bits for every scene */
+    int64_t paths_per_batch;  /* at most this many paths in one batch of any scene
+                                 (0 = each scene's own paths_in_flight) */
 } pg_batch_options;
 typedef struct pg_batch_result {
     double ms;        /* wall time of the batch on `stream` */

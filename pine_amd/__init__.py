@@ -97,7 +97,7 @@ class _RenderOptions(ctypes.Structure):
 
 
 class _BatchOptions(ctypes.Structure):
-    _fields_ = [("streams", ctypes.c_int32), ("flags", ctypes.c_int32)]
+    _fields_ = [("streams", ctypes.c_int32), ("flags", ctypes.c_int32), ("paths_per_batch", ctypes.c_int64)]
 
 
 class _BatchResult(ctypes.Structure):
@@ -439,16 +439,17 @@ class Scene:
             pass
 
 
-def render_batch(scenes, spp=None, streams=4, graph=True):
+def render_batch(scenes, spp=None, streams=4, graph=True, paths_per_batch=0):
     """Renders a list of Scene objects as one batch (pg_render_batch); returns
     the batch wall time, camera samples and rays.  Every scene's film and info
-    hold its own result afterwards."""
+    hold its own result afterwards.  paths_per_batch > 0 caps every scene's
+    batch (0: each scene's own paths in flight)."""
     n = len(scenes)
     arr = (ctypes.c_void_p * max(n, 1))(*[s.h for s in scenes])
     sp = None
     if spp is not None:
         sp = (ctypes.c_int32 * max(n, 1))(*[int(x) for x in spp])
-    o, r = _BatchOptions(streams, 0 if graph else RENDER_NO_GRAPH), _BatchResult()
+    o, r = _BatchOptions(streams, 0 if graph else RENDER_NO_GRAPH, int(paths_per_batch)), _BatchResult()
     check(lib().pg_render_batch(arr, n, sp, ctypes.byref(o), _stream_handle(), ctypes.byref(r)))
     return {k: getattr(r, k) for k, _ in _BatchResult._fields_}
 

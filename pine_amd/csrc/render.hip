@@ -28,10 +28,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -153,9 +157,7 @@ struct PathsDev {
     int32_t* occluded;
     // Film contributions a sample makes to a neighbouring pixel (a sample on a
     // pixel border): (pixel, w*r, w*g, w*b, w), applied after the main pass.
-    int32_t* spill_count;
-    float* spill;  // 5 floats per entry (pixel index stored as bits)
-    int32_t spill_capacity;
+    uint8_t* border;  // per pixel: 1 if one of its samples also feeds a neighbour (k_film)
     // General mode (media or material-less shapes): the bounce is split at the
     // shadow ray, because GridMedium::Tr consumes sampler dimensions between
     // the light sample and the BSDF / phase sample.
@@ -2468,6 +2470,7 @@ __global__ __launch_bounds__(256) void k_film(SceneDev s, PathsDev P, int32_t n_
     const int px = pixel % s.width, py = pixel / s.width;
     const size_t own = (size_t)(s.height - 1 - py) * s.width + px;
     float acc[4] = {film[4 * own + 0], film[4 * own + 1], film[4 * own + 2], film[4 * own + 3]};
+    bool border = false;
     for (int sl = 0; sl < spb; sl++) {
         const int32_t i = pixel * spb + sl;
         const float fx = P.pfilm[2 * i] * (float)s.width - 0.5f;
@@ -2493,19 +2496,67 @@ __global__ __launch_bounds__(256) void k_film(SceneDev s, PathsDev P, int32_t n_
                     acc[2] += L2 * w;
                     acc[3] += w;
                 } else {
-                    // A sample exactly on a pixel border also feeds the neighbour;
-                    // the neighbour's own pass may be running, so defer it.
-                    int slot = atomicAdd(P.spill_count, 1);
-                    if (slot < P.spill_capacity) {
-                        P.spill[5 * slot + 0] = __int_as_float((int)pix);
-                        P.spill[5 * slot + 1] = L0 * w;
-                        P.spill[5 * slot + 2] = L1 * w;
-                        P.spill[5 * slot + 3] = L2 * w;
-                        P.spill[5 * slot + 4] = w;
-                    }
+                    // A sample exactly on a pixel border also feeds the
+                    // neighbour; the neighbour's own pass may be running, so
+                    // k_film_border adds it afterwards.
+                    border = true;
                 }
             }
     }
+    film[4 * own + 0] = acc[0];
+    film[4 * own + 1] = acc[1];
+    film[4 * own + 2] = acc[2];
+    film[4 * own + 3] = acc[3];
+    P.border[pixel] = border;
+}
+
+// The border samples k_film left out, as a gather: each film pixel visits the
+// (at most eight) neighbours k_film marked, finds their samples whose
+// [Ceil(pf - r), Floor(pf + r)] footprint (computed as k_film computes it)
+// covers it, and adds them in a fixed order (neighbour rows, columns, samples),
+// so the film does not depend on scheduling.  Filters of radius at most 1/2:
+// a sample lies within half a pixel (plus rounding) of its own pixel's centre,
+// so its footprint reaches one pixel at most.
+__global__ __launch_bounds__(256) void k_film_border(SceneDev s, PathsDev P, int32_t n_pixels, int32_t spb,
+                                                     float* film) {
+    const int32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pixel >= n_pixels)
+        return;
+    const int x = pixel % s.width, y = pixel / s.width;
+    const int qy0 = y > 0 ? y - 1 : 0, qy1 = y < s.height - 1 ? y + 1 : y;
+    const int qx0 = x > 0 ? x - 1 : 0, qx1 = x < s.width - 1 ? x + 1 : x;
+    bool any = false;
+    for (int qy = qy0; qy <= qy1; qy++)
+        for (int qx = qx0; qx <= qx1; qx++)
+            any |= (qx != x || qy != y) && P.border[qy * s.width + qx];
+    if (!any)
+        return;
+    const float rx = s.filter_radius[0], ry = s.filter_radius[1];
+    const size_t own = (size_t)(s.height - 1 - y) * s.width + x;
+    float acc[4] = {film[4 * own + 0], film[4 * own + 1], film[4 * own + 2], film[4 * own + 3]};
+    for (int qy = qy0; qy <= qy1; qy++)
+        for (int qx = qx0; qx <= qx1; qx++) {
+            const int32_t q = qy * s.width + qx;
+            if ((qx == x && qy == y) || !P.border[q])
+                continue;
+            for (int sl = 0; sl < spb; sl++) {
+                const int32_t i = q * spb + sl;
+                const float fx = P.pfilm[2 * i] * (float)s.width - 0.5f;
+                const float fy = P.pfilm[2 * i + 1] * (float)s.height - 0.5f;
+                const int x0 = (int)pstd::ceil(fx - rx), y0 = (int)pstd::ceil(fy - ry);
+                const int x1 = (int)pstd::floor(fx + rx), y1 = (int)pstd::floor(fy + ry);
+                if (x < x0 || x > x1 || y < y0 || y > y1)
+                    continue;
+                const float dx = fabsp((float)x - fx) / rx, dy = fabsp((float)y - fy) / ry;
+                const int tx = (int)(16.0f * (dx < kOneMinusEps ? dx : kOneMinusEps));
+                const int ty = (int)(16.0f * (dy < kOneMinusEps ? dy : kOneMinusEps));
+                const float w = s.filter_table[ty * 16 + tx];
+                acc[0] += P.L[3 * i] * w;
+                acc[1] += P.L[3 * i + 1] * w;
+                acc[2] += P.L[3 * i + 2] * w;
+                acc[3] += w;
+            }
+        }
     film[4 * own + 0] = acc[0];
     film[4 * own + 1] = acc[1];
     film[4 * own + 2] = acc[2];
@@ -2555,23 +2606,31 @@ __global__ __launch_bounds__(256) void k_film_gather(SceneDev s, PathsDev P, int
     film[4 * own + 3] = acc[3];
 }
 
-__global__ __launch_bounds__(256) void k_film_spill(PathsDev P, float* film) {
-    const int32_t n = min(*P.spill_count, P.spill_capacity);
-    for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        const size_t pix = (size_t)__float_as_int(P.spill[5 * k]);
-        atomicAdd(&film[4 * pix + 0], P.spill[5 * k + 1]);
-        atomicAdd(&film[4 * pix + 1], P.spill[5 * k + 2]);
-        atomicAdd(&film[4 * pix + 2], P.spill[5 * k + 3]);
-        atomicAdd(&film[4 * pix + 3], P.spill[5 * k + 4]);
-    }
-}
-
 }  // namespace
 
 // ----------------------------------------------------------------------------
 // Host: scene loading (fileio.cpp:251-281 and the Create* factories)
 
 enum { kIntPath = 0, kIntAO = 1, kIntViz = 2, kIntRandomWalk = 3 };
+
+// Per-path buffers of one batch.  Scenes do not own them: each device keeps a
+// few sets (slot 0 for single-scene renders, one per stream of a batched
+// render), reused by every scene rendered on that slot and grown on demand, so
+// the memory held is bounded by the slots' largest batches, not by the number
+// of live scenes.  Each kind of buffer has its own capacity, so one set serves
+// fused, split-bounce, node-material, spectral and guided scenes alike.
+struct PathState {
+    PathsDev P{};
+    int64_t core = 0, general = 0, nodes = 0, spectral = 0, guide = 0, records = 0;
+    std::vector<void*> core_allocs, general_allocs, node_allocs, spectral_allocs, guide_allocs, record_allocs;
+    int64_t gen = 0;  // changes with every reallocation, unique over all sets
+    ~PathState() {
+        for (const std::vector<void*>* l :
+             {&core_allocs, &general_allocs, &node_allocs, &spectral_allocs, &guide_allocs, &record_allocs})
+            for (void* p : *l)
+                (void)hipFree(p);
+    }
+};
 
 struct pg_scene {
     std::string dir;
@@ -2599,8 +2658,9 @@ struct pg_scene {
     // device buffers owned by the scene
     std::vector<void*> allocations;
     float* d_film = nullptr;
-    PathsDev paths{};
-    int64_t path_capacity = 0;
+    PathsDev paths{};   // the current render's view of its path-state slot
+    int device = -1;    // device the scene was loaded on (-1 until registered)
+    unsigned long long* d_lookups = nullptr;  // density lookups of the last render
     int32_t* d_viz_counts = nullptr;  // Viz Bvh: nodes and primitives per camera ray
     int64_t viz_counts_capacity = 0;
     double last_render_ms = 0.0;
@@ -2613,7 +2673,6 @@ struct pg_scene {
     bool general = false;
     int materialless_shapes = 0;  // shapes without a material (the split bounce's pass-throughs)
     float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};  // bounds of the finite shapes
-    int64_t guide_capacity = 0, record_capacity = 0;
     double last_guide_ms = 0.0;
     int64_t guide_steps = 0, guide_records = 0;
     // The fused pipeline's bounce loop recorded as a HIP graph (counts stay in
@@ -2633,16 +2692,19 @@ struct pg_scene {
     bool keep_film = false;  // the next render adds to the film instead of clearing it
     int integrator = 0;                  // kIntPath / kIntAO / kIntViz / kIntRandomWalk
     bool spectral = false;               // this render in spectral mode (PG_RENDER_SPECTRAL)
-    int64_t spectral_capacity = 0;
     GraphKey graph_key;
     hipGraphExec_t graph_exec = nullptr;
     hipStream_t capture_stream = nullptr;
     int32_t* g_counts = nullptr;         // 4 per bounce after the batch's path count
     unsigned long long* g_rays = nullptr;  // rays traced: [0] extension + shadow, [1] extension
     int64_t last_ext_rays = 0;           // extension rays (path vertices) of the last render
-    int64_t paths_gen = 0;               // bumped when path or guide state is reallocated
+    int64_t paths_gen = 0;               // generation of the path-state slot of the current render
     int64_t graph_captures = 0;
-    int64_t batch_paths = 4 << 20;       // paths per batch when not given (tunable)
+    // Paths per batch when not given (tunable), bounded by free device memory.
+    // Large batches amortise each bounce's fixed cost (launches, and the split
+    // bounce's count read-back) over the short tail bounces, and put a pixel's
+    // samples in adjacent lanes, so a wave's rays start coherent.
+    int64_t batch_paths = 64 << 20;
     pg_build_options build_opts{};       // the accel's build options (rebuilds keep them)
     // Guide inference kernel time and queries of the last launch-by-launch
     // render (events around each k_guide_infer launch; graph replays skip it).
@@ -2666,8 +2728,62 @@ struct pg_scene {
             (void)hipFree(p);
         if (accel)
             pg_accel_release(accel);
+        unregister_paths();
     }
+    void unregister_paths();
 };
+
+namespace {
+
+// The device's path-state slots and the lock that runs its renders one at a
+// time (a render holds its slot from the first launch to the wait at its end).
+struct DevicePaths {
+    std::mutex render;
+    std::vector<std::unique_ptr<PathState>> slots;
+    int scenes = 0;
+};
+std::mutex g_paths_mu;
+std::atomic<int64_t> g_paths_gen{0};
+
+// Never destroyed: slots are freed when a device's last scene goes, not by
+// static destructors that may run after the HIP runtime has shut down.
+std::map<int, DevicePaths>& device_paths_map() {
+    static auto* m = new std::map<int, DevicePaths>();
+    return *m;
+}
+
+DevicePaths& device_paths(int dev) {
+    std::lock_guard<std::mutex> lk(g_paths_mu);
+    return device_paths_map()[dev];
+}
+
+PathState& path_slot(DevicePaths& D, int k) {
+    std::lock_guard<std::mutex> lk(g_paths_mu);
+    while ((int)D.slots.size() <= k)
+        D.slots.push_back(std::make_unique<PathState>());
+    return *D.slots[k];
+}
+
+int register_paths(pg_scene* S) {
+    int dev = 0;
+    PG_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_paths_mu);
+    device_paths_map()[dev].scenes++;
+    S->device = dev;
+    return PG_OK;
+}
+
+}  // namespace
+
+void pg_scene::unregister_paths() {
+    if (device < 0)
+        return;
+    std::lock_guard<std::mutex> lk(g_paths_mu);
+    DevicePaths& D = device_paths_map()[device];
+    if (--D.scenes == 0)
+        D.slots.clear();
+    device = -1;
+}
 
 namespace {
 
@@ -3694,11 +3810,29 @@ int upload(pg_scene& S, const std::vector<T>& v, const T** out) {
 }
 
 template <typename T>
-int dalloc(pg_scene& S, int64_t n, T** out) {
+int dalloc(std::vector<void*>& list, int64_t n, T** out) {
     void* p = nullptr;
     PG_HIP(hipMalloc(&p, std::max<int64_t>(n, 1) * sizeof(T)));
-    S.allocations.push_back(p);
+    list.push_back(p);
     *out = (T*)p;
+    return PG_OK;
+}
+
+template <typename T>
+int dalloc(pg_scene& S, int64_t n, T** out) {
+    return dalloc(S.allocations, n, out);
+}
+
+// Frees one list of per-path buffers before a larger set replaces it, after
+// the device has finished every launch that reads them.  The recorded bounce
+// graph that points at them is re-recorded first (paths_gen changes).
+int release_state(std::vector<void*>& list) {
+    if (list.empty())
+        return PG_OK;
+    PG_HIP(hipDeviceSynchronize());
+    for (void* p : list)
+        PG_HIP(hipFree(p));
+    list.clear();
     return PG_OK;
 }
 
@@ -3966,83 +4100,139 @@ int finish_scene(pg_scene& S, const pg_build_options* opts) {
     return PG_OK;
 }
 
-int ensure_paths(pg_scene& S, int64_t n) {
-    if (n <= S.path_capacity)
+// Grows one kind of per-path buffer of a slot to n entries: frees the old
+// set first (release_state waits for every launch that may read it) and moves
+// the slot to a new generation, so recorded bounce graphs are re-recorded.
+template <typename F>
+int grow_state(PathState& W, int64_t& cap, std::vector<void*>& list, int64_t n, F&& alloc) {
+    if (n <= cap)
         return PG_OK;
-    PathsDev& P = S.paths;
-    int rc;
-    if ((rc = dalloc(S, 3 * n, &P.L)) || (rc = dalloc(S, 3 * n, &P.beta)) ||
-        (rc = dalloc(S, n, &P.bsdf_pdf)) || (rc = dalloc(S, n, &P.sidx)) || (rc = dalloc(S, n, &P.sdim)) ||
-        (rc = dalloc(S, n, &P.depth)) || (rc = dalloc(S, 2 * n, &P.pfilm)) ||
-        (rc = dalloc(S, n, &P.queue_in)) || (rc = dalloc(S, n, &P.queue_out)) ||
-        (rc = dalloc(S, n, &P.rays_in)) || (rc = dalloc(S, n, &P.rays_out)) ||
-        (rc = dalloc(S, n, &P.hits)) || (rc = dalloc(S, n, &P.uvs)) || (rc = dalloc(S, 8, &P.counters)) ||
-        (rc = dalloc(S, n, &P.shadow_rays)) || (rc = dalloc(S, n, &P.shadow_path)) ||
-        (rc = dalloc(S, 3 * n, &P.shadow_contrib)) || (rc = dalloc(S, n, &P.occluded)) ||
-        (rc = dalloc(S, 1, &P.spill_count)))
+    W.gen = ++g_paths_gen;
+    cap = 0;
+    if (int rc = release_state(list))
         return rc;
-    S.paths_gen++;
-    P.spill_capacity = (int32_t)std::min<int64_t>(n, 1 << 20);
-    if ((rc = dalloc(S, 5 * (int64_t)P.spill_capacity, &P.spill)))
+    if (int rc = alloc(list))
+        return rc;
+    cap = n;
+    return PG_OK;
+}
+
+int ensure_paths(PathState& W, const pg_scene& S, int64_t n) {
+    PathsDev& P = W.P;
+    int rc = grow_state(W, W.core, W.core_allocs, n, [&](std::vector<void*>& A) {
+        int r;
+        if ((r = dalloc(A, 3 * n, &P.L)) || (r = dalloc(A, 3 * n, &P.beta)) ||
+            (r = dalloc(A, n, &P.bsdf_pdf)) || (r = dalloc(A, n, &P.sidx)) || (r = dalloc(A, n, &P.sdim)) ||
+            (r = dalloc(A, n, &P.depth)) || (r = dalloc(A, 2 * n, &P.pfilm)) ||
+            (r = dalloc(A, n, &P.queue_in)) || (r = dalloc(A, n, &P.queue_out)) ||
+            (r = dalloc(A, n, &P.rays_in)) || (r = dalloc(A, n, &P.rays_out)) ||
+            (r = dalloc(A, n, &P.hits)) || (r = dalloc(A, n, &P.uvs)) || (r = dalloc(A, 8, &P.counters)) ||
+            (r = dalloc(A, n, &P.shadow_rays)) || (r = dalloc(A, n, &P.shadow_path)) ||
+            (r = dalloc(A, 3 * n, &P.shadow_contrib)) || (r = dalloc(A, n, &P.occluded)) ||
+            (r = dalloc(A, n, &P.border)))
+            return r;
+        return PG_OK;
+    });
+    if (rc)
         return rc;
     if (S.general) {
-        if ((rc = dalloc(S, n, &P.medium)) || (rc = dalloc(S, 9 * n, &P.vtx)) ||
-            (rc = dalloc(S, 4 * n, &P.vinfo)) || (rc = dalloc(S, n, &P.queue_b)) ||
-            (rc = dalloc(S, 16 * n, &P.sh_rec)) || (rc = dalloc(S, 3 * n, &P.sh_tr)) ||
-            (rc = dalloc(S, n, &P.seg_rays[0])) || (rc = dalloc(S, n, &P.seg_rays[1])) ||
-            (rc = dalloc(S, n, &P.seg_rec[0])) || (rc = dalloc(S, n, &P.seg_rec[1])) ||
-            (rc = dalloc(S, n, &P.seg_med[0])) || (rc = dalloc(S, n, &P.seg_med[1])) ||
-            (rc = dalloc(S, n, &P.seg_hits)) || (rc = dalloc(S, n, &P.seg_uvs)) ||
-            (rc = dalloc(S, 1, &P.lookups)) || (rc = dalloc(S, 2 * n, &P.mev)) ||
-            (rc = dalloc(S, n, &P.seg_trf)))
+        rc = grow_state(W, W.general, W.general_allocs, n, [&](std::vector<void*>& A) {
+            int r;
+            if ((r = dalloc(A, n, &P.medium)) || (r = dalloc(A, 9 * n, &P.vtx)) ||
+                (r = dalloc(A, 4 * n, &P.vinfo)) || (r = dalloc(A, n, &P.queue_b)) ||
+                (r = dalloc(A, 16 * n, &P.sh_rec)) || (r = dalloc(A, 3 * n, &P.sh_tr)) ||
+                (r = dalloc(A, n, &P.seg_rays[0])) || (r = dalloc(A, n, &P.seg_rays[1])) ||
+                (r = dalloc(A, n, &P.seg_rec[0])) || (r = dalloc(A, n, &P.seg_rec[1])) ||
+                (r = dalloc(A, n, &P.seg_med[0])) || (r = dalloc(A, n, &P.seg_med[1])) ||
+                (r = dalloc(A, n, &P.seg_hits)) || (r = dalloc(A, n, &P.seg_uvs)) ||
+                (r = dalloc(A, 2 * n, &P.mev)) || (r = dalloc(A, n, &P.seg_trf)))
+                return r;
+            return PG_OK;
+        });
+        if (rc)
             return rc;
     }
     if (!S.node_progs.empty()) {
-        if ((rc = dalloc(S, n, &P.node_mat)) || (rc = dalloc(S, nodes::kStack * n, &P.node_stack)))
-            return rc;
+        rc = grow_state(W, W.nodes, W.node_allocs, n, [&](std::vector<void*>& A) {
+            int r;
+            if ((r = dalloc(A, n, &P.node_mat)) || (r = dalloc(A, nodes::kStack * n, &P.node_stack)))
+                return r;
+            return PG_OK;
+        });
     }
-    S.path_capacity = n;
-    return PG_OK;
+    return rc;
 }
 
-int ensure_spectral(pg_scene& S, int64_t n) {
-    if (n <= S.spectral_capacity)
+int ensure_spectral(PathState& W, int64_t n) {
+    PathsDev& P = W.P;
+    return grow_state(W, W.spectral, W.spectral_allocs, n, [&](std::vector<void*>& A) {
+        int r;
+        if ((r = dalloc(A, n, &P.lam)) || (r = dalloc(A, 4 * n, &P.L4)) || (r = dalloc(A, 4 * n, &P.beta4)) ||
+            (r = dalloc(A, 4 * n, &P.sh4)))
+            return r;
         return PG_OK;
-    PathsDev& P = S.paths;
-    int rc;
-    if ((rc = dalloc(S, n, &P.lam)) || (rc = dalloc(S, 4 * n, &P.L4)) || (rc = dalloc(S, 4 * n, &P.beta4)) ||
-        (rc = dalloc(S, 4 * n, &P.sh4)))
-        return rc;
-    S.spectral_capacity = n;
-    S.paths_gen++;
-    return PG_OK;
+    });
 }
 
-int ensure_guide(pg_scene& S, int64_t n, int rec_depth) {
-    PathsDev& P = S.paths;
-    int rc;
-    if (n > S.guide_capacity) {
-        if ((rc = dalloc(S, (int64_t)guide::kParamStride * n, &P.gparams)) || (rc = dalloc(S, n, &P.gflag)) ||
-            (rc = dalloc(S, n, &P.shadow_rec)) || (rc = dalloc(S, (int64_t)guide::kIn * n, &P.gx)))
-            return rc;
-        S.guide_capacity = n;
-        S.paths_gen++;
-    }
+int ensure_guide(PathState& W, int64_t n, int rec_depth) {
+    PathsDev& P = W.P;
+    int rc = grow_state(W, W.guide, W.guide_allocs, n, [&](std::vector<void*>& A) {
+        int r;
+        if ((r = dalloc(A, (int64_t)guide::kParamStride * n, &P.gparams)) || (r = dalloc(A, n, &P.gflag)) ||
+            (r = dalloc(A, n, &P.shadow_rec)) || (r = dalloc(A, (int64_t)guide::kIn * n, &P.gx)))
+            return r;
+        return PG_OK;
+    });
+    if (rc)
+        return rc;
     const int64_t slots = n * rec_depth;
-    if (slots > S.record_capacity) {
-        S.paths_gen++;
-        if ((rc = dalloc(S, (int64_t)guide::kIn * slots, &P.rec_x)) || (rc = dalloc(S, 3 * slots, &P.rec_dir)) ||
-            (rc = dalloc(S, slots, &P.rec_w)) || (rc = dalloc(S, slots, &P.rec_pb)) ||
-            (rc = dalloc(S, slots, &P.rec_pdf)) || (rc = dalloc(S, 3 * slots, &P.rec_lb)) ||
-            (rc = dalloc(S, 3 * slots, &P.rec_beta)) || (rc = dalloc(S, 2, &P.rec_acc)) ||
-            (rc = dalloc(S, (int64_t)guide::kIn * slots, &P.c_x)) || (rc = dalloc(S, 3 * slots, &P.c_dir)) ||
-            (rc = dalloc(S, slots, &P.c_w)) || (rc = dalloc(S, slots, &P.c_pb)) ||
-            (rc = dalloc(S, (slots + 255) / 256, &P.rec_bcnt)) || (rc = dalloc(S, (slots + 255) / 256, &P.rec_bsum)) ||
-            (rc = dalloc(S, (slots + 255) / 256, &P.rec_boff)) || (rc = dalloc(S, 1, &P.rec_n)))
-            return rc;
-        S.record_capacity = slots;
+    return grow_state(W, W.records, W.record_allocs, slots, [&](std::vector<void*>& A) {
+        int r;
+        if ((r = dalloc(A, (int64_t)guide::kIn * slots, &P.rec_x)) || (r = dalloc(A, 3 * slots, &P.rec_dir)) ||
+            (r = dalloc(A, slots, &P.rec_w)) || (r = dalloc(A, slots, &P.rec_pb)) ||
+            (r = dalloc(A, slots, &P.rec_pdf)) || (r = dalloc(A, 3 * slots, &P.rec_lb)) ||
+            (r = dalloc(A, 3 * slots, &P.rec_beta)) || (r = dalloc(A, 2, &P.rec_acc)) ||
+            (r = dalloc(A, (int64_t)guide::kIn * slots, &P.c_x)) || (r = dalloc(A, 3 * slots, &P.c_dir)) ||
+            (r = dalloc(A, slots, &P.c_w)) || (r = dalloc(A, slots, &P.c_pb)) ||
+            (r = dalloc(A, (slots + 255) / 256, &P.rec_bcnt)) || (r = dalloc(A, (slots + 255) / 256, &P.rec_bsum)) ||
+            (r = dalloc(A, (slots + 255) / 256, &P.rec_boff)) || (r = dalloc(A, 1, &P.rec_n)))
+            return r;
+        return PG_OK;
+    });
+}
+
+// Device bytes per path of the buffers ensure_paths, ensure_spectral and
+// ensure_guide allocate for a batch (rounded up; the small fixed buffers aside).
+int64_t path_state_bytes(const pg_scene& S, bool guided, int rec_depth) {
+    const int64_t ray = sizeof(pg_ray), hit = sizeof(pg_hit) + sizeof(pg_hit_uv);
+    int64_t b = 4 * (3 + 3 + 1 + 2 + 3) + 8 + 4 * (1 + 1 + 2 + 1 + 1) + 3 * ray + hit + 1;
+    if (S.general)
+        b += 4 * (1 + 9 + 4 + 1 + 16 + 3 + 2 + 2 + 1) + 2 * ray + hit + 2 * sizeof(float4);
+    if (!S.node_progs.empty())
+        b += sizeof(MaterialDev) + nodes::kStack * sizeof(f3);
+    if (S.spectral)
+        b += 4 * (1 + 4 + 4 + 4);
+    if (guided)
+        b += 4 * (guide::kParamStride + 1 + 1 + guide::kIn) +
+             (int64_t)rec_depth * 4 * (2 * guide::kIn + 3 + 3 + 1 + 1 + 1 + 3 + 3 + 1 + 1);
+    return b;
+}
+
+// Samples per pixel of one batch when the caller leaves it open: the scene's
+// batch_paths, at most half of the free device memory (counting what the
+// slot's buffers would free on growing), at least one sample per pixel.
+int auto_batch_spp(const PathState& W, const pg_scene& S, int spp, int64_t npix, bool guided, int rec_depth) {
+    int64_t want = std::min<int64_t>((int64_t)spp * npix, S.batch_paths);
+    const int64_t per_path = path_state_bytes(S, guided, rec_depth);
+    if (want > W.core) {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+            const int64_t held = W.core * per_path;
+            want = std::min<int64_t>(want, ((int64_t)free_b + held) / 2 / per_path);
+        }
     }
-    return PG_OK;
+    want = std::min<int64_t>(want, INT32_MAX);
+    return (int)std::max<int64_t>(1, std::min<int64_t>(spp, want / std::max<int64_t>(npix, 1)));
 }
 
 }  // namespace
@@ -4060,6 +4250,10 @@ static int load_common(pg_scene* S, const std::string& text, pg_scene** out,
         return pg_fail(PG_E_PARSE, e.what());
     }
     if (int rc = finish_scene(*S, opts)) {
+        delete S;
+        return rc;
+    }
+    if (int rc = register_paths(S)) {
         delete S;
         return rc;
     }
@@ -4449,30 +4643,39 @@ struct RenderTicket {
 // Enqueues a whole render on st.  Without training and in the fused pipeline
 // nothing waits on the host (graph replay per batch); training passes and the
 // split bounce read counts back per bounce or per batch.
-static int render_enqueue(pg_scene* S, int spp, int spb_req, hipStream_t st, GuideTraining* tr,
+// W: the path-state slot the render uses; the caller holds the device's
+// render lock until the render has finished on its stream.
+static int render_enqueue(pg_scene* S, PathState& W, int spp, int spb_req, hipStream_t st, GuideTraining* tr,
                           int sample_offset, RenderTicket* t) {
     const int64_t npix = (int64_t)S->dev.width * S->dev.height;
-    int spb = spb_req;
-    if (spb <= 0) {
-        // About 4M paths per batch (unless tuned): enough lanes for 256 CUs, bounded state.
-        spb = (int)std::max<int64_t>(1, std::min<int64_t>(spp, S->batch_paths / std::max<int64_t>(npix, 1)));
-    }
+    if (S->spectral && S->dev.guided)
+        return pg_fail(PG_E_ARG, "spectral rendering is not combined with the learned guide");
+    int spb = spb_req > 0 ? spb_req : auto_batch_spp(W, *S, spp, npix, S->dev.guided, tr ? tr->rec_depth : 0);
     spb = std::min(spb, spp);
     const int64_t n_paths = npix * spb;
     if (n_paths > (int64_t)INT32_MAX)
         return pg_fail(PG_E_ARG, "pg_render: batch too large");
-    if (int rc = ensure_paths(*S, n_paths))
+    if (int rc = ensure_paths(W, *S, n_paths))
         return rc;
     if (S->dev.guided) {
-        if (int rc = ensure_guide(*S, n_paths, tr ? tr->rec_depth : 0))
+        if (int rc = ensure_guide(W, n_paths, tr ? tr->rec_depth : 0))
             return rc;
     }
     if (S->spectral) {
-        if (S->dev.guided)
-            return pg_fail(PG_E_ARG, "spectral rendering is not combined with the learned guide");
-        if (int rc = ensure_spectral(*S, n_paths))
+        if (int rc = ensure_spectral(W, n_paths))
             return rc;
     }
+    if (S->general && !S->d_lookups) {
+        if (int rc = dalloc(*S, 1, &S->d_lookups))
+            return rc;
+    }
+    // The scene's view of the slot: buffers of kinds this scene does not use
+    // are left out (k_raygen writes a path's medium only where one is kept).
+    S->paths = W.P;
+    S->paths.lookups = S->general ? S->d_lookups : nullptr;
+    if (!S->general)
+        S->paths.medium = nullptr;
+    S->paths_gen = W.gen;
     PG_HIP(hipEventCreate(&t->e0));
     PG_HIP(hipEventCreate(&t->e1));
     PG_HIP(hipEventRecord(t->e0, st));
@@ -4675,10 +4878,10 @@ static int render_enqueue(pg_scene* S, int spp, int spb_req, hipStream_t st, Gui
             hipLaunchKernelGGL(k_film_gather, dim3((npix + B - 1) / B), dim3(B), 0, st, S->dev, P, (int32_t)npix,
                                nb, S->d_film);
         } else {
-            PG_HIP(hipMemsetAsync(P.spill_count, 0, sizeof(int32_t), st));
             hipLaunchKernelGGL(k_film, dim3((npix + B - 1) / B), dim3(B), 0, st, S->dev, P, (int32_t)npix,
                                nb, S->d_film);
-            hipLaunchKernelGGL(k_film_spill, dim3(64), dim3(B), 0, st, P, S->d_film);
+            hipLaunchKernelGGL(k_film_border, dim3((npix + B - 1) / B), dim3(B), 0, st, S->dev, P, (int32_t)npix,
+                               nb, S->d_film);
         }
     }
     PG_HIP(hipGetLastError());
@@ -4706,18 +4909,26 @@ static int render_finish(pg_scene* S, RenderTicket* t, double* ms_out, int64_t* 
     *ms_out = ms;
     *rays_out = rays;
     S->last_lookups = 0;
-    if (S->paths.lookups) {
+    if (S->d_lookups) {
         unsigned long long n = 0;
-        PG_HIP(hipMemcpy(&n, S->paths.lookups, sizeof(n), hipMemcpyDeviceToHost));
+        PG_HIP(hipMemcpy(&n, S->d_lookups, sizeof(n), hipMemcpyDeviceToHost));
         S->last_lookups = (int64_t)n;
     }
     return PG_OK;
 }
 
-static int render_impl(pg_scene* S, int spp, int spb_req, hipStream_t st, GuideTraining* tr,
+// Holds the device's render lock for one entry point's renders.
+struct RenderLock {
+    DevicePaths& D;
+    std::lock_guard<std::mutex> lk;
+    explicit RenderLock(const pg_scene* S) : D(device_paths(S->device)), lk(D.render) {}
+    PathState& slot(int k) { return path_slot(D, k); }
+};
+
+static int render_impl(pg_scene* S, PathState& W, int spp, int spb_req, hipStream_t st, GuideTraining* tr,
                        double* ms_out, int64_t* rays_out, int sample_offset = 0) {
     RenderTicket t;
-    if (int rc = render_enqueue(S, spp, spb_req, st, tr, sample_offset, &t)) {
+    if (int rc = render_enqueue(S, W, spp, spb_req, st, tr, sample_offset, &t)) {
         (void)hipStreamSynchronize(st);
         t.release();
         return rc;
@@ -4740,7 +4951,8 @@ extern "C" int pg_render(pg_scene* S, const pg_render_options* opts, void* strea
     S->lane_tracking = opts && (opts->flags & PG_RENDER_LANE_TRACKING);
     double ms = 0.0;
     int64_t rays = 0;
-    const int rc = render_impl(S, spp, spb, (hipStream_t)stream, nullptr, &ms, &rays, offset);
+    RenderLock lock(S);
+    const int rc = render_impl(S, lock.slot(0), spp, spb, (hipStream_t)stream, nullptr, &ms, &rays, offset);
     S->use_graph = true;
     S->spectral = false;
     S->lane_tracking = false;
@@ -4772,8 +4984,20 @@ extern "C" int pg_render_batch(pg_scene* const* scenes, int32_t n, const int32_t
     for (int32_t i = 0; i < n; i++)
         if (!scenes[i])
             return pg_fail(PG_E_ARG, "pg_render_batch: null scene");
+        else if (scenes[i]->device != scenes[0]->device)
+            return pg_fail(PG_E_ARG, "pg_render_batch: scenes loaded on different devices");
+    if (n == 0) {
+        if (out)
+            *out = pg_batch_result{};
+        return PG_OK;
+    }
+    // Stream k renders its scenes one after another on path-state slot k + 1.
+    RenderLock lock(scenes[0]);
     const int K = o && o->streams > 0 ? std::min(o->streams, 16) : 4;
     const int flags = o ? o->flags : 0;
+    if (o && o->paths_per_batch < 0)
+        return pg_fail(PG_E_ARG, "pg_render_batch: paths_per_batch must be >= 0");
+    const int64_t cap = o && o->paths_per_batch > 0 ? o->paths_per_batch : 0;
     hipStream_t st = (hipStream_t)stream;
     std::vector<hipStream_t> ws(K, nullptr);
     std::vector<hipEvent_t> done(K, nullptr);
@@ -4826,7 +5050,11 @@ extern "C" int pg_render_batch(pg_scene* const* scenes, int32_t n, const int32_t
         S->dev.rr_depth = 2;
         S->use_graph = !(flags & PG_RENDER_NO_GRAPH);
         S->lane_tracking = (flags & PG_RENDER_LANE_TRACKING) != 0;
-        rc = render_enqueue(S, sp, 0, ws[k], nullptr, 0, &tickets[i]);
+        const int64_t own = S->batch_paths;
+        if (cap > 0)
+            S->batch_paths = std::min(own, cap);
+        rc = render_enqueue(S, lock.slot(k + 1), sp, 0, ws[k], nullptr, 0, &tickets[i]);
+        S->batch_paths = own;
         S->use_graph = true;
         S->lane_tracking = false;
     }
@@ -4865,11 +5093,11 @@ extern "C" int pg_render_batch(pg_scene* const* scenes, int32_t n, const int32_t
 }
 
 // One probe render of the scene as configured; its time on the stream.
-static int probe(pg_scene* S, int spp, hipStream_t st, double* ms) {
+static int probe(pg_scene* S, PathState& W, int spp, hipStream_t st, double* ms) {
     S->dev.guided = 0;
     S->dev.rr_depth = 2;
     int64_t rays = 0;
-    return render_impl(S, spp, 0, st, nullptr, ms, &rays, 0);
+    return render_impl(S, W, spp, 0, st, nullptr, ms, &rays, 0);
 }
 
 extern "C" int pg_scene_autotune(pg_scene* S, const pg_tune_options* o, void* stream, pg_tune_result* out) {
@@ -4880,11 +5108,13 @@ extern "C" int pg_scene_autotune(pg_scene* S, const pg_tune_options* o, void* st
     pg_tune_result r{};
     int rc;
     double ms = 0.0, best = 0.0;
+    RenderLock lock(S);
+    PathState& W = lock.slot(0);
     auto measure = [&](int spp, double* t) -> int {
         double a = 0.0, b = 0.0;
-        if (int e = probe(S, spp, st, &a))  // first run records the graph
+        if (int e = probe(S, W, spp, st, &a))  // first run records the graph
             return e;
-        if (int e = probe(S, spp, st, &b))
+        if (int e = probe(S, W, spp, st, &b))
             return e;
         r.probes += 2;
         *t = std::min(a, b);
@@ -4933,7 +5163,7 @@ extern "C" int pg_scene_autotune(pg_scene* S, const pg_tune_options* o, void* st
     pg_accel_set_launch(S->accel, best_block, best_flags);
     // Paths in flight: probes large enough to fill the largest candidate.
     const int64_t npix = (int64_t)S->dev.width * S->dev.height;
-    const int64_t cands[4] = {1 << 20, 2 << 20, 4 << 20, 8 << 20};
+    const int64_t cands[4] = {4 << 20, 16 << 20, 32 << 20, 64 << 20};
     const int qspp = (int)std::min<int64_t>(S->spp, std::max<int64_t>(pspp, (cands[3] + npix - 1) / npix));
     if (qspp * npix > cands[0]) {
         const int64_t start = S->batch_paths;
@@ -4980,6 +5210,8 @@ extern "C" int pg_render_guided(pg_scene* S, pg_guide* G, const pg_guided_option
     const int final_spp = o && o->final_spp > 0 ? o->final_spp : S->spp;
     const int offset = o && o->sample_offset > 0 ? o->sample_offset : 0;
     const int keep = o && o->keep_passes > 0 ? std::min(o->keep_passes, passes) : 0;
+    RenderLock lock(S);
+    PathState& W = lock.slot(0);
     SceneDev& d = S->dev;
     d.g_depth = o && o->guide_depth > 0 ? o->guide_depth : (1 << 30);
     d.rr_depth = o && o->rr_depth > 0 ? o->rr_depth : 2;
@@ -5004,14 +5236,14 @@ extern "C" int pg_render_guided(pg_scene* S, pg_guide* G, const pg_guided_option
     // cover one contiguous range of sample indices.
     for (int p = 0; p < passes && rc == PG_OK; p++) {
         S->keep_film = p > passes - keep;
-        rc = render_impl(S, train_spp, 0, st, &tr, &ms, &rays, offset + p * train_spp);
+        rc = render_impl(S, W, train_spp, (int)per_batch, st, &tr, &ms, &rays, offset + p * train_spp);
         train_ms += ms;
         all_rays += rays;
     }
     if (rc == PG_OK) {
         S->use_graph = !(o && (o->flags & PG_RENDER_NO_GRAPH));
         S->keep_film = keep > 0;
-        rc = render_impl(S, final_spp, 0, st, nullptr, &ms, &rays, offset + passes * train_spp);
+        rc = render_impl(S, W, final_spp, 0, st, nullptr, &ms, &rays, offset + passes * train_spp);
         S->use_graph = true;
     }
     S->keep_film = false;

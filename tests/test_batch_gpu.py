@@ -88,6 +88,28 @@ def test_batch_renders_each_scene_as_alone(pg, cuda, scenes):
         s.release()
 
 
+def test_batch_paths_cap_matches_the_same_split_alone(pg, cuda, scenes):
+    """paths_per_batch caps each scene's batch: a scene then renders in
+    batches of floor(cap / pixels) samples per pixel, as it does alone with
+    that samples_per_batch (same film bit for bit, same rays); one sample per
+    pixel is the floor."""
+    texts, d = scenes
+    ss = [pg.Scene(text=t, directory=d) for t in texts]
+    cap = 3 * max(s.info()["width"] * s.info()["height"] for s in ss) // 2
+    pg.render_batch(ss, streams=2, paths_per_batch=cap)
+    got = [(s.film(), s.info()["last_rays"]) for s in ss]
+    for s, (film, rays) in zip(ss, got):
+        inf = s.info()
+        spb = max(1, cap // (inf["width"] * inf["height"]))
+        s.render(samples_per_batch=spb)
+        np.testing.assert_array_equal(s.film(), film)
+        assert s.info()["last_rays"] == rays
+    with pytest.raises(pg.PineError):
+        pg.render_batch(ss, paths_per_batch=-1)
+    for s in ss:
+        s.release()
+
+
 def test_autotune_keeps_the_image(pg, cuda, scenes):
     texts, d = scenes
     s = pg.Scene(text=texts[1], directory=d)
@@ -168,3 +190,36 @@ def test_full_64_scene_batch_equals_each_scene_alone(pg, cuda, tmp_path):
     print(f"64-scene batch: {res['ms']:.1f} ms, {res['samples']} samples, {res['rays']} rays")
     for s in scenes:
         s.release()
+
+
+def test_film_border_samples_are_deterministic(pg, cuda, tmp_path):
+    """Samples on a pixel border also feed the neighbouring pixel, after the
+    batch's own-pixel pass; those additions run in a fixed order, so a batch of
+    58.7M paths (1024^2 x 56 spp, one batch) gives the same film bit for bit on
+    every render, on any path-state slot, alone or batched.  The same render in
+    4-spp batches adds the border samples between batches instead, which
+    changes the rounding only of the pixels that receive them."""
+    import hashlib
+    from pine_amd import workloads as W
+    it = W.batch_workload()[63]
+    W.write_box_obj(str(tmp_path / W.BOX_OBJ))
+    v, i = W.make_mesh(it["mesh"])
+    pg.write_obj_fixed(str(tmp_path / "b63.obj"), v, i)
+    s = pg.Scene(text=W.batch_scene_text(it, "b63.obj"), directory=str(tmp_path))
+    films = []
+    for _ in range(3):
+        s.render()
+        films.append(s.film().copy())
+    assert s.info()["paths_in_flight"] >= it["width"] * it["height"] * it["spp"]
+    pg.render_batch([s], streams=1)
+    films.append(s.film().copy())
+    pg.render_batch([s], streams=2)
+    films.append(s.film().copy())
+    digests = {hashlib.sha256(f.tobytes()).hexdigest() for f in films}
+    assert len(digests) == 1
+    s.render(samples_per_batch=4)
+    small = s.film()
+    differ = np.any(small != films[0], axis=-1)
+    assert 0 < differ.sum() < 0.01 * differ.size
+    np.testing.assert_allclose(small, films[0], rtol=1e-5, atol=1e-6)
+    s.release()

@@ -350,75 +350,77 @@ __global__ __launch_bounds__(kBlock) void k_trace(const float4* __restrict__ nod
                                                   const int32_t* __restrict__ perm) {
     extern __shared__ int32_t lds_stack[];
     const int lane = threadIdx.x;
-    const int64_t i = logical_block(xcd_remap != 0) * kBlock + lane;
-    // d_n: the ray count in device memory (a launch sized for a queue's
-    // capacity, as in a captured bounce loop); the blocks past it exit.
-    if (i >= (d_n ? (int64_t)*d_n : n))
-        return;
-    // With reordering, slot i traces ray perm[i] (key order) and its result
-    // goes to that ray's slot.
-    const int64_t o = perm ? (int64_t)perm[i] : i;
-    const RayIn r = load_ray(rays, o);
-    float best_t = r.tmax, bu = 0.0f, bv = 0.0f;
-    int32_t best_prim = -1;
-    Stack st{lds_stack, lane, kBlock, 0};
-    int32_t node = 0;
-    bool live = true, found_any = false;
-    int visited = 0, tested = 0;
-    if (!kPostpone) {
-        while (live) {
-            if (node >= 0) {
-                visited++;
-                visit<W>(nodes, r, kMode == kAnyHit ? r.tmax : best_t, st, node, live);
-            } else {
-                if (leaf_test<kMode>(node, tris, ana, r, best_t, best_prim, bu, bv, tested)) {
-                    found_any = true;
-                    break;
-                }
-                live = st.pop(node);
-            }
-        }
-    } else {
-        int32_t leaf = 0;  // postponed leaf code; leaf codes are negative, 0 = none
-        while (live) {
-            while (live && node >= 0) {
-                visited++;
-                visit<W>(nodes, r, kMode == kAnyHit ? r.tmax : best_t, st, node, live);
-                if (live && node < 0 && leaf == 0) {
-                    leaf = node;
-                    live = st.pop(node);
-                }
-                if (__all(leaf != 0))
-                    break;
-            }
-            while (leaf != 0) {
-                if (leaf_test<kMode>(leaf, tris, ana, r, best_t, best_prim, bu, bv, tested)) {
-                    found_any = true;
-                    live = false;
-                    break;
-                }
-                leaf = 0;
-                if (live && node < 0) {
-                    leaf = node;
+    // d_n: the ray count in device memory (a launch for a queue's capacity, as
+    // in a captured bounce loop): the grid is capped (launch_grid) and strides
+    // over the count, so a bounce with few rays costs a small launch.  With a
+    // host count the grid covers the rays and each lane traces one.
+    const int64_t cnt = d_n ? (int64_t)*d_n : n, step = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = logical_block(xcd_remap != 0) * kBlock + lane; i < cnt; i += step) {
+        // With reordering, slot i traces ray perm[i] (key order) and its result
+        // goes to that ray's slot.
+        const int64_t o = perm ? (int64_t)perm[i] : i;
+        const RayIn r = load_ray(rays, o);
+        float best_t = r.tmax, bu = 0.0f, bv = 0.0f;
+        int32_t best_prim = -1;
+        Stack st{lds_stack, lane, kBlock, 0};
+        int32_t node = 0;
+        bool live = true, found_any = false;
+        int visited = 0, tested = 0;
+        if (!kPostpone) {
+            while (live) {
+                if (node >= 0) {
+                    visited++;
+                    visit<W>(nodes, r, kMode == kAnyHit ? r.tmax : best_t, st, node, live);
+                } else {
+                    if (leaf_test<kMode>(node, tris, ana, r, best_t, best_prim, bu, bv, tested)) {
+                        found_any = true;
+                        break;
+                    }
                     live = st.pop(node);
                 }
             }
+        } else {
+            int32_t leaf = 0;  // postponed leaf code; leaf codes are negative, 0 = none
+            while (live) {
+                while (live && node >= 0) {
+                    visited++;
+                    visit<W>(nodes, r, kMode == kAnyHit ? r.tmax : best_t, st, node, live);
+                    if (live && node < 0 && leaf == 0) {
+                        leaf = node;
+                        live = st.pop(node);
+                    }
+                    if (__all(leaf != 0))
+                        break;
+                }
+                while (leaf != 0) {
+                    if (leaf_test<kMode>(leaf, tris, ana, r, best_t, best_prim, bu, bv, tested)) {
+                        found_any = true;
+                        live = false;
+                        break;
+                    }
+                    leaf = 0;
+                    if (live && node < 0) {
+                        leaf = node;
+                        live = st.pop(node);
+                    }
+                }
+            }
         }
-    }
-    if (kStats)
-        counts[o] = make_int2(visited, tested);
-    if (kMode != kClosest) {
-        occluded[o] = found_any ? 1 : 0;
-    } else {
-        pg_hit h;
-        h.prim = best_prim;
-        h.t = best_prim >= 0 ? best_t : r.tmax;
-        hits[o] = h;
-        if (uvs) {
-            pg_hit_uv uv;
-            uv.u = bu;
-            uv.v = bv;
-            uvs[o] = uv;
+        if (kStats)
+            counts[o] = make_int2(visited, tested);
+        if (kMode != kClosest) {
+            occluded[o] = found_any ? 1 : 0;
+        } else {
+            pg_hit h;
+            h.prim = best_prim;
+            h.t = best_prim >= 0 ? best_t : r.tmax;
+            hits[o] = h;
+            if (uvs) {
+                pg_hit_uv uv;
+                uv.u = bu;
+                uv.v = bv;
+                uvs[o] = uv;
+            }
         }
     }
 }
@@ -511,73 +513,73 @@ __global__ __launch_bounds__(kBlock) void k_trace_stackless(const float4* __rest
                                                             const int32_t* __restrict__ d_n,
                                                             const int32_t* __restrict__ perm,
                                                             const int2* __restrict__ keytab, int keybits) {
-    const int64_t i = logical_block(xcd_remap != 0) * kBlock + threadIdx.x;
-    if (i >= (d_n ? (int64_t)*d_n : n))
-        return;
-    // With reordering, slot i traces ray perm[i] (key order) and its result
-    // goes to that ray's slot.
-    const int64_t o = perm ? (int64_t)perm[i] : i;
-    const RayIn r = load_ray(rays, o);
-    float best_t = r.tmax, bu = 0.0f, bv = 0.0f;
-    int32_t best_prim = -1;
-    int visited = 0, tested = 0;
-    bool found_any = false;
-    int32_t node = 0;    // the current item: inner node index or leaf code
-    uint32_t key = 1u;   // its path key
-    int level = 0;       // its depth (root 0)
-    uint32_t trail = 0;  // bit L: the far child chosen at level L is pending
-    // The far child set aside last (key 0: none): the deepest pending one as
-    // long as nothing was set aside after it, which saves the table probe on
-    // about 60% of backtracks.
-    uint32_t far_key = 0u;
-    int32_t far_code = 0;
-    bool live = true;
-    while (live) {
-        if (node >= 0) {
-            visited++;
-            float4 a, b, c, d;
-            fetch2(nodes, node, level, a, b, c, d);
-            const int32_t c0 = __float_as_int(d.x), c1 = __float_as_int(d.y);
-            const float lim = kMode == kAnyHit ? r.tmax : best_t;
-            float t0, t1;
-            const bool h0 = slab(r, a.x, a.y, a.z, a.w, b.x, b.y, lim, t0) && c0 != kEmptyChild;
-            const bool h1 = slab(r, b.z, b.w, c.x, c.y, c.z, c.w, lim, t1) && c1 != kEmptyChild;
-            if (h0 || h1) {
-                const uint32_t pick = h0 && h1 ? (t0 <= t1 ? 0u : 1u) : (h0 ? 0u : 1u);
-                if (h0 && h1) {
-                    trail |= 1u << level;
-                    far_key = 2u * key + (1u - pick);
-                    far_code = pick ? c0 : c1;
+    const int64_t cnt = d_n ? (int64_t)*d_n : n, step = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = logical_block(xcd_remap != 0) * kBlock + threadIdx.x; i < cnt; i += step) {
+        // With reordering, slot i traces ray perm[i] (key order) and its result
+        // goes to that ray's slot.
+        const int64_t o = perm ? (int64_t)perm[i] : i;
+        const RayIn r = load_ray(rays, o);
+        float best_t = r.tmax, bu = 0.0f, bv = 0.0f;
+        int32_t best_prim = -1;
+        int visited = 0, tested = 0;
+        bool found_any = false;
+        int32_t node = 0;    // the current item: inner node index or leaf code
+        uint32_t key = 1u;   // its path key
+        int level = 0;       // its depth (root 0)
+        uint32_t trail = 0;  // bit L: the far child chosen at level L is pending
+        // The far child set aside last (key 0: none): the deepest pending one as
+        // long as nothing was set aside after it, which saves the table probe on
+        // about 60% of backtracks.
+        uint32_t far_key = 0u;
+        int32_t far_code = 0;
+        bool live = true;
+        while (live) {
+            if (node >= 0) {
+                visited++;
+                float4 a, b, c, d;
+                fetch2(nodes, node, level, a, b, c, d);
+                const int32_t c0 = __float_as_int(d.x), c1 = __float_as_int(d.y);
+                const float lim = kMode == kAnyHit ? r.tmax : best_t;
+                float t0, t1;
+                const bool h0 = slab(r, a.x, a.y, a.z, a.w, b.x, b.y, lim, t0) && c0 != kEmptyChild;
+                const bool h1 = slab(r, b.z, b.w, c.x, c.y, c.z, c.w, lim, t1) && c1 != kEmptyChild;
+                if (h0 || h1) {
+                    const uint32_t pick = h0 && h1 ? (t0 <= t1 ? 0u : 1u) : (h0 ? 0u : 1u);
+                    if (h0 && h1) {
+                        trail |= 1u << level;
+                        far_key = 2u * key + (1u - pick);
+                        far_code = pick ? c0 : c1;
+                    }
+                    node = pick ? c1 : c0;
+                    key = 2u * key + pick;
+                    level++;
+                    continue;
                 }
-                node = pick ? c1 : c0;
-                key = 2u * key + pick;
-                level++;
-                continue;
+            } else if (leaf_test<kMode>(node, tris, ana, r, best_t, best_prim, bu, bv, tested)) {
+                found_any = true;
+                break;
             }
-        } else if (leaf_test<kMode>(node, tris, ana, r, best_t, best_prim, bu, bv, tested)) {
-            found_any = true;
-            break;
+            // Every far child that was hit is in the table; a missing key
+            // (kEmptyChild) is skipped rather than followed.
+            do {
+                live = backtrack(trail, key, level, node, far_key, far_code, keytab, keybits);
+            } while (live && node == kEmptyChild);
         }
-        // Every far child that was hit is in the table; a missing key
-        // (kEmptyChild) is skipped rather than followed.
-        do {
-            live = backtrack(trail, key, level, node, far_key, far_code, keytab, keybits);
-        } while (live && node == kEmptyChild);
-    }
-    if (kStats)
-        counts[o] = make_int2(visited, tested);
-    if (kMode != kClosest) {
-        occluded[o] = found_any ? 1 : 0;
-    } else {
-        pg_hit h;
-        h.prim = best_prim;
-        h.t = best_prim >= 0 ? best_t : r.tmax;
-        hits[o] = h;
-        if (uvs) {
-            pg_hit_uv uv;
-            uv.u = bu;
-            uv.v = bv;
-            uvs[o] = uv;
+        if (kStats)
+            counts[o] = make_int2(visited, tested);
+        if (kMode != kClosest) {
+            occluded[o] = found_any ? 1 : 0;
+        } else {
+            pg_hit h;
+            h.prim = best_prim;
+            h.t = best_prim >= 0 ? best_t : r.tmax;
+            hits[o] = h;
+            if (uvs) {
+                pg_hit_uv uv;
+                uv.u = bu;
+                uv.v = bv;
+                uvs[o] = uv;
+            }
         }
     }
 }
@@ -719,6 +721,17 @@ hipError_t enqueue_reorder(const pg_accel* a, const pg_ray* rays, int64_t n, con
                         reinterpret_cast<uint32_t*>(a->d_perm), n, end_bit, s);
 }
 
+// Workgroups of a traversal launch: enough for every ray of a host count; for
+// a device-held count (at most n) kDevCountLanes lanes, about two per lane
+// the GPU can hold at once, striding over the count: an empty or short queue
+// then costs a short launch instead of one workgroup per capacity slot.
+constexpr int64_t kDevCountLanes = 1 << 20;
+
+int64_t launch_grid(int64_t n, int block, bool device_count) {
+    const int64_t g = (n + block - 1) / block;
+    return device_count ? std::min<int64_t>(g, kDevCountLanes / block) : g;
+}
+
 // Launches the traversal over rays[0, n) (or the first *d_n).  With
 // PG_LAUNCH_REORDER and at least kReorderMinRays rays it first sorts the rays
 // by key; host-count launches may allocate the sort's buffers, device-count
@@ -728,7 +741,7 @@ template <int kMode, bool kStats>
 hipError_t launch_trace(pg_accel* a, const pg_ray* rays, int64_t n, pg_hit* hits,
                         pg_hit_uv* uvs, int32_t* occ, int2* counts, hipStream_t s,
                         const int32_t* d_n = nullptr) {
-    const int64_t grid = (n + a->block - 1) / a->block;
+    const int64_t grid = launch_grid(n, a->block, d_n != nullptr);
     const size_t lds = (size_t)a->stack_entries * a->block * sizeof(int32_t);
     const bool postpone = (a->launch_flags & PG_LAUNCH_POSTPONE_LEAVES) != 0;
     const int32_t* perm = nullptr;

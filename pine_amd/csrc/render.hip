@@ -157,7 +157,7 @@ struct PathsDev {
     int32_t* occluded;
     // Film contributions a sample makes to a neighbouring pixel (a sample on a
     // pixel border): (pixel, w*r, w*g, w*b, w), applied after the main pass.
-    uint8_t* border;  // per pixel: 1 if one of its samples also feeds a neighbour (k_film)
+    int2* border;     // per pixel: first and last of its samples that also feed a neighbour, -1 if none (k_film)
     // General mode (media or material-less shapes): the bounce is split at the
     // shadow ray, because GridMedium::Tr consumes sampler dimensions between
     // the light sample and the BSDF / phase sample.
@@ -925,22 +925,24 @@ __global__ __launch_bounds__(kSlotBlock) void k_shade(SceneDev s, PathsDev P, in
 }
 
 __global__ __launch_bounds__(256) void k_resolve_shadow(PathsDev P, int32_t n_shadow, const int32_t* d_shadow) {
-    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (d_shadow ? *d_shadow : n_shadow) || P.occluded[k])
-        return;
-    const int32_t i = P.shadow_path[k];
-    // One shadow ray per path per bounce: no two lanes share a path here.
-    P.L[3 * i + 0] += P.shadow_contrib[3 * k + 0];
-    P.L[3 * i + 1] += P.shadow_contrib[3 * k + 1];
-    P.L[3 * i + 2] += P.shadow_contrib[3 * k + 2];
-    // A recorded guided vertex's own light sample is not radiance along its
-    // sampled direction: it joins the record's path radiance before it.
-    if (P.rec_depth > 0 && P.shadow_rec) {
-        const int32_t r = P.shadow_rec[k];
-        if (r >= 0) {
-            P.rec_lb[3 * (int64_t)r + 0] += P.shadow_contrib[3 * k + 0];
-            P.rec_lb[3 * (int64_t)r + 1] += P.shadow_contrib[3 * k + 1];
-            P.rec_lb[3 * (int64_t)r + 2] += P.shadow_contrib[3 * k + 2];
+    const int32_t n = d_shadow ? *d_shadow : n_shadow;
+    for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        if (P.occluded[k])
+            continue;
+        const int32_t i = P.shadow_path[k];
+        // One shadow ray per path per bounce: no two lanes share a path here.
+        P.L[3 * i + 0] += P.shadow_contrib[3 * k + 0];
+        P.L[3 * i + 1] += P.shadow_contrib[3 * k + 1];
+        P.L[3 * i + 2] += P.shadow_contrib[3 * k + 2];
+        // A recorded guided vertex's own light sample is not radiance along its
+        // sampled direction: it joins the record's path radiance before it.
+        if (P.rec_depth > 0 && P.shadow_rec) {
+            const int32_t r = P.shadow_rec[k];
+            if (r >= 0) {
+                P.rec_lb[3 * (int64_t)r + 0] += P.shadow_contrib[3 * k + 0];
+                P.rec_lb[3 * (int64_t)r + 1] += P.shadow_contrib[3 * k + 1];
+                P.rec_lb[3 * (int64_t)r + 2] += P.shadow_contrib[3 * k + 2];
+            }
         }
     }
 }
@@ -2256,25 +2258,25 @@ __global__ __launch_bounds__(kSlotBlock) void k_shade_gen_b(SceneDev s, PathsDev
 // per queue slot).  Its dependent loads (queue, hit, shape, triangle,
 // material) run here at full occupancy rather than in the network kernel.
 __global__ __launch_bounds__(256) void k_guide_gather(SceneDev s, PathsDev P, int32_t n_in, const int32_t* d_n) {
-    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (d_n ? *d_n : n_in))
-        return;
-    const int32_t i = P.queue_in[k];
-    const pg_hit hit = P.hits[k];
-    int use = 0;
-    if (hit.prim >= 0 && P.depth[i] + 1 < s.max_depth && P.depth[i] < s.g_depth) {
-        const float4* r4 = reinterpret_cast<const float4*>(&P.rays_in[k]);
-        const float4 ra = r4[0], rb = r4[1];
-        const f3 ro = mk(ra.x, ra.y, ra.z), rd = mk(rb.x, rb.y, rb.z);
-        const pg_hit_uv uv = P.uvs[k];
-        const Hit h = surface(s, hit.prim, hit.t, uv.u, uv.v, ro, rd);
-        const int mat = s.shapes[h.shape].material;
-        if (mat >= 0 && guidable(s, make_int4(0, mat, 0, 0))) {
-            use = 1;
-            guide_input(s, h.p, h.n, -rd, P.gx + guide::kIn * (int64_t)k);
+    const int32_t n = d_n ? *d_n : n_in;
+    for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int32_t i = P.queue_in[k];
+        const pg_hit hit = P.hits[k];
+        int use = 0;
+        if (hit.prim >= 0 && P.depth[i] + 1 < s.max_depth && P.depth[i] < s.g_depth) {
+            const float4* r4 = reinterpret_cast<const float4*>(&P.rays_in[k]);
+            const float4 ra = r4[0], rb = r4[1];
+            const f3 ro = mk(ra.x, ra.y, ra.z), rd = mk(rb.x, rb.y, rb.z);
+            const pg_hit_uv uv = P.uvs[k];
+            const Hit h = surface(s, hit.prim, hit.t, uv.u, uv.v, ro, rd);
+            const int mat = s.shapes[h.shape].material;
+            if (mat >= 0 && guidable(s, make_int4(0, mat, 0, 0))) {
+                use = 1;
+                guide_input(s, h.p, h.n, -rd, P.gx + guide::kIn * (int64_t)k);
+            }
         }
+        P.gflag[k] = use;
     }
-    P.gflag[k] = use;
 }
 
 constexpr int kInferThreads = 512;
@@ -2462,61 +2464,105 @@ __global__ void k_fill_f32(float* p, int64_t n, float v) {
         p[k] = v;
 }
 
-// Film::AddSample (film.h:26-45) for every path of a pixel, in sample order.
-__global__ __launch_bounds__(256) void k_film(SceneDev s, PathsDev P, int32_t n_pixels, int32_t spb, float* film) {
-    const int32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pixel >= n_pixels)
-        return;
-    const int px = pixel % s.width, py = pixel / s.width;
-    const size_t own = (size_t)(s.height - 1 - py) * s.width + px;
-    float acc[4] = {film[4 * own + 0], film[4 * own + 1], film[4 * own + 2], film[4 * own + 3]};
-    bool border = false;
-    for (int sl = 0; sl < spb; sl++) {
-        const int32_t i = pixel * spb + sl;
-        const float fx = P.pfilm[2 * i] * (float)s.width - 0.5f;
-        const float fy = P.pfilm[2 * i + 1] * (float)s.height - 0.5f;
-        const float rx = s.filter_radius[0], ry = s.filter_radius[1];
-        int x0 = (int)pstd::ceil(fx - rx), y0 = (int)pstd::ceil(fy - ry);
-        int x1 = (int)pstd::floor(fx + rx), y1 = (int)pstd::floor(fy + ry);
-        x0 = x0 > 0 ? x0 : 0;
-        y0 = y0 > 0 ? y0 : 0;
-        x1 = x1 < s.width - 1 ? x1 : s.width - 1;
-        y1 = y1 < s.height - 1 ? y1 : s.height - 1;
-        const float L0 = P.L[3 * i], L1 = P.L[3 * i + 1], L2 = P.L[3 * i + 2];
-        for (int y = y0; y <= y1; y++)
-            for (int x = x0; x <= x1; x++) {
-                float dx = fabsp((float)x - fx) / rx, dy = fabsp((float)y - fy) / ry;
-                int tx = (int)(16.0f * (dx < kOneMinusEps ? dx : kOneMinusEps));
-                int ty = (int)(16.0f * (dy < kOneMinusEps ? dy : kOneMinusEps));
-                const float w = s.filter_table[ty * 16 + tx];
-                const size_t pix = (size_t)(s.height - 1 - y) * s.width + x;
-                if (pix == own) {
-                    acc[0] += L0 * w;
-                    acc[1] += L1 * w;
-                    acc[2] += L2 * w;
-                    acc[3] += w;
-                } else {
-                    // A sample exactly on a pixel border also feeds the
-                    // neighbour; the neighbour's own pass may be running, so
-                    // k_film_border adds it afterwards.
-                    border = true;
-                }
-            }
-    }
-    film[4 * own + 0] = acc[0];
-    film[4 * own + 1] = acc[1];
-    film[4 * own + 2] = acc[2];
-    film[4 * own + 3] = acc[3];
-    P.border[pixel] = border;
+// Film::AddSample (film.h:26-45) for every path of a pixel, in sample order,
+// for filters of radius at most 1/2.  A workgroup takes G consecutive pixels
+// (their samples are consecutive paths) and stages their samples' own-pixel
+// contributions through LDS, kFilmTile at a time, so the path reads are
+// coalesced; each pixel's thread then adds its samples to the film value in
+// sample order, as pine's AddSample does.  A sample exactly on a pixel border
+// also feeds a neighbour, whose own pass may be running: the pixel records the
+// range of such samples and k_film_border adds them afterwards.
+constexpr int kFilmTile = 2048;
+
+int film_pixels_per_block(int spb) {
+    return std::max(1, std::min(256, kFilmTile / std::max(spb, 1)));
 }
 
-// The border samples k_film left out, as a gather: each film pixel visits the
-// (at most eight) neighbours k_film marked, finds their samples whose
-// [Ceil(pf - r), Floor(pf + r)] footprint (computed as k_film computes it)
-// covers it, and adds them in a fixed order (neighbour rows, columns, samples),
-// so the film does not depend on scheduling.  Filters of radius at most 1/2:
-// a sample lies within half a pixel (plus rounding) of its own pixel's centre,
-// so its footprint reaches one pixel at most.
+__global__ __launch_bounds__(256) void k_film(SceneDev s, PathsDev P, int32_t n_pixels, int32_t spb, int32_t G,
+                                              float* film) {
+    __shared__ float4 contrib[kFilmTile];
+    __shared__ uint8_t kind[kFilmTile];  // bit 0: the own pixel is in the footprint; bit 1: another pixel is
+    const int32_t p0 = blockIdx.x * G;
+    const int32_t np = min(G, n_pixels - p0);
+    const int64_t base = (int64_t)p0 * spb, count = (int64_t)np * spb;
+    const int j = threadIdx.x;
+    const bool owner = j < np;
+    const float rx = s.filter_radius[0], ry = s.filter_radius[1];
+    size_t own = 0;
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    int32_t first = -1, last = -1;
+    if (owner) {
+        const int32_t pixel = p0 + j;
+        const int px = pixel % s.width, py = pixel / s.width;
+        own = (size_t)(s.height - 1 - py) * s.width + px;
+        for (int c = 0; c < 4; c++)
+            acc[c] = film[4 * own + c];
+    }
+    for (int64_t c0 = 0; c0 < count; c0 += kFilmTile) {
+        const int n = (int)min((int64_t)kFilmTile, count - c0);
+        __syncthreads();  // the previous tile has been read
+        for (int t = j; t < n; t += 256) {
+            const int64_t sl = c0 + t;
+            const int32_t q = p0 + (int32_t)(sl / spb);
+            const int64_t i = base + sl;
+            const int qx = q % s.width, qy = q / s.width;
+            const float fx = P.pfilm[2 * i] * (float)s.width - 0.5f;
+            const float fy = P.pfilm[2 * i + 1] * (float)s.height - 0.5f;
+            int x0 = (int)pstd::ceil(fx - rx), y0 = (int)pstd::ceil(fy - ry);
+            int x1 = (int)pstd::floor(fx + rx), y1 = (int)pstd::floor(fy + ry);
+            x0 = x0 > 0 ? x0 : 0;
+            y0 = y0 > 0 ? y0 : 0;
+            x1 = x1 < s.width - 1 ? x1 : s.width - 1;
+            y1 = y1 < s.height - 1 ? y1 : s.height - 1;
+            uint8_t k = 0;
+            float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (qx >= x0 && qx <= x1 && qy >= y0 && qy <= y1) {
+                const float dx = fabsp((float)qx - fx) / rx, dy = fabsp((float)qy - fy) / ry;
+                const int tx = (int)(16.0f * (dx < kOneMinusEps ? dx : kOneMinusEps));
+                const int ty = (int)(16.0f * (dy < kOneMinusEps ? dy : kOneMinusEps));
+                const float w = s.filter_table[ty * 16 + tx];
+                c = make_float4(P.L[3 * i] * w, P.L[3 * i + 1] * w, P.L[3 * i + 2] * w, w);
+                k = 1;
+            }
+            if (x0 <= x1 && y0 <= y1 && !(x0 == x1 && y0 == y1 && x0 == qx && y0 == qy))
+                k |= 2;
+            contrib[t] = c;
+            kind[t] = k;
+        }
+        __syncthreads();
+        if (owner) {
+            const int64_t lo = max((int64_t)j * spb, c0), hi = min((int64_t)(j + 1) * spb, c0 + n);
+            for (int64_t sl = lo; sl < hi; sl++) {
+                const uint8_t k = kind[sl - c0];
+                if (k & 1) {
+                    const float4 c = contrib[sl - c0];
+                    acc[0] += c.x;
+                    acc[1] += c.y;
+                    acc[2] += c.z;
+                    acc[3] += c.w;
+                }
+                if (k & 2) {
+                    last = (int32_t)(sl - (int64_t)j * spb);
+                    if (first < 0)
+                        first = last;
+                }
+            }
+        }
+    }
+    if (owner) {
+        for (int c = 0; c < 4; c++)
+            film[4 * own + c] = acc[c];
+        P.border[p0 + j] = make_int2(first, last);
+    }
+}
+
+// The border samples k_film left out, as a gather: each film pixel visits its
+// (at most eight) neighbours' recorded ranges of border samples, finds those
+// whose [Ceil(pf - r), Floor(pf + r)] footprint (computed as k_film computes
+// it) covers it, and adds them in a fixed order (neighbour rows, columns,
+// samples), so the film does not depend on scheduling.  Filters of radius at
+// most 1/2: a sample lies within half a pixel (plus rounding) of its own
+// pixel's centre, so its footprint reaches one pixel at most.
 __global__ __launch_bounds__(256) void k_film_border(SceneDev s, PathsDev P, int32_t n_pixels, int32_t spb,
                                                      float* film) {
     const int32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2528,7 +2574,7 @@ __global__ __launch_bounds__(256) void k_film_border(SceneDev s, PathsDev P, int
     bool any = false;
     for (int qy = qy0; qy <= qy1; qy++)
         for (int qx = qx0; qx <= qx1; qx++)
-            any |= (qx != x || qy != y) && P.border[qy * s.width + qx];
+            any |= (qx != x || qy != y) && P.border[qy * s.width + qx].x >= 0;
     if (!any)
         return;
     const float rx = s.filter_radius[0], ry = s.filter_radius[1];
@@ -2537,10 +2583,11 @@ __global__ __launch_bounds__(256) void k_film_border(SceneDev s, PathsDev P, int
     for (int qy = qy0; qy <= qy1; qy++)
         for (int qx = qx0; qx <= qx1; qx++) {
             const int32_t q = qy * s.width + qx;
-            if ((qx == x && qy == y) || !P.border[q])
+            const int2 r = P.border[q];
+            if ((qx == x && qy == y) || r.x < 0)
                 continue;
-            for (int sl = 0; sl < spb; sl++) {
-                const int32_t i = q * spb + sl;
+            for (int sl = r.x; sl <= r.y; sl++) {
+                const int64_t i = (int64_t)q * spb + sl;
                 const float fx = P.pfilm[2 * i] * (float)s.width - 0.5f;
                 const float fy = P.pfilm[2 * i + 1] * (float)s.height - 0.5f;
                 const int x0 = (int)pstd::ceil(fx - rx), y0 = (int)pstd::ceil(fy - ry);
@@ -4205,7 +4252,7 @@ int ensure_guide(PathState& W, int64_t n, int rec_depth) {
 // ensure_guide allocate for a batch (rounded up; the small fixed buffers aside).
 int64_t path_state_bytes(const pg_scene& S, bool guided, int rec_depth) {
     const int64_t ray = sizeof(pg_ray), hit = sizeof(pg_hit) + sizeof(pg_hit_uv);
-    int64_t b = 4 * (3 + 3 + 1 + 2 + 3) + 8 + 4 * (1 + 1 + 2 + 1 + 1) + 3 * ray + hit + 1;
+    int64_t b = 4 * (3 + 3 + 1 + 2 + 3) + 8 + 4 * (1 + 1 + 2 + 1 + 1) + 3 * ray + hit + 8;
     if (S.general)
         b += 4 * (1 + 9 + 4 + 1 + 16 + 3 + 2 + 2 + 1) + 2 * ray + hit + 2 * sizeof(float4);
     if (!S.node_progs.empty())
@@ -4519,6 +4566,15 @@ __global__ void k_count_rays(const int32_t* d_active, const int32_t* d_shadow, u
     }
 }
 
+// Workgroups of a launch over a queue whose length is read in device memory
+// (at most cap): about two lanes per lane the GPU holds at once, striding over
+// the queue, so the bounces that find few or no paths cost short launches.
+constexpr int64_t kDevCountLanes = 1 << 20;
+
+unsigned dev_count_grid(int64_t cap, int block) {
+    return (unsigned)std::min<int64_t>((cap + block - 1) / block, kDevCountLanes / block);
+}
+
 // The fused pipeline's bounce loop for up to `cap` paths with every count in
 // device memory: bounce b reads its path count from the previous bounce's
 // counters (bounce 0 from g_counts[0], written by k_raygen) and writes its
@@ -4529,7 +4585,10 @@ __global__ void k_count_rays(const int32_t* d_active, const int32_t* d_shadow, u
 static int record_bounces(pg_scene* S, const PathsDev& P, int64_t cap, hipStream_t st) {
     const int B = 256;
     const int nb = S->max_depth;
-    const unsigned grid = (unsigned)((cap + B - 1) / B);
+    const unsigned grid = dev_count_grid(cap, B);
+    // k_shade takes one tile per workgroup (a strided loop costs it registers:
+    // 152 VGPRs, 3 waves per SIMD), so its grid covers the capacity.
+    const unsigned shade_grid = (unsigned)((cap + B - 1) / B);
     hipLaunchKernelGGL(k_fill_i32, dim3((4 * nb + B - 1) / B), dim3(B), 0, st, S->g_counts + 4, 4 * nb, 0);
     PathsDev Q = P;
     for (int b = 0; b < nb; b++) {
@@ -4541,9 +4600,9 @@ static int record_bounces(pg_scene* S, const PathsDev& P, int64_t cap, hipStream
             hipLaunchKernelGGL(k_guide_gather, dim3(grid), dim3(B), 0, st, S->dev, Q, 0, d_active);
             hipLaunchKernelGGL(k_guide_infer<1>, dim3(infer_blocks((int32_t)cap)), dim3(kInferThreads), 0, st,
                                S->dev, Q, 0, d_active);
-            hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(B), 0, st, S->dev, Q, 0, d_active);
+            hipLaunchKernelGGL(k_shade<true>, dim3(shade_grid), dim3(B), 0, st, S->dev, Q, 0, d_active);
         } else {
-            hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(B), 0, st, S->dev, Q, 0, d_active);
+            hipLaunchKernelGGL(k_shade<false>, dim3(shade_grid), dim3(B), 0, st, S->dev, Q, 0, d_active);
         }
         if (int rc = pg_trace_occluded_dev(S->accel, Q.shadow_rays, Q.counters + 1, cap, Q.occluded, st))
             return rc;
@@ -4878,8 +4937,9 @@ static int render_enqueue(pg_scene* S, PathState& W, int spp, int spb_req, hipSt
             hipLaunchKernelGGL(k_film_gather, dim3((npix + B - 1) / B), dim3(B), 0, st, S->dev, P, (int32_t)npix,
                                nb, S->d_film);
         } else {
-            hipLaunchKernelGGL(k_film, dim3((npix + B - 1) / B), dim3(B), 0, st, S->dev, P, (int32_t)npix,
-                               nb, S->d_film);
+            const int G = film_pixels_per_block(nb);
+            hipLaunchKernelGGL(k_film, dim3((unsigned)((npix + G - 1) / G)), dim3(B), 0, st, S->dev, P,
+                               (int32_t)npix, nb, G, S->d_film);
             hipLaunchKernelGGL(k_film_border, dim3((npix + B - 1) / B), dim3(B), 0, st, S->dev, P, (int32_t)npix,
                                nb, S->d_film);
         }

@@ -1,5 +1,7 @@
 """Renders one benchmark workload in a process of its own and prints its
 rate: for profiling a workload's kernels without the rest of the benchmark.
+The workload renders twice, a warm render and the timed one, both at full
+size.
 
     python -m pine_amd.render_workload {spheres,mesh,cloud,box} [--spp N] [--res R]
 
@@ -44,7 +46,7 @@ def main(argv=None):
         W.write_obj(os.path.join(work, "clutter.obj"), v, i)
         text = W.occluded_box_scene_text("clutter.obj", width=res, height=res, spp=spp, max_depth=8)
     s = pg.Scene(text=text, directory=work)
-    s.render(spp=2)
+    s.render()  # warm: sizes the path state and records the bounce loop for the batch size
     inf = s.render()
     ms = inf["last_render_ms"]
     print(f"{args.workload} {res}x{res}, {spp} spp: {ms:.1f} ms, "

@@ -722,10 +722,13 @@ hipError_t enqueue_reorder(const pg_accel* a, const pg_ray* rays, int64_t n, con
 }
 
 // Workgroups of a traversal launch: enough for every ray of a host count; for
-// a device-held count (at most n) kDevCountLanes lanes, about two per lane
-// the GPU can hold at once, striding over the count: an empty or short queue
-// then costs a short launch instead of one workgroup per capacity slot.
-constexpr int64_t kDevCountLanes = 1 << 20;
+// a device-held count (at most n) at most kDevCountLanes lanes striding over
+// the count.  A bounce's queue is sized for its capacity but often holds far
+// fewer rays.  4M lanes is about eight per lane the GPU holds at once.
+// Measured on the 512^2 x 64-spp spheres and 2^20-triangle mesh scenes
+// (16M-path batches, M samples/s): 1M lanes 3,274 and 966, 4M 3,345 and
+// 1,001, no cap 3,270 and 1,020.
+constexpr int64_t kDevCountLanes = 1 << 22;
 
 int64_t launch_grid(int64_t n, int block, bool device_count) {
     const int64_t g = (n + block - 1) / block;

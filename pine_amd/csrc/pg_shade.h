@@ -68,13 +68,26 @@ struct HaltonDev {
     const uint16_t* perms;  // ComputeRadicalInversePermutations
     const int* primes;      // first 1000 primes
     const int* primeSums;   // prefix sums of primes
-    // Per dimension, two 16-byte records: prime base, offset of its permutation
-    // in perms, 1/base and the series term perm[0] / (base + 1), as floats'
-    // bits (the two quotients computed once on the host, IEEE as the device
-    // would); then the base's division magic m and post-shift l - 1 (see
-    // udiv_magic).
+    // Per dimension, three 16-byte records: prime base, offset of its
+    // permutation in perms, 1/base and the series term perm[0] / (base + 1),
+    // as floats' bits (the two quotients computed once on the host, IEEE as
+    // the device would); then the base's division magic m and post-shift
+    // l - 1 (see udiv_magic), the chunk length k (0: no chunk tables) and the
+    // offset of the dimension's words in chunk32; then B = base^k, B's magic
+    // and post-shift, and the offset of its tables in chunk16.
     const int4* dims;
+    // Digit-chunk tables of the bases with base^2 <= kChunkEntries (see
+    // scrambled_radical_inverse32): per dimension, in chunk16, the natural
+    // table (B entries: the permuted digits of x < B folded as the digit loop
+    // folds them, with the digit count in the top 4 bits) and the full table
+    // (B entries: all k digits of x, leading zeros included); in chunk32,
+    // base^n for n <= k, then the bits of invBase^n for n <= 32 as the loop
+    // forms them (one rounded multiplication per digit).
+    const uint16_t* chunk16;
+    const uint32_t* chunk32;
 };
+
+constexpr int kChunkEntries = 4096, kMaxFullChunks = 3;
 
 // Unsigned 32-bit division by an invariant b >= 2 (Granlund and Montgomery,
 // "Division by invariant integers using multiplication", 1994): with
@@ -121,8 +134,50 @@ PG_HD uint64_t halton_index(const HaltonDev& h, int px, int py, int sampleIndex)
 // ScrambledRadicalInverse for an index below 2^32: the digit loop in 32 bits,
 // four digits a round so that their permutation loads are in flight together
 // (the digits are combined in the same order as one at a time).
+//
+// Bases with chunk tables (base^k = B <= kChunkEntries, k >= 2) take the
+// index k digits at a time: each full chunk x = a mod B (the rest still
+// nonzero, so all k digits count, zeros included) contributes
+// reversed = reversed * B + full[x], and the last part (a < B, only its own
+// digits) reversed * base^nd + natural[a]: the same integer as digit by
+// digit, and invBase^(digits) from the table, the same float.  At most
+// kMaxFullChunks full chunks for a 32-bit index (the host checks).
 PG_HD float scrambled_radical_inverse32(const HaltonDev& h, int dim, uint32_t a) {
-    const int4 t = h.dims[2 * dim], dm = h.dims[2 * dim + 1];
+    const int4 t = h.dims[3 * dim], dm = h.dims[3 * dim + 1];
+    if (dm.z > 0) {
+        const int4 c = h.dims[3 * dim + 2];
+        const uint32_t B = (uint32_t)c.x, k = (uint32_t)dm.z;
+        const uint16_t* natural = h.chunk16 + c.w;
+        const uint16_t* full = natural + B;
+        const uint32_t* words = h.chunk32 + dm.w;
+        uint32_t xs[kMaxFullChunks];
+        int nf = 0;
+#pragma unroll
+        for (int j = 0; j < kMaxFullChunks; j++) {
+            xs[j] = 0u;
+            if (a >= B) {
+                const uint32_t next = udiv_magic(a, (uint32_t)c.y, (uint32_t)c.z);
+                xs[j] = a - next * B;
+                a = next;
+                nf++;
+            }
+        }
+        uint32_t fv[kMaxFullChunks];
+#pragma unroll
+        for (int j = 0; j < kMaxFullChunks; j++)
+            fv[j] = j < nf ? (uint32_t)full[xs[j]] : 0u;
+        const uint32_t e = natural[a];
+        uint64_t reversed = 0;
+#pragma unroll
+        for (int j = 0; j < kMaxFullChunks; j++)
+            if (j < nf)
+                reversed = reversed * (uint64_t)B + fv[j];
+        const uint32_t nd = e >> 12;
+        reversed = reversed * (uint64_t)words[nd] + (e & 0xFFFu);
+        const float invBaseN = __builtin_bit_cast(float, words[k + 1 + (uint32_t)nf * k + nd]);
+        const float v = ((float)reversed + __builtin_bit_cast(float, t.w)) * invBaseN;
+        return v < kOneMinusEps ? v : kOneMinusEps;
+    }
     const uint32_t b = (uint32_t)t.x, m = (uint32_t)dm.x, post = (uint32_t)dm.y;
     const uint16_t* perm = h.perms + t.y;
     const float invBase = __builtin_bit_cast(float, t.z);

@@ -4058,22 +4058,74 @@ int finish_scene(pg_scene& S, const pg_build_options* opts) {
                 perms[o + j] = (uint16_t)j;
     }
     // Per-dimension records of the radical inverse (base, permutation offset,
-    // 1/base, perm[0] / (base + 1)).
-    std::vector<int4> dimtab(2 * primes.size());
+    // 1/base, perm[0] / (base + 1); division magic; digit-chunk tables).
+    std::vector<int4> dimtab(3 * primes.size());
+    std::vector<uint16_t> chunk16;
+    std::vector<uint32_t> chunk32;
+    // udiv_magic's constants for a divisor b: l = ceil(log2 b), m, post-shift l - 1.
+    auto magic = [](uint64_t b, uint32_t& m, int& post) {
+        int l = 0;
+        while ((uint64_t(1) << l) < b)
+            l++;
+        m = (uint32_t)(((uint64_t(1) << 32) * ((uint64_t(1) << l) - b)) / b + 1);
+        post = l - 1;
+    };
     for (size_t d = 0; d < primes.size(); d++) {
         const float inv_base = 1.0f / (float)primes[d];
         const float series = (float)perms[sums[d]] / ((float)primes[d] + 1.0f);
         int ib, sb;
         std::memcpy(&ib, &inv_base, 4);
         std::memcpy(&sb, &series, 4);
-        dimtab[2 * d] = make_int4(primes[d], sums[d], ib, sb);
-        // udiv_magic's constants for the base: l = ceil(log2 b), m, post-shift l - 1.
+        dimtab[3 * d] = make_int4(primes[d], sums[d], ib, sb);
         const uint64_t b = (uint64_t)primes[d];
-        int l = 0;
-        while ((uint64_t(1) << l) < b)
-            l++;
-        const uint32_t m = (uint32_t)(((uint64_t(1) << 32) * ((uint64_t(1) << l) - b)) / b + 1);
-        dimtab[2 * d + 1] = make_int4((int32_t)m, l - 1, 0, 0);
+        uint32_t m = 0;
+        int post = 0;
+        magic(b, m, post);
+        dimtab[3 * d + 1] = make_int4((int32_t)m, post, 0, 0);
+        dimtab[3 * d + 2] = make_int4(0, 0, 0, 0);
+        // Chunk tables: k digits at a time, B = b^k <= kChunkEntries, for the
+        // bases where k >= 2 and a 32-bit index has at most kMaxFullChunks
+        // full chunks before its last part.
+        int k = 0;
+        uint64_t B = 1;
+        while (B * b <= (uint64_t)kChunkEntries) {
+            B *= b;
+            k++;
+        }
+        int digits = 0;
+        for (uint64_t v = 0xFFFFFFFFull; v; v /= b)
+            digits++;
+        if (k < 2 || (digits + k - 1) / k - 1 > kMaxFullChunks)
+            continue;
+        const uint16_t* perm = perms.data() + sums[d];
+        const size_t off16 = chunk16.size(), off32 = chunk32.size();
+        for (uint64_t x = 0; x < B; x++) {  // natural: the digits x has
+            uint64_t v = 0, y = x;
+            uint32_t nd = 0;
+            for (; y; y /= b, nd++)
+                v = v * b + perm[y % b];
+            chunk16.push_back((uint16_t)(v | (nd << 12)));
+        }
+        for (uint64_t x = 0; x < B; x++) {  // full: all k digits
+            uint64_t v = 0, y = x;
+            for (int j = 0; j < k; j++, y /= b)
+                v = v * b + perm[y % b];
+            chunk16.push_back((uint16_t)v);
+        }
+        for (uint64_t n = 0, p = 1; n <= (uint64_t)k; n++, p *= b)
+            chunk32.push_back((uint32_t)p);
+        float ipow = 1.0f;
+        for (int n = 0; n <= 32; n++) {
+            uint32_t bits;
+            std::memcpy(&bits, &ipow, 4);
+            chunk32.push_back(bits);
+            ipow *= inv_base;
+        }
+        uint32_t mB = 0;
+        int postB = 0;
+        magic(B, mB, postB);
+        dimtab[3 * d + 1] = make_int4((int32_t)m, post, k, (int32_t)off32);
+        dimtab[3 * d + 2] = make_int4((int32_t)B, (int32_t)mB, postB, (int32_t)off16);
     }
     SamplerDev& sd = S.dev.sampler;
     sd.kind = S.sampler_kind;
@@ -4098,7 +4150,8 @@ int finish_scene(pg_scene& S, const pg_build_options* opts) {
             return rc;
     }
     if ((rc = upload(S, perms, &sd.halton.perms)) || (rc = upload(S, primes, &sd.halton.primes)) ||
-        (rc = upload(S, dimtab, &sd.halton.dims)) ||
+        (rc = upload(S, dimtab, &sd.halton.dims)) || (rc = upload(S, chunk16, &sd.halton.chunk16)) ||
+        (rc = upload(S, chunk32, &sd.halton.chunk32)) ||
         (rc = upload(S, sums, &sd.halton.primeSums)) || (rc = upload(S, S.prim_shape, &S.dev.prim_shape)) ||
         (rc = upload(S, S.tri_verts, &S.dev.tri_verts)) || (rc = upload(S, S.shapes, &S.dev.shapes)) ||
         (rc = upload(S, S.materials, &S.dev.materials)) || (rc = upload(S, S.lights, &S.dev.lights)) ||

@@ -232,27 +232,27 @@ def test_reordered_device_count_launch(pg, oracle, cuda):
 
 @pytest.mark.parametrize("width,flags", [(2, 0), (2, 1), (4, 0), (2, 4)])
 def test_device_count_launch_strides_past_its_grid(pg, cuda, width, flags):
-    """A device-count launch caps its grid at 2^20 lanes and strides over the
-    count: with 2.6M rays (pine's golden rays repeated, ragged tail) every
-    lane traces several rays.  The first *d_n slots hold the host-count
+    """A device-count launch caps its grid at 2^22 lanes and strides over the
+    count: with 5.2M rays (pine's golden rays repeated, ragged tail) some
+    lanes trace two rays.  The first *d_n slots hold the host-count
     launch's hits and occlusion (one ray per lane; pine's t, bit for bit),
     for plain, postponed-leaf, 4-wide and stackless traversal, and the slots
     past the count are untouched."""
     import ctypes
     fx = golden("trace_82k")
     v, i, rays = _mesh_rays(fx)
-    reps = -(-(5 << 19) // len(rays))
+    reps = -(-(5 << 20) // len(rays))
     def tile(x):
         return np.concatenate([x] * reps + [x[:333]])
     big, gprim, gt = tile(rays), tile(fx["prim"]), tile(fx["t"])
-    assert len(big) > 5 << 19
+    assert len(big) > 5 << 20
     a = _accel(pg, v, i, width, 128, flags)
     d = cuda.DeviceArray.from_numpy(big)
     full, full_occ = a.trace_closest(d).numpy(), a.trace_occluded(d).numpy()
     hit = gprim >= 0
     assert np.array_equal(full[:, 0] >= 0, hit)
     assert np.array_equal(full[hit, 1], gt[hit].view(np.int32))
-    for count in (len(big), (5 << 19) - 3, 1000, 0):
+    for count in (len(big), (5 << 20) - 3, 1000, 0):
         dn = cuda.DeviceArray.from_numpy(np.array([count], np.int32))
         hits = cuda.DeviceArray.from_numpy(np.full((len(big), 2), -7, np.int32))
         pg.check(pg.lib().pg_trace_closest_dev(a.h, d.ptr, dn.ptr, ctypes.c_int64(len(big)),

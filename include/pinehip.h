@@ -276,7 +276,8 @@ typedef struct pg_batch_options {
     int32_t streams;          /* concurrent streams, 1..16 (0 = 4) */
     int32_t flags;            /* PG_RENDER_* bits for every scene */
     int64_t paths_per_batch;  /* at most this many paths in one batch of any scene
-                                 (0 = each scene's own paths_in_flight) */
+                                 (0 = each scene's own paths_in_flight, at most 64M:
+                                 every stream's path state is live at once) */
 } pg_batch_options;
 typedef struct pg_batch_result {
     double ms;        /* wall time of the batch on `stream` */

@@ -2751,7 +2751,7 @@ struct pg_scene {
     // Large batches amortise each bounce's fixed cost (launches, and the split
     // bounce's count read-back) over the short tail bounces, and put a pixel's
     // samples in adjacent lanes, so a wave's rays start coherent.
-    int64_t batch_paths = 64 << 20;
+    int64_t batch_paths = 256 << 20;
     pg_build_options build_opts{};       // the accel's build options (rebuilds keep them)
     // Guide inference kernel time and queries of the last launch-by-launch
     // render (events around each k_guide_infer launch; graph replays skip it).
@@ -5030,6 +5030,11 @@ static int render_finish(pg_scene* S, RenderTicket* t, double* ms_out, int64_t* 
     return PG_OK;
 }
 
+// Default cap on a scene's batch in pg_render_batch (its streams' path-state
+// slots are live together): 64M paths, about 15 GB per slot for the fused
+// bounce.
+constexpr int64_t kBatchStreamPaths = 64 << 20;
+
 // Holds the device's render lock for one entry point's renders.
 struct RenderLock {
     DevicePaths& D;
@@ -5110,7 +5115,9 @@ extern "C" int pg_render_batch(pg_scene* const* scenes, int32_t n, const int32_t
     const int flags = o ? o->flags : 0;
     if (o && o->paths_per_batch < 0)
         return pg_fail(PG_E_ARG, "pg_render_batch: paths_per_batch must be >= 0");
-    const int64_t cap = o && o->paths_per_batch > 0 ? o->paths_per_batch : 0;
+    // Each stream's path-state slot is live at once: by default a scene's
+    // batch is at most kBatchStreamPaths paths.
+    const int64_t cap = o && o->paths_per_batch > 0 ? o->paths_per_batch : kBatchStreamPaths;
     hipStream_t st = (hipStream_t)stream;
     std::vector<hipStream_t> ws(K, nullptr);
     std::vector<hipEvent_t> done(K, nullptr);
@@ -5164,8 +5171,7 @@ extern "C" int pg_render_batch(pg_scene* const* scenes, int32_t n, const int32_t
         S->use_graph = !(flags & PG_RENDER_NO_GRAPH);
         S->lane_tracking = (flags & PG_RENDER_LANE_TRACKING) != 0;
         const int64_t own = S->batch_paths;
-        if (cap > 0)
-            S->batch_paths = std::min(own, cap);
+        S->batch_paths = std::min(own, cap);
         rc = render_enqueue(S, lock.slot(k + 1), sp, 0, ws[k], nullptr, 0, &tickets[i]);
         S->batch_paths = own;
         S->use_graph = true;
@@ -5276,7 +5282,7 @@ extern "C" int pg_scene_autotune(pg_scene* S, const pg_tune_options* o, void* st
     pg_accel_set_launch(S->accel, best_block, best_flags);
     // Paths in flight: probes large enough to fill the largest candidate.
     const int64_t npix = (int64_t)S->dev.width * S->dev.height;
-    const int64_t cands[4] = {4 << 20, 16 << 20, 32 << 20, 64 << 20};
+    const int64_t cands[4] = {8 << 20, 32 << 20, 64 << 20, 256 << 20};
     const int qspp = (int)std::min<int64_t>(S->spp, std::max<int64_t>(pspp, (cands[3] + npix - 1) / npix));
     if (qspp * npix > cands[0]) {
         const int64_t start = S->batch_paths;

@@ -699,12 +699,16 @@ __device__ __forceinline__ bool guided_direction(const SceneDev& s, PathsDev& P,
 // order, lanes in order).  Every lane of the workgroup must call them (no early
 // return before); they begin with a barrier so that they can follow each other.
 constexpr int kSlotBlock = 256, kSlotWaves = kSlotBlock / 64;
+// k_shade's workgroup: the fused bounce launches it for the batch's capacity,
+// so a near-empty bounce dispatches a workgroup per kShadeBlock slots.
+constexpr int kShadeBlock = 512;
 
 // Two adjacent counters ctr[0], ctr[1] (8-byte aligned) in one 64-bit atomic:
 // the low counter never carries into the high one (counts stay below 2^31).
+template <int kWaves = kSlotWaves>
 __device__ __forceinline__ void block_slots2(bool want_a, bool want_b, int32_t* ctr, int32_t& slot_a,
                                              int32_t& slot_b) {
-    __shared__ uint32_t cnt[2][kSlotWaves];
+    __shared__ uint32_t cnt[2][kWaves];
     __shared__ uint32_t base[2];
     __syncthreads();
     const int lane = (int)__lane_id(), wave = (int)(threadIdx.x >> 6);
@@ -716,7 +720,7 @@ __device__ __forceinline__ void block_slots2(bool want_a, bool want_b, int32_t* 
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t ta = 0, tb = 0;
-        for (int w = 0; w < kSlotWaves; w++) {
+        for (int w = 0; w < kWaves; w++) {
             ta += cnt[0][w];
             tb += cnt[1][w];
         }
@@ -762,7 +766,7 @@ __device__ __forceinline__ int32_t block_slot(bool want, int32_t* ctr) {
 }
 
 template <bool kGuided>
-__global__ __launch_bounds__(kSlotBlock) void k_shade(SceneDev s, PathsDev P, int32_t n_active, const int32_t* d_active) {
+__global__ __launch_bounds__(kShadeBlock) void k_shade(SceneDev s, PathsDev P, int32_t n_active, const int32_t* d_active) {
     const int32_t n = d_active ? *d_active : n_active;
     if ((int64_t)blockIdx.x * blockDim.x >= n)
         return;  // the whole workgroup
@@ -908,7 +912,7 @@ __global__ __launch_bounds__(kSlotBlock) void k_shade(SceneDev s, PathsDev P, in
     P.depth[i] = depth;
     }
     int32_t slot = 0, sslot = 0;
-    block_slots2(alive, shadow, P.counters, slot, sslot);
+    block_slots2<kShadeBlock / 64>(alive, shadow, P.counters, slot, sslot);
     if (alive) {
         P.queue_out[slot] = i;
         store_ray(&P.rays_out[slot], nextO, nextD, 0.0f, kFloatMax);
@@ -4641,7 +4645,7 @@ static int record_bounces(pg_scene* S, const PathsDev& P, int64_t cap, hipStream
     const unsigned grid = dev_count_grid(cap, B);
     // k_shade takes one tile per workgroup (a strided loop costs it registers:
     // 152 VGPRs, 3 waves per SIMD), so its grid covers the capacity.
-    const unsigned shade_grid = (unsigned)((cap + B - 1) / B);
+    const unsigned shade_grid = (unsigned)((cap + kShadeBlock - 1) / kShadeBlock);
     hipLaunchKernelGGL(k_fill_i32, dim3((4 * nb + B - 1) / B), dim3(B), 0, st, S->g_counts + 4, 4 * nb, 0);
     PathsDev Q = P;
     for (int b = 0; b < nb; b++) {
@@ -4653,9 +4657,9 @@ static int record_bounces(pg_scene* S, const PathsDev& P, int64_t cap, hipStream
             hipLaunchKernelGGL(k_guide_gather, dim3(grid), dim3(B), 0, st, S->dev, Q, 0, d_active);
             hipLaunchKernelGGL(k_guide_infer<1>, dim3(infer_blocks((int32_t)cap)), dim3(kInferThreads), 0, st,
                                S->dev, Q, 0, d_active);
-            hipLaunchKernelGGL(k_shade<true>, dim3(shade_grid), dim3(B), 0, st, S->dev, Q, 0, d_active);
+            hipLaunchKernelGGL(k_shade<true>, dim3(shade_grid), dim3(kShadeBlock), 0, st, S->dev, Q, 0, d_active);
         } else {
-            hipLaunchKernelGGL(k_shade<false>, dim3(shade_grid), dim3(B), 0, st, S->dev, Q, 0, d_active);
+            hipLaunchKernelGGL(k_shade<false>, dim3(shade_grid), dim3(kShadeBlock), 0, st, S->dev, Q, 0, d_active);
         }
         if (int rc = pg_trace_occluded_dev(S->accel, Q.shadow_rays, Q.counters + 1, cap, Q.occluded, st))
             return rc;
@@ -4914,11 +4918,11 @@ static int render_enqueue(pg_scene* S, PathState& W, int spp, int spb_req, hipSt
                     hipLaunchKernelGGL(k_guide_infer<1>, dim3(infer_blocks(active)), dim3(kInferThreads), 0, st,
                                        S->dev, Q, active, nullptr);
                     PG_HIP(hipEventRecord(S->infer_ev[1], st));
-                    hipLaunchKernelGGL(k_shade<true>, dim3((active + B - 1) / B), dim3(B), 0, st, S->dev, Q,
-                                       active, nullptr);
+                    hipLaunchKernelGGL(k_shade<true>, dim3((active + kShadeBlock - 1) / kShadeBlock),
+                                       dim3(kShadeBlock), 0, st, S->dev, Q, active, nullptr);
                 } else {
-                    hipLaunchKernelGGL(k_shade<false>, dim3((active + B - 1) / B), dim3(B), 0, st, S->dev, Q,
-                                       active, nullptr);
+                    hipLaunchKernelGGL(k_shade<false>, dim3((active + kShadeBlock - 1) / kShadeBlock),
+                                       dim3(kShadeBlock), 0, st, S->dev, Q, active, nullptr);
                 }
                 int32_t counts[2];
                 PG_HIP(hipMemcpyAsync(counts, Q.counters, sizeof(counts), hipMemcpyDeviceToHost, st));

@@ -672,13 +672,14 @@ def guided_leg(args, pg, W, work):
 
     pt, pe = plain(0)
     qt, qe = plain(GUIDED_RR_DEPTH)
-    gt, ge, train_ms = [], [], []
+    gt, ge, train_ms, step_ms = [], [], [], []
     for spp in spps:
         g = pg.Guide(seed=3, learning_rate=3e-3)
         inf = s.render_guided(g, final_spp=spp - GUIDED_KEPT_SPP, sample_offset=1 << 21,
                               rr_depth=GUIDED_RR_DEPTH, **GUIDED_TRAIN)
         gt.append(inf["last_guided_ms"])
         train_ms.append(inf["last_guided_ms"] - inf["last_render_ms"])
+        step_ms.append(inf["guide_step_ms"])
         score("guided", ge)
         g.release()
     # Guide inference on the matrix cores: one more guided render of the same
@@ -724,7 +725,8 @@ def guided_leg(args, pg, W, work):
         "plain": series(pt, pe, "plain_rr0"),
         f"plain_rr{GUIDED_RR_DEPTH}": series(qt, qe, f"plain_rr{GUIDED_RR_DEPTH}"),
         "guided": dict(series(gt, ge, "guided"), final_spp=[x - GUIDED_KEPT_SPP for x in spps],
-                       training_ms=round(float(np.mean(train_ms)), 2)),
+                       training_ms=round(float(np.mean(train_ms)), 2),
+                       training_adam_steps_ms=round(float(np.mean(step_ms)), 2)),
         "target_relmse": float(f"{target:.4g}"),
         "plain_ms_to_target": round(tp, 2) if tp else None,
         f"plain_rr{GUIDED_RR_DEPTH}_ms_to_target": round(tq, 2) if tq else None,

@@ -224,12 +224,15 @@ typedef struct pg_scene_info {
     double guide_infer_ms;        /* last guided render drawn launch by launch (PG_RENDER_NO_GRAPH):
                                      time of its guide-inference kernels on the GPU */
     int64_t guide_infer_queries;  /* ... and the network evaluations they made */
+    double guide_step_ms;         /* last pg_render_guided: GPU time of its Adam steps (the
+                                     training passes' network updates, not their rendering) */
 } pg_scene_info;
 int pg_scene_get_info(const pg_scene* s, pg_scene_info* info);
 
 typedef struct pg_render_options {
     int32_t spp;                /* 0 = the scene's samplesPerPixel */
-    int32_t samples_per_batch;  /* 0 = automatic (the scene's paths in flight, 4M unless tuned) */
+    int32_t samples_per_batch;  /* 0 = automatic (the scene's paths in flight, 256M unless tuned,
+                                   at most half the free device memory) */
     int32_t sample_offset;      /* first sample index per pixel (progressive rendering) */
     int32_t flags;              /* PG_RENDER_* bits */
     int32_t rr_depth;           /* Russian roulette (path.cpp:76-82) only after this depth; 0 =

@@ -78,7 +78,7 @@ class _SceneInfo(ctypes.Structure):
                 ("trace_block", ctypes.c_int32), ("trace_flags", ctypes.c_int32),
                 ("bvh_width", ctypes.c_int32), ("last_extension_rays", ctypes.c_int64),
                 ("apply_tone_mapping", ctypes.c_int32), ("guide_infer_ms", ctypes.c_double),
-                ("guide_infer_queries", ctypes.c_int64)]
+                ("guide_infer_queries", ctypes.c_int64), ("guide_step_ms", ctypes.c_double)]
 
 
 class _GuidedOptions(ctypes.Structure):

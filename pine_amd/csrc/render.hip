@@ -2766,6 +2766,7 @@ struct pg_scene {
     hipEvent_t infer_ev[2] = {nullptr, nullptr};
     double infer_ms = 0.0;
     int64_t infer_queries = 0;
+    double guide_step_ms = 0.0;  // last pg_render_guided: GPU time of its Adam steps
 
     ~pg_scene() {
         for (hipEvent_t e : infer_ev)
@@ -4415,6 +4416,7 @@ extern "C" int pg_scene_get_info(const pg_scene* S, pg_scene_info* info) {
     info->apply_tone_mapping = S->tone_map ? 1 : 0;
     info->guide_infer_ms = S->infer_ms;
     info->guide_infer_queries = S->infer_queries;
+    info->guide_step_ms = S->guide_step_ms;
     pg_accel_stats as{};
     if (pg_accel_get_stats(S->accel, &as) == PG_OK) {
         info->bvh_width = as.width;
@@ -4728,6 +4730,24 @@ struct GuideTraining {
     int batch;
     int64_t steps;         // Adam steps taken
     int64_t last_records;  // nonzero-weight records trained on
+    // Events around each batch's Adam steps, read once the render is done
+    // (no extra synchronisation inside the passes).
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> step_events;
+    double step_ms() const {
+        double t = 0.0;
+        for (const auto& e : step_events) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess)
+                t += ms;
+        }
+        return t;
+    }
+    ~GuideTraining() {
+        for (const auto& e : step_events) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+    }
 };
 
 static uint64_t gcd64(uint64_t a, uint64_t b) {
@@ -4977,6 +4997,11 @@ static int render_enqueue(pg_scene* S, PathState& W, int spp, int spb_req, hipSt
                     count = (n_train + steps - 1) / steps;
                 }
                 int64_t first = 0;
+                hipEvent_t e0 = nullptr, e1 = nullptr;
+                PG_HIP(hipEventCreate(&e0));
+                PG_HIP(hipEventCreate(&e1));
+                tr->step_events.emplace_back(e0, e1);
+                PG_HIP(hipEventRecord(e0, st));
                 for (int64_t k = 0; k < steps; k++) {
                     const int64_t take = std::min<int64_t>(count, n_train - first);
                     if (int rc = guide_train_step(tr->G, P.c_x, P.c_dir, P.c_w, P.c_pb, n_train, first, take, a, st))
@@ -4985,6 +5010,7 @@ static int render_enqueue(pg_scene* S, PathState& W, int spp, int spb_req, hipSt
                     if (first >= n_train)
                         first = 0;
                 }
+                PG_HIP(hipEventRecord(e1, st));
                 tr->steps += steps;
             }
         }
@@ -5352,7 +5378,7 @@ extern "C" int pg_render_guided(pg_scene* S, pg_guide* G, const pg_guided_option
     const int64_t npix = (int64_t)d.width * d.height;
     const int64_t per_batch = std::max<int64_t>(1, std::min<int64_t>(train_spp, (4 << 20) / std::max<int64_t>(npix, 1)));
     const int nbatches = (int)((train_spp + per_batch - 1) / per_batch);
-    GuideTraining tr{G, depth, steps > 0 ? std::max(1, (steps + nbatches - 1) / nbatches) : 0, batch, 0, 0};
+    GuideTraining tr{G, depth, steps > 0 ? std::max(1, (steps + nbatches - 1) / nbatches) : 0, batch, 0, 0, {}};
     int rc = PG_OK;
     // The last `keep` training passes stay in the film: each is an unbiased
     // guided render of its own sample range, and with the final render they
@@ -5378,6 +5404,7 @@ extern "C" int pg_render_guided(pg_scene* S, pg_guide* G, const pg_guided_option
     S->last_guide_ms = train_ms + ms;
     S->guide_steps = tr.steps;
     S->guide_records = tr.last_records;
+    S->guide_step_ms = tr.step_ms();  // every pass has finished (render_impl waited)
     return PG_OK;
 }
 

@@ -676,7 +676,7 @@ def guided_leg(args, pg, W, work):
     for spp in spps:
         g = pg.Guide(seed=3, learning_rate=3e-3)
         inf = s.render_guided(g, final_spp=spp - GUIDED_KEPT_SPP, sample_offset=1 << 21,
-                              rr_depth=GUIDED_RR_DEPTH, **GUIDED_TRAIN)
+                              rr_depth=GUIDED_RR_DEPTH, steps_per_pass=args.guide_steps, **GUIDED_TRAIN)
         gt.append(inf["last_guided_ms"])
         train_ms.append(inf["last_guided_ms"] - inf["last_render_ms"])
         step_ms.append(inf["guide_step_ms"])
@@ -686,7 +686,8 @@ def guided_leg(args, pg, W, work):
     # size drawn launch by launch, its inference kernels timed by HIP events.
     g = pg.Guide(seed=3, learning_rate=3e-3)
     inf = s.render_guided(g, final_spp=64 - GUIDED_KEPT_SPP, sample_offset=1 << 21,
-                          rr_depth=GUIDED_RR_DEPTH, graph=False, **GUIDED_TRAIN)
+                          rr_depth=GUIDED_RR_DEPTH, graph=False, steps_per_pass=args.guide_steps,
+                          **GUIDED_TRAIN)
     g.release()
     iq, ims = int(inf["guide_infer_queries"]), float(inf["guide_infer_ms"])
     tflops = iq * GUIDE_FLOP_PER_QUERY / (ims * 1e-3) / 1e12 if ims > 0 else None
@@ -772,6 +773,9 @@ def main():
                     help="occluded-box resolution of the guiding leg")
     ap.add_argument("--guide-ref-chunks", type=int, default=64,
                     help="1024-spp ranges in the guiding leg's reference")
+    ap.add_argument("--guide-steps", type=int, default=0,
+                    help="Adam steps per training pass of the guiding leg (0: one epoch of its "
+                         "records in at most 32 steps)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one device-to-device copy of the ray buffer (512 MiB), the "
                          "known-size kernel that scales rocprof's FETCH_SIZE/WRITE_SIZE")

@@ -628,8 +628,10 @@ def batch_leg(args, pg, W, work):
 
 
 GUIDED_RR_DEPTH = 4      # Russian roulette from depth 4 in the guided renders (pine: 2)
-GUIDED_TRAIN = dict(train_passes=8, train_spp=4, keep_passes=4)
-GUIDED_KEPT_SPP = GUIDED_TRAIN["train_spp"] * GUIDED_TRAIN["keep_passes"]
+# Training schedule of the guiding leg: 6 passes of 2 spp, the last 3 kept in the image
+# (time to the plain tracer's 512-spp error, 16-range reference: 8 x 4 spp kept 4: 330 ms;
+# 8 x 3: 292; 6 x 3: 273; 8 x 2: 280; 4 x 4: 283; 6 x 2: 261).
+GUIDED_TRAIN_DEFAULT = dict(train_passes=6, train_spp=2, keep_passes=3)
 
 
 def guided_leg(args, pg, W, work):
@@ -637,13 +639,16 @@ def guided_leg(args, pg, W, work):
     tracers, each at five sample counts, relMSE against a reference of the
     plain tracer (pine_amd.metrics: mean of 1024-spp ranges without singular
     ones): the plain tracer (pine's Russian roulette from depth 2), the plain
-    tracer with roulette from depth 4, and the guided tracer (8 training
-    passes of 4 spp, the last 4 kept in the image, roulette from depth 4) with
-    the same samples per pixel in the image as the plain one.  Reported: time
+    tracer with roulette from depth 4, and the guided tracer (6 training
+    passes of 2 spp by default, the last 3 kept in the image, roulette from
+    depth 4) with the same samples per pixel in the image as the plain one.  Reported: time
     for each to reach the plain tracer's error at 512 spp, and the split of
     the gain into the later roulette (plain rr4 vs plain) and guiding itself
     (guided vs plain rr4)."""
     from pine_amd import metrics as M
+    train_cfg = dict(train_passes=args.guide_passes, train_spp=args.guide_train_spp,
+                        keep_passes=args.guide_keep)
+    kept_spp = train_cfg["train_spp"] * train_cfg["keep_passes"]
     res = args.guide_res
     v, i = W.occluded_box_clutter()
     W.write_obj(os.path.join(work, "clutter.obj"), v, i)
@@ -675,8 +680,8 @@ def guided_leg(args, pg, W, work):
     gt, ge, train_ms, step_ms = [], [], [], []
     for spp in spps:
         g = pg.Guide(seed=3, learning_rate=3e-3)
-        inf = s.render_guided(g, final_spp=spp - GUIDED_KEPT_SPP, sample_offset=1 << 21,
-                              rr_depth=GUIDED_RR_DEPTH, steps_per_pass=args.guide_steps, **GUIDED_TRAIN)
+        inf = s.render_guided(g, final_spp=spp - kept_spp, sample_offset=1 << 21,
+                              rr_depth=GUIDED_RR_DEPTH, steps_per_pass=args.guide_steps, **train_cfg)
         gt.append(inf["last_guided_ms"])
         train_ms.append(inf["last_guided_ms"] - inf["last_render_ms"])
         step_ms.append(inf["guide_step_ms"])
@@ -685,9 +690,9 @@ def guided_leg(args, pg, W, work):
     # Guide inference on the matrix cores: one more guided render of the same
     # size drawn launch by launch, its inference kernels timed by HIP events.
     g = pg.Guide(seed=3, learning_rate=3e-3)
-    inf = s.render_guided(g, final_spp=64 - GUIDED_KEPT_SPP, sample_offset=1 << 21,
+    inf = s.render_guided(g, final_spp=64 - kept_spp, sample_offset=1 << 21,
                           rr_depth=GUIDED_RR_DEPTH, graph=False, steps_per_pass=args.guide_steps,
-                          **GUIDED_TRAIN)
+                          **train_cfg)
     g.release()
     iq, ims = int(inf["guide_infer_queries"]), float(inf["guide_infer_ms"])
     tflops = iq * GUIDE_FLOP_PER_QUERY / (ims * 1e-3) / 1e12 if ims > 0 else None
@@ -715,8 +720,8 @@ def guided_leg(args, pg, W, work):
 
     out = {
         "workload": f"occluded-light box (512,000 clutter triangles), {res}x{res}, maxDepth 8; "
-                    f"guide trained {GUIDED_TRAIN['train_passes']} passes x "
-                    f"{GUIDED_TRAIN['train_spp']} spp, the last {GUIDED_TRAIN['keep_passes']} "
+                    f"guide trained {train_cfg['train_passes']} passes x "
+                    f"{train_cfg['train_spp']} spp, the last {train_cfg['keep_passes']} "
                     f"kept in the image; Russian roulette from depth {GUIDED_RR_DEPTH} when guided",
         "reference": f"plain tracer, {args.guide_ref_chunks} x 1024 spp ({ref_s:.0f} s), "
                      f"per-pixel mean of the ranges without singular ones",
@@ -725,7 +730,7 @@ def guided_leg(args, pg, W, work):
         "spp_in_image": list(spps),
         "plain": series(pt, pe, "plain_rr0"),
         f"plain_rr{GUIDED_RR_DEPTH}": series(qt, qe, f"plain_rr{GUIDED_RR_DEPTH}"),
-        "guided": dict(series(gt, ge, "guided"), final_spp=[x - GUIDED_KEPT_SPP for x in spps],
+        "guided": dict(series(gt, ge, "guided"), final_spp=[x - kept_spp for x in spps],
                        training_ms=round(float(np.mean(train_ms)), 2),
                        training_adam_steps_ms=round(float(np.mean(step_ms)), 2)),
         "target_relmse": float(f"{target:.4g}"),
@@ -773,6 +778,12 @@ def main():
                     help="occluded-box resolution of the guiding leg")
     ap.add_argument("--guide-ref-chunks", type=int, default=64,
                     help="1024-spp ranges in the guiding leg's reference")
+    ap.add_argument("--guide-passes", type=int, default=GUIDED_TRAIN_DEFAULT["train_passes"],
+                    help="training passes of the guiding leg")
+    ap.add_argument("--guide-train-spp", type=int, default=GUIDED_TRAIN_DEFAULT["train_spp"],
+                    help="samples per pixel of each training pass")
+    ap.add_argument("--guide-keep", type=int, default=GUIDED_TRAIN_DEFAULT["keep_passes"],
+                    help="last training passes kept in the image")
     ap.add_argument("--guide-steps", type=int, default=0,
                     help="Adam steps per training pass of the guiding leg (0: one epoch of its "
                          "records in at most 32 steps)")

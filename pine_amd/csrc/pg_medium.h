@@ -227,9 +227,12 @@ PG_HD float fast_log2(float x) {
 #endif
 }
 
-// A free flight from t at extinction `rate`: t - ln(1 - u) / rate.
-PG_HD float brick_flight(float t, float u, float rate) {
-    return t - (fast_log2(1.0f - u) * 0.693147182f) / rate;
+// A free flight from t at extinction `rate`: t - ln(1 - u) / rate, with the
+// hardware log2 and scale = ln 2 / rate (brick_flight_scale), divided once per
+// brick instead of once per flight.
+PG_HD float brick_flight_scale(float rate) { return 0.693147182f / rate; }
+PG_HD float brick_flight(float t, float u, float scale) {
+    return t - fast_log2(1.0f - u) * scale;
 }
 
 // GridMedium::Density (medium.cpp:139-155) in float: trilinear weights as
@@ -414,9 +417,9 @@ PG_HD MediumEvent medium_sample(const MediumDev& m, const float* maj, const Samp
         const float mj = maj[brick_index(m, w)];
         const float rate = m.sigma_t * mj;
         if (mj > 0.0f && rate > 0.0f) {
-            const float inv_mj = 1.0f / mj;
+            const float inv_mj = 1.0f / mj, scale = brick_flight_scale(rate);
             while (true) {
-                t = brick_flight(t, track_uniform<kS>(hs, ss), rate);
+                t = brick_flight(t, track_uniform<kS>(hs, ss), scale);
                 if (!(t < texit))
                     break;
                 const f3 pm = ray_at(r, t);
@@ -492,9 +495,9 @@ PG_HD f3 medium_tr(const MediumDev& m, const float* maj, const SamplerDev& hs, S
         const float mj = maj[brick_index(m, w)];
         const float rate = m.sigma_t * mj;
         if (mj > 0.0f && rate > 0.0f) {
-            const float inv_mj = 1.0f / mj;
+            const float inv_mj = 1.0f / mj, scale = brick_flight_scale(rate);
             while (true) {
-                t = brick_flight(t, track_uniform<kS>(hs, ss), rate);
+                t = brick_flight(t, track_uniform<kS>(hs, ss), scale);
                 if (!(t < texit))
                     break;
                 lookups++;

@@ -1514,7 +1514,7 @@ enum TrackPhase : int { kEnter = 0, kFlight = 1, kLeave = 2 };
 struct LaneTrack {
     MRay r;
     BrickWalk w;
-    float t, t1, texit, rate, inv_mj;
+    float t, t1, texit, rate, inv_mj, scale;
     int phase;
 };
 
@@ -1530,13 +1530,14 @@ __device__ __forceinline__ int track_step_bricks(const MediumDev& m, const float
         L.rate = m.sigma_t * mj;
         if (mj > 0.0f && L.rate > 0.0f) {
             L.inv_mj = 1.0f / mj;
+            L.scale = brick_flight_scale(L.rate);
             L.phase = kFlight;
         } else {
             L.phase = kLeave;
         }
     }
     if (L.phase == kFlight) {
-        L.t = brick_flight(L.t, track_uniform<kS>(hs, ss), L.rate);
+        L.t = brick_flight(L.t, track_uniform<kS>(hs, ss), L.scale);
         if (L.t < L.texit)
             return 2;
         L.phase = kLeave;

@@ -2479,17 +2479,22 @@ __global__ void k_fill_f32(float* p, int64_t n, float v) {
 // range of such samples and k_film_border adds them afterwards.
 constexpr int kFilmTile = 2048;
 
+// Pixels per k_film workgroup: at least 64 (a wave of pixel threads for the
+// in-order sums), all of a tile's samples when a pixel has few.
 int film_pixels_per_block(int spb) {
-    return std::max(1, std::min(256, kFilmTile / std::max(spb, 1)));
+    return std::max(64, std::min(256, kFilmTile / std::max(spb, 1)));
 }
 
+// G pixels per workgroup; each tile holds C consecutive samples of each of
+// them (C = kFilmTile / G, at most spb), so that G threads, one per pixel, add
+// their pixel's samples in sample order, chunk after chunk.
 __global__ __launch_bounds__(256) void k_film(SceneDev s, PathsDev P, int32_t n_pixels, int32_t spb, int32_t G,
                                               float* film) {
     __shared__ float4 contrib[kFilmTile];
     __shared__ uint8_t kind[kFilmTile];  // bit 0: the own pixel is in the footprint; bit 1: another pixel is
     const int32_t p0 = blockIdx.x * G;
     const int32_t np = min(G, n_pixels - p0);
-    const int64_t base = (int64_t)p0 * spb, count = (int64_t)np * spb;
+    const int C = min(spb, kFilmTile / G);
     const int j = threadIdx.x;
     const bool owner = j < np;
     const float rx = s.filter_radius[0], ry = s.filter_radius[1];
@@ -2503,13 +2508,13 @@ __global__ __launch_bounds__(256) void k_film(SceneDev s, PathsDev P, int32_t n_
         for (int c = 0; c < 4; c++)
             acc[c] = film[4 * own + c];
     }
-    for (int64_t c0 = 0; c0 < count; c0 += kFilmTile) {
-        const int n = (int)min((int64_t)kFilmTile, count - c0);
-        __syncthreads();  // the previous tile has been read
-        for (int t = j; t < n; t += 256) {
-            const int64_t sl = c0 + t;
-            const int32_t q = p0 + (int32_t)(sl / spb);
-            const int64_t i = base + sl;
+    for (int s0 = 0; s0 < spb; s0 += C) {
+        const int cs = min(C, spb - s0);  // samples of each pixel in this tile
+        __syncthreads();                  // the previous tile has been read
+        for (int t = j; t < np * cs; t += 256) {
+            const int pl = t / cs, sl = s0 + (t - pl * cs);
+            const int32_t q = p0 + pl;
+            const int64_t i = (int64_t)q * spb + sl;
             const int qx = q % s.width, qy = q / s.width;
             const float fx = P.pfilm[2 * i] * (float)s.width - 0.5f;
             const float fy = P.pfilm[2 * i + 1] * (float)s.height - 0.5f;
@@ -2536,18 +2541,21 @@ __global__ __launch_bounds__(256) void k_film(SceneDev s, PathsDev P, int32_t n_
         }
         __syncthreads();
         if (owner) {
-            const int64_t lo = max((int64_t)j * spb, c0), hi = min((int64_t)(j + 1) * spb, c0 + n);
-            for (int64_t sl = lo; sl < hi; sl++) {
-                const uint8_t k = kind[sl - c0];
+            // The pixel's cs samples of this tile, in sample order; the LDS
+            // reads do not depend on the sums, so they are issued ahead.
+            const int t0 = j * cs;
+#pragma unroll 4
+            for (int t = 0; t < cs; t++) {
+                const uint8_t k = kind[t0 + t];
+                const float4 c = contrib[t0 + t];
                 if (k & 1) {
-                    const float4 c = contrib[sl - c0];
                     acc[0] += c.x;
                     acc[1] += c.y;
                     acc[2] += c.z;
                     acc[3] += c.w;
                 }
                 if (k & 2) {
-                    last = (int32_t)(sl - (int64_t)j * spb);
+                    last = s0 + t;
                     if (first < 0)
                         first = last;
                 }

@@ -107,28 +107,47 @@ PG_HD uint32_t udiv_magic(uint32_t a, uint32_t m, uint32_t post) {
     return (t + ((a - t) >> 1)) >> post;
 }
 
-PG_HD uint64_t inverse_radical_inverse(uint64_t inverse, int base, int nDigits) {
-    uint64_t index = 0;
+// InverseRadicalInverse (sampler.cpp:32-41) for the pixel offsets of
+// StartPixel, whose digits fit 32 bits: base 2 reverses the low nDigits bits,
+// base 3 takes nDigits digits least significant first.
+PG_HD uint32_t inverse_radical_inverse2(uint32_t inverse, int nDigits) {
+    if (nDigits <= 0)
+        return 0u;
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_bitreverse32(inverse) >> (32 - nDigits);
+#else
+    uint32_t index = 0;
+    for (int i = 0; i < nDigits; i++, inverse >>= 1)
+        index = (index << 1) | (inverse & 1u);
+    return index;
+#endif
+}
+PG_HD uint32_t inverse_radical_inverse3(uint32_t inverse, int nDigits) {
+    uint32_t index = 0;
     for (int i = 0; i < nDigits; i++) {
-        uint64_t digit = inverse % (uint64_t)base;
-        inverse /= (uint64_t)base;
-        index = index * (uint64_t)base + digit;
+        const uint32_t next = inverse / 3u;
+        index = index * 3u + (inverse - 3u * next);
+        inverse = next;
     }
     return index;
 }
 
-// HaltonSampler::StartPixel: the Halton index of sample `sampleIndex` of pixel p.
+// HaltonSampler::StartPixel (sampler.cpp:57-70): the Halton index of sample
+// `sampleIndex` of pixel p.  The per-pixel offset is formed in 32 bits, which
+// holds every intermediate of pine's 64-bit sum: the scales are at most 128
+// (base 2) and 243 (base 3), each offset is below its own scale and each
+// multiplicative inverse below the other scale, so each term is below
+// 243 * 128 * 243 < 2^23 and the sum below 2^24.
 PG_HD uint64_t halton_index(const HaltonDev& h, int px, int py, int sampleIndex) {
-    int64_t hi = 0;
+    uint32_t hi = 0;
     if (h.stride > 1) {
-        const int pm[2] = {px % 128, py % 128};
-        for (int i = 0; i < 2; i++) {
-            uint64_t off = inverse_radical_inverse((uint64_t)pm[i], i == 0 ? 2 : 3, h.baseExponents[i]);
-            hi += (int64_t)(off * (uint64_t)h.baseScales[1 - i] * (uint64_t)h.multInverse[1 - i]);
-        }
-        hi %= h.stride;
+        const uint32_t o0 = inverse_radical_inverse2((uint32_t)(px % 128), h.baseExponents[0]);
+        const uint32_t o1 = inverse_radical_inverse3((uint32_t)(py % 128), h.baseExponents[1]);
+        hi = (o0 * (uint32_t)h.baseScales[1] * (uint32_t)h.multInverse[1] +
+              o1 * (uint32_t)h.baseScales[0] * (uint32_t)h.multInverse[0]) %
+             (uint32_t)h.stride;
     }
-    return (uint64_t)(hi + (int64_t)sampleIndex * h.stride);
+    return (uint64_t)((int64_t)hi + (int64_t)sampleIndex * h.stride);
 }
 
 // ScrambledRadicalInverse for an index below 2^32: the digit loop in 32 bits,

@@ -244,10 +244,10 @@ __global__ __launch_bounds__(256) void k_raygen(SceneDev s, PathsDev P, int32_t 
         *d_np = n_paths;
     if (i >= n_paths)
         return;
-    const int32_t pixel = i / spb, sl = i - pixel * spb;
-    const int px = pixel % s.width, py = pixel / s.width;
+    const uint32_t pixel = (uint32_t)i / (uint32_t)spb, sl = (uint32_t)i - pixel * (uint32_t)spb;
+    const int py = (int)(pixel / (uint32_t)s.width), px = (int)pixel - py * s.width;
     SamplerState ss;
-    ss.index = sampler_start(s.sampler, px, py, sample0 + sl);
+    ss.index = sampler_start(s.sampler, px, py, sample0 + (int)sl);
     ss.dim = 2;
     float u0, u1, l0, l1;
     get_2d(s.sampler, ss, u0, u1);
@@ -272,7 +272,7 @@ __global__ __launch_bounds__(256) void k_raygen(SceneDev s, PathsDev P, int32_t 
     if (P.lam) {
         // Spectral mode: the hero-wavelength variable from a hash of (pixel,
         // sample index), so a render is deterministic and batch-independent.
-        uint32_t h = (uint32_t)pixel * 0x9E3779B1u ^ ((uint32_t)(sample0 + sl) + 0x7F4A7C15u) * 0x85EBCA77u;
+        uint32_t h = pixel * 0x9E3779B1u ^ ((uint32_t)sample0 + sl + 0x7F4A7C15u) * 0x85EBCA77u;
         h ^= h >> 15;
         h *= 0x2C1B3C6Du;
         h ^= h >> 12;
